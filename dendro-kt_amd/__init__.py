@@ -1,0 +1,238 @@
+"""Python host side of the MI355X adaptive k-tree FEM matvec.
+
+A thin ctypes layer over the C ABI in include/dkt.h (libdkt.so, built in-tree by
+build.py).  It mirrors the reference's host interfaces for this path:
+
+* ``balance_tree`` / ``construct_tree`` / ``sort_tree``  -- SFC_Tree::distTree*
+  (src/tsort.cpp:201-878)
+* ``DA``        -- ot::DA<dim> (include/oda.h:40-525): CG nodes in the
+  reference's order, boundary ids, and the flattened traversal
+* ``Plan``      -- the device matvec, fem::matvec / feMatrix::matVec
+  (FEM/include/matvec.h:231-548, FEM/include/feMatrix.h:189-259)
+
+Device buffers are passed as raw pointers (e.g. torch tensors' ``data_ptr()``)
+with a HIP stream handle; torch is plumbing here, never the compute path.  If
+the library is missing, every call raises: there is no fallback.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBNAME = "libdkt.so"
+
+CURVE_MORTON = 0
+CURVE_HILBERT = 1
+OP_STIFFNESS = 0
+OP_MASS = 1
+OP_IDENTITY = 2
+OP_CAPTURED = 3
+
+_lib = None
+
+
+class DktError(RuntimeError):
+    pass
+
+
+def lib_path():
+    return os.environ.get("DKT_LIB", os.path.join(HERE, LIBNAME))
+
+
+def load(path=None):
+    """Loads libdkt.so (once).  Raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    p = path or lib_path()
+    if not os.path.exists(p):
+        raise DktError(f"{p} not found: run __graft_entry__.build() (hipcc) first")
+    L = ctypes.CDLL(p)
+    c = ctypes
+    P = c.c_void_p
+    u32p = c.POINTER(c.c_uint32)
+    L.dkt_last_error.restype = c.c_char_p
+    L.dkt_octs_create.argtypes = [c.c_int, c.c_int, c.c_int, u32p, c.c_size_t, c.POINTER(P)]
+    L.dkt_octs_destroy.argtypes = [P]
+    L.dkt_octs_size.argtypes = [P]
+    L.dkt_octs_size.restype = c.c_size_t
+    L.dkt_octs_copy.argtypes = [P, u32p]
+    L.dkt_tree_sort.argtypes = [P]
+    L.dkt_tree_construct.argtypes = [P, c.c_size_t, c.POINTER(P)]
+    L.dkt_tree_balance.argtypes = [P, c.c_size_t, c.POINTER(P)]
+    L.dkt_tree_sphere.argtypes = [c.c_int, c.c_int, c.c_int, c.c_int, c.c_int, c.POINTER(c.c_double),
+                                  c.c_double, c.POINTER(P)]
+    L.dkt_points_gauss.argtypes = [c.c_int, c.c_int, c.c_int, c.c_size_t, c.c_int, c.c_double, c.c_uint64,
+                                   c.POINTER(P)]
+    L.dkt_keyed_uniform.argtypes = [c.c_uint64, c.c_int, u32p, c.c_size_t, c.POINTER(c.c_double)]
+    L.dkt_da_create.argtypes = [P, c.c_int, c.POINTER(P)]
+    L.dkt_da_destroy.argtypes = [P]
+    L.dkt_da_count.argtypes = [P, c.c_int]
+    L.dkt_da_count.restype = c.c_size_t
+    L.dkt_da_nodes.argtypes = [P, u32p]
+    L.dkt_da_elements.argtypes = [P, u32p]
+    L.dkt_da_boundary.argtypes = [P, u32p]
+    L.dkt_da_traversal.argtypes = [P, u32p, c.POINTER(c.c_uint8), c.POINTER(c.c_int32),
+                                   c.POINTER(c.c_int32), c.POINTER(c.c_int32)]
+    L.dkt_da_hanging_gather.argtypes = [P, c.POINTER(c.c_int64), c.POINTER(c.c_int32), c.POINTER(c.c_double)]
+    _bind_device(L)
+    _lib = L
+    return L
+
+
+def _bind_device(L):
+    c = ctypes
+    P = c.c_void_p
+    if not hasattr(L, "dkt_plan_create"):
+        return
+    L.dkt_plan_create.argtypes = [P, c.c_int, P, c.c_int, c.POINTER(P)]
+    L.dkt_plan_destroy.argtypes = [P]
+    L.dkt_plan_matvec.argtypes = [P, c.c_int, P, P, c.c_int, c.c_int, P]
+    L.dkt_plan_info.argtypes = [P, c.POINTER(c.c_double), c.c_int]
+
+
+def _chk(rc):
+    if rc != 0:
+        raise DktError(load().dkt_last_error().decode())
+
+
+def _u32(a):
+    a = np.ascontiguousarray(a, dtype=np.uint32)
+    return a, a.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32))
+
+
+class Octs:
+    """A host list of cells or points (uint32 [n, dim+1])."""
+
+    def __init__(self, handle, dim):
+        self.h = handle
+        self.dim = dim
+
+    @classmethod
+    def from_array(cls, cells, dim, maxdepth, curve=CURVE_HILBERT):
+        L = load()
+        a, ptr = _u32(cells)
+        h = ctypes.c_void_p()
+        _chk(L.dkt_octs_create(dim, maxdepth, curve, ptr, a.shape[0], ctypes.byref(h)))
+        return cls(h, dim)
+
+    def __len__(self):
+        return load().dkt_octs_size(self.h)
+
+    def array(self):
+        out = np.empty((len(self), self.dim + 1), dtype=np.uint32)
+        _chk(load().dkt_octs_copy(self.h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32))))
+        return out
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.dkt_octs_destroy(self.h)
+            self.h = None
+
+
+def sort_tree(octs):
+    _chk(load().dkt_tree_sort(octs.h))
+    return octs
+
+
+def construct_tree(pts, max_pts=1):
+    h = ctypes.c_void_p()
+    _chk(load().dkt_tree_construct(pts.h, max_pts, ctypes.byref(h)))
+    return Octs(h, pts.dim)
+
+
+def balance_tree(pts, max_pts=1):
+    h = ctypes.c_void_p()
+    _chk(load().dkt_tree_balance(pts.h, max_pts, ctypes.byref(h)))
+    return Octs(h, pts.dim)
+
+
+def sphere_tree(dim, maxdepth, base_level, max_level, center, radius, curve=CURVE_HILBERT):
+    c = (ctypes.c_double * dim)(*center)
+    h = ctypes.c_void_p()
+    _chk(load().dkt_tree_sphere(dim, maxdepth, curve, base_level, max_level, c, radius, ctypes.byref(h)))
+    return Octs(h, dim)
+
+
+def gauss_points(dim, maxdepth, n, ncomp, sigma, seed, curve=CURVE_HILBERT):
+    h = ctypes.c_void_p()
+    _chk(load().dkt_points_gauss(dim, maxdepth, curve, n, ncomp, sigma, seed, ctypes.byref(h)))
+    return Octs(h, dim)
+
+
+def keyed_uniform(seed, nodes):
+    a, ptr = _u32(nodes)
+    x = np.empty(a.shape[0], dtype=np.float64)
+    _chk(load().dkt_keyed_uniform(seed, a.shape[1] - 1, ptr, a.shape[0],
+                                  x.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+    return x
+
+
+class DA:
+    """Mesh over a balanced tree: ot::DA<dim>(tree, nEle, comm, order) on one rank."""
+
+    def __init__(self, tree, order=1):
+        self.dim = tree.dim
+        self.order = order
+        h = ctypes.c_void_p()
+        _chk(load().dkt_da_create(tree.h, order, ctypes.byref(h)))
+        self.h = h
+
+    def count(self, what):
+        return load().dkt_da_count(self.h, what)
+
+    @property
+    def n_elements(self):
+        return self.count(0)
+
+    @property
+    def n_nodes(self):
+        return self.count(1)
+
+    @property
+    def npe(self):
+        return self.count(5)
+
+    def nodes(self):
+        out = np.empty((self.n_nodes, self.dim + 1), dtype=np.uint32)
+        _chk(load().dkt_da_nodes(self.h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32))))
+        return out
+
+    def elements(self):
+        out = np.empty((self.n_elements, self.dim + 1), dtype=np.uint32)
+        _chk(load().dkt_da_elements(self.h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32))))
+        return out
+
+    def boundary(self):
+        out = np.empty(self.count(2), dtype=np.uint32)
+        _chk(load().dkt_da_boundary(self.h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32))))
+        return out
+
+    def traversal(self):
+        L, H, n = self.count(3), self.count(4), self.npe
+        cells = np.empty((L, self.dim + 1), dtype=np.uint32)
+        child = np.empty(L, dtype=np.uint8)
+        slot = np.empty((L, n), dtype=np.int32)
+        hang = np.empty(L, dtype=np.int32)
+        pslot = np.empty((H, n), dtype=np.int32)
+        c = ctypes
+        _chk(load().dkt_da_traversal(self.h, cells.ctypes.data_as(c.POINTER(c.c_uint32)),
+                                     child.ctypes.data_as(c.POINTER(c.c_uint8)),
+                                     slot.ctypes.data_as(c.POINTER(c.c_int32)),
+                                     hang.ctypes.data_as(c.POINTER(c.c_int32)),
+                                     pslot.ctypes.data_as(c.POINTER(c.c_int32))))
+        nh = self.count(7)
+        off = np.empty(H * n + 1, dtype=np.int64)
+        hnode = np.empty(nh, dtype=np.int32)
+        hw = np.empty(nh, dtype=np.float64)
+        _chk(load().dkt_da_hanging_gather(self.h, off.ctypes.data_as(c.POINTER(c.c_int64)),
+                                          hnode.ctypes.data_as(c.POINTER(c.c_int32)),
+                                          hw.ctypes.data_as(c.POINTER(c.c_double))))
+        return dict(cells=cells, child=child, slot=slot, hang_of=hang, pslot=pslot,
+                    hg_off=off, hg_node=hnode, hg_w=hw)
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.dkt_da_destroy(self.h)
+            self.h = None

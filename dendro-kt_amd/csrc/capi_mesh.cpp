@@ -1,0 +1,244 @@
+// C ABI over the host mesh layer (include/dkt.h).
+#include <cstring>
+#include <exception>
+#include <string>
+
+#include "../../include/dkt.h"
+#include "capi_common.h"
+#include "mesh.h"
+
+using namespace dkt;
+
+namespace dkt {
+thread_local std::string g_last_error;
+}
+
+extern "C" const char *dkt_last_error(void) { return dkt::g_last_error.c_str(); }
+
+struct dkt_octs {
+  MeshParams mp;
+  std::vector<Oct> v;
+};
+
+struct dkt_da {
+  MeshParams mp;
+  std::vector<Oct> elems;
+  NodeSet ns;
+  Traversal tr;
+};
+
+static int check_dims(int dim, int maxdepth) {
+  if (dim < 2 || dim > 4) throw std::invalid_argument("dim must be 2, 3 or 4");
+  if (maxdepth < 1 || maxdepth > 30) throw std::invalid_argument("maxdepth must be 1..30");
+  return 0;
+}
+
+extern "C" int dkt_octs_create(int dim, int maxdepth, int curve, const uint32_t *cells, size_t n, dkt_octs **out) {
+  return guarded([&] {
+    check_dims(dim, maxdepth);
+    if (!out || (n && !cells)) throw std::invalid_argument("null pointer");
+    auto *o = new dkt_octs;
+    o->mp.dim = dim;
+    o->mp.maxd = maxdepth;
+    o->mp.curve = curve;
+    o->v.resize(n);
+    const uint32_t top = 1u << maxdepth;
+    for (size_t i = 0; i < n; i++) {
+      Oct c{};
+      for (int d = 0; d < dim; d++) {
+        c.x[d] = cells[i * (dim + 1) + d];
+        if (c.x[d] >= top) {
+          delete o;
+          throw std::invalid_argument("cell coordinate outside [0, 2^maxdepth)");
+        }
+      }
+      c.lev = cells[i * (dim + 1) + dim];
+      if ((int)c.lev > maxdepth) {
+        delete o;
+        throw std::invalid_argument("cell level above maxdepth");
+      }
+      const uint32_t mask = (c.lev == (uint32_t)maxdepth) ? 0u : ((1u << (maxdepth - c.lev)) - 1u);
+      for (int d = 0; d < dim; d++) c.x[d] &= ~mask;
+      o->v[i] = c;
+    }
+    *out = o;
+  });
+}
+
+extern "C" int dkt_octs_destroy(dkt_octs *o) {
+  delete o;
+  return 0;
+}
+extern "C" size_t dkt_octs_size(const dkt_octs *o) { return o ? o->v.size() : 0; }
+extern "C" int dkt_octs_copy(const dkt_octs *o, uint32_t *cells) {
+  return guarded([&] {
+    if (!o || !cells) throw std::invalid_argument("null pointer");
+    const int dim = o->mp.dim;
+    for (size_t i = 0; i < o->v.size(); i++) {
+      for (int d = 0; d < dim; d++) cells[i * (dim + 1) + d] = o->v[i].x[d];
+      cells[i * (dim + 1) + dim] = o->v[i].lev;
+    }
+  });
+}
+
+extern "C" int dkt_tree_sort(dkt_octs *o) {
+  return guarded([&] {
+    if (!o) throw std::invalid_argument("null pointer");
+    sort_octants(o->mp, o->v, true);
+  });
+}
+
+extern "C" int dkt_tree_construct(const dkt_octs *pts, size_t max_pts, dkt_octs **out) {
+  return guarded([&] {
+    if (!pts || !out) throw std::invalid_argument("null pointer");
+    auto *t = new dkt_octs;
+    t->mp = pts->mp;
+    std::vector<Oct> tmp = pts->v;
+    t->v = construct_tree(pts->mp, tmp, max_pts);
+    *out = t;
+  });
+}
+
+extern "C" int dkt_tree_balance(const dkt_octs *pts, size_t max_pts, dkt_octs **out) {
+  return guarded([&] {
+    if (!pts || !out) throw std::invalid_argument("null pointer");
+    auto *t = new dkt_octs;
+    t->mp = pts->mp;
+    std::vector<Oct> tmp = pts->v;
+    t->v = balance_tree(pts->mp, tmp, max_pts);
+    *out = t;
+  });
+}
+
+extern "C" int dkt_tree_sphere(int dim, int maxdepth, int curve, int base_level, int max_level, const double *center,
+                               double radius, dkt_octs **out) {
+  return guarded([&] {
+    check_dims(dim, maxdepth);
+    if (!center || !out) throw std::invalid_argument("null pointer");
+    auto *t = new dkt_octs;
+    t->mp.dim = dim;
+    t->mp.maxd = maxdepth;
+    t->mp.curve = curve;
+    t->v = sphere_tree(t->mp, base_level, max_level, center, radius);
+    *out = t;
+  });
+}
+
+extern "C" int dkt_points_gauss(int dim, int maxdepth, int curve, size_t n, int ncomp, double sigma, uint64_t seed,
+                                dkt_octs **out) {
+  return guarded([&] {
+    check_dims(dim, maxdepth);
+    if (!out || ncomp < 1) throw std::invalid_argument("bad argument");
+    auto *t = new dkt_octs;
+    t->mp.dim = dim;
+    t->mp.maxd = maxdepth;
+    t->mp.curve = curve;
+    t->v = gauss_points(t->mp, n, ncomp, sigma, seed);
+    *out = t;
+  });
+}
+
+extern "C" int dkt_keyed_uniform(uint64_t seed, int dim, const uint32_t *nodes, size_t n, double *x) {
+  return guarded([&] {
+    if (dim < 1 || dim > 4 || (n && (!nodes || !x))) throw std::invalid_argument("bad argument");
+    for (size_t i = 0; i < n; i++) x[i] = keyed_uniform(seed, nodes + i * (dim + 1), dim);
+  });
+}
+
+extern "C" int dkt_da_create(const dkt_octs *tree, int order, dkt_da **out) {
+  return guarded([&] {
+    if (!tree || !out) throw std::invalid_argument("null pointer");
+    if (order < 1 || order > 2) throw std::invalid_argument("order must be 1 or 2");
+    if (tree->v.empty()) throw std::invalid_argument("empty tree");
+    auto *da = new dkt_da;
+    try {
+      da->mp = tree->mp;
+      da->mp.order = order;
+      da->elems = tree->v;
+      for (const Oct &e : da->elems)
+        if (order > 1 && (int)e.lev >= da->mp.maxd)
+          throw std::invalid_argument("order 2 needs every element level below maxdepth");
+      da->ns = enumerate_nodes(da->mp, da->elems);
+      da->tr = flatten_traversal(da->mp, da->ns.nodes);
+    } catch (...) {
+      delete da;
+      throw;
+    }
+    *out = da;
+  });
+}
+
+extern "C" int dkt_da_destroy(dkt_da *da) {
+  delete da;
+  return 0;
+}
+
+extern "C" size_t dkt_da_count(const dkt_da *da, int what) {
+  if (!da) return 0;
+  switch (what) {
+    case 0: return da->elems.size();
+    case 1: return da->ns.nodes.size();
+    case 2: return da->ns.bdy.size();
+    case 3: return da->tr.leaves.size();
+    case 4: return da->tr.pslot.size() / (size_t)da->tr.nPe;
+    case 5: return (size_t)da->tr.nPe;
+    case 6: return da->tr.stale_needed;
+    case 7: return da->tr.hg_node.size();
+    default: return 0;
+  }
+}
+
+static void copy_octs(const std::vector<Oct> &v, int dim, uint32_t *out) {
+  for (size_t i = 0; i < v.size(); i++) {
+    for (int d = 0; d < dim; d++) out[i * (dim + 1) + d] = v[i].x[d];
+    out[i * (dim + 1) + dim] = v[i].lev;
+  }
+}
+
+extern "C" int dkt_da_nodes(const dkt_da *da, uint32_t *out) {
+  return guarded([&] {
+    if (!da || !out) throw std::invalid_argument("null pointer");
+    copy_octs(da->ns.nodes, da->mp.dim, out);
+  });
+}
+extern "C" int dkt_da_elements(const dkt_da *da, uint32_t *out) {
+  return guarded([&] {
+    if (!da || !out) throw std::invalid_argument("null pointer");
+    copy_octs(da->elems, da->mp.dim, out);
+  });
+}
+extern "C" int dkt_da_boundary(const dkt_da *da, uint32_t *out) {
+  return guarded([&] {
+    if (!da || !out) throw std::invalid_argument("null pointer");
+    std::memcpy(out, da->ns.bdy.data(), da->ns.bdy.size() * sizeof(uint32_t));
+  });
+}
+extern "C" int dkt_da_traversal(const dkt_da *da, uint32_t *cells, uint8_t *child, int32_t *slot, int32_t *hang_of,
+                                int32_t *pslot) {
+  return guarded([&] {
+    if (!da) throw std::invalid_argument("null pointer");
+    const Traversal &t = da->tr;
+    if (cells) copy_octs(t.leaves, da->mp.dim, cells);
+    if (child) std::memcpy(child, t.child.data(), t.child.size());
+    if (slot) std::memcpy(slot, t.slot.data(), t.slot.size() * sizeof(int32_t));
+    if (hang_of) std::memcpy(hang_of, t.hang_of.data(), t.hang_of.size() * sizeof(int32_t));
+    if (pslot) std::memcpy(pslot, t.pslot.data(), t.pslot.size() * sizeof(int32_t));
+  });
+}
+
+extern "C" int dkt_da_hanging_gather(const dkt_da *da, int64_t *off, int32_t *node, double *w) {
+  return guarded([&] {
+    if (!da) throw std::invalid_argument("null pointer");
+    const Traversal &t = da->tr;
+    if (off) std::memcpy(off, t.hg_off.data(), t.hg_off.size() * sizeof(int64_t));
+    if (node) std::memcpy(node, t.hg_node.data(), t.hg_node.size() * sizeof(int32_t));
+    if (w) std::memcpy(w, t.hg_w.data(), t.hg_w.size() * sizeof(double));
+  });
+}
+
+// Internal accessors for the device plan (plan.cpp).
+namespace dkt {
+const MeshParams &da_params(const dkt_da *da) { return da->mp; }
+const NodeSet &da_nodes(const dkt_da *da) { return da->ns; }
+const Traversal &da_traversal(const dkt_da *da) { return da->tr; }
+}  // namespace dkt

@@ -1,0 +1,840 @@
+// Host mesh layer (see mesh.h).  Citations are to the reference Dendro-KT
+// sources whose behaviour each function reproduces.
+#include "mesh.h"
+
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <map>
+#include <mutex>
+#include <numeric>
+#include <parallel/algorithm>
+#include <stdexcept>
+#include <cstdio>
+
+namespace dkt {
+
+// ---------------------------------------------------------------------------
+// Space-filling curve tables.
+//
+// Hilbert: the K-dimensional harmonious Hilbert curve of Haverkort (2012,
+// "Harmonious Hilbert curves and other extradimensional space-filling curves",
+// p.23), the curve the reference tabulates (gen/RotationTableHilbert.cpp,
+// src/KDhcurvedata_DATA.cpp).  An orientation is a permutation `a` of the K
+// axes plus a reflection bit vector `m`; bit strings are written with
+// Haverkort's index 0 as the most significant of K bits.  Orientations are
+// numbered in first-visit order of a depth-first refinement from the identity,
+// which yields the same row numbering as the reference tables.
+// ---------------------------------------------------------------------------
+namespace {
+
+struct Orient {
+  std::array<int, 4> a{};
+  unsigned m = 0;
+  int K = 0;
+  unsigned permute(unsigned bits) const {
+    unsigned out = 0;
+    for (int i = 0; i < K; i++) out = (out << 1) | ((bits >> (K - 1 - a[i])) & 1u);
+    return out;
+  }
+  unsigned apply(unsigned loc) const { return permute(loc) ^ m; }
+  // (this) o (q): permutations compose as r.a[i] = q.a[a[i]], reflections as m ^ permute(q.m).
+  Orient compose(const Orient &q) const {
+    Orient r;
+    r.K = K;
+    for (int i = 0; i < K; i++) r.a[i] = q.a[a[i]];
+    r.m = m ^ permute(q.m);
+    return r;
+  }
+  unsigned code() const {
+    unsigned c = m;
+    for (int i = 0; i < K; i++) c = c * 8 + (unsigned)a[i];
+    return c;
+  }
+};
+
+// Haverkort's refinement operator: location (Gray code) and orientation of the
+// child at curve rank r.
+void refine(int K, int r, unsigned &loc, Orient &o) {
+  const unsigned c = (unsigned)(r ^ (r >> 1));
+  loc = c;
+  o.K = K;
+  if (r == 0) {
+    o.m = 0;
+  } else {
+    const unsigned cm1 = (unsigned)((r - 1) ^ ((r - 1) >> 1));
+    o.m = (cm1 & ~1u) | ((~c) & 1u);
+  }
+  const int endr = r & 1;
+  int front = 0;
+  for (int i = 0; i < K; i++) front += (((r >> (K - 1 - i)) & 1) != endr);
+  int L = front - 1, R = K - 1;
+  for (int i = 0; i < K; i++) {
+    if (((r >> (K - 1 - i)) & 1) != endr)
+      o.a[i] = L--;
+    else
+      o.a[i] = R--;
+  }
+}
+
+SfcTables build_hilbert(int K) {
+  const int nch = 1 << K;
+  std::vector<unsigned> locs(nch);
+  std::vector<Orient> kids(nch);
+  for (int r = 0; r < nch; r++) refine(K, r, locs[r], kids[r]);
+
+  Orient id;
+  id.K = K;
+  for (int i = 0; i < K; i++) id.a[i] = i;
+  id.m = 0;
+
+  std::map<unsigned, int> rank;
+  std::vector<Orient> byRank;
+  // Depth-first, children in curve order; numbering on first visit.
+  std::vector<std::pair<Orient, int>> stack;
+  rank[id.code()] = 0;
+  byRank.push_back(id);
+  stack.push_back({id, 0});
+  while (!stack.empty()) {
+    auto &top = stack.back();
+    if (top.second >= nch) {
+      stack.pop_back();
+      continue;
+    }
+    const Orient c = top.first.compose(kids[top.second]);
+    top.second++;
+    if (rank.find(c.code()) == rank.end()) {
+      rank[c.code()] = (int)byRank.size();
+      byRank.push_back(c);
+      stack.push_back({c, 0});
+    }
+  }
+  SfcTables t;
+  t.dim = K;
+  t.nch = nch;
+  t.nrot = (int)byRank.size();
+  t.perm.assign((size_t)t.nrot * nch, 0);
+  t.inv.assign((size_t)t.nrot * nch, 0);
+  t.child.assign((size_t)t.nrot * nch, 0);
+  for (int p = 0; p < t.nrot; p++) {
+    const Orient &po = byRank[p];
+    for (int cs = 0; cs < nch; cs++) {
+      const unsigned mort = po.apply(locs[cs]);
+      const Orient co = po.compose(kids[cs]);
+      t.perm[p * nch + cs] = (uint8_t)mort;
+      t.inv[p * nch + mort] = (uint8_t)cs;
+      t.child[p * nch + mort] = rank.at(co.code());
+    }
+  }
+  return t;
+}
+
+SfcTables build_morton(int K) {
+  SfcTables t;
+  t.dim = K;
+  t.nch = 1 << K;
+  t.nrot = 1;
+  t.perm.resize(t.nch);
+  t.inv.resize(t.nch);
+  t.child.assign(t.nch, 0);
+  for (int c = 0; c < t.nch; c++) t.perm[c] = t.inv[c] = (uint8_t)c;
+  return t;
+}
+
+}  // namespace
+
+const SfcTables &sfc_tables(int dim, int curve) {
+  static std::mutex mu;
+  static std::map<int, SfcTables> cache;
+  if (dim < 1 || dim > 4) throw std::invalid_argument("dim must be 1..4");
+  std::lock_guard<std::mutex> g(mu);
+  const int key = dim * 2 + (curve == CURVE_HILBERT ? 1 : 0);
+  auto it = cache.find(key);
+  if (it != cache.end()) return it->second;
+  SfcTables t = (curve == CURVE_HILBERT && dim >= 2) ? build_hilbert(dim) : build_morton(dim);
+  return cache.emplace(key, std::move(t)).first->second;
+}
+
+// ---------------------------------------------------------------------------
+// Keys and sorting.
+// ---------------------------------------------------------------------------
+static inline unsigned morton_digit(const Oct &o, int dim, int maxd, int lev) {
+  unsigned m = 0;
+  const int sh = maxd - lev;
+  for (int d = 0; d < dim; d++) m |= ((o.x[d] >> sh) & 1u) << d;
+  return m;
+}
+
+u128 sfc_key(const MeshParams &mp, const Oct &o) {
+  const SfcTables &t = sfc_tables(mp.dim, mp.curve);
+  int rot = 0;
+  u128 key = 0;
+  for (int l = 1; l <= mp.maxd; l++) {
+    unsigned digit = 0;
+    if (l <= (int)o.lev) {
+      const unsigned m = morton_digit(o, mp.dim, mp.maxd, l);
+      digit = t.inv[rot * t.nch + m];
+      rot = t.child[rot * t.nch + m];
+    }
+    key = (key << mp.dim) | digit;
+  }
+  return key;
+}
+
+namespace {
+struct KeyedIdx {
+  u128 key;
+  u32 lev;
+  u32 idx;
+};
+inline bool keyed_less(const KeyedIdx &a, const KeyedIdx &b) {
+  if (a.key != b.key) return a.key < b.key;
+  if (a.lev != b.lev) return a.lev < b.lev;
+  return a.idx < b.idx;
+}
+inline bool is_ancestor(const Oct &a, const Oct &b, int dim, int maxd) {
+  if (a.lev >= b.lev) return false;
+  const int sh = maxd - (int)a.lev;
+  for (int d = 0; d < dim; d++)
+    if (sh < 32 ? ((a.x[d] >> sh) != (b.x[d] >> sh)) : false) return false;
+  return true;
+}
+inline bool same_oct(const Oct &a, const Oct &b, int dim) {
+  if (a.lev != b.lev) return false;
+  for (int d = 0; d < dim; d++)
+    if (a.x[d] != b.x[d]) return false;
+  return true;
+}
+
+std::vector<KeyedIdx> keyed(const MeshParams &mp, const std::vector<Oct> &octs) {
+  std::vector<KeyedIdx> k(octs.size());
+#pragma omp parallel for schedule(static)
+  for (size_t i = 0; i < octs.size(); i++) k[i] = {sfc_key(mp, octs[i]), octs[i].lev, (u32)i};
+  __gnu_parallel::sort(k.begin(), k.end(), keyed_less);
+  return k;
+}
+}  // namespace
+
+void sort_octants(const MeshParams &mp, std::vector<Oct> &octs, bool remove_ancestors) {
+  auto k = keyed(mp, octs);
+  std::vector<Oct> out;
+  out.reserve(octs.size());
+  for (size_t i = 0; i < k.size(); i++) {
+    const Oct &cur = octs[k[i].idx];
+    if (i + 1 < k.size()) {
+      const Oct &nx = octs[k[i + 1].idx];
+      if (same_oct(cur, nx, mp.dim)) continue;
+      if (remove_ancestors && is_ancestor(cur, nx, mp.dim, mp.maxd)) continue;
+    }
+    out.push_back(cur);
+  }
+  octs.swap(out);
+}
+
+// ---------------------------------------------------------------------------
+// Tree construction (locTreeConstruction) and balancing (distTreeBalancing).
+// ---------------------------------------------------------------------------
+namespace {
+struct Builder {
+  const MeshParams &mp;
+  const SfcTables &t;
+  const std::vector<KeyedIdx> &k;
+  size_t max_pts;
+  std::vector<Oct> &out;
+
+  void rec(size_t b, size_t e, const Oct &cell, int rot) {
+    const int l = (int)cell.lev + 1;  // level of the children
+    while (b < e && (int)k[b].lev < l) b++;  // the cell itself / ancestors precede in pre-order
+    const int sh = mp.dim * (mp.maxd - l);
+    const u32 len = 1u << (mp.maxd - l);
+    size_t s = b;
+    for (int cs = 0; cs < t.nch; cs++) {
+      size_t f = s;
+      while (f < e && (unsigned)((k[f].key >> sh) & (u128)(t.nch - 1)) == (unsigned)cs) f++;
+      const unsigned m = t.perm[rot * t.nch + cs];
+      Oct ch;
+      ch.lev = (u32)l;
+      for (int d = 0; d < 4; d++) ch.x[d] = d < mp.dim ? cell.x[d] + (((m >> d) & 1u) ? len : 0u) : 0u;
+      if (l < mp.maxd && f - s > max_pts)
+        rec(s, f, ch, t.child[rot * t.nch + m]);
+      else
+        out.push_back(ch);
+      s = f;
+    }
+  }
+};
+}  // namespace
+
+std::vector<Oct> construct_tree(const MeshParams &mp, std::vector<Oct> &pts, size_t max_pts) {
+  const SfcTables &t = sfc_tables(mp.dim, mp.curve);
+  auto k = keyed(mp, pts);
+  std::vector<Oct> out;
+  Oct root{};
+  root.lev = 0;
+  Builder b{mp, t, k, max_pts, out};
+  b.rec(0, k.size(), root, 0);
+  return out;
+}
+
+namespace {
+inline u128 pack(const Oct &o, int dim) {
+  u128 v = 0;
+  for (int d = 0; d < dim; d++) v = (v << 32) | o.x[d];
+  return v;
+}
+void dedupe_level(std::vector<Oct> &v, int dim) {
+  std::vector<std::pair<u128, u32>> k(v.size());
+  for (size_t i = 0; i < v.size(); i++) k[i] = {pack(v[i], dim), (u32)i};
+  __gnu_parallel::sort(k.begin(), k.end());
+  std::vector<Oct> out;
+  out.reserve(v.size());
+  for (size_t i = 0; i < k.size(); i++)
+    if (i == 0 || k[i].first != k[i - 1].first) out.push_back(v[k[i].second]);
+  v.swap(out);
+}
+}  // namespace
+
+std::vector<Oct> balance_tree(const MeshParams &mp, std::vector<Oct> &pts, size_t max_pts) {
+  std::vector<Oct> t0 = construct_tree(mp, pts, max_pts);
+  // propagateNeighbours (src/tsort.cpp:777-820): bottom-up, every octant adds its
+  // parent and the parent's in-domain neighbours (all 3^dim-1) at the parent level.
+  std::vector<std::vector<Oct>> lv(mp.maxd + 1);
+  for (const Oct &o : t0) lv[o.lev].push_back(o);
+  const u32 top = (u32)1 << mp.maxd;
+  int nbr = 1;
+  for (int d = 0; d < mp.dim; d++) nbr *= 3;
+  for (int l = mp.maxd; l > 0; l--) {
+    std::vector<Oct> par;
+    par.reserve(lv[l].size());
+    const u32 plen = 1u << (mp.maxd - (l - 1));
+    for (const Oct &o : lv[l]) {
+      Oct p = o;
+      p.lev = (u32)(l - 1);
+      for (int d = 0; d < mp.dim; d++) p.x[d] = o.x[d] & ~(plen - 1);
+      par.push_back(p);
+    }
+    dedupe_level(par, mp.dim);
+    std::vector<Oct> &dst = lv[l - 1];
+    dst.reserve(dst.size() + par.size() * nbr);
+    for (const Oct &p : par) {
+      for (int code = 0; code < nbr; code++) {
+        Oct q = p;
+        int c = code;
+        bool ok = true;
+        for (int d = 0; d < mp.dim; d++) {
+          const int off = c % 3 - 1;
+          c /= 3;
+          if (off < 0) {
+            if (p.x[d] < plen) ok = false;
+            else q.x[d] = p.x[d] - plen;
+          } else if (off > 0) {
+            if (p.x[d] >= top - plen) ok = false;
+            else q.x[d] = p.x[d] + plen;
+          }
+        }
+        if (ok) dst.push_back(q);
+      }
+    }
+    dedupe_level(dst, mp.dim);
+  }
+  std::vector<Oct> seeds;
+  size_t tot = 0;
+  for (auto &v : lv) tot += v.size();
+  seeds.reserve(tot);
+  for (auto &v : lv) seeds.insert(seeds.end(), v.begin(), v.end());
+  lv.clear();
+  return construct_tree(mp, seeds, 1);
+}
+
+// ---------------------------------------------------------------------------
+// CG node enumeration (SFC_NodeSort::countCGNodes, include/nsort.tcc:879-1313,
+// single-rank compaction :519-533; DA::construct src/oda.cpp:90-107).
+// ---------------------------------------------------------------------------
+namespace {
+struct Inst {
+  u128 pos;   // packed coordinates
+  u32 lev;
+  u32 elem;
+};
+
+inline bool on_boundary(const Oct &o, int dim, int maxd) {
+  const u32 mask = (1u << maxd) - 1u;
+  for (int d = 0; d < dim; d++)
+    if ((o.x[d] & mask) == 0) return true;
+  return false;
+}
+
+// Number of axes along which the point is interior to its cell at level lev.
+inline int cell_dim(const Oct &o, int dim, int maxd, int lev) {
+  const u32 mask = (1u << (maxd - lev)) - 1u;
+  int c = 0;
+  for (int d = 0; d < dim; d++) c += (o.x[d] & mask) != 0;
+  return c;
+}
+
+// Full point key: child_sfc digits of the point's coordinate bits for levels
+// 1..maxd (rotation chain from the root), as locTreeSortAsPoints orders points.
+inline u128 point_key(const SfcTables &t, const Oct &o, int dim, int maxd) {
+  int rot = 0;
+  u128 key = 0;
+  for (int l = 1; l <= maxd; l++) {
+    const unsigned m = morton_digit(o, dim, maxd, l);
+    key = (key << dim) | t.inv[rot * t.nch + m];
+    rot = t.child[rot * t.nch + m];
+  }
+  return key;
+}
+
+struct InteriorKey {
+  u128 key;
+  int lc;   // level of the finest open container (getFinestOpenContainer)
+  int hp;   // first incident hyperplane at level lc+1 (get_firstIncidentHyperplane)
+  u32 idx;
+};
+}  // namespace
+
+NodeSet enumerate_nodes(const MeshParams &mp, const std::vector<Oct> &elems) {
+  const int dim = mp.dim, maxd = mp.maxd, p = mp.order;
+  if (p < 1) throw std::invalid_argument("order must be >= 1");
+  int npe = 1, nint = 1;
+  for (int d = 0; d < dim; d++) npe *= (p + 1), nint *= (p - 1);
+  const int next = npe - nint;
+
+  // 1. Exterior node instances (Element::appendExteriorNodes, nsort.tcc:360-389).
+  std::vector<Oct> inst(elems.size() * (size_t)next);
+  std::vector<int> extIdx;
+  for (int r = 0; r < npe; r++) {
+    bool ext = false;
+    int rr = r;
+    for (int d = 0; d < dim; d++) {
+      const int i = rr % (p + 1);
+      rr /= (p + 1);
+      if (i == 0 || i == p) ext = true;
+    }
+    if (ext) extIdx.push_back(r);
+  }
+#pragma omp parallel for schedule(static)
+  for (size_t e = 0; e < elems.size(); e++) {
+    const Oct &el = elems[e];
+    const u64 len = (u64)1 << (maxd - el.lev);
+    for (int j = 0; j < next; j++) {
+      int rr = extIdx[j];
+      Oct o{};
+      o.lev = el.lev;
+      for (int d = 0; d < dim; d++) {
+        const int i = rr % (p + 1);
+        rr /= (p + 1);
+        o.x[d] = (u32)(len * (u64)i / (u64)p + el.x[d]);
+      }
+      inst[e * next + j] = o;
+    }
+  }
+
+  // 2. Resolve duplicates and hanging nodes per location
+  //    (resolveInterface_lowOrder nsort.tcc:1266-1313; boundary: coarsest wins, :913-926).
+  if (p > 2) throw std::invalid_argument("orders above 2 are not supported");
+  std::vector<std::pair<u128, u32>> srt(inst.size());
+#pragma omp parallel for schedule(static)
+  for (size_t i = 0; i < inst.size(); i++) srt[i] = {(pack(inst[i], dim) << 5) | inst[i].lev, (u32)i};
+  __gnu_parallel::sort(srt.begin(), srt.end());
+  std::vector<u32> sel;
+  sel.reserve(inst.size() / 2);
+  for (size_t b = 0; b < srt.size();) {
+    size_t e = b + 1;
+    const u128 pos = srt[b].first >> 5;
+    while (e < srt.size() && (srt[e].first >> 5) == pos) e++;
+    const Oct &first = inst[srt[b].second];  // coarsest instance (sorted by level)
+    const bool mixed = inst[srt[e - 1].second].lev != first.lev;
+    bool keep;
+    if (on_boundary(first, dim, maxd) || mixed) {
+      keep = true;
+    } else {
+      const int cd = cell_dim(first, dim, maxd, first.lev);
+      keep = (e - b) == ((size_t)1 << (dim - cd));
+    }
+    if (keep) sel.push_back(srt[b].second);
+    b = e;
+  }
+  srt.clear();
+  srt.shrink_to_fit();
+
+  // 3. Order: interior nodes by (container post-order, hyperplane, point key);
+  //    boundary nodes after them, by (root bucket, point key).
+  const SfcTables &t = sfc_tables(dim, mp.curve);
+  std::vector<InteriorKey> ik;
+  std::vector<std::pair<std::pair<int, u128>, u32>> bk;
+  for (u32 s : sel) {
+    const Oct &o = inst[s];
+    if (on_boundary(o, dim, maxd)) {
+      unsigned m0 = 0;
+      for (int d = 0; d < dim; d++) m0 |= ((o.x[d] >> maxd) & 1u) << d;
+      bk.push_back({{(int)t.inv[m0], point_key(t, o, dim, maxd)}, s});
+    } else {
+      int maxlow = 0;
+      for (int d = 0; d < dim; d++) {
+        const int low = __builtin_ctz(o.x[d]);
+        maxlow = std::max(maxlow, low);
+      }
+      const int lc = maxd - maxlow - 1;
+      const u32 hmask = (1u << (maxd - (lc + 1))) - 1u;
+      int hp = dim;
+      for (int d = 0; d < dim; d++)
+        if ((o.x[d] & hmask) == 0) {
+          hp = d;
+          break;
+        }
+      ik.push_back({point_key(t, o, dim, maxd), lc, hp, s});
+    }
+  }
+  const int D = dim, MD = maxd;
+  __gnu_parallel::sort(ik.begin(), ik.end(), [D, MD](const InteriorKey &a, const InteriorKey &b) {
+    const int l = std::min(a.lc, b.lc);
+    const int sh = D * (MD - l);
+    const u128 pa = sh >= 128 ? 0 : (a.key >> sh), pb = sh >= 128 ? 0 : (b.key >> sh);
+    if (pa != pb) return pa < pb;
+    if (a.lc != b.lc) return a.lc > b.lc;
+    if (a.hp != b.hp) return a.hp < b.hp;
+    return a.key < b.key;
+  });
+  __gnu_parallel::sort(bk.begin(), bk.end());
+
+  NodeSet ns;
+  ns.nodes.reserve(sel.size() + elems.size() * (size_t)nint);
+  for (const auto &k : ik) ns.nodes.push_back(inst[k.idx]);
+  for (const auto &k : bk) ns.nodes.push_back(inst[k.second]);
+  ns.n_exterior = ns.nodes.size();
+
+  // 4. Element-interior nodes, appended in element order (oda.cpp:100-101).
+  if (nint > 0) {
+    for (const Oct &el : elems) {
+      const u64 len = (u64)1 << (maxd - el.lev);
+      for (int j = 0; j < nint; j++) {
+        int rr = j;
+        Oct o{};
+        o.lev = el.lev;
+        for (int d = 0; d < dim; d++) {
+          const int i = rr % (p - 1);
+          rr /= (p - 1);
+          o.x[d] = (u32)(len * (u64)(i + 1) / (u64)p + el.x[d]);
+        }
+        ns.nodes.push_back(o);
+      }
+    }
+  }
+  for (size_t i = 0; i < ns.nodes.size(); i++)
+    if (on_boundary(ns.nodes[i], dim, maxd)) ns.bdy.push_back((u32)i);
+  return ns;
+}
+
+// ---------------------------------------------------------------------------
+// Traversal flattening (fem::matvec_rec, FEM/include/matvec.h:243-548).
+// ---------------------------------------------------------------------------
+void interp_1d(int p, int c, double *ip) {
+  // Parent nodes at t_k = k/p on [0,1]; child c covers [c/2, (c+1)/2].
+  const int n = p + 1;
+  for (int k = 0; k < n; k++)
+    for (int i = 0; i < n; i++) {
+      const double x = 0.5 * ((double)c + (double)i / (double)p);
+      double l = 1.0;
+      for (int j = 0; j < n; j++)
+        if (j != k) l *= (x - (double)j / p) / ((double)k / p - (double)j / p);
+      ip[k * n + i] = l;
+    }
+}
+
+namespace {
+struct Flattener {
+  const MeshParams &mp;
+  const SfcTables &t;
+  const std::vector<Oct> &nodes;
+  Traversal &tr;
+  int npe;
+  std::vector<std::vector<int32_t>> buf;  // per level: duplicated child buckets
+  std::vector<int32_t> scratch;
+  double ip[2][9];
+  size_t stale_needed = 0;
+  using Combo = std::vector<std::pair<int32_t, double>>;
+  std::vector<Combo> pbuf, nbuf;                                    // emulated parentEleBuffer
+  std::vector<std::vector<std::vector<std::pair<int, double>>>> wnz;  // [child][r] -> (s, weight != 0)
+
+  void init_weights() {
+    const int n1 = mp.order + 1;
+    const int nch = 1 << mp.dim;
+    wnz.assign(nch, std::vector<std::vector<std::pair<int, double>>>(npe));
+    for (int c = 0; c < nch; c++)
+      for (int r = 0; r < npe; r++)
+        for (int s = 0; s < npe; s++) {
+          double w = 1.0;
+          int rr = r, ss = s;
+          for (int d = 0; d < mp.dim; d++) {
+            w *= ip[(c >> d) & 1][(ss % n1) * n1 + (rr % n1)];
+            rr /= n1;
+            ss /= n1;
+          }
+          if (w != 0.0) wnz[c][r].push_back({s, w});
+        }
+    pbuf.assign(npe, Combo());
+    nbuf.assign(npe, Combo());
+  }
+
+  // Lexicographic rank of node x on cell (TNPoint::get_lexNodeRank, nsort.tcc:283-303).
+  int lex_rank(const Oct &x, const Oct &cell) const {
+    const u64 len = (u64)1 << (mp.maxd - cell.lev);
+    int rank = 0, stride = 1;
+    for (int d = 0; d < mp.dim; d++) {
+      const u64 off = (u64)x.x[d] - (u64)cell.x[d];
+      if ((u64)x.x[d] < (u64)cell.x[d] || off > len || (off * mp.order) % len != 0)
+        throw std::runtime_error("node is not on the lattice of its traversal cell");
+      const int i = (int)(off * mp.order / len);
+      rank += i * stride;
+      stride *= (mp.order + 1);
+    }
+    return rank;
+  }
+
+  void leaf(const Oct &cell, const int32_t *ids, size_t n, const int32_t *pids, size_t pn, const Oct &parent) {
+    const size_t L = tr.leaves.size();
+    tr.leaves.push_back(cell);
+    const unsigned cm = morton_digit(cell, mp.dim, mp.maxd, cell.lev);
+    tr.child.push_back((uint8_t)cm);
+    tr.slot.resize((L + 1) * npe, -1);
+    int32_t *sl = &tr.slot[L * npe];
+    for (size_t i = 0; i < n; i++) sl[lex_rank(nodes[ids[i]], cell)] = ids[i];
+    bool all = true;
+    for (int r = 0; r < npe; r++) all = all && sl[r] >= 0;
+    if (all) {
+      tr.hang_of.push_back(-1);
+      return;
+    }
+    if (pids == nullptr || pn == 0) throw std::runtime_error("hanging leaf without parent nodes");
+    const size_t H = tr.pslot.size() / npe;
+    tr.hang_of.push_back((int32_t)H);
+    tr.pslot.resize((H + 1) * npe, -1);
+    int32_t *ps = &tr.pslot[H * npe];
+    for (size_t i = 0; i < pn; i++) {
+      const Oct &x = nodes[pids[i]];
+      if ((int)x.lev != (int)cell.lev - 1) continue;
+      ps[lex_rank(x, parent)] = pids[i];
+    }
+    // Emulate the reference's static parent buffer (matvec.h:283,411-456): the
+    // present parent-level nodes overwrite their ranks, the buffer is then
+    // interpolated in place (IKD_Parent2Child with in == out, refel.h:214-240),
+    // and missing leaf nodes take the interpolated values.  Ranks whose parent
+    // node is absent keep what earlier hanging leaves of this traversal left
+    // there (zero before the first one).
+    bool stale = false;
+    for (int s = 0; s < npe; s++)
+      if (ps[s] >= 0) pbuf[s].assign(1, {ps[s], 1.0});
+    for (int r = 0; r < npe; r++) {
+      Combo &c = nbuf[r];
+      c.clear();
+      for (const auto &sw : wnz[cm][r]) {
+        if (ps[sw.first] < 0 && sl[r] < 0) stale = true;
+        for (const auto &e : pbuf[sw.first]) c.push_back({e.first, e.second * sw.second});
+      }
+      if (c.size() > 1) {
+        std::sort(c.begin(), c.end(), [](const std::pair<int32_t, double> &a, const std::pair<int32_t, double> &b) {
+          return a.first < b.first;
+        });
+        size_t w = 0;
+        for (size_t i = 0; i < c.size(); i++) {
+          if (w > 0 && c[w - 1].first == c[i].first)
+            c[w - 1].second += c[i].second;
+          else
+            c[w++] = c[i];
+        }
+        c.resize(w);
+      }
+    }
+    pbuf.swap(nbuf);
+    if (stale) stale_needed++;
+    for (int r = 0; r < npe; r++) {
+      if (sl[r] < 0)
+        for (const auto &e : pbuf[r]) {
+          tr.hg_node.push_back(e.first);
+          tr.hg_w.push_back(e.second);
+        }
+      tr.hg_off.push_back((int64_t)tr.hg_node.size());
+    }
+  }
+
+  void rec(const Oct &cell, int rot, const int32_t *ids, size_t n, const int32_t *pids, size_t pn,
+           const Oct &parent) {
+    if (n == 0) return;
+    const int lev = (int)cell.lev;
+    bool isLeaf = true;
+    for (size_t i = 0; i < n && isLeaf; i++) isLeaf = (int)nodes[ids[i]].lev <= lev;
+    if (isLeaf) {
+      leaf(cell, ids, n, pids, pn, parent);
+      return;
+    }
+    // Bucket with duplication onto closed children (top_down, matvec.h:74-199).
+    const int nch = t.nch;
+    const u32 h = 1u << (mp.maxd - lev - 1);
+    std::vector<int32_t> &b = buf[lev];
+    int cnt[16] = {0}, off[17];
+    scratch.resize(n);
+    // First pass: counts. Children of node i are encoded as (lo mask, dup mask).
+    std::vector<std::pair<unsigned, unsigned>> cc(n);
+    for (size_t i = 0; i < n; i++) {
+      const Oct &x = nodes[ids[i]];
+      unsigned lo = 0, dup = 0;
+      for (int d = 0; d < mp.dim; d++) {
+        const u32 mid = cell.x[d] + h;
+        if (x.x[d] == mid)
+          dup |= 1u << d;
+        else if (x.x[d] > mid)
+          lo |= 1u << d;
+      }
+      cc[i] = {lo, dup};
+      for (unsigned sub = dup;; sub = (sub - 1) & dup) {
+        cnt[t.inv[rot * nch + (lo | sub)]]++;
+        if (sub == 0) break;
+      }
+    }
+    off[0] = 0;
+    for (int c = 0; c < nch; c++) off[c + 1] = off[c] + cnt[c];
+    b.resize(off[nch]);
+    int fill[16];
+    for (int c = 0; c < nch; c++) fill[c] = off[c];
+    for (size_t i = 0; i < n; i++) {
+      const unsigned lo = cc[i].first, dup = cc[i].second;
+      // Ascending child order within the duplicate set, as the reference enumerates it.
+      for (unsigned sub = 0;; sub = (sub - dup) & dup) {
+        b[fill[t.inv[rot * nch + (lo | sub)]]++] = ids[i];
+        if (sub == dup) break;
+      }
+    }
+    // Keep this level's bucket alive while recursing (deeper levels use other buffers).
+    std::vector<int32_t> mine;
+    mine.swap(b);
+    for (int cs = 0; cs < nch; cs++) {
+      const unsigned m = t.perm[rot * nch + cs];
+      Oct ch = cell;
+      ch.lev = (u32)(lev + 1);
+      for (int d = 0; d < mp.dim; d++) ch.x[d] = cell.x[d] + (((m >> d) & 1u) ? h : 0u);
+      rec(ch, t.child[rot * nch + m], mine.data() + off[cs], (size_t)cnt[cs], ids, n, cell);
+    }
+    mine.swap(b);
+  }
+};
+}  // namespace
+
+Traversal flatten_traversal(const MeshParams &mp, const std::vector<Oct> &nodes) {
+  Traversal tr;
+  int npe = 1;
+  for (int d = 0; d < mp.dim; d++) npe *= (mp.order + 1);
+  tr.nPe = npe;
+  Flattener f{mp, sfc_tables(mp.dim, mp.curve), nodes, tr, npe, {}, {}, {}};
+  f.buf.resize(mp.maxd + 2);
+  interp_1d(mp.order, 0, f.ip[0]);
+  interp_1d(mp.order, 1, f.ip[1]);
+  f.init_weights();
+  tr.hg_off.push_back(0);
+  std::vector<int32_t> all(nodes.size());
+  std::iota(all.begin(), all.end(), 0);
+  Oct root{};
+  root.lev = 0;
+  f.rec(root, 0, all.data(), all.size(), nullptr, 0, root);
+  tr.stale_needed = f.stale_needed;
+  return tr;
+}
+
+// ---------------------------------------------------------------------------
+// Synthetic inputs (identical to tests/golden/gen_inputs.py).
+// ---------------------------------------------------------------------------
+static inline u64 mix64(u64 z) {
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+static const u64 kGolden = 0x9E3779B97F4A7C15ull;
+
+double keyed_uniform(u64 seed, const u32 *c, int dim) {
+  u64 s = mix64(seed + kGolden);
+  for (int d = 0; d < dim; d++) s = mix64(s ^ ((u64)c[d] * kGolden + (u64)d));
+  return 2.0 * ((double)(s >> 11) * (1.0 / 9007199254740992.0)) - 1.0;
+}
+
+static bool sphere_split(const MeshParams &mp, const Oct &c, int base, int maxl, const double *ctr, double r) {
+  if ((int)c.lev < base) return true;
+  if ((int)c.lev >= maxl) return false;
+  const double scale = 1.0 / (double)((u64)1 << mp.maxd);
+  const double h = (double)((u64)1 << (mp.maxd - c.lev)) * scale;
+  double mind2 = 0.0, maxd2 = 0.0;
+  for (int d = 0; d < mp.dim; d++) {
+    const double a = (double)c.x[d] * scale;
+    const double lo = a - ctr[d];
+    const double hi = (a + h) - ctr[d];
+    const double gap = std::max(0.0, std::max(lo, -hi));
+    const double g2 = gap * gap;
+    mind2 = mind2 + g2;
+    const double l2 = lo * lo, h2 = hi * hi;
+    maxd2 = maxd2 + std::max(l2, h2);
+  }
+  const double r2 = r * r;
+  return mind2 <= r2 && r2 <= maxd2;
+}
+
+std::vector<Oct> sphere_tree(const MeshParams &mp, int base_level, int max_level, const double *center,
+                             double radius) {
+  std::vector<Oct> out;
+  const int nch = 1 << mp.dim;
+  std::vector<Oct> stack;
+  Oct root{};
+  root.lev = 0;
+  // Morton-order depth-first emission (the order is irrelevant: the tree is sorted later).
+  struct F {
+    static void rec(const MeshParams &mp, const Oct &c, int base, int maxl, const double *ctr, double r,
+                    std::vector<Oct> &out, int nch) {
+      if (c.lev > 0 && !sphere_split(mp, c, base, maxl, ctr, r)) {
+        out.push_back(c);
+        return;
+      }
+      const u32 h = 1u << (mp.maxd - c.lev - 1);
+      for (int ch = 0; ch < nch; ch++) {
+        Oct k = c;
+        k.lev = c.lev + 1;
+        for (int d = 0; d < mp.dim; d++) k.x[d] = c.x[d] + (((ch >> d) & 1) ? h : 0u);
+        rec(mp, k, base, maxl, ctr, r, out, nch);
+      }
+    }
+  };
+  F::rec(mp, root, base_level, max_level, center, radius, out, nch);
+  return out;
+}
+
+std::vector<Oct> gauss_points(const MeshParams &mp, size_t n, int ncomp, double sigma, u64 seed) {
+  const int dim = mp.dim;
+  const int per = 1 + 12 * dim;
+  auto u32at = [seed](u64 i) -> u64 {
+    return mix64((seed * kGolden) ^ (i * kGolden + 0x632BE59BD9B4E019ull)) >> 32;
+  };
+  std::vector<double> ctr((size_t)ncomp * dim);
+  for (int i = 0; i < ncomp * dim; i++) ctr[i] = (double)u32at((u64)i) * (1.0 / 4294967296.0);
+  std::vector<Oct> out(n);
+  const double top = (double)((u64)1 << mp.maxd);
+  const u64 base = (u64)ncomp * dim;
+#pragma omp parallel for schedule(static)
+  for (size_t i = 0; i < n; i++) {
+    const u64 b = base + (u64)i * per;
+    const int comp = (int)(u32at(b) % (u64)ncomp);
+    Oct o{};
+    o.lev = (u32)mp.maxd;
+    for (int d = 0; d < dim; d++) {
+      int64_t s = 0;
+      for (int k = 0; k < 12; k++) s += (int64_t)u32at(b + 1 + (u64)d * 12 + k);
+      const double z = (double)(s - (int64_t)6 * 4294967296LL) * (1.0 / 4294967296.0);
+      const double sz = sigma * z;
+      double x = ctr[comp * dim + d] + sz;
+      x = std::min(std::max(x, 0.0), 1.0);
+      double xi = std::floor(x * top);
+      xi = std::min(xi, top - 1.0);
+      o.x[d] = (u32)xi;
+    }
+    out[i] = o;
+  }
+  return out;
+}
+
+}  // namespace dkt

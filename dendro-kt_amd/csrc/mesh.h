@@ -1,0 +1,100 @@
+// Host mesh layer: SFC tables, tree construction and 2:1 balancing, CG node
+// enumeration in Dendro-KT's node order, and the flattening of the reference's
+// recursive matvec traversal into per-leaf gather maps.
+//
+// Everything here runs once per mesh on the host.  The results are plain
+// arrays handed to the device (dkt_plan in plan.h).
+#pragma once
+#include <cstddef>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace dkt {
+
+using u32 = uint32_t;
+using u64 = uint64_t;
+using u128 = unsigned __int128;
+
+enum Curve { CURVE_MORTON = 0, CURVE_HILBERT = 1 };
+
+// Child-order tables of a space-filling curve (the reference's rotations[] and
+// HILBERT_TABLE[], src/KDhcurvedata.cpp:25-60).  Row r is an orientation.
+struct SfcTables {
+  int dim = 0, nch = 0, nrot = 0;
+  std::vector<uint8_t> perm;   // [nrot][nch]: child_sfc -> Morton child
+  std::vector<uint8_t> inv;    // [nrot][nch]: Morton child -> child_sfc
+  std::vector<int32_t> child;  // [nrot][nch]: Morton child -> child orientation
+};
+const SfcTables &sfc_tables(int dim, int curve);
+
+// A cell or point: anchor coordinates (integer grid 2^maxd) and level.
+struct Oct {
+  u32 x[4];
+  u32 lev;
+};
+
+struct MeshParams {
+  int dim = 3;
+  int maxd = 30;   // the reference's m_uiMaxDepth
+  int order = 1;
+  int curve = CURVE_HILBERT;
+};
+
+// SFC key of an octant: child_sfc digits for levels 1..lev (dim bits each, level
+// 1 most significant), zero below the octant's level.
+u128 sfc_key(const MeshParams &mp, const Oct &o);
+
+// Sorts octants in SFC order with ancestors before descendants (the order of
+// SFC_Tree::locTreeSort, include/tsort.tcc:14-80) and removes duplicates and
+// ancestors of later octants (SFC_Tree::locRemoveDuplicates, src/tsort.cpp:718-741).
+void sort_octants(const MeshParams &mp, std::vector<Oct> &octs, bool remove_ancestors);
+
+// Complete tree whose leaves hold at most max_pts of the given octants/points
+// (SFC_Tree::locTreeConstruction semantics, src/tsort.cpp:569-645; leaf level
+// capped at maxd).  `pts` is sorted in place.
+std::vector<Oct> construct_tree(const MeshParams &mp, std::vector<Oct> &pts, size_t max_pts);
+
+// 2:1 balanced complete tree refining the tree built from `pts`
+// (SFC_Tree::distTreeBalancing on one rank, src/tsort.cpp:860-878).
+std::vector<Oct> balance_tree(const MeshParams &mp, std::vector<Oct> &pts, size_t max_pts);
+
+// CG nodes of a balanced tree, in the reference's single-rank DA order
+// (ot::DA::construct, src/oda.cpp:53-151).
+struct NodeSet {
+  std::vector<Oct> nodes;   // exterior nodes in countCGNodes order, then element-interior nodes
+  size_t n_exterior = 0;
+  std::vector<u32> bdy;     // indices of domain-boundary nodes (DA::m_uiBdyNodeIds)
+};
+NodeSet enumerate_nodes(const MeshParams &mp, const std::vector<Oct> &elems);
+
+// Flattened traversal (fem::matvec_rec, FEM/include/matvec.h:243-548): one
+// record per traversal leaf, in traversal (SFC) order.
+struct Traversal {
+  int nPe = 0;
+  std::vector<Oct> leaves;        // leaf cells
+  std::vector<uint8_t> child;     // Morton index of the leaf within its parent
+  std::vector<int32_t> slot;      // [L][nPe]: node index, or -1 if the leaf lacks that node
+  std::vector<int32_t> hang_of;   // [L]: index into pslot rows, -1 if the leaf has all nodes
+  std::vector<int32_t> pslot;     // [H][nPe]: parent-level node at parent lexicographic rank, or -1
+  // Gather of missing nodes, per (hanging leaf h, rank r): x-weights
+  // hg_node/hg_w[hg_off[h*nPe+r] .. hg_off[h*nPe+r+1]) (empty for present ranks).
+  std::vector<int64_t> hg_off;
+  std::vector<int32_t> hg_node;
+  std::vector<double> hg_w;
+  size_t stale_needed = 0;        // hanging leaves that read a stale parent-buffer entry
+};
+Traversal flatten_traversal(const MeshParams &mp, const std::vector<Oct> &nodes);
+
+// Integer-exact synthetic inputs (also restated in numpy by the tests).
+std::vector<Oct> sphere_tree(const MeshParams &mp, int base_level, int max_level, const double *center,
+                             double radius);
+std::vector<Oct> gauss_points(const MeshParams &mp, size_t n, int ncomp, double sigma, u64 seed);
+double keyed_uniform(u64 seed, const u32 *coords, int dim);
+
+// 1D Lagrange interpolation from a parent to child `c` (0/1) at equispaced
+// nodes of order p: ip[k*(p+1)+i] = l_k(child point i)  (RefElement ip_1D_c,
+// FEM/src/refel.cpp:158-165).
+void interp_1d(int p, int c, double *ip);
+
+}  // namespace dkt

@@ -1,0 +1,112 @@
+/*
+ * dkt.h -- C ABI of the MI355X-native adaptive k-tree FEM matvec.
+ *
+ * Plain C: pointers, sizes and int status codes (0 = success, nonzero = error,
+ * message in dkt_last_error()).  No torch or C++ types cross this boundary.
+ * Every handle owns its memory; calls on different handles are independent
+ * (no global mutable state besides the per-thread error string), and device
+ * work is ordered on the caller's HIP stream.
+ *
+ * Each entry point names the reference interface it replaces
+ * (paths relative to the Dendro-KT source tree).
+ */
+#ifndef DKT_H
+#define DKT_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Space-filling curve of the tree order (reference: CMake option
+ * HILBERT_ORDERING, CMakeLists.txt:38,214-216; tables src/KDhcurvedata.cpp:25-60). */
+#define DKT_CURVE_MORTON 0
+#define DKT_CURVE_HILBERT 1
+
+/* Elemental operators with device implementations.
+ * DKT_OP_STIFFNESS: Poisson stiffness K_e = h^(dim-2) K_ref (equal to
+ *   HeatEq::HeatMat<3>::elementalMatVec, FEM/examples/src/heatMat.cpp:46-117, in 3D).
+ * DKT_OP_MASS: mass matrix h^dim M_ref (the operator of HeatVec's RHS,
+ *   FEM/examples/src/heatVec.cpp:31-78).
+ * DKT_OP_IDENTITY: identity elemental op (test/testMatvec.cpp:187-198).
+ * DKT_OP_CAPTURED: dense per-level elemental matrices supplied by the caller
+ *   (captured from a user elementalMatVec, FEM/include/feMatrix.h:56). */
+#define DKT_OP_STIFFNESS 0
+#define DKT_OP_MASS 1
+#define DKT_OP_IDENTITY 2
+#define DKT_OP_CAPTURED 3
+
+typedef struct dkt_octs dkt_octs; /* host list of cells/points */
+typedef struct dkt_da dkt_da;     /* host mesh: tree + CG nodes + traversal */
+typedef struct dkt_plan dkt_plan; /* device-resident matvec plan */
+
+/* Last error message of the calling thread ("" if none). */
+const char *dkt_last_error(void);
+
+/* ---- Octant lists ------------------------------------------------------- */
+/* A list of cells or points of a dim-dimensional tree with integer
+ * coordinates in [0, 2^maxdepth].  Layout of `cells`: uint32 [n][dim+1] =
+ * (anchor coords..., level); a point is a cell of level maxdepth. */
+int dkt_octs_create(int dim, int maxdepth, int curve, const uint32_t *cells, size_t n, dkt_octs **out);
+int dkt_octs_destroy(dkt_octs *o);
+size_t dkt_octs_size(const dkt_octs *o);
+int dkt_octs_copy(const dkt_octs *o, uint32_t *cells_out);
+
+/* Sorts in SFC order, ancestors first, removing duplicates.
+ * Replaces SFC_Tree::distTreeSort + locRemoveDuplicates (src/tsort.cpp:201-224,718-741). */
+int dkt_tree_sort(dkt_octs *o);
+
+/* Complete tree whose leaves contain at most max_pts of the given cells/points.
+ * Replaces SFC_Tree::distTreeConstruction (src/tsort.cpp:648-715). */
+int dkt_tree_construct(const dkt_octs *pts, size_t max_pts, dkt_octs **tree_out);
+
+/* 2:1 balanced complete tree refining construct(pts, max_pts).
+ * Replaces SFC_Tree::distTreeBalancing (src/tsort.cpp:860-878). */
+int dkt_tree_balance(const dkt_octs *pts, size_t max_pts, dkt_octs **tree_out);
+
+/* Synthetic inputs: tree refined around a sphere surface (level < base_level
+ * always split; level < max_level split when the cell meets the sphere), and
+ * Gaussian-mixture points (Irwin-Hall normals, integer exact). */
+int dkt_tree_sphere(int dim, int maxdepth, int curve, int base_level, int max_level, const double *center,
+                    double radius, dkt_octs **out);
+int dkt_points_gauss(int dim, int maxdepth, int curve, size_t n, int ncomp, double sigma, uint64_t seed,
+                     dkt_octs **out);
+/* U[-1,1) values keyed by node coordinates: x[i] = f(seed, nodes[i]). */
+int dkt_keyed_uniform(uint64_t seed, int dim, const uint32_t *nodes, size_t n, double *x_out);
+
+/* ---- Mesh (DA) ---------------------------------------------------------- */
+/* Builds the CG node list of a sorted, 2:1 balanced complete tree in the
+ * reference's node order, and flattens the matvec traversal over it.
+ * Replaces ot::DA<dim>::DA(tree, nEle, comm, order) (src/oda.cpp:45-151). */
+int dkt_da_create(const dkt_octs *tree, int order, dkt_da **out);
+int dkt_da_destroy(dkt_da *da);
+/* Counts: 0 elements, 1 nodes (DA::getLocalNodalSz), 2 boundary nodes,
+ * 3 traversal leaves, 4 hanging leaves, 5 nodes per element, 6 hanging
+ * leaves that read a stale parent-buffer entry, 7 hanging-gather entries. */
+size_t dkt_da_count(const dkt_da *da, int what);
+/* uint32 [N][dim+1] node coords + level (DA::getTNCoords, oda.h:252). */
+int dkt_da_nodes(const dkt_da *da, uint32_t *out);
+/* uint32 [E][dim+1] (the tree). */
+int dkt_da_elements(const dkt_da *da, uint32_t *out);
+/* Boundary node ids (DA::getBoundaryNodeIndices, oda.h:264). */
+int dkt_da_boundary(const dkt_da *da, uint32_t *out);
+/* Flattened traversal: leaf cells uint32 [L][dim+1], child index uint8 [L],
+ * slots int32 [L][nPe] (node id or -1), hang_of int32 [L] (-1 or row of
+ * pslot), pslot int32 [H][nPe] (parent-level node per parent rank or -1).
+ * Any output pointer may be NULL. */
+int dkt_da_traversal(const dkt_da *da, uint32_t *leaf_cells, uint8_t *child, int32_t *slot, int32_t *hang_of,
+                     int32_t *pslot);
+
+/* Gather of the missing nodes of hanging leaves, as the reference computes it
+ * (parent-level nodes interpolated in its static parent buffer,
+ * FEM/include/matvec.h:403-457): for hanging leaf h and rank r, the input
+ * value is sum_k w[k] x[node[k]] over k in [off[h*nPe+r], off[h*nPe+r+1]).
+ * off: int64 [H*nPe+1]; node: int32 [count 7]; w: double [count 7]. */
+int dkt_da_hanging_gather(const dkt_da *da, int64_t *off, int32_t *node, double *w);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* DKT_H */
