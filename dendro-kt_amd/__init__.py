@@ -48,6 +48,14 @@ def load(path=None):
     p = path or lib_path()
     if not os.path.exists(p):
         raise DktError(f"{p} not found: run __graft_entry__.build() (hipcc) first")
+    # One HIP runtime per process: when torch is present its bundled
+    # libamdhip64 (soname libamdhip64.so.7) must be loaded first so that
+    # libdkt binds to the same instance; two runtimes cannot share the device,
+    # and torch's stream/synchronize calls must see libdkt's kernels.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = ctypes.CDLL(p)
     c = ctypes
     P = c.c_void_p
@@ -235,4 +243,52 @@ class DA:
     def __del__(self):
         if getattr(self, "h", None) and _lib is not None:
             _lib.dkt_da_destroy(self.h)
+            self.h = None
+
+
+class Plan:
+    """Device-resident matvec plan over a DA (fem::matvec / feMatrix::matVec).
+
+    ``matvec(x_ptr, y_ptr, nrhs, dtype, masked, stream)`` takes raw device
+    pointers; ``matvec_torch`` is a convenience for torch tensors (plumbing only).
+    """
+
+    def __init__(self, da, op=OP_STIFFNESS, op_data=None):
+        L = load()
+        if not hasattr(L, "dkt_plan_create"):
+            raise DktError("libdkt.so was built without the device plan")
+        self.da = da
+        self.n_nodes = da.n_nodes
+        h = ctypes.c_void_p()
+        ptr = None
+        if op_data is not None:
+            self._op_data = np.ascontiguousarray(op_data, dtype=np.float64)
+            ptr = self._op_data.ctypes.data_as(ctypes.c_void_p)
+        _chk(L.dkt_plan_create(da.h, op, ptr, 0, ctypes.byref(h)))
+        self.h = h
+
+    def info(self):
+        out = (ctypes.c_double * 6)()
+        _chk(load().dkt_plan_info(self.h, out, 6))
+        return dict(zip(["nodes", "leaves", "hanging", "npe", "map_bytes", "contrib_entries"], list(out)))
+
+    def matvec(self, x_ptr, y_ptr, nrhs=1, dtype=0, masked=False, stream=None):
+        _chk(load().dkt_plan_matvec(self.h, dtype, ctypes.c_void_p(x_ptr), ctypes.c_void_p(y_ptr), nrhs,
+                                    1 if masked else 0, ctypes.c_void_p(stream or 0)))
+
+    def matvec_torch(self, x, y, masked=False, stream=None):
+        import torch
+        if x.dtype != y.dtype or x.dtype not in (torch.float64, torch.float32):
+            raise DktError("x and y must both be float64 or float32")
+        if not (x.is_cuda and y.is_cuda and x.is_contiguous() and y.is_contiguous()):
+            raise DktError("x and y must be contiguous device tensors")
+        nrhs = 1 if x.dim() == 1 else x.shape[1]
+        if x.shape[0] != self.n_nodes or y.shape != x.shape:
+            raise DktError("x/y shape does not match the plan")
+        s = stream if stream is not None else torch.cuda.current_stream().cuda_stream
+        self.matvec(x.data_ptr(), y.data_ptr(), nrhs, 0 if x.dtype == torch.float64 else 1, masked, s)
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.dkt_plan_destroy(self.h)
             self.h = None

@@ -106,6 +106,29 @@ int dkt_da_traversal(const dkt_da *da, uint32_t *leaf_cells, uint8_t *child, int
  * off: int64 [H*nPe+1]; node: int32 [count 7]; w: double [count 7]. */
 int dkt_da_hanging_gather(const dkt_da *da, int64_t *off, int32_t *node, double *w);
 
+/* ---- Device plan and matvec -------------------------------------------- */
+/* Uploads the flattened traversal of `da` to the current HIP device.
+ * op: DKT_OP_*; op_data: for DKT_OP_CAPTURED, double [(maxdepth+1)][nPe][nPe]
+ * elemental matrices indexed by leaf level (row-major, out = K in), else NULL.
+ * flags: reserved, pass 0.
+ * Replaces the per-call setup of fem::matvec (FEM/include/matvec.h:231-240):
+ * the traversal is flattened once per mesh instead of per call. */
+int dkt_plan_create(const dkt_da *da, int op, const double *op_data, int flags, dkt_plan **out);
+int dkt_plan_destroy(dkt_plan *plan);
+
+/* y = A x on device, ordered on `stream` (a hipStream_t; NULL = default).
+ * dtype 0: double, 1: float (float accumulation).  x, y: device arrays
+ * [N][nrhs] (node-major, the reference's dof-interleaved layout), not
+ * aliased.  masked = 0: fem::matvec (FEM/include/matvec.h:231-240);
+ * masked = 1: feMatrix::matVec with HeatMat's Dirichlet hooks
+ * (FEM/include/feMatrix.h:189-259, FEM/examples/src/heatMat.cpp:120-139):
+ * boundary entries of x are read as zero and boundary entries of y are zero. */
+int dkt_plan_matvec(dkt_plan *plan, int dtype, const void *x, void *y, int nrhs, int masked, void *stream);
+
+/* info[0..5] = nodes, leaves, hanging leaves, nodes per element, leaf-map
+ * bytes, node-side contribution entries. */
+int dkt_plan_info(const dkt_plan *plan, double *info, int n);
+
 #ifdef __cplusplus
 }
 #endif
