@@ -28,6 +28,7 @@ OP_STIFFNESS = 0
 OP_MASS = 1
 OP_IDENTITY = 2
 OP_CAPTURED = 3
+PLAN_SIMPLE = 1
 
 _lib = None
 
@@ -83,7 +84,8 @@ def load(path=None):
     L.dkt_da_boundary.argtypes = [P, u32p]
     L.dkt_da_traversal.argtypes = [P, u32p, c.POINTER(c.c_uint8), c.POINTER(c.c_int32),
                                    c.POINTER(c.c_int32), c.POINTER(c.c_int32)]
-    L.dkt_da_hanging_gather.argtypes = [P, c.POINTER(c.c_int64), c.POINTER(c.c_int32), c.POINTER(c.c_double)]
+    L.dkt_da_stale_terms.argtypes = [P, c.POINTER(c.c_int64), c.POINTER(c.c_uint8), c.POINTER(c.c_int32),
+                                     c.POINTER(c.c_double)]
     _bind_device(L)
     _lib = L
     return L
@@ -231,14 +233,16 @@ class DA:
                                      hang.ctypes.data_as(c.POINTER(c.c_int32)),
                                      pslot.ctypes.data_as(c.POINTER(c.c_int32))))
         nh = self.count(7)
-        off = np.empty(H * n + 1, dtype=np.int64)
+        off = np.empty(H + 1, dtype=np.int64)
+        hrank = np.empty(nh, dtype=np.uint8)
         hnode = np.empty(nh, dtype=np.int32)
         hw = np.empty(nh, dtype=np.float64)
-        _chk(load().dkt_da_hanging_gather(self.h, off.ctypes.data_as(c.POINTER(c.c_int64)),
-                                          hnode.ctypes.data_as(c.POINTER(c.c_int32)),
-                                          hw.ctypes.data_as(c.POINTER(c.c_double))))
+        _chk(load().dkt_da_stale_terms(self.h, off.ctypes.data_as(c.POINTER(c.c_int64)),
+                                       hrank.ctypes.data_as(c.POINTER(c.c_uint8)),
+                                       hnode.ctypes.data_as(c.POINTER(c.c_int32)),
+                                       hw.ctypes.data_as(c.POINTER(c.c_double))))
         return dict(cells=cells, child=child, slot=slot, hang_of=hang, pslot=pslot,
-                    hg_off=off, hg_node=hnode, hg_w=hw)
+                    hc_off=off, hc_rank=hrank, hc_node=hnode, hc_w=hw)
 
     def __del__(self):
         if getattr(self, "h", None) and _lib is not None:
@@ -253,7 +257,7 @@ class Plan:
     pointers; ``matvec_torch`` is a convenience for torch tensors (plumbing only).
     """
 
-    def __init__(self, da, op=OP_STIFFNESS, op_data=None):
+    def __init__(self, da, op=OP_STIFFNESS, op_data=None, simple=False):
         L = load()
         if not hasattr(L, "dkt_plan_create"):
             raise DktError("libdkt.so was built without the device plan")
@@ -264,13 +268,15 @@ class Plan:
         if op_data is not None:
             self._op_data = np.ascontiguousarray(op_data, dtype=np.float64)
             ptr = self._op_data.ctypes.data_as(ctypes.c_void_p)
-        _chk(L.dkt_plan_create(da.h, op, ptr, 0, ctypes.byref(h)))
+        _chk(L.dkt_plan_create(da.h, op, ptr, PLAN_SIMPLE if simple else 0, ctypes.byref(h)))
         self.h = h
 
     def info(self):
-        out = (ctypes.c_double * 6)()
-        _chk(load().dkt_plan_info(self.h, out, 6))
-        return dict(zip(["nodes", "leaves", "hanging", "npe", "map_bytes", "contrib_entries"], list(out)))
+        keys = ["nodes", "leaves", "hanging", "npe", "map_bytes", "tiles", "shared_nodes", "partials",
+                "max_tile_nodes"]
+        out = (ctypes.c_double * len(keys))()
+        _chk(load().dkt_plan_info(self.h, out, len(keys)))
+        return dict(zip(keys, list(out)))
 
     def matvec(self, x_ptr, y_ptr, nrhs=1, dtype=0, masked=False, stream=None):
         _chk(load().dkt_plan_matvec(self.h, dtype, ctypes.c_void_p(x_ptr), ctypes.c_void_p(y_ptr), nrhs,

@@ -183,7 +183,7 @@ extern "C" size_t dkt_da_count(const dkt_da *da, int what) {
     case 4: return da->tr.pslot.size() / (size_t)da->tr.nPe;
     case 5: return (size_t)da->tr.nPe;
     case 6: return da->tr.stale_needed;
-    case 7: return da->tr.hg_node.size();
+    case 7: return da->tr.hc_node.size();
     default: return 0;
   }
 }
@@ -226,13 +226,14 @@ extern "C" int dkt_da_traversal(const dkt_da *da, uint32_t *cells, uint8_t *chil
   });
 }
 
-extern "C" int dkt_da_hanging_gather(const dkt_da *da, int64_t *off, int32_t *node, double *w) {
+extern "C" int dkt_da_stale_terms(const dkt_da *da, int64_t *off, uint8_t *rank, int32_t *node, double *w) {
   return guarded([&] {
     if (!da) throw std::invalid_argument("null pointer");
     const Traversal &t = da->tr;
-    if (off) std::memcpy(off, t.hg_off.data(), t.hg_off.size() * sizeof(int64_t));
-    if (node) std::memcpy(node, t.hg_node.data(), t.hg_node.size() * sizeof(int32_t));
-    if (w) std::memcpy(w, t.hg_w.data(), t.hg_w.size() * sizeof(double));
+    if (off) std::memcpy(off, t.hc_off.data(), t.hc_off.size() * sizeof(int64_t));
+    if (rank) std::memcpy(rank, t.hc_rank.data(), t.hc_rank.size());
+    if (node) std::memcpy(node, t.hc_node.data(), t.hc_node.size() * sizeof(int32_t));
+    if (w) std::memcpy(w, t.hc_w.data(), t.hc_w.size() * sizeof(double));
   });
 }
 

@@ -8,7 +8,9 @@
 #include <map>
 #include <mutex>
 #include <numeric>
+#include <functional>
 #include <parallel/algorithm>
+#include <unordered_map>
 #include <stdexcept>
 #include <cstdio>
 
@@ -553,8 +555,6 @@ struct Flattener {
   std::vector<int32_t> scratch;
   double ip[2][9];
   size_t stale_needed = 0;
-  using Combo = std::vector<std::pair<int32_t, double>>;
-  std::vector<Combo> pbuf, nbuf;                                    // emulated parentEleBuffer
   std::vector<std::vector<std::vector<std::pair<int, double>>>> wnz;  // [child][r] -> (s, weight != 0)
 
   void init_weights() {
@@ -573,8 +573,6 @@ struct Flattener {
           }
           if (w != 0.0) wnz[c][r].push_back({s, w});
         }
-    pbuf.assign(npe, Combo());
-    nbuf.assign(npe, Combo());
   }
 
   // Lexicographic rank of node x on cell (TNPoint::get_lexNodeRank, nsort.tcc:283-303).
@@ -615,46 +613,6 @@ struct Flattener {
       const Oct &x = nodes[pids[i]];
       if ((int)x.lev != (int)cell.lev - 1) continue;
       ps[lex_rank(x, parent)] = pids[i];
-    }
-    // Emulate the reference's static parent buffer (matvec.h:283,411-456): the
-    // present parent-level nodes overwrite their ranks, the buffer is then
-    // interpolated in place (IKD_Parent2Child with in == out, refel.h:214-240),
-    // and missing leaf nodes take the interpolated values.  Ranks whose parent
-    // node is absent keep what earlier hanging leaves of this traversal left
-    // there (zero before the first one).
-    bool stale = false;
-    for (int s = 0; s < npe; s++)
-      if (ps[s] >= 0) pbuf[s].assign(1, {ps[s], 1.0});
-    for (int r = 0; r < npe; r++) {
-      Combo &c = nbuf[r];
-      c.clear();
-      for (const auto &sw : wnz[cm][r]) {
-        if (ps[sw.first] < 0 && sl[r] < 0) stale = true;
-        for (const auto &e : pbuf[sw.first]) c.push_back({e.first, e.second * sw.second});
-      }
-      if (c.size() > 1) {
-        std::sort(c.begin(), c.end(), [](const std::pair<int32_t, double> &a, const std::pair<int32_t, double> &b) {
-          return a.first < b.first;
-        });
-        size_t w = 0;
-        for (size_t i = 0; i < c.size(); i++) {
-          if (w > 0 && c[w - 1].first == c[i].first)
-            c[w - 1].second += c[i].second;
-          else
-            c[w++] = c[i];
-        }
-        c.resize(w);
-      }
-    }
-    pbuf.swap(nbuf);
-    if (stale) stale_needed++;
-    for (int r = 0; r < npe; r++) {
-      if (sl[r] < 0)
-        for (const auto &e : pbuf[r]) {
-          tr.hg_node.push_back(e.first);
-          tr.hg_w.push_back(e.second);
-        }
-      tr.hg_off.push_back((int64_t)tr.hg_node.size());
     }
   }
 
@@ -720,6 +678,110 @@ struct Flattener {
 };
 }  // namespace
 
+// The reference interpolates missing nodes from its static parent buffer
+// (matvec.h:283,411-456): present parent-level nodes overwrite their ranks, the
+// buffer is interpolated in place (IKD_Parent2Child with in == out,
+// refel.h:214-240), and missing leaf nodes read the result.  A rank whose
+// parent-level node is absent keeps the value the previous hanging leaf's
+// interpolation left there (zero before the first hanging leaf of a call).
+// The device interpolates from present parent nodes; this pass computes, per
+// hanging leaf, the extra terms that the stale entries contribute, as sparse
+// (rank, node, weight) lists.  Buffer values are evaluated lazily, only where
+// a stale entry is read.
+namespace {
+using Combo = std::vector<std::pair<int32_t, double>>;
+
+void add_scaled(Combo &dst, const Combo &src, double w) {
+  for (const auto &e : src) dst.push_back({e.first, e.second * w});
+}
+void compact(Combo &c) {
+  std::sort(c.begin(), c.end(), [](const std::pair<int32_t, double> &a, const std::pair<int32_t, double> &b) {
+    return a.first < b.first;
+  });
+  size_t w = 0;
+  for (size_t i = 0; i < c.size(); i++) {
+    if (w > 0 && c[w - 1].first == c[i].first)
+      c[w - 1].second += c[i].second;
+    else
+      c[w++] = c[i];
+  }
+  c.resize(w);
+}
+}  // namespace
+
+void stale_corrections(const MeshParams &mp, Traversal &tr,
+                       const std::vector<std::vector<std::vector<std::pair<int, double>>>> &wnz) {
+  const int npe = tr.nPe;
+  const size_t H = tr.pslot.size() / npe;
+  std::vector<size_t> leafOf(H);
+  for (size_t e = 0; e < tr.hang_of.size(); e++)
+    if (tr.hang_of[e] >= 0) leafOf[tr.hang_of[e]] = e;
+  // after(k, s): buffer rank s after hanging leaf k's interpolation.
+  std::unordered_map<uint64_t, Combo> memo;
+  auto key = [npe](size_t k, int s) { return (uint64_t)k * (uint64_t)npe + (uint64_t)s; };
+  std::function<const Combo &(size_t, int)> after;
+  std::vector<std::pair<size_t, int>> stack;
+  after = [&](size_t k0, int s0) -> const Combo & {
+    auto it = memo.find(key(k0, s0));
+    if (it != memo.end()) return it->second;
+    stack.clear();
+    stack.push_back({k0, s0});
+    while (!stack.empty()) {
+      const size_t k = stack.back().first;
+      const int s = stack.back().second;
+      if (memo.count(key(k, s))) {
+        stack.pop_back();
+        continue;
+      }
+      const int c = tr.child[leafOf[k]];
+      const int32_t *ps = &tr.pslot[k * npe];
+      bool ready = true;
+      for (const auto &tw : wnz[c][s])
+        if (ps[tw.first] < 0 && k > 0 && !memo.count(key(k - 1, tw.first))) {
+          stack.push_back({k - 1, tw.first});
+          ready = false;
+        }
+      if (!ready) continue;
+      Combo v;
+      for (const auto &tw : wnz[c][s]) {
+        if (ps[tw.first] >= 0)
+          v.push_back({ps[tw.first], tw.second});
+        else if (k > 0)
+          add_scaled(v, memo[key(k - 1, tw.first)], tw.second);
+      }
+      compact(v);
+      memo.emplace(key(k, s), std::move(v));
+      stack.pop_back();
+    }
+    return memo[key(k0, s0)];
+  };
+  tr.hc_off.assign(1, 0);
+  tr.stale_needed = 0;
+  for (size_t h = 0; h < H; h++) {
+    const size_t e = leafOf[h];
+    const int c = tr.child[e];
+    const int32_t *sl = &tr.slot[e * npe];
+    const int32_t *ps = &tr.pslot[h * npe];
+    bool stale = false;
+    for (int r = 0; r < npe; r++) {
+      if (sl[r] >= 0) continue;
+      Combo corr;
+      for (const auto &sw : wnz[c][r])
+        if (ps[sw.first] < 0 && h > 0) add_scaled(corr, after(h - 1, sw.first), sw.second);
+      compact(corr);
+      for (const auto &e2 : corr) {
+        if (e2.second == 0.0) continue;
+        tr.hc_rank.push_back((uint8_t)r);
+        tr.hc_node.push_back(e2.first);
+        tr.hc_w.push_back(e2.second);
+        stale = true;
+      }
+    }
+    if (stale) tr.stale_needed++;
+    tr.hc_off.push_back((int64_t)tr.hc_node.size());
+  }
+}
+
 Traversal flatten_traversal(const MeshParams &mp, const std::vector<Oct> &nodes) {
   Traversal tr;
   int npe = 1;
@@ -730,13 +792,12 @@ Traversal flatten_traversal(const MeshParams &mp, const std::vector<Oct> &nodes)
   interp_1d(mp.order, 0, f.ip[0]);
   interp_1d(mp.order, 1, f.ip[1]);
   f.init_weights();
-  tr.hg_off.push_back(0);
   std::vector<int32_t> all(nodes.size());
   std::iota(all.begin(), all.end(), 0);
   Oct root{};
   root.lev = 0;
   f.rec(root, 0, all.data(), all.size(), nullptr, 0, root);
-  tr.stale_needed = f.stale_needed;
+  stale_corrections(mp, tr, f.wnz);
   return tr;
 }
 

@@ -77,11 +77,12 @@ struct Traversal {
   std::vector<int32_t> slot;      // [L][nPe]: node index, or -1 if the leaf lacks that node
   std::vector<int32_t> hang_of;   // [L]: index into pslot rows, -1 if the leaf has all nodes
   std::vector<int32_t> pslot;     // [H][nPe]: parent-level node at parent lexicographic rank, or -1
-  // Gather of missing nodes, per (hanging leaf h, rank r): x-weights
-  // hg_node/hg_w[hg_off[h*nPe+r] .. hg_off[h*nPe+r+1]) (empty for present ranks).
-  std::vector<int64_t> hg_off;
-  std::vector<int32_t> hg_node;
-  std::vector<double> hg_w;
+  // Stale parent-buffer terms of hanging leaf h: for k in [hc_off[h], hc_off[h+1]),
+  // missing rank hc_rank[k] adds hc_w[k] * x[hc_node[k]] to its interpolated value.
+  std::vector<int64_t> hc_off;
+  std::vector<uint8_t> hc_rank;
+  std::vector<int32_t> hc_node;
+  std::vector<double> hc_w;
   size_t stale_needed = 0;        // hanging leaves that read a stale parent-buffer entry
 };
 Traversal flatten_traversal(const MeshParams &mp, const std::vector<Oct> &nodes);

@@ -38,6 +38,9 @@ extern "C" {
 #define DKT_OP_IDENTITY 2
 #define DKT_OP_CAPTURED 3
 
+/* dkt_plan_create flags. */
+#define DKT_PLAN_SIMPLE 1
+
 typedef struct dkt_octs dkt_octs; /* host list of cells/points */
 typedef struct dkt_da dkt_da;     /* host mesh: tree + CG nodes + traversal */
 typedef struct dkt_plan dkt_plan; /* device-resident matvec plan */
@@ -84,7 +87,7 @@ int dkt_da_create(const dkt_octs *tree, int order, dkt_da **out);
 int dkt_da_destroy(dkt_da *da);
 /* Counts: 0 elements, 1 nodes (DA::getLocalNodalSz), 2 boundary nodes,
  * 3 traversal leaves, 4 hanging leaves, 5 nodes per element, 6 hanging
- * leaves that read a stale parent-buffer entry, 7 hanging-gather entries. */
+ * leaves that read a stale parent-buffer entry, 7 stale-term entries. */
 size_t dkt_da_count(const dkt_da *da, int what);
 /* uint32 [N][dim+1] node coords + level (DA::getTNCoords, oda.h:252). */
 int dkt_da_nodes(const dkt_da *da, uint32_t *out);
@@ -99,18 +102,22 @@ int dkt_da_boundary(const dkt_da *da, uint32_t *out);
 int dkt_da_traversal(const dkt_da *da, uint32_t *leaf_cells, uint8_t *child, int32_t *slot, int32_t *hang_of,
                      int32_t *pslot);
 
-/* Gather of the missing nodes of hanging leaves, as the reference computes it
- * (parent-level nodes interpolated in its static parent buffer,
- * FEM/include/matvec.h:403-457): for hanging leaf h and rank r, the input
- * value is sum_k w[k] x[node[k]] over k in [off[h*nPe+r], off[h*nPe+r+1]).
- * off: int64 [H*nPe+1]; node: int32 [count 7]; w: double [count 7]. */
-int dkt_da_hanging_gather(const dkt_da *da, int64_t *off, int32_t *node, double *w);
+/* Missing nodes of a hanging leaf are interpolated from its parent-level
+ * nodes (pslot) as the reference does in its static parent buffer
+ * (FEM/include/matvec.h:403-457).  Where a parent rank has no node at the
+ * parent level the reference reads the value a previous hanging leaf left in
+ * that buffer entry; these stale terms are returned here: for hanging leaf h
+ * and k in [off[h], off[h+1]), missing rank rank[k] adds w[k] * x[node[k]].
+ * off: int64 [H+1]; rank: uint8, node: int32, w: double [count 7]. */
+int dkt_da_stale_terms(const dkt_da *da, int64_t *off, uint8_t *rank, int32_t *node, double *w);
 
 /* ---- Device plan and matvec -------------------------------------------- */
 /* Uploads the flattened traversal of `da` to the current HIP device.
  * op: DKT_OP_*; op_data: for DKT_OP_CAPTURED, double [(maxdepth+1)][nPe][nPe]
  * elemental matrices indexed by leaf level (row-major, out = K in), else NULL.
- * flags: reserved, pass 0.
+ * flags: 0 = tiled kernels (default); DKT_PLAN_SIMPLE = per-leaf/per-node
+ * kernels through a global contribution buffer (an independent check path;
+ * always used for DKT_OP_CAPTURED).
  * Replaces the per-call setup of fem::matvec (FEM/include/matvec.h:231-240):
  * the traversal is flattened once per mesh instead of per call. */
 int dkt_plan_create(const dkt_da *da, int op, const double *op_data, int flags, dkt_plan **out);

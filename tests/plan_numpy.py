@@ -2,7 +2,8 @@
 
 Applies the per-leaf maps exported by dkt_da_traversal/dkt_da_hanging_gather
 with the semantics of the reference's leaf step (FEM/include/matvec.h:380-522):
-gather present nodes and the hanging gather, apply the elemental matrix, add
+gather present nodes, interpolate missing ones from the parent-level nodes
+(plus the stale parent-buffer terms), apply the elemental matrix, add
 present ranks to their nodes, and add the transposed interpolation of the
 missing ranks to the parent-level nodes at ranks the leaf lacks.  Used to pin
 the maps against the reference's matvec outputs independently of the GPU code.
@@ -70,7 +71,7 @@ def apply(tr, x, dim, p, maxd, op="poisson", masked_bdy=None):
         x[masked_bdy] = 0.0
     y = np.zeros_like(x)
     slot, hang, pslot = tr["slot"], tr["hang_of"], tr["pslot"]
-    off, hn, hw = tr["hg_off"], tr["hg_node"], tr["hg_w"]
+    off, hr, hn, hw = tr["hc_off"], tr["hc_rank"], tr["hc_node"], tr["hc_w"]
     npe = slot.shape[1]
     levels = tr["cells"][:, dim].astype(np.int64)
     Ws = [interp_matrix(dim, p, c) for c in range(1 << dim)]
@@ -79,10 +80,12 @@ def apply(tr, x, dim, p, maxd, op="poisson", masked_bdy=None):
         u = np.where(s >= 0, x[np.maximum(s, 0)], 0.0)
         h = hang[e]
         if h >= 0:
-            for r in range(npe):
-                a, b = off[h * npe + r], off[h * npe + r + 1]
-                if b > a:
-                    u[r] = np.dot(hw[a:b], x[hn[a:b]])
+            ps = pslot[h]
+            xp = np.where(ps >= 0, x[np.maximum(ps, 0)], 0.0)
+            ui = Ws[tr["child"][e]] @ xp
+            a, b = off[h], off[h + 1]
+            np.add.at(ui, hr[a:b], hw[a:b] * x[hn[a:b]])
+            u = np.where(s >= 0, u, ui)
         scale = 1.0 if hp is None else (2.0 ** (-float(levels[e]))) ** hp
         v = scale * (A @ u)
         pres = s >= 0

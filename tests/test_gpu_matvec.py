@@ -2,6 +2,7 @@
 
 FP64: relative L2 <= 1e-12 against the reference matvec on the same tree and
 input.  FP32 (float accumulation): relative L2 <= 2e-6 against the same.
+Both device paths (tiled, simple) are checked.
 """
 import numpy as np
 import pytest
@@ -21,8 +22,9 @@ def _torch():
     return torch
 
 
+@pytest.mark.parametrize("simple", [False, True], ids=["tiled", "simple"])
 @pytest.mark.parametrize("name", fx.names())
-def test_matvec_matches_reference(dkt, name):
+def test_matvec_matches_reference(dkt, name, simple):
     torch = _torch()
     da, z, meta = fx.build_da(dkt, name)
     nodes = da.nodes()
@@ -30,36 +32,36 @@ def test_matvec_matches_reference(dkt, name):
     for key, op, xs, masked in fx.y_cases(z):
         code = fx.op_code(dkt, op)
         if code not in plans:
-            plans[code] = dkt.Plan(da, code)
+            plans[code] = dkt.Plan(da, code, simple=simple)
         plan = plans[code]
         x = fx.make_x(xs, nodes, meta["dim"])
         ref = z[key]
         xd = torch.tensor(x, dtype=torch.float64, device="cuda")
-        yd = torch.empty_like(xd)
+        yd = torch.full_like(xd, float("nan"))
         plan.matvec_torch(xd, yd, masked=masked)
         torch.cuda.synchronize()
         y = yd.cpu().numpy()
         err = np.linalg.norm(y - ref) / max(np.linalg.norm(ref), 1e-300)
-        print(f"{name} {key} fp64 rel_l2 {err:.2e}", flush=True)
+        print(f"{name} {key} simple={simple} fp64 rel_l2 {err:.2e}", flush=True)
         assert err <= TOL64, (name, key, err)
         x32 = xd.float()
-        y32 = torch.empty_like(x32)
+        y32 = torch.full_like(x32, float("nan"))
         plan.matvec_torch(x32, y32, masked=masked)
         torch.cuda.synchronize()
         err32 = np.linalg.norm(y32.cpu().double().numpy() - ref) / max(np.linalg.norm(ref), 1e-300)
-        print(f"{name} {key} fp32 rel_l2 {err32:.2e}", flush=True)
+        print(f"{name} {key} simple={simple} fp32 rel_l2 {err32:.2e}", flush=True)
         assert err32 <= TOL32, (name, key, err32)
 
 
-def test_multi_rhs_columns_match_single(dkt):
+@pytest.mark.parametrize("nrhs", [3, 8, 24])
+def test_multi_rhs_columns_match_single(dkt, nrhs):
     torch = _torch()
     da, z, meta = fx.build_da(dkt, "gauss4d_small")
     nodes = da.nodes()
     plan = dkt.Plan(da, dkt.OP_STIFFNESS)
-    nrhs = 8
     X = np.stack([fx.make_x(f"hash{s}", nodes, meta["dim"]) for s in range(nrhs)], axis=1)
     xd = torch.tensor(X, dtype=torch.float64, device="cuda").contiguous()
-    yd = torch.empty_like(xd)
+    yd = torch.full_like(xd, float("nan"))
     plan.matvec_torch(xd, yd)
     for q in range(nrhs):
         xq = xd[:, q].contiguous()
