@@ -100,6 +100,8 @@ def _bind_device(L):
     L.dkt_plan_destroy.argtypes = [P]
     L.dkt_plan_matvec.argtypes = [P, c.c_int, P, P, c.c_int, c.c_int, P]
     L.dkt_plan_info.argtypes = [P, c.POINTER(c.c_double), c.c_int]
+    L.dkt_plan_profile.argtypes = [P, c.c_int]
+    L.dkt_plan_profile_read.argtypes = [P, c.POINTER(c.c_double), c.POINTER(c.c_double), c.POINTER(c.c_int)]
 
 
 def _chk(rc):
@@ -273,10 +275,21 @@ class Plan:
 
     def info(self):
         keys = ["nodes", "leaves", "hanging", "npe", "map_bytes", "tiles", "shared_nodes", "partials",
-                "max_tile_nodes"]
+                "max_tile_nodes", "readers_mean", "readers_max", "carry_terms_mean", "carry_terms_max",
+                "groups_mean", "tile_nodes_mean", "hanging_entries", "cone_mean", "tile_leaves"]
         out = (ctypes.c_double * len(keys))()
         _chk(load().dkt_plan_info(self.h, out, len(keys)))
         return dict(zip(keys, list(out)))
+
+    def profile(self, enable=True):
+        """Records HIP events around the tile kernel of each subsequent matvec."""
+        _chk(load().dkt_plan_profile(self.h, 1 if enable else 0))
+
+    def profile_read(self):
+        """(tile kernel ms summed, whole matvec ms summed, calls); clears the record."""
+        t, tot, n = ctypes.c_double(), ctypes.c_double(), ctypes.c_int()
+        _chk(load().dkt_plan_profile_read(self.h, ctypes.byref(t), ctypes.byref(tot), ctypes.byref(n)))
+        return t.value, tot.value, n.value
 
     def matvec(self, x_ptr, y_ptr, nrhs=1, dtype=0, masked=False, stream=None):
         _chk(load().dkt_plan_matvec(self.h, dtype, ctypes.c_void_p(x_ptr), ctypes.c_void_p(y_ptr), nrhs,

@@ -182,8 +182,8 @@ extern "C" size_t dkt_da_count(const dkt_da *da, int what) {
     case 3: return da->tr.leaves.size();
     case 4: return da->tr.pslot.size() / (size_t)da->tr.nPe;
     case 5: return (size_t)da->tr.nPe;
-    case 6: return da->tr.stale_needed;
-    case 7: return da->tr.hc_node.size();
+    case 6: compute_stale_terms(da->mp, const_cast<dkt_da *>(da)->tr); return da->tr.stale_needed;
+    case 7: compute_stale_terms(da->mp, const_cast<dkt_da *>(da)->tr); return da->tr.hc_node.size();
     default: return 0;
   }
 }
@@ -229,6 +229,7 @@ extern "C" int dkt_da_traversal(const dkt_da *da, uint32_t *cells, uint8_t *chil
 extern "C" int dkt_da_stale_terms(const dkt_da *da, int64_t *off, uint8_t *rank, int32_t *node, double *w) {
   return guarded([&] {
     if (!da) throw std::invalid_argument("null pointer");
+    compute_stale_terms(da->mp, const_cast<dkt_da *>(da)->tr);
     const Traversal &t = da->tr;
     if (off) std::memcpy(off, t.hc_off.data(), t.hc_off.size() * sizeof(int64_t));
     if (rank) std::memcpy(rank, t.hc_rank.data(), t.hc_rank.size());
@@ -242,4 +243,5 @@ namespace dkt {
 const MeshParams &da_params(const dkt_da *da) { return da->mp; }
 const NodeSet &da_nodes(const dkt_da *da) { return da->ns; }
 const Traversal &da_traversal(const dkt_da *da) { return da->tr; }
+Traversal &da_traversal_mut(const dkt_da *da) { return const_cast<dkt_da *>(da)->tr; }
 }  // namespace dkt

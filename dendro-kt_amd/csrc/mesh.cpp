@@ -684,13 +684,8 @@ struct Flattener {
 // refel.h:214-240), and missing leaf nodes read the result.  A rank whose
 // parent-level node is absent keeps the value the previous hanging leaf's
 // interpolation left there (zero before the first hanging leaf of a call).
-// The device interpolates from present parent nodes; this pass computes, per
-// hanging leaf, the extra terms that the stale entries contribute, as sparse
-// (rank, node, weight) lists.  Buffer values are evaluated lazily, only where
-// a stale entry is read.
+// StaleBuffer evaluates that buffer lazily, as sparse node combinations.
 namespace {
-using Combo = std::vector<std::pair<int32_t, double>>;
-
 void add_scaled(Combo &dst, const Combo &src, double w) {
   for (const auto &e : src) dst.push_back({e.first, e.second * w});
 }
@@ -709,56 +704,77 @@ void compact(Combo &c) {
 }
 }  // namespace
 
-void stale_corrections(const MeshParams &mp, Traversal &tr,
-                       const std::vector<std::vector<std::vector<std::pair<int, double>>>> &wnz) {
+StaleBuffer::StaleBuffer(const MeshParams &mp, const Traversal &tr) : tr_(tr), npe_(tr.nPe) {
+  const size_t H = tr.pslot.size() / npe_;
+  leaf_of_.resize(H);
+  for (size_t e = 0; e < tr.hang_of.size(); e++)
+    if (tr.hang_of[e] >= 0) leaf_of_[tr.hang_of[e]] = e;
+  double ip[2][9];
+  interp_1d(mp.order, 0, ip[0]);
+  interp_1d(mp.order, 1, ip[1]);
+  const int n1 = mp.order + 1, nch = 1 << mp.dim;
+  wnz_.assign(nch, std::vector<std::vector<std::pair<int, double>>>(npe_));
+  for (int c = 0; c < nch; c++)
+    for (int r = 0; r < npe_; r++)
+      for (int s = 0; s < npe_; s++) {
+        double w = 1.0;
+        int rr = r, ss = s;
+        for (int d = 0; d < mp.dim; d++) {
+          w *= ip[(c >> d) & 1][(ss % n1) * n1 + (rr % n1)];
+          rr /= n1;
+          ss /= n1;
+        }
+        if (w != 0.0) wnz_[c][r].push_back({s, w});
+      }
+}
+
+const Combo &StaleBuffer::after(size_t k0, int s0) {
+  auto key = [this](size_t k, int s) { return (uint64_t)k * (uint64_t)npe_ + (uint64_t)s; };
+  auto it = memo_.find(key(k0, s0));
+  if (it != memo_.end()) return it->second;
+  std::vector<std::pair<size_t, int>> stack{{k0, s0}};
+  while (!stack.empty()) {
+    const size_t k = stack.back().first;
+    const int s = stack.back().second;
+    if (memo_.count(key(k, s))) {
+      stack.pop_back();
+      continue;
+    }
+    const int c = tr_.child[leaf_of_[k]];
+    const int32_t *ps = &tr_.pslot[k * npe_];
+    bool ready = true;
+    for (const auto &tw : wnz_[c][s])
+      if (ps[tw.first] < 0 && k > 0 && !memo_.count(key(k - 1, tw.first))) {
+        stack.push_back({k - 1, tw.first});
+        ready = false;
+      }
+    if (!ready) continue;
+    Combo v;
+    for (const auto &tw : wnz_[c][s]) {
+      if (ps[tw.first] >= 0)
+        v.push_back({ps[tw.first], tw.second});
+      else if (k > 0)
+        add_scaled(v, memo_[key(k - 1, tw.first)], tw.second);
+    }
+    compact(v);
+    memo_.emplace(key(k, s), std::move(v));
+    stack.pop_back();
+  }
+  return memo_[key(k0, s0)];
+}
+
+void compute_stale_terms(const MeshParams &mp, Traversal &tr) {
+  if (tr.hc_ready) return;
   const int npe = tr.nPe;
   const size_t H = tr.pslot.size() / npe;
-  std::vector<size_t> leafOf(H);
-  for (size_t e = 0; e < tr.hang_of.size(); e++)
-    if (tr.hang_of[e] >= 0) leafOf[tr.hang_of[e]] = e;
-  // after(k, s): buffer rank s after hanging leaf k's interpolation.
-  std::unordered_map<uint64_t, Combo> memo;
-  auto key = [npe](size_t k, int s) { return (uint64_t)k * (uint64_t)npe + (uint64_t)s; };
-  std::function<const Combo &(size_t, int)> after;
-  std::vector<std::pair<size_t, int>> stack;
-  after = [&](size_t k0, int s0) -> const Combo & {
-    auto it = memo.find(key(k0, s0));
-    if (it != memo.end()) return it->second;
-    stack.clear();
-    stack.push_back({k0, s0});
-    while (!stack.empty()) {
-      const size_t k = stack.back().first;
-      const int s = stack.back().second;
-      if (memo.count(key(k, s))) {
-        stack.pop_back();
-        continue;
-      }
-      const int c = tr.child[leafOf[k]];
-      const int32_t *ps = &tr.pslot[k * npe];
-      bool ready = true;
-      for (const auto &tw : wnz[c][s])
-        if (ps[tw.first] < 0 && k > 0 && !memo.count(key(k - 1, tw.first))) {
-          stack.push_back({k - 1, tw.first});
-          ready = false;
-        }
-      if (!ready) continue;
-      Combo v;
-      for (const auto &tw : wnz[c][s]) {
-        if (ps[tw.first] >= 0)
-          v.push_back({ps[tw.first], tw.second});
-        else if (k > 0)
-          add_scaled(v, memo[key(k - 1, tw.first)], tw.second);
-      }
-      compact(v);
-      memo.emplace(key(k, s), std::move(v));
-      stack.pop_back();
-    }
-    return memo[key(k0, s0)];
-  };
+  StaleBuffer sb(mp, tr);
   tr.hc_off.assign(1, 0);
+  tr.hc_rank.clear();
+  tr.hc_node.clear();
+  tr.hc_w.clear();
   tr.stale_needed = 0;
   for (size_t h = 0; h < H; h++) {
-    const size_t e = leafOf[h];
+    const size_t e = sb.leaf_of(h);
     const int c = tr.child[e];
     const int32_t *sl = &tr.slot[e * npe];
     const int32_t *ps = &tr.pslot[h * npe];
@@ -766,8 +782,8 @@ void stale_corrections(const MeshParams &mp, Traversal &tr,
     for (int r = 0; r < npe; r++) {
       if (sl[r] >= 0) continue;
       Combo corr;
-      for (const auto &sw : wnz[c][r])
-        if (ps[sw.first] < 0 && h > 0) add_scaled(corr, after(h - 1, sw.first), sw.second);
+      for (const auto &sw : sb.weights(c, r))
+        if (ps[sw.first] < 0 && h > 0) add_scaled(corr, sb.after(h - 1, sw.first), sw.second);
       compact(corr);
       for (const auto &e2 : corr) {
         if (e2.second == 0.0) continue;
@@ -780,6 +796,7 @@ void stale_corrections(const MeshParams &mp, Traversal &tr,
     if (stale) tr.stale_needed++;
     tr.hc_off.push_back((int64_t)tr.hc_node.size());
   }
+  tr.hc_ready = true;
 }
 
 Traversal flatten_traversal(const MeshParams &mp, const std::vector<Oct> &nodes) {
@@ -797,7 +814,6 @@ Traversal flatten_traversal(const MeshParams &mp, const std::vector<Oct> &nodes)
   Oct root{};
   root.lev = 0;
   f.rec(root, 0, all.data(), all.size(), nullptr, 0, root);
-  stale_corrections(mp, tr, f.wnz);
   return tr;
 }
 

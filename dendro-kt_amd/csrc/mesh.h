@@ -8,6 +8,8 @@
 #include <cstddef>
 #include <cstdint>
 #include <string>
+#include <unordered_map>
+#include <utility>
 #include <vector>
 
 namespace dkt {
@@ -84,8 +86,33 @@ struct Traversal {
   std::vector<int32_t> hc_node;
   std::vector<double> hc_w;
   size_t stale_needed = 0;        // hanging leaves that read a stale parent-buffer entry
+  bool hc_ready = false;          // hc_* computed (compute_stale_terms)
 };
 Traversal flatten_traversal(const MeshParams &mp, const std::vector<Oct> &nodes);
+
+using Combo = std::vector<std::pair<int32_t, double>>;
+
+// Lazy evaluation of the reference's static parent buffer: after(k, s) is the
+// value of buffer rank s after the k-th hanging leaf (traversal order) has
+// interpolated in place, as a sparse combination of x entries.
+class StaleBuffer {
+ public:
+  StaleBuffer(const MeshParams &mp, const Traversal &tr);
+  const Combo &after(size_t k, int s);
+  size_t leaf_of(size_t h) const { return leaf_of_[h]; }
+  // Nonzero interpolation weights (parent rank s, w) of child rank r, child c.
+  const std::vector<std::pair<int, double>> &weights(int c, int r) const { return wnz_[c][r]; }
+
+ private:
+  const Traversal &tr_;
+  int npe_;
+  std::vector<size_t> leaf_of_;
+  std::vector<std::vector<std::vector<std::pair<int, double>>>> wnz_;
+  std::unordered_map<uint64_t, Combo> memo_;
+};
+
+// Per-leaf stale terms hc_* (all hanging leaves); computed on demand.
+void compute_stale_terms(const MeshParams &mp, Traversal &tr);
 
 // Integer-exact synthetic inputs (also restated in numpy by the tests).
 std::vector<Oct> sphere_tree(const MeshParams &mp, int base_level, int max_level, const double *center,

@@ -1,338 +1,52 @@
-// Device matvec plan and CDNA4 kernels for the flattened k-tree traversal.
+// Device matvec plan: host construction of the tiled layout (tile_kernel.h),
+// the simple check path, launch and the C ABI (include/dkt.h).
 //
 // The reference computes y = A x by a recursive top-down/bottom-up traversal of
-// the node list (fem::matvec_rec, FEM/include/matvec.h:243-548).  Here the
-// traversal is flattened once per mesh (mesh.cpp) into per-leaf maps and each
-// matvec is a gather -> elemental apply -> deterministic reduction on device.
+// the node list (fem::matvec_rec, FEM/include/matvec.h:243-548).  The
+// traversal is flattened once per mesh (mesh.cpp); each matvec is then the
+// tile kernel plus a small fix-up kernel on the caller's stream.
 //
-// Tiled path (default): leaves are cut into SFC-contiguous tiles, one tile per
-// workgroup.  Per tile:
-//   1. the tile's distinct nodes are gathered from x into LDS (x_loc);
-//   2. one work-item per leaf builds its element vector from x_loc (16-bit
-//      local indices), interpolating missing (hanging) ranks from its parent
-//      group's nodes in registers (IKD_Parent2Child, FEM/include/refel.h:214-240)
-//      plus the rare stale parent-buffer terms;
-//   3. v = h^k K_ref u in registers;
-//   4. direct outputs are added into an LDS accumulator y_loc in nPe phases, one
-//      per rank (two disjoint leaves never share a node at the same rank, so a
-//      phase has no write conflicts);
-//   5. the transposed interpolation of missing ranks (matvec.h:482-522) is
-//      summed per parent group and added in host-coloured phases;
-//   6. nodes touched by this tile only are written to y; nodes shared with
-//      other tiles are written as partials, summed in tile order by
-//      fixup_kernel.
-// The reduction order is fixed by the plan: results are bitwise reproducible,
-// with no atomics.
-//
-// Simple path (flag DKT_PLAN_SIMPLE): a per-leaf kernel writing every
-// contribution to a buffer, and a per-node kernel summing them through a CSR.
-// It is kept as an independent device check of the tiled path.
+// Simple path (DKT_PLAN_SIMPLE): a per-leaf kernel writing every contribution
+// to a buffer and a per-node kernel summing them through a CSR in leaf order;
+// an independent device check of the tiled path, also used for per-level
+// captured operators.
 //
 // Multi-RHS vectors are node-major, [node][nrhs] (the reference's
 // dof-interleaved layout, include/oda.h:296-322).
-#include <hip/hip_runtime.h>
-
 #include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <memory>
-#include <stdexcept>
-#include <string>
+#include <set>
 #include <unordered_map>
 #include <vector>
 
 #include "../../include/dkt.h"
 #include "capi_common.h"
 #include "mesh.h"
+#include "plan_common.h"
+#include "tile_kernel.h"
 
 namespace dkt {
 const MeshParams &da_params(const dkt_da *da);
 const NodeSet &da_nodes(const dkt_da *da);
-const Traversal &da_traversal(const dkt_da *da);
+Traversal &da_traversal_mut(const dkt_da *da);
+thread_local bool g_dry_run = false;
 }  // namespace dkt
 
 using namespace dkt;
 
-#define HIP_OK(expr)                                                                    \
-  do {                                                                                  \
-    hipError_t e_ = (expr);                                                             \
-    if (e_ != hipSuccess)                                                               \
-      throw std::runtime_error(std::string(#expr " failed: ") + hipGetErrorString(e_)); \
-  } while (0)
-
 namespace {
 
-constexpr uint16_t kNone = 0xFFFF;
-constexpr int kTileThreads = 256;  // one leaf per work-item
-
 template <typename T>
-struct DevBuf {
-  T *p = nullptr;
-  size_t n = 0;
-  void alloc(size_t count) {
-    release();
-    n = count;
-    if (count) HIP_OK(hipMalloc((void **)&p, count * sizeof(T)));
-  }
-  void upload(const std::vector<T> &v) {
-    alloc(v.size());
-    if (n) HIP_OK(hipMemcpy(p, v.data(), n * sizeof(T), hipMemcpyHostToDevice));
-  }
-  void release() {
-    if (p) (void)hipFree(p);
-    p = nullptr;
-    n = 0;
-  }
-  ~DevBuf() { release(); }
-};
-
-// ---------------------------------------------------------------------------
-// Per-leaf tensor-product interpolation in registers.
-// ip tables: IP[c][k][i] = l_k(child point i) for child half c (k parent rank).
-// ---------------------------------------------------------------------------
-template <int P>
-struct Ip;
-template <>
-struct Ip<1> {
-  __device__ static constexpr double w(int c, int k, int i) {
-    // c=0: points {0,1/2}; c=1: points {1/2,1}.
-    return (c == 0) ? (k == 0 ? (i == 0 ? 1.0 : 0.5) : (i == 0 ? 0.0 : 0.5))
-                    : (k == 0 ? (i == 0 ? 0.5 : 0.0) : (i == 0 ? 0.5 : 1.0));
-  }
-};
-template <>
-struct Ip<2> {
-  // Quadratic Lagrange on nodes {0,1/2,1} evaluated at child points.
-  __device__ static constexpr double w(int c, int k, int i) {
-    const double t = 0.5 * (c + 0.5 * i);  // child point in parent coords
-    return k == 0 ? (t - 0.5) * (t - 1.0) / 0.5 : (k == 1 ? t * (t - 1.0) / -0.25 : t * (t - 0.5) / 0.5);
-  }
-};
-
-template <int N, int D>
-struct Pow {
-  static constexpr int v = N * Pow<N, D - 1>::v;
-};
-template <int N>
-struct Pow<N, 0> {
-  static constexpr int v = 1;
-};
-
-// out[r] = sum_s W_c[r][s] in[s]   (forward: parent ranks -> child ranks)
-template <typename T, int DIM, int P>
-__device__ __forceinline__ void interp_fwd(const T *in, T *out, int c) {
-  constexpr int n1 = P + 1, NPE = Pow<P + 1, DIM>::v;
-  T a[NPE], b[NPE];
-#pragma unroll
-  for (int i = 0; i < NPE; i++) a[i] = in[i];
-#pragma unroll
-  for (int d = 0; d < DIM; d++) {
-    const int cb = (c >> d) & 1;
-    constexpr int strides[4] = {1, n1, n1 * n1, n1 * n1 * n1};
-    const int st = strides[d];
-#pragma unroll
-    for (int r = 0; r < NPE; r++) {
-      const int id = (r / st) % n1;
-      const int base = r - id * st;
-      T acc = T(0);
-#pragma unroll
-      for (int k = 0; k < n1; k++) acc += (T)Ip<P>::w(cb, k, id) * a[base + k * st];
-      b[r] = acc;
-    }
-#pragma unroll
-    for (int i = 0; i < NPE; i++) a[i] = b[i];
-  }
-#pragma unroll
-  for (int i = 0; i < NPE; i++) out[i] = a[i];
-}
-
-// out[s] = sum_r W_c[r][s] in[r]   (transpose: child ranks -> parent ranks)
-template <typename T, int DIM, int P>
-__device__ __forceinline__ void interp_bwd(const T *in, T *out, int c) {
-  constexpr int n1 = P + 1, NPE = Pow<P + 1, DIM>::v;
-  T a[NPE], b[NPE];
-#pragma unroll
-  for (int i = 0; i < NPE; i++) a[i] = in[i];
-#pragma unroll
-  for (int d = 0; d < DIM; d++) {
-    const int cb = (c >> d) & 1;
-    constexpr int strides[4] = {1, n1, n1 * n1, n1 * n1 * n1};
-    const int st = strides[d];
-#pragma unroll
-    for (int s = 0; s < NPE; s++) {
-      const int kd = (s / st) % n1;
-      const int base = s - kd * st;
-      T acc = T(0);
-#pragma unroll
-      for (int i = 0; i < n1; i++) acc += (T)Ip<P>::w(cb, kd, i) * a[base + i * st];
-      b[s] = acc;
-    }
-#pragma unroll
-    for (int i = 0; i < NPE; i++) a[i] = b[i];
-  }
-#pragma unroll
-  for (int i = 0; i < NPE; i++) out[i] = a[i];
-}
-
-// ---------------------------------------------------------------------------
-// Tiled kernel.
-// ---------------------------------------------------------------------------
-struct TileHdr {
-  int32_t leaf0, nleaf;   // leaf range
-  int32_t node0, nnode;   // local node list range (tile_nodes / tile_out)
-  int32_t grp0, ngrp;     // parent groups
-  int32_t hph0, nhph;     // hanging-phase offsets range (hph_off has nhph+1 entries from hph0)
-  int32_t st0, nst;       // stale terms
-};
-
-struct GroupRec {
-  uint16_t first;  // first leaf (tile-local)
-  uint8_t count;   // number of consecutive leaves
-  uint8_t pad;
-};
-
-// Leaf aux word: bits 0-4 level, 5-8 child index, 9 hanging, 16-31 group (tile-local).
-__device__ __forceinline__ int aux_level(uint32_t a) { return a & 31u; }
-__device__ __forceinline__ int aux_child(uint32_t a) { return (a >> 5) & 15u; }
-__device__ __forceinline__ bool aux_hang(uint32_t a) { return (a >> 9) & 1u; }
-__device__ __forceinline__ int aux_group(uint32_t a) { return a >> 16; }
-
-template <typename T, int DIM, int P>
-__global__ void __launch_bounds__(kTileThreads) tile_kernel(
-    const TileHdr *__restrict__ tiles, const int32_t *__restrict__ tile_nodes, const int32_t *__restrict__ tile_out,
-    const uint16_t *__restrict__ leaf_loc, const uint32_t *__restrict__ leaf_aux, const GroupRec *__restrict__ groups,
-    const uint16_t *__restrict__ group_ploc, const int32_t *__restrict__ hph_off, const uint32_t *__restrict__ hentries,
-    const uint32_t *__restrict__ stale_key, const T *__restrict__ stale_w, const T *__restrict__ kref, int hp,
-    const uint8_t *__restrict__ dirichlet, int nrhs, int ld, const T *__restrict__ x, T *__restrict__ y,
-    T *__restrict__ partial) {
-  constexpr int NPE = Pow<P + 1, DIM>::v;
-  extern __shared__ unsigned char smem_raw[];
-  const TileHdr th = tiles[blockIdx.x];
-  T *x_loc = reinterpret_cast<T *>(smem_raw);
-  T *y_loc = x_loc + (size_t)th.nnode * nrhs;
-  T *w_loc = y_loc + (size_t)th.nnode * nrhs;  // [nleaf][NPE] transposed hanging outputs (one rhs at a time)
-  const int tid = threadIdx.x;
-
-  // 1. Stage the tile's nodes.
-  for (int j = tid; j < th.nnode; j += blockDim.x) {
-    const int g = tile_nodes[th.node0 + j];
-    const bool z = dirichlet && dirichlet[g];
-    for (int q = 0; q < nrhs; q++) {
-      x_loc[j * nrhs + q] = z ? T(0) : x[(size_t)g * ld + q];
-      y_loc[j * nrhs + q] = T(0);
-    }
-  }
-  __syncthreads();
-
-  const bool active = tid < th.nleaf;
-  uint16_t loc[NPE];
-  uint32_t aux = 0;
-  if (active) {
-    const uint16_t *lp = leaf_loc + (size_t)(th.leaf0 + tid) * NPE;
-#pragma unroll
-    for (int r = 0; r < NPE; r++) loc[r] = lp[r];
-    aux = leaf_aux[th.leaf0 + tid];
-  } else {
-#pragma unroll
-    for (int r = 0; r < NPE; r++) loc[r] = kNone;
-  }
-  const bool hang = active && aux_hang(aux);
-  const int child = aux_child(aux);
-  const T scale = (T)ldexp(1.0, -aux_level(aux) * hp);
-  const uint16_t *ploc = hang ? group_ploc + (size_t)(th.grp0 + aux_group(aux)) * NPE : nullptr;
-
-  for (int q = 0; q < nrhs; q++) {
-    T u[NPE], v[NPE];
-    if (active) {
-      if (hang) {
-        T xp[NPE], xi[NPE];
-#pragma unroll
-        for (int s = 0; s < NPE; s++) xp[s] = ploc[s] != kNone ? x_loc[ploc[s] * nrhs + q] : T(0);
-        interp_fwd<T, DIM, P>(xp, xi, child);
-        for (int k = 0; k < th.nst; k++) {  // stale parent-buffer terms (rare)
-          const uint32_t key = stale_key[th.st0 + k];
-          if ((int)(key >> 24) == tid) {
-            const int r = (key >> 16) & 255, j = key & 0xFFFF;
-#pragma unroll
-            for (int rr = 0; rr < NPE; rr++)
-              if (rr == r) xi[rr] += stale_w[th.st0 + k] * x_loc[j * nrhs + q];
-          }
-        }
-#pragma unroll
-        for (int r = 0; r < NPE; r++) u[r] = loc[r] != kNone ? x_loc[loc[r] * nrhs + q] : xi[r];
-      } else {
-#pragma unroll
-        for (int r = 0; r < NPE; r++) u[r] = loc[r] != kNone ? x_loc[loc[r] * nrhs + q] : T(0);
-      }
-#pragma unroll
-      for (int i = 0; i < NPE; i++) {
-        T acc = T(0);
-#pragma unroll
-        for (int j = 0; j < NPE; j++) acc += kref[i * NPE + j] * u[j];
-        v[i] = scale * acc;
-      }
-    }
-    // 4. Direct outputs, one phase per rank.
-#pragma unroll
-    for (int r = 0; r < NPE; r++) {
-      __syncthreads();
-      if (loc[r] != kNone) y_loc[loc[r] * nrhs + q] += v[r];
-    }
-    // 5. Transposed interpolation of the missing ranks.
-    if (th.nhph > 0) {
-      if (active) {
-        T vm[NPE], wv[NPE];
-#pragma unroll
-        for (int r = 0; r < NPE; r++) vm[r] = (hang && loc[r] == kNone) ? v[r] : T(0);
-        interp_bwd<T, DIM, P>(vm, wv, child);
-#pragma unroll
-        for (int s = 0; s < NPE; s++) w_loc[tid * NPE + s] = (hang && loc[s] == kNone) ? wv[s] : T(0);
-      }
-      for (int ph = 0; ph < th.nhph; ph++) {
-        __syncthreads();
-        const int a = hph_off[th.hph0 + ph], b = hph_off[th.hph0 + ph + 1];
-        for (int k = a + tid; k < b; k += blockDim.x) {
-          const uint32_t ent = hentries[k];
-          const int g = ent >> 8, s = ent & 255;
-          const GroupRec gr = groups[th.grp0 + g];
-          T acc = T(0);
-          for (int l = 0; l < gr.count; l++) acc += w_loc[(gr.first + l) * NPE + s];
-          const int j = group_ploc[(size_t)(th.grp0 + g) * NPE + s];
-          y_loc[j * nrhs + q] += acc;
-        }
-      }
-    }
-  }
-  __syncthreads();
-  // 6. Write out: owned nodes to y, shared nodes to their partial slot.
-  for (int j = tid; j < th.nnode; j += blockDim.x) {
-    const int o = tile_out[th.node0 + j];
-    if (o == INT32_MIN) continue;  // gathered only, no contribution from this tile
-    if (o >= 0) {
-      const bool z = dirichlet && dirichlet[o];
-      for (int q = 0; q < nrhs; q++) y[(size_t)o * ld + q] = z ? T(0) : y_loc[j * nrhs + q];
-    } else {
-      const size_t slot = (size_t)(-o - 1);
-      for (int q = 0; q < nrhs; q++) partial[slot * nrhs + q] = y_loc[j * nrhs + q];
-    }
-  }
+std::vector<T> nonempty(std::vector<T> v, T fill = T()) {
+  if (v.empty()) v.push_back(fill);
+  return v;
 }
 
 template <typename T>
-__global__ void __launch_bounds__(256) fixup_kernel(int ns, const int32_t *__restrict__ snode,
-                                                    const int32_t *__restrict__ soff, const T *__restrict__ partial,
-                                                    const uint8_t *__restrict__ dirichlet, int nrhs, int ld,
-                                                    T *__restrict__ y) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= ns) return;
-  const int n = snode[i];
-  const int a = soff[i], b = soff[i + 1];
-  const bool z = dirichlet && dirichlet[n];
-  for (int q = 0; q < nrhs; q++) {
-    T acc = T(0);
-    for (int k = a; k < b; k++) acc += partial[(size_t)k * nrhs + q];
-    y[(size_t)n * ld + q] = z ? T(0) : acc;
-  }
+std::vector<T> cast(const std::vector<double> &v) {
+  return std::vector<T>(v.begin(), v.end());
 }
 
 // ---------------------------------------------------------------------------
@@ -360,18 +74,19 @@ __global__ void __launch_bounds__(256) leaf_kernel(
   for (int q = 0; q < nrhs; q++) {
     T u[NPE];
     if (h >= 0) {
-      T xp[NPE], xi[NPE];
+      T xi[NPE];
 #pragma unroll
       for (int t = 0; t < NPE; t++) {
         const int n = pslot[(size_t)h * NPE + t];
-        xp[t] = n >= 0 ? xv(n, q) : T(0);
+        xi[t] = n >= 0 ? xv(n, q) : T(0);
       }
-      interp_fwd<T, DIM, P>(xp, xi, c);
+      interp<T, DIM, P, false>(xi, c);
       for (int k = hc_off[h]; k < hc_off[h + 1]; k++) {
         const int r = hc_rank[k];
+        const T add = hc_w[k] * xv(hc_node[k], q);
 #pragma unroll
         for (int rr = 0; rr < NPE; rr++)
-          if (rr == r) xi[rr] += hc_w[k] * xv(hc_node[k], q);
+          if (rr == r) xi[rr] += add;
       }
 #pragma unroll
       for (int r = 0; r < NPE; r++) u[r] = s[r] >= 0 ? xv(s[r], q) : xi[r];
@@ -390,12 +105,12 @@ __global__ void __launch_bounds__(256) leaf_kernel(
 #pragma unroll
     for (int r = 0; r < NPE; r++) vout[((size_t)e * NPE + r) * nrhs + q] = v[r];
     if (h >= 0) {
-      T vm[NPE], wv[NPE];
+      T w[NPE];
 #pragma unroll
-      for (int r = 0; r < NPE; r++) vm[r] = s[r] < 0 ? v[r] : T(0);
-      interp_bwd<T, DIM, P>(vm, wv, c);
+      for (int r = 0; r < NPE; r++) w[r] = s[r] < 0 ? v[r] : T(0);
+      interp<T, DIM, P, true>(w, c);
 #pragma unroll
-      for (int t = 0; t < NPE; t++) wout[((size_t)h * NPE + t) * nrhs + q] = wv[t];
+      for (int t = 0; t < NPE; t++) wout[((size_t)h * NPE + t) * nrhs + q] = w[t];
     }
   }
 }
@@ -416,7 +131,7 @@ __global__ void __launch_bounds__(256) node_kernel(int N, int nrhs, const int64_
 }
 
 // ---------------------------------------------------------------------------
-// Host-side operator matrices.
+// Reference-element matrices (unit cube, axis 0 fastest).
 // ---------------------------------------------------------------------------
 void one_d(int p, std::vector<double> &K, std::vector<double> &M) {
   if (p == 1) {
@@ -428,8 +143,8 @@ void one_d(int p, std::vector<double> &K, std::vector<double> &M) {
   }
 }
 
-// Reference-element matrix (unit cube, axis 0 fastest): stiffness = sum_t of
-// tensor products with the 1D stiffness on axis t and 1D mass elsewhere.
+// Stiffness = sum over axes t of tensor products with the 1D stiffness on t and
+// the 1D mass elsewhere; mass = tensor product of 1D masses.
 std::vector<double> ref_matrix(int dim, int p, int op) {
   const int n1 = p + 1;
   int npe = 1;
@@ -461,17 +176,12 @@ std::vector<double> ref_matrix(int dim, int p, int op) {
   return A;
 }
 
-template <typename T>
-std::vector<T> cast(const std::vector<double> &v) {
-  return std::vector<T>(v.begin(), v.end());
-}
-
 }  // namespace
 
 struct dkt_plan {
   int dim = 0, order = 0, npe = 0, op = 0, hp = 0, maxd = 0;
   size_t N = 0, L = 0, H = 0;
-  bool per_level = false, simple = false;
+  bool per_level = false, simple = false, dry = false;
   DevBuf<uint8_t> dirichlet;
   DevBuf<double> kmat64;
   DevBuf<float> kmat32;
@@ -483,28 +193,49 @@ struct dkt_plan {
   DevBuf<int64_t> csr_off;
   size_t nbuf = 0;
   // tiled path
-  size_t ntiles = 0, nshared = 0, npartial = 0, max_tile_nodes = 0;
   int tile_leaves = kTileThreads;
+  size_t ntiles = 0, nshared = 0, npartial = 0;
+  int max_nodes = 0, max_grp = 0, max_he = 0, max_rd = 0;
   DevBuf<TileHdr> tiles;
-  DevBuf<int32_t> tile_nodes, tile_out, hph_off, snode, soff;
+  DevBuf<int32_t> tile_nodes, tile_out, snode, soff;
   DevBuf<uint16_t> leaf_loc, group_ploc;
-  DevBuf<uint32_t> leaf_aux, hentries, stale_key;
-  DevBuf<GroupRec> groups;
-  DevBuf<double> stale_w64;
-  DevBuf<float> stale_w32;
+  DevBuf<uint8_t> hlist;
+  DevBuf<uint32_t> leaf_aux, carry_key;
+  DevBuf<uint64_t> hentries, readers;
+  DevBuf<double> carry_w64;
+  DevBuf<float> carry_w32;
   // scratch
   DevBuf<double> scratch64;
   DevBuf<float> scratch32;
   double map_bytes = 0;
+  // Tile statistics: mean readers, max readers, mean carry terms, max carry terms,
+  // mean groups, mean nodes, hanging-transpose entries, mean cone length.
+  std::vector<double> stats = std::vector<double>(8, 0.0);
+  // Optional event timing (dkt_plan_profile): (start, after tile kernel, end) per launch.
+  bool profiling = false;
+  std::vector<hipEvent_t> ev_pool;
+  size_t ev_used = 0;
+  hipEvent_t next_event() {
+    if (ev_used == ev_pool.size()) {
+      hipEvent_t e;
+      HIP_OK(hipEventCreate(&e));
+      ev_pool.push_back(e);
+    }
+    return ev_pool[ev_used++];
+  }
+  ~dkt_plan() {
+    for (hipEvent_t e : ev_pool) (void)hipEventDestroy(e);
+  }
 };
 
 namespace {
 
 // ---------------------------------------------------------------------------
-// Plan construction.
+// Simple path construction.
 // ---------------------------------------------------------------------------
-void build_simple(dkt_plan *pl, const Traversal &tr) {
+void build_simple(dkt_plan *pl, const MeshParams &mp, Traversal &tr) {
   const int npe = pl->npe;
+  compute_stale_terms(mp, tr);
   std::vector<uint8_t> lev(pl->L);
   for (size_t e = 0; e < pl->L; e++) lev[e] = (uint8_t)tr.leaves[e].lev;
   pl->slot.upload(tr.slot);
@@ -512,12 +243,11 @@ void build_simple(dkt_plan *pl, const Traversal &tr) {
   pl->hang_of.upload(tr.hang_of);
   pl->child.upload(tr.child);
   pl->pslot.upload(tr.pslot.empty() ? std::vector<int32_t>(npe, -1) : tr.pslot);
-  std::vector<int32_t> hco(tr.hc_off.begin(), tr.hc_off.end());
-  pl->hc_off.upload(hco);
-  pl->hc_rank.upload(tr.hc_rank.empty() ? std::vector<uint8_t>{0} : tr.hc_rank);
-  pl->hc_node.upload(tr.hc_node.empty() ? std::vector<int32_t>{0} : tr.hc_node);
-  pl->hc_w64.upload(tr.hc_w.empty() ? std::vector<double>{0} : tr.hc_w);
-  pl->hc_w32.upload(cast<float>(tr.hc_w.empty() ? std::vector<double>{0} : tr.hc_w));
+  pl->hc_off.upload(std::vector<int32_t>(tr.hc_off.begin(), tr.hc_off.end()));
+  pl->hc_rank.upload(nonempty(tr.hc_rank));
+  pl->hc_node.upload(nonempty(tr.hc_node));
+  pl->hc_w64.upload(nonempty(tr.hc_w));
+  pl->hc_w32.upload(cast<float>(nonempty(tr.hc_w)));
   // Node-side CSR over contribution positions, in leaf SFC order then rank:
   // direct outputs at e*nPe+r, hanging transposes at L*nPe + h*nPe + s.
   std::vector<int64_t> cnt(pl->N + 1, 0);
@@ -549,38 +279,45 @@ void build_simple(dkt_plan *pl, const Traversal &tr) {
   pl->nbuf = (pl->L + pl->H) * (size_t)npe;
 }
 
-void build_tiled(dkt_plan *pl, const Traversal &tr) {
-  const int npe = pl->npe;
-  const size_t L = pl->L;
-  // Leaves per tile: the per-leaf LDS rows (nPe values) must leave room for the
-  // tile's nodes within 64 KiB.
-  const int T = npe > 16 ? kTileThreads / 2 : kTileThreads;
-  pl->tile_leaves = T;
+// ---------------------------------------------------------------------------
+// Tiled path construction.
+// ---------------------------------------------------------------------------
+struct TiledHost {
+  std::vector<TileHdr> hdr;
+  std::vector<int32_t> tnodes, tout, snode, soff;
+  std::vector<uint16_t> lloc, gploc;
+  std::vector<uint8_t> hl;
+  std::vector<uint32_t> laux, ckey;
+  std::vector<uint64_t> hent, rdr;
+  std::vector<double> cw;
+  int max_nodes = 0, max_grp = 0, max_he = 0, max_rd = 0;
+  size_t nshared = 0, npartial = 0;
+  std::vector<double> stats;
+};
+
+TiledHost build_tiled_host(const MeshParams &mp, const Traversal &tr, size_t N, int T) {
+  const int npe = tr.nPe;
+  const int dim = mp.dim;
+  const size_t L = tr.leaves.size();
   const size_t ntiles = (L + T - 1) / T;
-  pl->ntiles = ntiles;
-  const int dim = pl->dim;
+  StaleBuffer sb(mp, tr);
+  TiledHost o;
+  o.hdr.resize(ntiles);
+  o.lloc.assign(L * (size_t)npe, kNone);
+  o.laux.assign(L, 0);
+  std::vector<int32_t> last_tile(N, -1), ntarget(N, 0), local(N, -1);
 
-  std::vector<TileHdr> hdr(ntiles);
-  std::vector<int32_t> tnodes, tout, hph;
-  std::vector<uint16_t> lloc(L * (size_t)npe, kNone), gploc;
-  std::vector<uint32_t> laux(L, 0), hent, skey;
-  std::vector<GroupRec> grps;
-  std::vector<double> sw;
-  // Tiles touching each node as a contribution target (ordered, de-duplicated).
-  std::vector<int32_t> last_tile(pl->N, -1), ntarget(pl->N, 0);
-  std::vector<int32_t> local(pl->N, -1);
-
-  // Pass 1: count target tiles per node.
+  // Pass 1: tiles that write each node (as a direct or hanging-transpose target).
   for (size_t t = 0; t < ntiles; t++) {
     const size_t l0 = t * T, l1 = std::min(L, l0 + T);
+    auto mark = [&](int32_t n) {
+      if (last_tile[n] != (int32_t)t) {
+        last_tile[n] = (int32_t)t;
+        ntarget[n]++;
+      }
+    };
     for (size_t e = l0; e < l1; e++) {
       const int32_t *sl = &tr.slot[e * npe];
-      auto mark = [&](int32_t n) {
-        if (last_tile[n] != (int32_t)t) {
-          last_tile[n] = (int32_t)t;
-          ntarget[n]++;
-        }
-      };
       for (int r = 0; r < npe; r++)
         if (sl[r] >= 0) mark(sl[r]);
       const int h = tr.hang_of[e];
@@ -592,32 +329,34 @@ void build_tiled(dkt_plan *pl, const Traversal &tr) {
     }
   }
   // Partial slots for shared nodes, consecutive per node, in tile order.
-  std::vector<int32_t> pbase(pl->N, -1), pfill(pl->N, 0);
-  std::vector<int32_t> snode, soff{0};
+  std::vector<int32_t> pbase(N, -1), pfill(N, 0);
+  o.soff.push_back(0);
   int64_t np = 0;
-  for (size_t n = 0; n < pl->N; n++)
+  for (size_t n = 0; n < N; n++)
     if (ntarget[n] > 1) {
       pbase[n] = (int32_t)np;
       np += ntarget[n];
-      snode.push_back((int32_t)n);
-      soff.push_back((int32_t)np);
+      o.snode.push_back((int32_t)n);
+      o.soff.push_back((int32_t)np);
       if (np >= INT32_MAX) throw std::runtime_error("too many partial sums");
     }
-  pl->nshared = snode.size();
-  pl->npartial = (size_t)np;
+  o.nshared = o.snode.size();
+  o.npartial = (size_t)np;
 
   // Pass 2: tiles.
   std::fill(last_tile.begin(), last_tile.end(), -1);
-  size_t max_nodes = 0;
+  double s_rd = 0, m_rd = 0, s_cr = 0, m_cr = 0, s_gr = 0, s_nn = 0, s_cone = 0, n_cone = 0;
   for (size_t t = 0; t < ntiles; t++) {
     const size_t l0 = t * T, l1 = std::min(L, l0 + T);
-    TileHdr &H = hdr[t];
+    TileHdr &H = o.hdr[t];
     H.leaf0 = (int32_t)l0;
     H.nleaf = (int32_t)(l1 - l0);
-    H.node0 = (int32_t)tnodes.size();
-    H.grp0 = (int32_t)grps.size();
-    H.hph0 = (int32_t)hph.size();
-    H.st0 = (int32_t)skey.size();
+    H.node0 = (int32_t)o.tnodes.size();
+    H.grp0 = (int32_t)(o.gploc.size() / npe);
+    H.he0 = (int32_t)o.hent.size();
+    H.hl0 = (int32_t)o.hl.size();
+    H.rd0 = (int32_t)o.rdr.size();
+    H.cr0 = (int32_t)o.ckey.size();
     std::vector<int32_t> touched;
     std::vector<uint8_t> is_target;
     auto loc_of = [&](int32_t n, bool target) -> uint16_t {
@@ -631,114 +370,197 @@ void build_tiled(dkt_plan *pl, const Traversal &tr) {
       if (local[n] >= 0xFFFF) throw std::runtime_error("tile has too many nodes");
       return (uint16_t)local[n];
     };
-    // Parent groups: consecutive hanging leaves with the same parent cell.
-    std::vector<std::pair<int, int>> gl;  // (group id, rank) hanging-transpose targets
+    std::vector<std::pair<int, int>> gl;  // (group, parent rank) hanging-transpose targets
+    std::vector<uint8_t> gfirst, gcount;
+    std::vector<size_t> hang_leaves;      // leaf ids of the tile's hanging leaves, in order
     int cur_group = -1;
     Oct cur_parent{};
     for (size_t e = l0; e < l1; e++) {
       const int32_t *sl = &tr.slot[e * npe];
       for (int r = 0; r < npe; r++)
-        if (sl[r] >= 0) lloc[e * npe + r] = loc_of(sl[r], true);
+        if (sl[r] >= 0) o.lloc[e * npe + r] = loc_of(sl[r], true);
       const Oct &c = tr.leaves[e];
       uint32_t aux = (c.lev & 31u) | ((uint32_t)tr.child[e] << 5);
       const int h = tr.hang_of[e];
-      if (h >= 0) {
-        aux |= 1u << 9;
-        // Parent cell: the leaf's anchor with its own level bit cleared.
-        Oct par = c;
-        par.lev = c.lev - 1;
-        const uint32_t pmask = ~(uint32_t)(((uint64_t)1 << (pl->maxd - (int)par.lev)) - 1);
-        for (int d = 0; d < dim; d++) par.x[d] = c.x[d] & pmask;
-        bool same = cur_group >= 0 && par.lev == cur_parent.lev;
-        for (int d = 0; same && d < dim; d++) same = par.x[d] == cur_parent.x[d];
-        const int32_t *ps = &tr.pslot[(size_t)h * npe];
-        if (!same || grps[H.grp0 + cur_group].count == 255) {
-          cur_group = (int)(grps.size() - H.grp0);
-          cur_parent = par;
-          GroupRec g;
-          g.first = (uint16_t)(e - l0);
-          g.count = 0;
-          g.pad = 0;
-          grps.push_back(g);
-          for (int s = 0; s < npe; s++) gploc.push_back(ps[s] >= 0 ? loc_of(ps[s], false) : kNone);
-        }
-        grps[H.grp0 + cur_group].count++;
-        aux |= (uint32_t)cur_group << 16;
-        uint16_t *gp = &gploc[(size_t)(H.grp0 + cur_group) * npe];
-        for (int s = 0; s < npe; s++)
-          if (ps[s] >= 0 && sl[s] < 0) {
-            loc_of(ps[s], true);
-            gl.push_back({cur_group, s});
-            (void)gp;
-          }
-        for (int64_t k = tr.hc_off[h]; k < tr.hc_off[h + 1]; k++) {
-          const uint16_t j = loc_of(tr.hc_node[k], false);
-          skey.push_back(((uint32_t)(e - l0) << 24) | ((uint32_t)tr.hc_rank[k] << 16) | j);
-          sw.push_back(tr.hc_w[k]);
-        }
-      } else {
+      if (h < 0) {
         cur_group = -1;
+        o.laux[e] = aux;
+        continue;
       }
-      laux[e] = aux;
+      aux |= 1u << 9;
+      Oct par = c;
+      par.lev = c.lev - 1;
+      const uint32_t pmask = ~(uint32_t)(((uint64_t)1 << (mp.maxd - (int)par.lev)) - 1);
+      for (int d = 0; d < dim; d++) par.x[d] = c.x[d] & pmask;
+      bool same = cur_group >= 0 && par.lev == cur_parent.lev && gcount[cur_group] < 255;
+      for (int d = 0; same && d < dim; d++) same = par.x[d] == cur_parent.x[d];
+      const int32_t *ps = &tr.pslot[(size_t)h * npe];
+      if (!same) {
+        cur_group = (int)gfirst.size();
+        cur_parent = par;
+        gfirst.push_back((uint8_t)(e - l0));
+        gcount.push_back(0);
+        for (int s = 0; s < npe; s++) o.gploc.push_back(ps[s] >= 0 ? loc_of(ps[s], false) : kNone);
+      }
+      gcount[cur_group]++;
+      aux |= (uint32_t)cur_group << 16;
+      o.laux[e] = aux;
+      for (int s = 0; s < npe; s++)
+        if (ps[s] >= 0 && sl[s] < 0) {
+          loc_of(ps[s], true);
+          gl.push_back({cur_group, s});
+        }
+      o.hl.push_back((uint8_t)(e - l0));
+      hang_leaves.push_back(e);
     }
-    if ((size_t)H.nleaf > 256) throw std::runtime_error("tile too large for the stale-term key");
-    H.ngrp = (int32_t)(grps.size() - H.grp0);
-    H.nst = (int32_t)(skey.size() - H.st0);
-    // Hanging-transpose entries: de-duplicate (group, rank), colour by target node.
+    // Stale parent-buffer readers and the start of their dependency cones.
+    for (size_t hi = 0; hi < hang_leaves.size(); hi++) {
+      const size_t e = hang_leaves[hi];
+      const int hg = tr.hang_of[e];
+      const int c = tr.child[e];
+      const int32_t *sl = &tr.slot[e * npe];
+      const int32_t *ps = &tr.pslot[(size_t)hg * npe];
+      std::set<int> cur;
+      for (int r = 0; r < npe; r++)
+        if (sl[r] < 0)
+          for (const auto &tw : sb.weights(c, r))
+            if (ps[tw.first] < 0) cur.insert(tw.first);
+      if (cur.empty()) continue;
+      int k = (int)hi - 1;
+      int start = -1;
+      bool carry = false;
+      const size_t cst = o.ckey.size() - (size_t)H.cr0;
+      while (true) {
+        if (k < 0) {  // the cone continues before the tile: carry its state in
+          carry = true;
+          start = 0;
+          const int before = hg - (int)hi - 1;  // global index of the last hanging leaf before the tile
+          if (before >= 0)
+            for (int s : cur)
+              for (const auto &e2 : sb.after((size_t)before, s)) {
+                if (e2.second == 0.0) continue;
+                o.ckey.push_back(((uint32_t)s << 16) | loc_of(e2.first, false));
+                o.cw.push_back(e2.second);
+              }
+          break;
+        }
+        const size_t ek = hang_leaves[k];
+        const int ck = tr.child[ek];
+        const int32_t *pk = &tr.pslot[(size_t)tr.hang_of[ek] * npe];
+        std::set<int> nxt;
+        for (int s : cur)
+          for (const auto &tw : sb.weights(ck, s))
+            if (pk[tw.first] < 0) nxt.insert(tw.first);
+        if (nxt.empty()) {
+          start = k;
+          break;
+        }
+        cur.swap(nxt);
+        k--;
+      }
+      const size_t ccnt = o.ckey.size() - (size_t)H.cr0 - cst;
+      if (cst >= 0xFFFF || ccnt >= 0xFFFF) throw std::runtime_error("carry-term list too long");
+      o.rdr.push_back((uint64_t)(e - l0) | ((uint64_t)start << 8) | ((uint64_t)hi << 16) |
+                      ((uint64_t)(carry ? 1 : 0) << 24) | ((uint64_t)cst << 32) | ((uint64_t)ccnt << 48));
+      s_cone += (double)hi - start + (carry ? 1 : 0);
+      n_cone += 1;
+    }
+    H.ngrp = (int32_t)gfirst.size();
+    H.nhl = (int32_t)(o.hl.size() - H.hl0);
+    H.nrd = (int32_t)(o.rdr.size() - H.rd0);
+    H.ncr = (int32_t)(o.ckey.size() - H.cr0);
+    if (H.nrd > 0xFFFE) throw std::runtime_error("too many readers in a tile");
+    // Hanging-transpose entries, sorted by target node for the segmented sum.
     std::sort(gl.begin(), gl.end());
     gl.erase(std::unique(gl.begin(), gl.end()), gl.end());
-    std::vector<int> phase_of(gl.size());
-    std::unordered_map<int, int> used;  // local node -> number of phases used
-    int nph = 0;
-    for (size_t i = 0; i < gl.size(); i++) {
-      const int j = gploc[(size_t)(H.grp0 + gl[i].first) * npe + gl[i].second];
-      int &u = used[j];
-      phase_of[i] = u++;
-      nph = std::max(nph, u);
+    std::vector<std::pair<uint32_t, uint64_t>> ent;
+    for (const auto &g : gl) {
+      const uint16_t j = o.gploc[(size_t)(H.grp0 + g.first) * npe + g.second];
+      const uint64_t v = (uint64_t)gfirst[g.first] | ((uint64_t)gcount[g.first] << 8) |
+                         ((uint64_t)g.second << 16) | ((uint64_t)j << 24);
+      ent.push_back({(uint32_t)j, v});
     }
-    H.nhph = nph;
-    std::vector<std::vector<uint32_t>> byph(nph);
-    for (size_t i = 0; i < gl.size(); i++)
-      byph[phase_of[i]].push_back(((uint32_t)gl[i].first << 8) | (uint32_t)gl[i].second);
-    int32_t base = (int32_t)hent.size();
-    hph.push_back(base);
-    for (int ph = 0; ph < nph; ph++) {
-      for (uint32_t v : byph[ph]) hent.push_back(v);
-      hph.push_back((int32_t)hent.size());
-    }
-    if (nph == 0) hph.pop_back();
+    std::stable_sort(ent.begin(), ent.end(),
+                     [](const std::pair<uint32_t, uint64_t> &a, const std::pair<uint32_t, uint64_t> &b) {
+                       return a.first < b.first;
+                     });
+    for (const auto &e : ent) o.hent.push_back(e.second);
+    H.nhe = (int32_t)ent.size();
     // Output classification of the tile's nodes.
     H.nnode = (int32_t)touched.size();
-    max_nodes = std::max(max_nodes, touched.size());
     for (size_t j = 0; j < touched.size(); j++) {
       const int32_t n = touched[j];
-      tnodes.push_back(n);
+      o.tnodes.push_back(n);
       if (!is_target[j])
-        tout.push_back(INT32_MIN);
+        o.tout.push_back(INT32_MIN);
       else if (ntarget[n] == 1)
-        tout.push_back(n);
+        o.tout.push_back(n);
       else
-        tout.push_back(-(pbase[n] + pfill[n]++) - 1);
+        o.tout.push_back(-(pbase[n] + pfill[n]++) - 1);
     }
+    o.max_nodes = std::max(o.max_nodes, H.nnode);
+    o.max_grp = std::max(o.max_grp, H.ngrp);
+    o.max_he = std::max(o.max_he, H.nhe);
+    o.max_rd = std::max(o.max_rd, H.nrd);
+    s_rd += H.nrd;
+    m_rd = std::max<double>(m_rd, H.nrd);
+    s_cr += H.ncr;
+    m_cr = std::max<double>(m_cr, H.ncr);
+    s_gr += H.ngrp;
+    s_nn += H.nnode;
   }
-  pl->max_tile_nodes = max_nodes;
-  pl->tiles.upload(hdr);
-  pl->tile_nodes.upload(tnodes);
-  pl->tile_out.upload(tout);
-  pl->leaf_loc.upload(lloc);
-  pl->leaf_aux.upload(laux);
-  pl->groups.upload(grps.empty() ? std::vector<GroupRec>(1) : grps);
-  pl->group_ploc.upload(gploc.empty() ? std::vector<uint16_t>(npe, kNone) : gploc);
-  pl->hph_off.upload(hph.empty() ? std::vector<int32_t>{0} : hph);
-  pl->hentries.upload(hent.empty() ? std::vector<uint32_t>{0} : hent);
-  pl->stale_key.upload(skey.empty() ? std::vector<uint32_t>{0} : skey);
-  pl->stale_w64.upload(sw.empty() ? std::vector<double>{0} : sw);
-  pl->stale_w32.upload(cast<float>(sw.empty() ? std::vector<double>{0} : sw));
-  pl->snode.upload(snode.empty() ? std::vector<int32_t>{0} : snode);
-  pl->soff.upload(soff);
-  pl->map_bytes = (double)(hdr.size() * sizeof(TileHdr) + tnodes.size() * 8 + lloc.size() * 2 + laux.size() * 4 +
-                           grps.size() * sizeof(GroupRec) + gploc.size() * 2 + hent.size() * 4 + skey.size() * 12 +
-                           snode.size() * 8);
+  const double nt = std::max<double>(1, (double)ntiles);
+  o.stats = {s_rd / nt, m_rd, s_cr / nt, m_cr, s_gr / nt, s_nn / nt, (double)o.hent.size(),
+             n_cone > 0 ? s_cone / n_cone : 0.0};
+  return o;
+}
+
+void build_tiled(dkt_plan *pl, const MeshParams &mp, const Traversal &tr) {
+  const int npe = pl->npe;
+  // Leaves per tile: as many as fit the LDS budget (FP64, one right-hand side).
+  int T = kTileThreads;
+  TiledHost o;
+  while (true) {
+    o = build_tiled_host(mp, tr, pl->N, T);
+    size_t lds = 0;
+    switch (npe) {
+      case 4: lds = tile_lds_bytes<double, 4>(o.max_nodes, o.max_grp, o.max_he, o.max_rd, T, 1); break;
+      case 8: lds = tile_lds_bytes<double, 8>(o.max_nodes, o.max_grp, o.max_he, o.max_rd, T, 1); break;
+      case 9: lds = tile_lds_bytes<double, 9>(o.max_nodes, o.max_grp, o.max_he, o.max_rd, T, 1); break;
+      case 16: lds = tile_lds_bytes<double, 16>(o.max_nodes, o.max_grp, o.max_he, o.max_rd, T, 1); break;
+      case 27: lds = tile_lds_bytes<double, 27>(o.max_nodes, o.max_grp, o.max_he, o.max_rd, T, 1); break;
+      default: throw std::invalid_argument("unsupported nodes per element");
+    }
+    if (lds <= kLdsLimit) break;
+    if (T <= 32) throw std::runtime_error("tile does not fit in LDS");
+    T /= 2;
+  }
+  pl->tile_leaves = T;
+  pl->ntiles = o.hdr.size();
+  pl->nshared = o.nshared;
+  pl->npartial = o.npartial;
+  pl->max_nodes = o.max_nodes;
+  pl->max_grp = o.max_grp;
+  pl->max_he = o.max_he;
+  pl->max_rd = o.max_rd;
+  pl->stats = o.stats;
+  pl->tiles.upload(o.hdr);
+  pl->tile_nodes.upload(o.tnodes);
+  pl->tile_out.upload(o.tout);
+  pl->leaf_loc.upload(o.lloc);
+  pl->leaf_aux.upload(o.laux);
+  pl->group_ploc.upload(nonempty(o.gploc, kNone));
+  pl->hentries.upload(nonempty(o.hent));
+  pl->hlist.upload(nonempty(o.hl));
+  pl->readers.upload(nonempty(o.rdr));
+  pl->carry_key.upload(nonempty(o.ckey));
+  pl->carry_w64.upload(nonempty(o.cw));
+  pl->carry_w32.upload(cast<float>(nonempty(o.cw)));
+  pl->snode.upload(nonempty(o.snode));
+  pl->soff.upload(o.soff);
+  pl->map_bytes = (double)(o.hdr.size() * sizeof(TileHdr) + o.tnodes.size() * 8 + o.lloc.size() * 2 +
+                           o.laux.size() * 4 + o.gploc.size() * 2 + o.hent.size() * 8 + o.hl.size() +
+                           o.rdr.size() * 8 + o.ckey.size() * 12 + o.snode.size() * 8);
 }
 
 // ---------------------------------------------------------------------------
@@ -778,8 +600,15 @@ void run_tiled(dkt_plan *pl, int nrhs, const T *x, T *y, bool masked, hipStream_
   constexpr int NPE = Pow<P + 1, DIM>::v;
   const bool f64 = sizeof(T) == 8;
   const T *kmat = f64 ? (const T *)pl->kmat64.p : (const T *)pl->kmat32.p;
-  const T *sw = f64 ? (const T *)pl->stale_w64.p : (const T *)pl->stale_w32.p;
-  const size_t need = std::max<size_t>(1, pl->npartial * (size_t)nrhs);
+  const T *cw = f64 ? (const T *)pl->carry_w64.p : (const T *)pl->carry_w32.p;
+  auto lds_for = [&](int nq) {
+    return tile_lds_bytes<T, NPE>(pl->max_nodes, pl->max_grp, pl->max_he, pl->max_rd, pl->tile_leaves, nq);
+  };
+  // Columns are processed in chunks whose LDS footprint stays within budget.
+  if (lds_for(1) > kLdsLimit) throw std::invalid_argument("tile does not fit in LDS");
+  int chunk = 1;
+  while (chunk < nrhs && lds_for(chunk + 1) <= kLdsLimit) chunk++;
+  const size_t need = std::max<size_t>(1, pl->npartial * (size_t)chunk);
   T *part;
   if (f64) {
     if (pl->scratch64.n < need) pl->scratch64.alloc(need);
@@ -788,41 +617,55 @@ void run_tiled(dkt_plan *pl, int nrhs, const T *x, T *y, bool masked, hipStream_
     if (pl->scratch32.n < need) pl->scratch32.alloc(need);
     part = (T *)pl->scratch32.p;
   }
-  // Columns are processed in chunks whose LDS footprint stays within 64 KiB.
-  const size_t fixed = (size_t)pl->tile_leaves * NPE * sizeof(T);
-  const size_t per_col = 2 * pl->max_tile_nodes * sizeof(T);
-  if (fixed + per_col > 64 * 1024) throw std::invalid_argument("tile does not fit in LDS");
-  const int chunk = (int)std::min<size_t>((size_t)nrhs, (64 * 1024 - fixed) / per_col);
-  const uint8_t *dir = masked ? pl->dirichlet.p : nullptr;
+  TileArgs A;
+  A.tiles = pl->tiles.p;
+  A.tile_nodes = pl->tile_nodes.p;
+  A.tile_out = pl->tile_out.p;
+  A.leaf_loc = pl->leaf_loc.p;
+  A.leaf_aux = pl->leaf_aux.p;
+  A.group_ploc = pl->group_ploc.p;
+  A.hentries = pl->hentries.p;
+  A.hlist = pl->hlist.p;
+  A.readers = pl->readers.p;
+  A.carry_key = pl->carry_key.p;
+  A.dirichlet = masked ? pl->dirichlet.p : nullptr;
+  A.hp = pl->hp;
+  A.ld = nrhs;
+  A.max_nodes = pl->max_nodes;
+  A.max_grp = pl->max_grp;
+  A.max_he = pl->max_he;
+  A.max_rd = pl->max_rd;
+  A.tile_leaves = pl->tile_leaves;
   for (int q0 = 0; q0 < nrhs; q0 += chunk) {
     const int nq = std::min(chunk, nrhs - q0);
-    const size_t lds = fixed + per_col * nq;
+    A.nq = nq;
+    if (pl->profiling) HIP_OK(hipEventRecord(pl->next_event(), st));
     if (pl->ntiles) {
-      hipLaunchKernelGGL((tile_kernel<T, DIM, P>), dim3(pl->ntiles), dim3(kTileThreads), lds, st, pl->tiles.p,
-                         pl->tile_nodes.p, pl->tile_out.p, pl->leaf_loc.p, pl->leaf_aux.p, pl->groups.p,
-                         pl->group_ploc.p, pl->hph_off.p, pl->hentries.p, pl->stale_key.p, sw, kmat, pl->hp, dir,
-                         nq, nrhs, x + q0, y + q0, part);
+      hipLaunchKernelGGL((tile_kernel<T, DIM, P>), dim3(pl->ntiles), dim3(kTileThreads), lds_for(nq), st, A, kmat,
+                         cw, x + q0, y + q0, part);
       HIP_OK(hipGetLastError());
     }
+    if (pl->profiling) HIP_OK(hipEventRecord(pl->next_event(), st));
     if (pl->nshared) {
       const int blk = 256;
       hipLaunchKernelGGL((fixup_kernel<T>), dim3((pl->nshared + blk - 1) / blk), dim3(blk), 0, st,
-                         (int)pl->nshared, pl->snode.p, pl->soff.p, part, dir, nq, nrhs, y + q0);
+                         (int)pl->nshared, pl->snode.p, pl->soff.p, part, A.dirichlet, nq, nrhs, y + q0);
       HIP_OK(hipGetLastError());
     }
+    if (pl->profiling) HIP_OK(hipEventRecord(pl->next_event(), st));
   }
 }
 
 template <typename T>
 void run_matvec(dkt_plan *pl, int nrhs, const T *x, T *y, bool masked, hipStream_t st) {
   const bool simple = pl->simple || pl->per_level;
-#define DKT_DISPATCH(D, P)                                   \
-  if (pl->dim == D && pl->order == P) {                      \
-    if (simple)                                              \
-      run_simple<T, D, P>(pl, nrhs, x, y, masked, st);       \
-    else                                                     \
-      run_tiled<T, D, P>(pl, nrhs, x, y, masked, st);        \
-    return;                                                  \
+#define DKT_DISPATCH(D, P)                             \
+  if (pl->dim == D && pl->order == P) {                \
+    if (simple)                                        \
+      run_simple<T, D, P>(pl, nrhs, x, y, masked, st); \
+    else                                               \
+      run_tiled<T, D, P>(pl, nrhs, x, y, masked, st);  \
+    return;                                            \
   }
   DKT_DISPATCH(2, 1)
   DKT_DISPATCH(3, 1)
@@ -840,7 +683,7 @@ extern "C" int dkt_plan_create(const dkt_da *da, int op, const double *op_data, 
     if (!da || !out) throw std::invalid_argument("null pointer");
     const MeshParams &mp = da_params(da);
     const NodeSet &ns = da_nodes(da);
-    const Traversal &tr = da_traversal(da);
+    Traversal &tr = da_traversal_mut(da);
     auto pl = std::make_unique<dkt_plan>();
     pl->dim = mp.dim;
     pl->order = mp.order;
@@ -850,7 +693,13 @@ extern "C" int dkt_plan_create(const dkt_da *da, int op, const double *op_data, 
     pl->N = ns.nodes.size();
     pl->L = tr.leaves.size();
     pl->H = tr.pslot.size() / tr.nPe;
-    pl->simple = (flags & 1) != 0;
+    pl->simple = (flags & DKT_PLAN_SIMPLE) != 0;
+    pl->dry = (flags & DKT_PLAN_DRYRUN) != 0;
+    struct DryGuard {
+      bool prev;
+      explicit DryGuard(bool on) : prev(g_dry_run) { g_dry_run = on; }
+      ~DryGuard() { g_dry_run = prev; }
+    } dry_guard(pl->dry);
     const int npe = tr.nPe;
     if (pl->N >= (size_t)INT32_MAX || (pl->L + pl->H) * (size_t)npe >= (size_t)INT32_MAX)
       throw std::invalid_argument("mesh too large for 32-bit indices");
@@ -871,10 +720,38 @@ extern "C" int dkt_plan_create(const dkt_da *da, int op, const double *op_data, 
     for (u32 b : ns.bdy) dir[b] = 1;
     pl->dirichlet.upload(dir);
     if (pl->simple || pl->per_level)
-      build_simple(pl.get(), tr);
+      build_simple(pl.get(), mp, tr);
     else
-      build_tiled(pl.get(), tr);
+      build_tiled(pl.get(), mp, tr);
     *out = pl.release();
+  });
+}
+
+extern "C" int dkt_plan_profile(dkt_plan *pl, int enable) {
+  return guarded([&] {
+    if (!pl) throw std::invalid_argument("null pointer");
+    pl->profiling = enable != 0;
+    pl->ev_used = 0;
+  });
+}
+
+extern "C" int dkt_plan_profile_read(dkt_plan *pl, double *tile_ms, double *total_ms, int *ncalls) {
+  return guarded([&] {
+    if (!pl || !tile_ms || !total_ms || !ncalls) throw std::invalid_argument("null pointer");
+    double t = 0, tot = 0;
+    const size_t n = pl->ev_used / 3;
+    for (size_t i = 0; i < n; i++) {
+      float a = 0, b = 0;
+      HIP_OK(hipEventSynchronize(pl->ev_pool[3 * i + 2]));
+      HIP_OK(hipEventElapsedTime(&a, pl->ev_pool[3 * i], pl->ev_pool[3 * i + 1]));
+      HIP_OK(hipEventElapsedTime(&b, pl->ev_pool[3 * i], pl->ev_pool[3 * i + 2]));
+      t += a;
+      tot += b;
+    }
+    *tile_ms = t;
+    *total_ms = tot;
+    *ncalls = (int)n;
+    pl->ev_used = 0;
   });
 }
 
@@ -887,6 +764,7 @@ extern "C" int dkt_plan_matvec(dkt_plan *pl, int dtype, const void *x, void *y, 
                                void *stream) {
   return guarded([&] {
     if (!pl || !x || !y) throw std::invalid_argument("null pointer");
+    if (pl->dry) throw std::invalid_argument("a dry-run plan cannot run a matvec");
     if (nrhs < 1 || nrhs > 64) throw std::invalid_argument("nrhs must be 1..64");
     if (x == y) throw std::invalid_argument("x and y must not alias");
     hipStream_t st = (hipStream_t)stream;
@@ -899,14 +777,16 @@ extern "C" int dkt_plan_matvec(dkt_plan *pl, int dtype, const void *x, void *y, 
   });
 }
 
-// info[0..]: nodes, leaves, hanging leaves, nPe, device map bytes, tiles, shared nodes, partial slots,
-// max nodes per tile.
+// info: nodes, leaves, hanging leaves, nPe, device map bytes, tiles, shared nodes,
+// partial slots, max nodes per tile, the tile statistics, leaves per tile.
 extern "C" int dkt_plan_info(const dkt_plan *pl, double *info, int n) {
   return guarded([&] {
     if (!pl || !info) throw std::invalid_argument("null pointer");
-    const double v[] = {(double)pl->N,        (double)pl->L,       (double)pl->H,
-                        (double)pl->npe,      pl->map_bytes,       (double)pl->ntiles,
-                        (double)pl->nshared,  (double)pl->npartial, (double)pl->max_tile_nodes};
-    for (int i = 0; i < n && i < (int)(sizeof v / sizeof v[0]); i++) info[i] = v[i];
+    std::vector<double> v = {(double)pl->N,       (double)pl->L,       (double)pl->H,
+                             (double)pl->npe,     pl->map_bytes,       (double)pl->ntiles,
+                             (double)pl->nshared, (double)pl->npartial, (double)pl->max_nodes};
+    v.insert(v.end(), pl->stats.begin(), pl->stats.end());
+    v.push_back((double)pl->tile_leaves);
+    for (int i = 0; i < n && i < (int)v.size(); i++) info[i] = v[i];
   });
 }

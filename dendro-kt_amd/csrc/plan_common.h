@@ -1,0 +1,143 @@
+// Shared pieces of the device plan: device buffers, tensor-product
+// interpolation in registers, the tile layout and the plan object.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#define HIP_OK(expr)                                                                    \
+  do {                                                                                  \
+    hipError_t e_ = (expr);                                                             \
+    if (e_ != hipSuccess)                                                               \
+      throw std::runtime_error(std::string(#expr " failed: ") + hipGetErrorString(e_)); \
+  } while (0)
+
+namespace dkt {
+
+constexpr uint16_t kNone = 0xFFFF;
+constexpr int kTileThreads = 256;
+constexpr size_t kLdsLimit = 64 * 1024;
+
+// Host-only plan construction (DKT_PLAN_DRYRUN): sizes are recorded, nothing is
+// allocated on the device.
+extern thread_local bool g_dry_run;
+
+template <typename T>
+struct DevBuf {
+  T *p = nullptr;
+  size_t n = 0;
+  void alloc(size_t count) {
+    release();
+    n = count;
+    if (count && !g_dry_run) HIP_OK(hipMalloc((void **)&p, count * sizeof(T)));
+  }
+  void upload(const std::vector<T> &v) {
+    alloc(v.size());
+    if (n && !g_dry_run) HIP_OK(hipMemcpy(p, v.data(), n * sizeof(T), hipMemcpyHostToDevice));
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  DevBuf() = default;
+  DevBuf(const DevBuf &) = delete;
+  DevBuf &operator=(const DevBuf &) = delete;
+  ~DevBuf() { release(); }
+};
+
+template <int N, int D>
+struct Pow {
+  static constexpr int v = N * Pow<N, D - 1>::v;
+};
+template <int N>
+struct Pow<N, 0> {
+  static constexpr int v = 1;
+};
+
+// Ip<P>::w(c, k, i) = l_k(child point i): Lagrange basis of parent rank k at
+// the i-th node of child half c (RefElement ip_1D_0/1, FEM/src/refel.cpp:158-165).
+template <int P>
+struct Ip;
+template <>
+struct Ip<1> {
+  __host__ __device__ static constexpr double w(int c, int k, int i) {
+    return (c == 0) ? (k == 0 ? (i == 0 ? 1.0 : 0.5) : (i == 0 ? 0.0 : 0.5))
+                    : (k == 0 ? (i == 0 ? 0.5 : 0.0) : (i == 0 ? 0.5 : 1.0));
+  }
+};
+template <>
+struct Ip<2> {
+  __host__ __device__ static constexpr double w(int c, int k, int i) {
+    const double t = 0.5 * (c + 0.5 * i);
+    return k == 0 ? (t - 0.5) * (t - 1.0) / 0.5 : (k == 1 ? t * (t - 1.0) / -0.25 : t * (t - 0.5) / 0.5);
+  }
+};
+
+// In place, axis 0 fastest (KroneckerProduct, FEM/include/tensor.h:181-203):
+//   forward  a[r] <- sum_s W_c[r][s] a[s]   (parent ranks -> child ranks)
+//   backward a[s] <- sum_r W_c[r][s] a[r]   (child ranks -> parent ranks)
+template <typename T, int DIM, int P, bool BWD>
+__device__ __forceinline__ void interp(T *a, int c) {
+  constexpr int n1 = P + 1, NPE = Pow<P + 1, DIM>::v;
+  T b[NPE];
+#pragma unroll
+  for (int d = 0; d < DIM; d++) {
+    const int cb = (c >> d) & 1;
+    constexpr int strides[4] = {1, n1, n1 * n1, n1 * n1 * n1};
+    const int st = strides[d];
+#pragma unroll
+    for (int r = 0; r < NPE; r++) {
+      const int id = (r / st) % n1;
+      const int base = r - id * st;
+      T acc = T(0);
+#pragma unroll
+      for (int k = 0; k < n1; k++)
+        acc += (T)(BWD ? Ip<P>::w(cb, id, k) : Ip<P>::w(cb, k, id)) * a[base + k * st];
+      b[r] = acc;
+    }
+#pragma unroll
+    for (int i = 0; i < NPE; i++) a[i] = b[i];
+  }
+}
+
+// One tile = one workgroup = up to kTileThreads SFC-consecutive leaves.
+struct TileHdr {
+  int32_t leaf0, nleaf;  // leaves
+  int32_t node0, nnode;  // local nodes (tile_nodes / tile_out)
+  int32_t grp0, ngrp;    // sibling groups (group_ploc rows)
+  int32_t he0, nhe;      // hanging-transpose entries, sorted by target node
+  int32_t hl0, nhl;      // hanging leaves in traversal order (tile-local leaf index)
+  int32_t rd0, nrd;      // stale-buffer readers
+  int32_t cr0, ncr;      // carry terms of the readers whose cone starts before the tile
+};
+
+// Leaf word: bits 0-4 level, 5-8 child index, 9 hanging, 16-31 group (tile-local).
+__host__ __device__ inline int aux_level(uint32_t a) { return a & 31u; }
+__host__ __device__ inline int aux_child(uint32_t a) { return (a >> 5) & 15u; }
+__host__ __device__ inline bool aux_hang(uint32_t a) { return (a >> 9) & 1u; }
+__host__ __device__ inline int aux_group(uint32_t a) { return a >> 16; }
+// Hanging-transpose entry: first leaf (8) | count (8) | parent rank (8) | target local node (16).
+__host__ __device__ inline int he_first(uint32_t lo) { return (int)(lo & 255u); }
+// Reader: leaf (8) | cone start, tile-local hanging index (8) | own hanging index (8) |
+// carry (1, bit 24); carry-term start (16) << 32 | carry-term count (16) << 48.
+
+struct TileArgs {
+  const TileHdr *tiles;
+  const int32_t *tile_nodes, *tile_out;
+  const uint16_t *leaf_loc;
+  const uint32_t *leaf_aux;
+  const uint16_t *group_ploc;
+  const uint64_t *hentries;
+  const uint8_t *hlist;
+  const uint64_t *readers;
+  const uint32_t *carry_key;  // buffer rank (8) << 16 | local node (16)
+  const uint8_t *dirichlet;
+  int hp, nq, ld;
+  int max_nodes, max_grp, max_he, max_rd, tile_leaves;
+};
+
+}  // namespace dkt

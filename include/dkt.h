@@ -38,8 +38,11 @@ extern "C" {
 #define DKT_OP_IDENTITY 2
 #define DKT_OP_CAPTURED 3
 
-/* dkt_plan_create flags. */
+/* dkt_plan_create flags.  DKT_PLAN_DRYRUN builds the host-side plan and its
+ * statistics (dkt_plan_info) without touching the device; such a plan cannot
+ * run a matvec. */
 #define DKT_PLAN_SIMPLE 1
+#define DKT_PLAN_DRYRUN 2
 
 typedef struct dkt_octs dkt_octs; /* host list of cells/points */
 typedef struct dkt_da dkt_da;     /* host mesh: tree + CG nodes + traversal */
@@ -132,9 +135,20 @@ int dkt_plan_destroy(dkt_plan *plan);
  * boundary entries of x are read as zero and boundary entries of y are zero. */
 int dkt_plan_matvec(dkt_plan *plan, int dtype, const void *x, void *y, int nrhs, int masked, void *stream);
 
-/* info[0..5] = nodes, leaves, hanging leaves, nodes per element, leaf-map
- * bytes, node-side contribution entries. */
+/* info[0..8] = nodes, leaves, hanging leaves, nodes per element, device map
+ * bytes, tiles, shared nodes, partial slots, max nodes per tile;
+ * info[9..16] = tile statistics of the tiled path: mean and max stale-buffer
+ * readers per tile, mean and max carry terms per tile, mean sibling groups,
+ * mean nodes per tile, hanging-transpose entries, mean replay length per
+ * reader; info[17] = leaves per tile. */
 int dkt_plan_info(const dkt_plan *plan, double *info, int n);
+
+/* Event timing of subsequent tiled matvecs on their stream (no counterpart in
+ * the reference; replaces the bench timers of matvec_bench_adaptive.cpp:199-216).
+ * profile_read synchronizes the recorded events and returns the summed tile
+ * kernel time and whole-matvec time in ms over ncalls launches, then clears. */
+int dkt_plan_profile(dkt_plan *plan, int enable);
+int dkt_plan_profile_read(dkt_plan *plan, double *tile_ms, double *total_ms, int *ncalls);
 
 #ifdef __cplusplus
 }
