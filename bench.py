@@ -10,10 +10,13 @@ resident in HBM.
 
 Prints one JSON line (rank 0):
   value            FP64 GDOF/s  = N_nodes / (ms per matvec)
-  roofline         algorithmic bytes per matvec / matvec time (HIP events on
-                   the launch stream), against 8 TB/s; bytes per matvec
-                   B = 2*s*N + L*(4*nPe + 4) + H*4*nPe  (x read, y written,
-                   int32 leaf maps + descriptor, parent maps of hanging leaves)
+  roofline         algorithmic bytes per matvec / average tile_kernel launch
+                   time (HIP events on the launch stream), against 8 TB/s;
+                   bytes per matvec B = 2*s*N + L*(4*nPe + 4) + H*4*nPe
+                   (x read, y written, int32 leaf maps + descriptor, parent
+                   maps of hanging leaves); traffic = HBM bytes per tile_kernel
+                   launch from FETCH_SIZE + WRITE_SIZE counter runs of this
+                   workload (profiles/tile_kernel_traffic.json), when present
   cpu_baseline     the reference traversal (oracle/_ref, built from the
                    reference sources) timed on this host on a bounded sample of
                    the same generator; falls back to the oracle's C
@@ -129,9 +132,12 @@ def main():
     ap.add_argument("--points", type=int, default=2_000_000)
     ap.add_argument("--maxdepth", type=int, default=9)
     ap.add_argument("--seed", type=int, default=1)
-    ap.add_argument("--cpu-points", type=int, default=4000)
+    ap.add_argument("--cpu-points", type=int, default=5000)
     ap.add_argument("--cpu-reps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calib", action="store_true",
+                    help="run one 256 MiB elementwise copy (torch.neg) first: the reference dispatch that scales "
+                         "FETCH_SIZE / WRITE_SIZE counters to bytes")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU design is documented, "
@@ -152,7 +158,12 @@ def main():
         f"{info['map_bytes'] / 1e6:.0f} MB device maps, built in {time.time() - t0:.1f}s")
     x = torch.tensor(dkt.keyed_uniform(0, da.nodes()), dtype=torch.float64, device="cuda")
     y = torch.empty_like(x)
-    stream = torch.cuda.current_stream()
+    if args.calib:
+        cal = torch.zeros(1 << 25, dtype=torch.float64, device="cuda")
+        cal2 = torch.empty_like(cal)
+        torch.neg(cal, out=cal2)
+        torch.cuda.synchronize()
+        del cal, cal2
 
     def timed(xv, yv, steps, warmup):
         for _ in range(warmup):
@@ -178,8 +189,17 @@ def main():
 
     bytes64 = 2 * 8 * N + L * (4 * npe + 4) + H * 4 * npe
     bytes32 = 2 * 4 * N + L * (4 * npe + 4) + H * 4 * npe
-    gbs64 = bytes64 / (ev64 * 1e-3) / 1e9
-    gbs32 = bytes32 / (ev32 * 1e-3) / 1e9
+    gbs64 = bytes64 / (tile64 * 1e-3) / 1e9
+    gbs32 = bytes32 / (tile32 * 1e-3) / 1e9
+    workload = (f"4D p=1 Poisson matvec (fem::matvec semantics), {args.points}-point 4-component Gaussian-mixture "
+                f"tree (seed {args.seed}), sigma 0.05, max level {args.maxdepth}, 2:1 balanced, Hilbert order")
+    traffic = None
+    tf = os.path.join(HERE, "profiles", "tile_kernel_traffic.json")
+    if os.path.exists(tf):
+        with open(tf) as f:
+            rec = json.load(f)
+        if rec.get("workload") == workload and rec.get("nodes") == N:
+            traffic = rec.get("hbm_bytes_per_launch_f64")
     gdof64 = N / (ms64 * 1e-3) / 1e9
     gdof32 = N / (ms32 * 1e-3) / 1e9
 
@@ -204,15 +224,14 @@ def main():
         "dtype": "f64",
         "data": "synthetic",
         "config": {
-            "workload": "4D p=1 Poisson matvec (fem::matvec semantics), 2e6-point 4-component Gaussian-mixture "
-                        "tree, sigma 0.05, max level 9, 2:1 balanced, Hilbert order",
+            "workload": workload,
             "points": args.points, "elements": da.n_elements, "nodes": N, "traversal_leaves": L,
             "hanging_leaves": H, "nrhs": 1, "mesh_build_s": round(mesh_s, 1),
         },
         "roofline": {"bound": "hbm", "achieved": round(gbs64, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(gbs64 / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": "tile_kernel + fixup_kernel (one matvec)", "bytes_per_launch": bytes64,
-                     "launch_ms": round(ev64, 4), "tile_kernel_ms": round(tile64, 4)},
+                     "frac": round(gbs64 / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "kernel": "tile_kernel<double,4,1>", "bytes_per_launch": bytes64,
+                     "launch_ms": round(tile64, 4), "matvec_ms": round(ev64, 4)},
         "hbm_gbs": round(gbs64, 1),
         "fp32": {"gdof_s": round(gdof32, 4), "ms_per_step": round(ms32, 4), "hbm_gbs": round(gbs32, 1),
                  "rel_l2_vs_fp64": rel32},

@@ -195,13 +195,12 @@ struct dkt_plan {
   // tiled path
   int tile_leaves = kTileThreads;
   size_t ntiles = 0, nshared = 0, npartial = 0;
-  int max_nodes = 0, max_grp = 0, max_he = 0, max_rd = 0;
+  int max_nodes = 0, max_grp = 0, max_he = 0, max_rd = 0, max_cr = 0;
   DevBuf<TileHdr> tiles;
   DevBuf<int32_t> tile_nodes, tile_out, snode, soff;
-  DevBuf<uint16_t> leaf_loc, group_ploc;
-  DevBuf<uint8_t> hlist;
+  DevBuf<uint16_t> leaf_loc, group_ploc, readers;
   DevBuf<uint32_t> leaf_aux, carry_key;
-  DevBuf<uint64_t> hentries, readers;
+  DevBuf<uint64_t> hentries, blocks;
   DevBuf<double> carry_w64;
   DevBuf<float> carry_w32;
   // scratch
@@ -209,8 +208,9 @@ struct dkt_plan {
   DevBuf<float> scratch32;
   double map_bytes = 0;
   // Tile statistics: mean readers, max readers, mean carry terms, max carry terms,
-  // mean groups, mean nodes, hanging-transpose entries, mean cone length.
-  std::vector<double> stats = std::vector<double>(8, 0.0);
+  // mean groups, mean nodes, hanging-transpose entries, mean cone length,
+  // mean and max longest replay of tiles with readers.
+  std::vector<double> stats = std::vector<double>(10, 0.0);
   // Optional event timing (dkt_plan_profile): (start, after tile kernel, end) per launch.
   bool profiling = false;
   std::vector<hipEvent_t> ev_pool;
@@ -286,16 +286,16 @@ struct TiledHost {
   std::vector<TileHdr> hdr;
   std::vector<int32_t> tnodes, tout, snode, soff;
   std::vector<uint16_t> lloc, gploc;
-  std::vector<uint8_t> hl;
   std::vector<uint32_t> laux, ckey;
-  std::vector<uint64_t> hent, rdr;
+  std::vector<uint64_t> hent, blk;
+  std::vector<uint16_t> rdr;
   std::vector<double> cw;
-  int max_nodes = 0, max_grp = 0, max_he = 0, max_rd = 0;
+  int max_nodes = 0, max_grp = 0, max_he = 0, max_rd = 0, max_cr = 0;
   size_t nshared = 0, npartial = 0;
   std::vector<double> stats;
 };
 
-TiledHost build_tiled_host(const MeshParams &mp, const Traversal &tr, size_t N, int T) {
+TiledHost build_tiled_host(const MeshParams &mp, const Traversal &tr, size_t N, int T, int n_groups) {
   const int npe = tr.nPe;
   const int dim = mp.dim;
   const size_t L = tr.leaves.size();
@@ -346,6 +346,7 @@ TiledHost build_tiled_host(const MeshParams &mp, const Traversal &tr, size_t N, 
   // Pass 2: tiles.
   std::fill(last_tile.begin(), last_tile.end(), -1);
   double s_rd = 0, m_rd = 0, s_cr = 0, m_cr = 0, s_gr = 0, s_nn = 0, s_cone = 0, n_cone = 0;
+  double s_span = 0, m_span = 0, n_rtiles = 0;
   for (size_t t = 0; t < ntiles; t++) {
     const size_t l0 = t * T, l1 = std::min(L, l0 + T);
     TileHdr &H = o.hdr[t];
@@ -354,8 +355,8 @@ TiledHost build_tiled_host(const MeshParams &mp, const Traversal &tr, size_t N, 
     H.node0 = (int32_t)o.tnodes.size();
     H.grp0 = (int32_t)(o.gploc.size() / npe);
     H.he0 = (int32_t)o.hent.size();
-    H.hl0 = (int32_t)o.hl.size();
     H.rd0 = (int32_t)o.rdr.size();
+    H.bk0 = (int32_t)o.blk.size();
     H.cr0 = (int32_t)o.ckey.size();
     std::vector<int32_t> touched;
     std::vector<uint8_t> is_target;
@@ -398,22 +399,29 @@ TiledHost build_tiled_host(const MeshParams &mp, const Traversal &tr, size_t N, 
       if (!same) {
         cur_group = (int)gfirst.size();
         cur_parent = par;
-        gfirst.push_back((uint8_t)(e - l0));
+        gfirst.push_back((uint8_t)hang_leaves.size());  // hanging index of the group's first leaf
         gcount.push_back(0);
         for (int s = 0; s < npe; s++) o.gploc.push_back(ps[s] >= 0 ? loc_of(ps[s], false) : kNone);
       }
       gcount[cur_group]++;
-      aux |= (uint32_t)cur_group << 16;
+      aux |= ((uint32_t)hang_leaves.size() << 10) | ((uint32_t)cur_group << 18);
       o.laux[e] = aux;
       for (int s = 0; s < npe; s++)
         if (ps[s] >= 0 && sl[s] < 0) {
           loc_of(ps[s], true);
           gl.push_back({cur_group, s});
         }
-      o.hl.push_back((uint8_t)(e - l0));
       hang_leaves.push_back(e);
     }
-    // Stale parent-buffer readers and the start of their dependency cones.
+    // Stale parent-buffer readers: leaves that read a parent rank with no node
+    // at the parent level.  A reader's cone starts at the last hanging leaf
+    // whose own parent values alone fix the slots the reader needs (start), or
+    // before the tile (carry: the buffer state at the tile start is computed
+    // from host-expanded terms).  Replaying from any earlier start is exact
+    // as well, so readers are grouped into blocks, one replay per block.
+    std::vector<std::pair<int, int>> rd;  // (start or -1, own hanging index)
+    std::vector<uint8_t> rleaf;
+    std::set<int> entry;
     for (size_t hi = 0; hi < hang_leaves.size(); hi++) {
       const size_t e = hang_leaves[hi];
       const int hg = tr.hang_of[e];
@@ -428,20 +436,9 @@ TiledHost build_tiled_host(const MeshParams &mp, const Traversal &tr, size_t N, 
       if (cur.empty()) continue;
       int k = (int)hi - 1;
       int start = -1;
-      bool carry = false;
-      const size_t cst = o.ckey.size() - (size_t)H.cr0;
       while (true) {
-        if (k < 0) {  // the cone continues before the tile: carry its state in
-          carry = true;
-          start = 0;
-          const int before = hg - (int)hi - 1;  // global index of the last hanging leaf before the tile
-          if (before >= 0)
-            for (int s : cur)
-              for (const auto &e2 : sb.after((size_t)before, s)) {
-                if (e2.second == 0.0) continue;
-                o.ckey.push_back(((uint32_t)s << 16) | loc_of(e2.first, false));
-                o.cw.push_back(e2.second);
-              }
+        if (k < 0) {
+          entry.insert(cur.begin(), cur.end());
           break;
         }
         const size_t ek = hang_leaves[k];
@@ -458,18 +455,65 @@ TiledHost build_tiled_host(const MeshParams &mp, const Traversal &tr, size_t N, 
         cur.swap(nxt);
         k--;
       }
-      const size_t ccnt = o.ckey.size() - (size_t)H.cr0 - cst;
-      if (cst >= 0xFFFF || ccnt >= 0xFFFF) throw std::runtime_error("carry-term list too long");
-      o.rdr.push_back((uint64_t)(e - l0) | ((uint64_t)start << 8) | ((uint64_t)hi << 16) |
-                      ((uint64_t)(carry ? 1 : 0) << 24) | ((uint64_t)cst << 32) | ((uint64_t)ccnt << 48));
-      s_cone += (double)hi - start + (carry ? 1 : 0);
+      rd.push_back({start, (int)hi});
+      rleaf.push_back((uint8_t)(e - l0));
+      s_cone += (double)hi - start;
       n_cone += 1;
     }
+    // Buffer state at the tile start on the slots carried readers need.
+    if (!entry.empty() && !hang_leaves.empty()) {
+      const int before = tr.hang_of[hang_leaves[0]] - 1;  // last hanging leaf before the tile
+      if (before >= 0)
+        for (int s : entry)
+          for (const auto &e2 : sb.after((size_t)before, s)) {
+            if (e2.second == 0.0) continue;
+            o.ckey.push_back(((uint32_t)s << 16) | loc_of(e2.first, false));
+            o.cw.push_back(e2.second);
+          }
+    }
+    // Blocks of consecutive readers (by own index) with the smallest maximum
+    // replay length, at most one block per lane group.
+    if (!rd.empty()) {
+      auto cut = [&](int span, std::vector<std::pair<size_t, size_t>> *out) {
+        size_t i = 0, blocks = 0;
+        while (i < rd.size()) {
+          int mn = rd[i].first;
+          size_t j = i + 1;
+          while (j < rd.size() && rd[j].second - std::min(mn, rd[j].first) <= span) mn = std::min(mn, rd[j++].first);
+          if (out) out->push_back({i, j});
+          blocks++;
+          i = j;
+        }
+        return blocks;
+      };
+      int lo = 0, hi = T + 1;
+      while (lo < hi) {
+        const int mid = (lo + hi) / 2;
+        if ((int)cut(mid, nullptr) <= n_groups) hi = mid;
+        else lo = mid + 1;
+      }
+      std::vector<std::pair<size_t, size_t>> blocks;
+      cut(lo, &blocks);
+      int mspan = 0;
+      for (const auto &b : blocks) {
+        int mn = rd[b.first].first, mx = 0;
+        for (size_t r = b.first; r < b.second; r++) {
+          mn = std::min(mn, rd[r].first);
+          mx = std::max(mx, rd[r].second);
+        }
+        mspan = std::max(mspan, mx - mn);
+        const uint64_t S = mn < 0 ? 0xFFFFu : (uint64_t)mn;
+        o.blk.push_back(S | ((uint64_t)mx << 16) | ((uint64_t)(o.rdr.size() - H.rd0 + b.first) << 32) |
+                        ((uint64_t)(b.second - b.first) << 48));
+      }
+      s_span += mspan;
+      m_span = std::max<double>(m_span, mspan);
+      for (size_t r = 0; r < rd.size(); r++) o.rdr.push_back((uint16_t)(rd[r].second | (rleaf[r] << 8)));
+    }
     H.ngrp = (int32_t)gfirst.size();
-    H.nhl = (int32_t)(o.hl.size() - H.hl0);
     H.nrd = (int32_t)(o.rdr.size() - H.rd0);
+    H.nbk = (int32_t)(o.blk.size() - H.bk0);
     H.ncr = (int32_t)(o.ckey.size() - H.cr0);
-    if (H.nrd > 0xFFFE) throw std::runtime_error("too many readers in a tile");
     // Hanging-transpose entries, sorted by target node for the segmented sum.
     std::sort(gl.begin(), gl.end());
     gl.erase(std::unique(gl.begin(), gl.end()), gl.end());
@@ -502,6 +546,8 @@ TiledHost build_tiled_host(const MeshParams &mp, const Traversal &tr, size_t N, 
     o.max_grp = std::max(o.max_grp, H.ngrp);
     o.max_he = std::max(o.max_he, H.nhe);
     o.max_rd = std::max(o.max_rd, H.nrd);
+    o.max_cr = std::max(o.max_cr, H.ncr);
+    if (H.nrd > 0) n_rtiles += 1;
     s_rd += H.nrd;
     m_rd = std::max<double>(m_rd, H.nrd);
     s_cr += H.ncr;
@@ -511,28 +557,34 @@ TiledHost build_tiled_host(const MeshParams &mp, const Traversal &tr, size_t N, 
   }
   const double nt = std::max<double>(1, (double)ntiles);
   o.stats = {s_rd / nt, m_rd, s_cr / nt, m_cr, s_gr / nt, s_nn / nt, (double)o.hent.size(),
-             n_cone > 0 ? s_cone / n_cone : 0.0};
+             n_cone > 0 ? s_cone / n_cone : 0.0, n_rtiles > 0 ? s_span / n_rtiles : 0.0, m_span};
   return o;
+}
+
+template <int NPE>
+size_t lds_f64(const TiledHost &o, int T) {
+  return tile_lds_bytes<double, NPE>(TileDims{o.max_nodes, o.max_grp, o.max_he, o.max_rd, o.max_cr, T}, 1);
 }
 
 void build_tiled(dkt_plan *pl, const MeshParams &mp, const Traversal &tr) {
   const int npe = pl->npe;
+  const int G = npe <= 4 ? 4 : (npe <= 8 ? 8 : (npe <= 16 ? 16 : 32));
   // Leaves per tile: as many as fit the LDS budget (FP64, one right-hand side).
   int T = kTileThreads;
   TiledHost o;
   while (true) {
-    o = build_tiled_host(mp, tr, pl->N, T);
+    o = build_tiled_host(mp, tr, pl->N, T, std::max(1, T / G));
     size_t lds = 0;
     switch (npe) {
-      case 4: lds = tile_lds_bytes<double, 4>(o.max_nodes, o.max_grp, o.max_he, o.max_rd, T, 1); break;
-      case 8: lds = tile_lds_bytes<double, 8>(o.max_nodes, o.max_grp, o.max_he, o.max_rd, T, 1); break;
-      case 9: lds = tile_lds_bytes<double, 9>(o.max_nodes, o.max_grp, o.max_he, o.max_rd, T, 1); break;
-      case 16: lds = tile_lds_bytes<double, 16>(o.max_nodes, o.max_grp, o.max_he, o.max_rd, T, 1); break;
-      case 27: lds = tile_lds_bytes<double, 27>(o.max_nodes, o.max_grp, o.max_he, o.max_rd, T, 1); break;
+      case 4: lds = lds_f64<4>(o, T); break;
+      case 8: lds = lds_f64<8>(o, T); break;
+      case 9: lds = lds_f64<9>(o, T); break;
+      case 16: lds = lds_f64<16>(o, T); break;
+      case 27: lds = lds_f64<27>(o, T); break;
       default: throw std::invalid_argument("unsupported nodes per element");
     }
     if (lds <= kLdsLimit) break;
-    if (T <= 32) throw std::runtime_error("tile does not fit in LDS");
+    if (T <= 64) throw std::runtime_error("tile does not fit in LDS");
     T /= 2;
   }
   pl->tile_leaves = T;
@@ -543,6 +595,7 @@ void build_tiled(dkt_plan *pl, const MeshParams &mp, const Traversal &tr) {
   pl->max_grp = o.max_grp;
   pl->max_he = o.max_he;
   pl->max_rd = o.max_rd;
+  pl->max_cr = o.max_cr;
   pl->stats = o.stats;
   pl->tiles.upload(o.hdr);
   pl->tile_nodes.upload(o.tnodes);
@@ -551,16 +604,16 @@ void build_tiled(dkt_plan *pl, const MeshParams &mp, const Traversal &tr) {
   pl->leaf_aux.upload(o.laux);
   pl->group_ploc.upload(nonempty(o.gploc, kNone));
   pl->hentries.upload(nonempty(o.hent));
-  pl->hlist.upload(nonempty(o.hl));
   pl->readers.upload(nonempty(o.rdr));
+  pl->blocks.upload(nonempty(o.blk));
   pl->carry_key.upload(nonempty(o.ckey));
   pl->carry_w64.upload(nonempty(o.cw));
   pl->carry_w32.upload(cast<float>(nonempty(o.cw)));
   pl->snode.upload(nonempty(o.snode));
   pl->soff.upload(o.soff);
   pl->map_bytes = (double)(o.hdr.size() * sizeof(TileHdr) + o.tnodes.size() * 8 + o.lloc.size() * 2 +
-                           o.laux.size() * 4 + o.gploc.size() * 2 + o.hent.size() * 8 + o.hl.size() +
-                           o.rdr.size() * 8 + o.ckey.size() * 12 + o.snode.size() * 8);
+                           o.laux.size() * 4 + o.gploc.size() * 2 + o.hent.size() * 8 + o.rdr.size() * 2 +
+                           o.blk.size() * 8 + o.ckey.size() * 12 + o.snode.size() * 8);
 }
 
 // ---------------------------------------------------------------------------
@@ -602,7 +655,8 @@ void run_tiled(dkt_plan *pl, int nrhs, const T *x, T *y, bool masked, hipStream_
   const T *kmat = f64 ? (const T *)pl->kmat64.p : (const T *)pl->kmat32.p;
   const T *cw = f64 ? (const T *)pl->carry_w64.p : (const T *)pl->carry_w32.p;
   auto lds_for = [&](int nq) {
-    return tile_lds_bytes<T, NPE>(pl->max_nodes, pl->max_grp, pl->max_he, pl->max_rd, pl->tile_leaves, nq);
+    return tile_lds_bytes<T, NPE>(
+        TileDims{pl->max_nodes, pl->max_grp, pl->max_he, pl->max_rd, pl->max_cr, pl->tile_leaves}, nq);
   };
   // Columns are processed in chunks whose LDS footprint stays within budget.
   if (lds_for(1) > kLdsLimit) throw std::invalid_argument("tile does not fit in LDS");
@@ -625,23 +679,19 @@ void run_tiled(dkt_plan *pl, int nrhs, const T *x, T *y, bool masked, hipStream_
   A.leaf_aux = pl->leaf_aux.p;
   A.group_ploc = pl->group_ploc.p;
   A.hentries = pl->hentries.p;
-  A.hlist = pl->hlist.p;
   A.readers = pl->readers.p;
   A.carry_key = pl->carry_key.p;
   A.dirichlet = masked ? pl->dirichlet.p : nullptr;
   A.hp = pl->hp;
   A.ld = nrhs;
-  A.max_nodes = pl->max_nodes;
-  A.max_grp = pl->max_grp;
-  A.max_he = pl->max_he;
-  A.max_rd = pl->max_rd;
-  A.tile_leaves = pl->tile_leaves;
+  A.blocks = pl->blocks.p;
+  A.dims = TileDims{pl->max_nodes, pl->max_grp, pl->max_he, pl->max_rd, pl->max_cr, pl->tile_leaves};
   for (int q0 = 0; q0 < nrhs; q0 += chunk) {
     const int nq = std::min(chunk, nrhs - q0);
     A.nq = nq;
     if (pl->profiling) HIP_OK(hipEventRecord(pl->next_event(), st));
     if (pl->ntiles) {
-      hipLaunchKernelGGL((tile_kernel<T, DIM, P>), dim3(pl->ntiles), dim3(kTileThreads), lds_for(nq), st, A, kmat,
+      hipLaunchKernelGGL((tile_kernel<T, DIM, P>), dim3(pl->ntiles), dim3(pl->tile_leaves), lds_for(nq), st, A, kmat,
                          cw, x + q0, y + q0, part);
       HIP_OK(hipGetLastError());
     }

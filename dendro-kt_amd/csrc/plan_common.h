@@ -110,20 +110,30 @@ struct TileHdr {
   int32_t node0, nnode;  // local nodes (tile_nodes / tile_out)
   int32_t grp0, ngrp;    // sibling groups (group_ploc rows)
   int32_t he0, nhe;      // hanging-transpose entries, sorted by target node
-  int32_t hl0, nhl;      // hanging leaves in traversal order (tile-local leaf index)
-  int32_t rd0, nrd;      // stale-buffer readers
-  int32_t cr0, ncr;      // carry terms of the readers whose cone starts before the tile
+  int32_t rd0, nrd;      // stale-buffer readers, grouped by replay block, by hanging index within a block
+  int32_t bk0, nbk;      // replay blocks
+  int32_t cr0, ncr;      // terms of the parent-buffer state at the tile start (carried slots)
 };
 
-// Leaf word: bits 0-4 level, 5-8 child index, 9 hanging, 16-31 group (tile-local).
+// Leaf word: bits 0-4 level, 5-8 child index, 9 hanging, 10-17 hanging index
+// within the tile, 18-31 sibling group (tile-local).
 __host__ __device__ inline int aux_level(uint32_t a) { return a & 31u; }
 __host__ __device__ inline int aux_child(uint32_t a) { return (a >> 5) & 15u; }
 __host__ __device__ inline bool aux_hang(uint32_t a) { return (a >> 9) & 1u; }
-__host__ __device__ inline int aux_group(uint32_t a) { return a >> 16; }
+__host__ __device__ inline int aux_hidx(uint32_t a) { return (a >> 10) & 255u; }
+__host__ __device__ inline int aux_group(uint32_t a) { return a >> 18; }
 // Hanging-transpose entry: first leaf (8) | count (8) | parent rank (8) | target local node (16).
 __host__ __device__ inline int he_first(uint32_t lo) { return (int)(lo & 255u); }
-// Reader: leaf (8) | cone start, tile-local hanging index (8) | own hanging index (8) |
-// carry (1, bit 24); carry-term start (16) << 32 | carry-term count (16) << 48.
+// Reader (uint16): hanging index (8) | tile-local leaf (8) << 8.
+// Replay block (uint64): start hanging index (16, 0xFFFF = from the carried
+// tile-start state) | last hanging index (16) << 16 | first reader (16) << 32 |
+// reader count (16) << 48.
+// Chain word (in LDS, per hanging index): child (8) | absent parent ranks (32) << 32.
+
+// Per-tile maxima that size the LDS carve.
+struct TileDims {
+  int max_nodes, max_grp, max_he, max_rd, max_cr, tile_leaves;
+};
 
 struct TileArgs {
   const TileHdr *tiles;
@@ -132,12 +142,12 @@ struct TileArgs {
   const uint32_t *leaf_aux;
   const uint16_t *group_ploc;
   const uint64_t *hentries;
-  const uint8_t *hlist;
-  const uint64_t *readers;
+  const uint16_t *readers;
+  const uint64_t *blocks;
   const uint32_t *carry_key;  // buffer rank (8) << 16 | local node (16)
   const uint8_t *dirichlet;
   int hp, nq, ld;
-  int max_nodes, max_grp, max_he, max_rd, tile_leaves;
+  TileDims dims;
 };
 
 }  // namespace dkt

@@ -10,10 +10,10 @@
 //      in place and reads absent parent ranks from what the previous hanging
 //      leaf left there (FEM/include/matvec.h:283,411-456).  For every leaf that
 //      reads such a value ("reader"), the host gives the start of its
-//      dependency cone; the reader's work-item replays the recurrence
-//      B_k = F_k + W_ck (Q_k B_{k-1}) forward from there (exact on the slots
-//      the reader needs), starting from host-computed carry terms when the
-//      cone begins before the tile;
+//      dependency cone; a group of lanes (one per buffer slot) replays the
+//      recurrence B_k = F_k + W_ck (Q_k B_{k-1}) forward from there (exact on
+//      the slots the reader needs), starting from host-computed carry terms
+//      when the cone begins before the tile;
 //   4. v = 2^(-level*hp) K_ref u in registers;
 //   5. direct outputs are added into the LDS accumulator y_loc in nPe phases,
 //      one per rank (two disjoint leaves never share a node at the same rank,
@@ -30,32 +30,113 @@
 
 namespace dkt {
 
+// Lanes that replay one stale-buffer reader together: one lane per buffer slot.
+template <int NPE>
+struct Group {
+  static constexpr int v = NPE <= 4 ? 4 : (NPE <= 8 ? 8 : (NPE <= 16 ? 16 : 32));
+};
+
+// Cross-lane moves inside a row of 16 lanes (DPP, a VALU operand modifier;
+// no LDS round trip).  Controls: quad_perm [1,0,3,2] = 0xB1, [2,3,0,1] = 0x4E,
+// [3,2,1,0] = 0x1B, row_half_mirror = 0x141, row_ror:8 = 0x128.
+template <int CTRL>
+__device__ __forceinline__ float dpp_mov(float v) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), CTRL, 0xF, 0xF, true));
+}
+template <int CTRL>
+__device__ __forceinline__ double dpp_mov(double v) {
+  const long long u = __builtin_bit_cast(long long, v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)(u & 0xFFFFFFFFll), CTRL, 0xF, 0xF, true);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(u >> 32), CTRL, 0xF, 0xF, true);
+  return __builtin_bit_cast(double, ((long long)hi << 32) | (unsigned int)lo);
+}
+// Value of lane (lane ^ (1 << D)) within the same row of 16.
+template <int D, typename T>
+__device__ __forceinline__ T xor_lane(T a) {
+  if constexpr (D == 0) return dpp_mov<0xB1>(a);
+  else if constexpr (D == 1) return dpp_mov<0x4E>(a);
+  else if constexpr (D == 2) return dpp_mov<0x1B>(dpp_mov<0x141>(a));  // (lane ^ 7) ^ 3
+  else return dpp_mov<0x128>(a);
+}
+
+// Forward parent-to-child interpolation with the buffer spread over a lane
+// group (lane s holds slot s): the same tensor-product map as interp(), one
+// axis at a time, with the other entries of an axis line fetched from their
+// lanes.  p = 1: child node i of axis d is the parent node when i equals the
+// child bit, else the midpoint 0.5 (a0 + a1).
+template <typename T, int DIM, int P>
+__device__ __forceinline__ T interp_lanes(T a, int lane, int c) {
+  constexpr int G = Group<Pow<P + 1, DIM>::v>::v;
+  if constexpr (P == 1) {
+    const bool k0 = (lane & 1) == (c & 1);
+    T b = xor_lane<0>(a);
+    a = k0 ? a : T(0.5) * (a + b);
+    if constexpr (DIM > 1) {
+      const bool k1 = ((lane >> 1) & 1) == ((c >> 1) & 1);
+      b = xor_lane<1>(a);
+      a = k1 ? a : T(0.5) * (a + b);
+    }
+    if constexpr (DIM > 2) {
+      const bool k2 = ((lane >> 2) & 1) == ((c >> 2) & 1);
+      b = xor_lane<2>(a);
+      a = k2 ? a : T(0.5) * (a + b);
+    }
+    if constexpr (DIM > 3) {
+      const bool k3 = ((lane >> 3) & 1) == ((c >> 3) & 1);
+      b = xor_lane<3>(a);
+      a = k3 ? a : T(0.5) * (a + b);
+    }
+  } else {
+    constexpr int n1 = P + 1;
+    int st = 1;
+#pragma unroll
+    for (int d = 0; d < DIM; d++) {
+      const int id = (lane / st) % n1;
+      const int base = lane - id * st;
+      const int cb = (c >> d) & 1;
+      T acc = T(0);
+#pragma unroll
+      for (int k = 0; k < n1; k++) acc += (T)Ip<P>::w(cb, k, id) * __shfl(a, base + k * st, G);
+      a = acc;
+      st *= n1;
+    }
+  }
+  return a;
+}
+
 template <typename T, int DIM, int P>
 __global__ void __launch_bounds__(kTileThreads) tile_kernel(TileArgs A, const T *__restrict__ kref,
                                                             const T *__restrict__ carry_w, const T *__restrict__ x,
                                                             T *__restrict__ y, T *__restrict__ partial) {
   constexpr int NPE = Pow<P + 1, DIM>::v;
+  constexpr int G = Group<NPE>::v;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const TileHdr th = A.tiles[blockIdx.x];
+  const TileDims D = A.dims;
   const int nq = A.nq;
-  const int TL = A.tile_leaves;
-  const int ulen = A.max_he > A.max_rd * NPE ? A.max_he : A.max_rd * NPE;
+  const int TL = D.tile_leaves;
+  const int FS = TL + 1;  // odd row stride of fbuf: a lane group reading one leaf's slots hits distinct banks
+  const int ulen = D.max_he > D.max_rd * NPE ? D.max_he : D.max_rd * NPE;
   size_t off = 0;
   auto carve = [&](size_t bytes) {
     unsigned char *p = smem_raw + off;
     off += (bytes + 15) & ~(size_t)15;
     return p;
   };
-  T *x_loc = reinterpret_cast<T *>(carve((size_t)A.max_nodes * nq * sizeof(T)));
-  T *y_loc = reinterpret_cast<T *>(carve((size_t)A.max_nodes * nq * sizeof(T)));
-  T *fbuf = reinterpret_cast<T *>(carve((size_t)TL * NPE * sizeof(T)));  // [leaf][NPE]
-  T *ubuf = reinterpret_cast<T *>(carve((size_t)ulen * sizeof(T)));       // reader corrections, then group sums
-  uint64_t *hent = reinterpret_cast<uint64_t *>(carve((size_t)A.max_he * 8));
-  uint32_t *aux_s = reinterpret_cast<uint32_t *>(carve((size_t)TL * 4));
-  uint16_t *gploc = reinterpret_cast<uint16_t *>(carve((size_t)A.max_grp * NPE * 2));
+  T *x_loc = reinterpret_cast<T *>(carve((size_t)D.max_nodes * nq * sizeof(T)));
+  T *y_loc = reinterpret_cast<T *>(carve((size_t)D.max_nodes * nq * sizeof(T)));
+  T *fbuf = reinterpret_cast<T *>(carve((size_t)FS * NPE * sizeof(T)));  // [slot][hanging index]
+  T *ubuf = reinterpret_cast<T *>(carve((size_t)ulen * sizeof(T)));      // [reader][slot], then group sums
+  T *cst = reinterpret_cast<T *>(carve((size_t)NPE * nq * sizeof(T)));   // parent buffer at the tile start
+  T *cwt_s = reinterpret_cast<T *>(carve((size_t)D.max_cr * sizeof(T)));
+  uint64_t *hent = reinterpret_cast<uint64_t *>(carve((size_t)D.max_he * 8));
+  uint64_t *cw_s = reinterpret_cast<uint64_t *>(carve((size_t)TL * 8));  // chain word per hanging index
+  uint32_t *ckey_s = reinterpret_cast<uint32_t *>(carve((size_t)D.max_cr * 4));
+  uint16_t *gploc = reinterpret_cast<uint16_t *>(carve((size_t)D.max_grp * NPE * 2));
   uint16_t *ridx_s = reinterpret_cast<uint16_t *>(carve((size_t)TL * 2));
-  uint8_t *hl_s = carve((size_t)TL);
+  uint16_t *rdl_s = reinterpret_cast<uint16_t *>(carve((size_t)D.max_rd * 2));
   const int tid = threadIdx.x;
+  const int nthr = blockDim.x;
 
   // 1. Stage.
   const bool active = tid < th.nleaf;
@@ -70,11 +151,8 @@ __global__ void __launch_bounds__(kTileThreads) tile_kernel(TileArgs A, const T 
 #pragma unroll
     for (int r = 0; r < NPE; r++) loc[r] = kNone;
   }
-  if (tid < TL) {
-    aux_s[tid] = aux;
-    ridx_s[tid] = kNone;
-  }
-  for (int j = tid; j < th.nnode; j += kTileThreads) {
+  if (tid < TL) ridx_s[tid] = kNone;
+  for (int j = tid; j < th.nnode; j += nthr) {
     const int g = A.tile_nodes[th.node0 + j];
     const bool z = A.dirichlet && A.dirichlet[g];
     for (int q = 0; q < nq; q++) {
@@ -82,69 +160,89 @@ __global__ void __launch_bounds__(kTileThreads) tile_kernel(TileArgs A, const T 
       y_loc[j * nq + q] = T(0);
     }
   }
-  for (int k = tid; k < th.ngrp * NPE; k += kTileThreads) gploc[k] = A.group_ploc[(size_t)th.grp0 * NPE + k];
-  for (int k = tid; k < th.nhe; k += kTileThreads) hent[k] = A.hentries[th.he0 + k];
-  for (int k = tid; k < th.nhl; k += kTileThreads) hl_s[k] = A.hlist[th.hl0 + k];
+  for (int k = tid; k < th.ngrp * NPE; k += nthr) gploc[k] = A.group_ploc[(size_t)th.grp0 * NPE + k];
+  for (int k = tid; k < th.nhe; k += nthr) hent[k] = A.hentries[th.he0 + k];
+  for (int k = tid; k < th.nrd; k += nthr) rdl_s[k] = A.readers[th.rd0 + k];
+  for (int k = tid; k < th.ncr; k += nthr) {
+    ckey_s[k] = A.carry_key[th.cr0 + k];
+    cwt_s[k] = carry_w[th.cr0 + k];
+  }
   __syncthreads();
 
   const bool hang = active && aux_hang(aux);
   const int child = aux_child(aux);
+  const int hidx = aux_hidx(aux);
   const T scale = (T)ldexp(1.0, -aux_level(aux) * A.hp);
   const uint16_t *ploc = gploc + (size_t)aux_group(aux) * NPE;
+  if (hang) {
+    uint64_t absent = 0;
+#pragma unroll
+    for (int s = 0; s < NPE; s++)
+      if (ploc[s] == kNone) absent |= (uint64_t)1 << s;
+    cw_s[hidx] = (uint64_t)child | (absent << 32);
+  }
+  const int lane = tid % G, grp = tid / G, ngroups = nthr / G;
+  const int sl = lane < NPE ? lane : 0;
 
   for (int q = 0; q < nq; q++) {
-    // 2. Interpolation of the present parent nodes.
+    // 2. Interpolation of the present parent nodes; parent-buffer state at the
+    //    tile start (terms sorted by slot).
     if (hang) {
       T f[NPE];
 #pragma unroll
       for (int s = 0; s < NPE; s++) f[s] = ploc[s] != kNone ? x_loc[ploc[s] * nq + q] : T(0);
       interp<T, DIM, P, false>(f, child);
 #pragma unroll
-      for (int r = 0; r < NPE; r++) fbuf[tid * NPE + r] = f[r];
+      for (int r = 0; r < NPE; r++) fbuf[r * FS + hidx] = f[r];
+    }
+    if (tid < NPE) {
+      T acc = T(0);
+      for (int c = 0; c < th.ncr; c++) {
+        const uint32_t key = ckey_s[c];
+        if ((int)(key >> 16) == tid) acc += cwt_s[c] * x_loc[(key & 0xFFFFu) * nq + q];
+      }
+      cst[tid * nq + q] = acc;
     }
     __syncthreads();
-    // 3. Stale parent-buffer reads, one work-item per reader.
-    for (int i = tid; i < th.nrd; i += kTileThreads) {
-      const uint64_t rec = A.readers[th.rd0 + i];
-      const int leaf = (int)(rec & 255u), start = (int)((rec >> 8) & 255u), own = (int)((rec >> 16) & 255u);
-      const bool carry = (rec >> 24) & 1u;
-      T B[NPE];
+    // 3. Stale parent-buffer reads: a lane group per replay block runs
+    //    B_k = F_k + W_ck (Q_k B_(k-1)) over the block's hanging leaves and
+    //    keeps W_ck (Q_k B_(k-1)) of each reader it passes.
+    for (int b0 = 0; b0 < th.nbk; b0 += ngroups) {
+      const int bi = b0 + grp;
+      if (bi >= th.nbk) continue;
+      const uint64_t blk = A.blocks[th.bk0 + bi];
+      const int S = (int)(blk & 0xFFFFu), E = (int)((blk >> 16) & 0xFFFFu);
+      int p = (int)((blk >> 32) & 0xFFFFu);
+      const int pend = p + (int)(blk >> 48);
+      int nxt = rdl_s[p] & 255;
+      T B;
       int k;
-      if (carry) {
-#pragma unroll
-        for (int s = 0; s < NPE; s++) B[s] = T(0);
-        const int c0 = th.cr0 + (int)((rec >> 32) & 0xFFFFu), c1 = c0 + (int)(rec >> 48);
-        for (int c = c0; c < c1; c++) {
-          const uint32_t key = A.carry_key[c];
-          const int s = (key >> 16) & 255, j = key & 0xFFFF;
-          const T add = carry_w[c] * x_loc[j * nq + q];
-#pragma unroll
-          for (int ss = 0; ss < NPE; ss++)
-            if (ss == s) B[ss] += add;
-        }
+      if (S == 0xFFFF) {
+        B = cst[sl * nq + q];
         k = 0;
       } else {
-        const int l = hl_s[start];
-#pragma unroll
-        for (int s = 0; s < NPE; s++) B[s] = fbuf[l * NPE + s];
-        k = start + 1;
+        B = fbuf[sl * FS + S];
+        k = S + 1;
       }
+      // Chain words and F values do not depend on B: loaded one step ahead.
+      uint64_t cw = cw_s[k];
+      T Fk = fbuf[sl * FS + k];
       for (;; k++) {
-        const int l = k < own ? hl_s[k] : leaf;
-        const uint32_t la = aux_s[l];
-        const uint16_t *pl = gploc + (size_t)aux_group(la) * NPE;
-        T b[NPE];
-#pragma unroll
-        for (int s = 0; s < NPE; s++) b[s] = pl[s] == kNone ? B[s] : T(0);
-        interp<T, DIM, P, false>(b, aux_child(la));
-        if (k >= own) {  // the reader itself: keep only the stale part
-#pragma unroll
-          for (int s = 0; s < NPE; s++) ubuf[i * NPE + s] = b[s];
-          ridx_s[leaf] = (uint16_t)i;
-          break;
+        const bool last = k >= E;
+        const uint64_t cwn = last ? cw : cw_s[k + 1];
+        const T Fn = fbuf[sl * FS + (last ? k : k + 1)];
+        const bool absent = lane < NPE && ((cw >> (32 + lane)) & 1u);
+        const T b = interp_lanes<T, DIM, P>(absent ? B : T(0), lane, (int)(cw & 15u));
+        if (k == nxt) {
+          if (lane < NPE) ubuf[p * NPE + lane] = b;
+          if (lane == 0) ridx_s[rdl_s[p] >> 8] = (uint16_t)p;
+          p++;
+          nxt = p < pend ? (rdl_s[p] & 255) : -1;
         }
-#pragma unroll
-        for (int s = 0; s < NPE; s++) B[s] = fbuf[l * NPE + s] + b[s];
+        if (last) break;
+        B = Fk + b;
+        cw = cwn;
+        Fk = Fn;
       }
     }
     __syncthreads();
@@ -156,7 +254,7 @@ __global__ void __launch_bounds__(kTileThreads) tile_kernel(TileArgs A, const T 
 #pragma unroll
       for (int r = 0; r < NPE; r++) {
         T hv = T(0);
-        if (hang) hv = fbuf[tid * NPE + r] + (ri != kNone ? ubuf[ri * NPE + r] : T(0));
+        if (hang) hv = fbuf[r * FS + hidx] + (ri != kNone ? ubuf[ri * NPE + r] : T(0));
         u[r] = loc[r] != kNone ? x_loc[loc[r] * nq + q] : hv;
       }
 #pragma unroll
@@ -181,18 +279,19 @@ __global__ void __launch_bounds__(kTileThreads) tile_kernel(TileArgs A, const T 
         for (int r = 0; r < NPE; r++) w[r] = (hang && loc[r] == kNone) ? v[r] : T(0);
         interp<T, DIM, P, true>(w, child);
 #pragma unroll
-        for (int s = 0; s < NPE; s++) fbuf[tid * NPE + s] = (hang && loc[s] == kNone) ? w[s] : T(0);
+        for (int s = 0; s < NPE; s++)
+          if (hang) fbuf[s * FS + hidx] = loc[s] == kNone ? w[s] : T(0);
       }
       __syncthreads();
-      for (int k = tid; k < th.nhe; k += kTileThreads) {
+      for (int k = tid; k < th.nhe; k += nthr) {
         const uint64_t e = hent[k];
         const int f0 = (int)(e & 255u), cnt = (int)((e >> 8) & 255u), s = (int)((e >> 16) & 255u);
         T acc = T(0);
-        for (int l = 0; l < cnt; l++) acc += fbuf[(f0 + l) * NPE + s];
+        for (int l = 0; l < cnt; l++) acc += fbuf[s * FS + f0 + l];
         ubuf[k] = acc;
       }
       __syncthreads();
-      for (int k = tid; k < th.nhe; k += kTileThreads) {
+      for (int k = tid; k < th.nhe; k += nthr) {
         const int j = (int)((hent[k] >> 24) & 0xFFFFu);
         if (k > 0 && (int)((hent[k - 1] >> 24) & 0xFFFFu) == j) continue;  // not a segment head
         T acc = T(0);
@@ -203,7 +302,7 @@ __global__ void __launch_bounds__(kTileThreads) tile_kernel(TileArgs A, const T 
     __syncthreads();
   }
   // 7. Write out: owned nodes to y, shared nodes to their partial slot.
-  for (int j = tid; j < th.nnode; j += kTileThreads) {
+  for (int j = tid; j < th.nnode; j += nthr) {
     const int o = A.tile_out[th.node0 + j];
     if (o == INT32_MIN) continue;  // gathered only
     if (o >= 0) {
@@ -235,12 +334,14 @@ __global__ void __launch_bounds__(256) fixup_kernel(int ns, const int32_t *__res
 
 // LDS bytes of one tile for nq right-hand sides (layout of tile_kernel).
 template <typename T, int NPE>
-inline size_t tile_lds_bytes(int max_nodes, int max_grp, int max_he, int max_rd, int tile_leaves, int nq) {
-  const size_t ulen = (size_t)(max_he > max_rd * NPE ? max_he : max_rd * NPE);
+inline size_t tile_lds_bytes(const TileDims &D, int nq) {
+  const size_t ulen = (size_t)(D.max_he > D.max_rd * NPE ? D.max_he : D.max_rd * NPE);
   auto r16 = [](size_t b) { return (b + 15) & ~(size_t)15; };
-  return 2 * r16((size_t)max_nodes * nq * sizeof(T)) + r16((size_t)tile_leaves * NPE * sizeof(T)) +
-         r16(ulen * sizeof(T)) + r16((size_t)max_he * 8) + r16((size_t)tile_leaves * 4) +
-         r16((size_t)max_grp * NPE * 2) + r16((size_t)tile_leaves * 2) + r16((size_t)tile_leaves);
+  const size_t TL = (size_t)D.tile_leaves;
+  return 2 * r16((size_t)D.max_nodes * nq * sizeof(T)) + r16((TL + 1) * NPE * sizeof(T)) + r16(ulen * sizeof(T)) +
+         r16((size_t)NPE * nq * sizeof(T)) + r16((size_t)D.max_cr * sizeof(T)) + r16((size_t)D.max_he * 8) +
+         r16(TL * 8) + r16((size_t)D.max_cr * 4) + r16((size_t)D.max_grp * NPE * 2) + r16(TL * 2) +
+         r16((size_t)D.max_rd * 2);
 }
 
 }  // namespace dkt

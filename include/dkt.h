@@ -137,10 +137,11 @@ int dkt_plan_matvec(dkt_plan *plan, int dtype, const void *x, void *y, int nrhs,
 
 /* info[0..8] = nodes, leaves, hanging leaves, nodes per element, device map
  * bytes, tiles, shared nodes, partial slots, max nodes per tile;
- * info[9..16] = tile statistics of the tiled path: mean and max stale-buffer
- * readers per tile, mean and max carry terms per tile, mean sibling groups,
- * mean nodes per tile, hanging-transpose entries, mean replay length per
- * reader; info[17] = leaves per tile. */
+ * info[9..18] = tile statistics of the tiled path: mean and max stale-buffer
+ * readers per tile, mean and max tile-start buffer terms per tile, mean
+ * sibling groups, mean nodes per tile, hanging-transpose entries, mean cone
+ * length per reader, mean and max longest replay per tile with readers;
+ * info[19] = leaves per tile. */
 int dkt_plan_info(const dkt_plan *plan, double *info, int n);
 
 /* Event timing of subsequent tiled matvecs on their stream (no counterpart in
