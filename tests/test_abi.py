@@ -1,0 +1,94 @@
+"""The C-ABI library (CPU): it loads, exports every function include/dkt.h
+declares, reports errors as status codes with a message (never aborts), and
+builds host-side plans without a device (DKT_PLAN_DRYRUN)."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared():
+    names = []
+    for h in sorted(os.listdir(os.path.join(REPO, "include"))):
+        if h.endswith(".h"):
+            text = open(os.path.join(REPO, "include", h)).read()
+            text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+            names += re.findall(r"^[A-Za-z_][\w \*]*?\b(dkt_\w+)\s*\(", text, flags=re.M)
+    return sorted(set(names))
+
+
+def test_header_declares_the_boundary():
+    names = _declared()
+    for required in ("dkt_da_create", "dkt_plan_create", "dkt_plan_matvec", "dkt_last_error", "dkt_tree_balance"):
+        assert required in names
+
+
+@pytest.mark.parametrize("name", _declared())
+def test_library_exports_declared_symbol(dkt, name):
+    assert hasattr(dkt.load(), name), name
+
+
+def test_errors_are_status_codes(dkt):
+    L = dkt.load()
+    h = ctypes.c_void_p()
+    cells = np.zeros((1, 6), dtype=np.uint32)
+    rc = L.dkt_octs_create(5, 10, 0, cells.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)), 1, ctypes.byref(h))
+    assert rc != 0 and b"dim" in L.dkt_last_error()
+    rc = L.dkt_da_create(None, 1, ctypes.byref(h))
+    assert rc != 0 and L.dkt_last_error()
+    bad = np.array([[1 << 20, 0, 3]], dtype=np.uint32)
+    rc = L.dkt_octs_create(2, 10, 0, bad.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)), 1, ctypes.byref(h))
+    assert rc != 0
+    with pytest.raises(dkt.DktError):
+        dkt.DA(dkt.sort_tree(dkt.Octs.from_array(np.array([[0, 0, 1]], dtype=np.uint32), 2, 4)), 7)
+
+
+def _dry_plan(dkt, da, op=None):
+    L = dkt.load()
+    h = ctypes.c_void_p()
+    rc = L.dkt_plan_create(da.h, dkt.OP_STIFFNESS if op is None else op, None, 2, ctypes.byref(h))
+    assert rc == 0, L.dkt_last_error()
+    keys = ["nodes", "leaves", "hanging", "npe", "map_bytes", "tiles", "shared_nodes", "partials",
+            "max_tile_nodes", "readers_mean", "readers_max", "carry_terms_mean", "carry_terms_max",
+            "groups_mean", "tile_nodes_mean", "hanging_entries", "cone_mean", "span_mean", "span_max",
+            "tile_leaves"]
+    out = (ctypes.c_double * len(keys))()
+    assert L.dkt_plan_info(h, out, len(keys)) == 0
+    return h, dict(zip(keys, list(out)))
+
+
+@pytest.mark.parametrize("dim,p", [(2, 1), (3, 1), (4, 1), (2, 2), (3, 2)])
+def test_dry_run_plan_layout(dkt, dim, p):
+    """Host-side tiling without a device: counts agree with the mesh, tiles cover the
+    leaves, every shared node has at least two partial slots, and a dry plan refuses
+    to run a matvec."""
+    import gen_inputs as gi
+    maxd = 8 if p == 1 else 10
+    if p == 1:
+        pts = gi.gauss_points(dim, 400, 3, 0.06, 9, maxd)
+    else:
+        pts = gi.cells_at_level(gi.gauss_points(dim, 400, 3, 0.06, 9, maxd - 2), maxd - 2, maxd)
+    da = dkt.DA(dkt.balance_tree(dkt.Octs.from_array(pts, dim, maxd, dkt.CURVE_HILBERT), 1), p)
+    h, info = _dry_plan(dkt, da)
+    L = dkt.load()
+    try:
+        assert info["nodes"] == da.n_nodes and info["leaves"] == da.count(3) and info["hanging"] == da.count(4)
+        assert info["npe"] == (p + 1) ** dim
+        assert info["tiles"] * info["tile_leaves"] >= info["leaves"] > (info["tiles"] - 1) * info["tile_leaves"]
+        assert info["partials"] >= 2 * info["shared_nodes"]
+        assert info["readers_max"] <= info["tile_leaves"]
+        x = (ctypes.c_double * 1)()
+        rc = L.dkt_plan_matvec(h, 0, ctypes.cast(x, ctypes.c_void_p), ctypes.cast(x, ctypes.c_void_p), 1, 0, None)
+        assert rc != 0
+    finally:
+        L.dkt_plan_destroy(h)
+
+
+def test_plan_matvec_argument_checks(dkt):
+    L = dkt.load()
+    rc = L.dkt_plan_matvec(None, 0, None, None, 1, 0, None)
+    assert rc != 0 and b"null" in L.dkt_last_error()
