@@ -29,6 +29,8 @@ OP_MASS = 1
 OP_IDENTITY = 2
 OP_CAPTURED = 3
 PLAN_SIMPLE = 1
+CG_F64 = 0
+CG_MIXED = 1
 
 _lib = None
 
@@ -102,6 +104,7 @@ def _bind_device(L):
     L.dkt_plan_info.argtypes = [P, c.POINTER(c.c_double), c.c_int]
     L.dkt_plan_profile.argtypes = [P, c.c_int]
     L.dkt_plan_profile_read.argtypes = [P, c.POINTER(c.c_double), c.POINTER(c.c_double), c.POINTER(c.c_int)]
+    L.dkt_cg_solve.argtypes = [P, c.c_int, P, P, c.c_int, c.c_double, c.c_int, c.POINTER(c.c_double), P]
 
 
 def _chk(rc):
@@ -307,6 +310,23 @@ class Plan:
             raise DktError("x/y shape does not match the plan")
         s = stream if stream is not None else torch.cuda.current_stream().cuda_stream
         self.matvec(x.data_ptr(), y.data_ptr(), nrhs, 0 if x.dtype == torch.float64 else 1, masked, s)
+
+    def cg_solve(self, b_ptr, x_ptr, mode=CG_F64, max_iter=10000, tol=1e-8, masked=True, stream=None):
+        """HeatMat::cgSolve on the device (x: initial guess in, solution out).  Returns
+        dict(iters, resid, refinements, matvecs)."""
+        st = (ctypes.c_double * 4)()
+        _chk(load().dkt_cg_solve(self.h, mode, ctypes.c_void_p(b_ptr), ctypes.c_void_p(x_ptr), max_iter, tol,
+                                 1 if masked else 0, st, ctypes.c_void_p(stream or 0)))
+        return dict(iters=int(st[0]), resid=st[1], refinements=int(st[2]), matvecs=int(st[3]))
+
+    def cg_solve_torch(self, b, x, mode=CG_F64, max_iter=10000, tol=1e-8, masked=True, stream=None):
+        import torch
+        if b.dtype != torch.float64 or x.dtype != torch.float64 or b.shape != (self.n_nodes,) or x.shape != b.shape:
+            raise DktError("b and x must be float64 device vectors of the plan's node count")
+        if not (b.is_cuda and x.is_cuda and b.is_contiguous() and x.is_contiguous()):
+            raise DktError("b and x must be contiguous device tensors")
+        s = stream if stream is not None else torch.cuda.current_stream().cuda_stream
+        return self.cg_solve(b.data_ptr(), x.data_ptr(), mode, max_iter, tol, masked, s)
 
     def __del__(self):
         if getattr(self, "h", None) and _lib is not None:

@@ -1,0 +1,372 @@
+// Conjugate gradients on the device, the caller of the matvec in the
+// reference (HeatMat::cgSolve, FEM/examples/src/heatMat.cpp:165-325):
+// unpreconditioned CG with one matvec per iteration, alpha = (r.r)/(p.Ap),
+// beta = (r1.r1)/(r0.r0), stopping when ||r||_inf / ||b||_inf <= tol
+// (normLInfty, include/mathUtils.tcc:120-127; dot, :204-221).
+//
+// Device design: the scalars never leave the device.  Dot products and max
+// norms are two-stage reductions over a fixed partition (kCgBlocks blocks of
+// kCgThreads work-items, fixed tree order): every block that needs alpha or
+// beta re-reduces the same partials in the same order, so all blocks agree
+// bit for bit and the solve is reproducible run to run.  Convergence raises a
+// device flag; the kernels of iterations queued after it (matvec included)
+// return at once, and the host only checks the flag every kCgChunk
+// iterations.
+//
+// Mixed precision (DKT_CG_MIXED): iterative refinement.  The outer loop keeps
+// x, the residual r = b - A x and its norm in FP64 (FP64 matvec); each inner
+// solve runs the same CG in FP32 (FP32 matvec and accumulation) on
+// A d = r / ||r||_inf to a relative residual of kInnerTol (or looser when
+// tol / ||r|| is larger), then x += ||r|| d.
+#include <algorithm>
+#include <cmath>
+
+#include "../../include/dkt.h"
+#include "capi_common.h"
+#include "plan_common.h"
+#include "plan_impl.h"
+
+namespace dkt {
+namespace {
+
+constexpr int kCgBlocks = 512;
+constexpr int kCgThreads = 256;
+constexpr int kCgChunk = 8;
+constexpr double kInnerTol = 1e-4;
+constexpr int kMaxOuter = 64;
+// Scalar slots.
+enum { S_RR0 = 0, S_RR1 = 1, S_NORMB = 2, S_RESID = 3, S_SCALE = 4, S_COUNT = 8 };
+
+template <typename T>
+__device__ __forceinline__ T block_sum(T v, T *sh) {
+  const int t = threadIdx.x;
+  __syncthreads();
+  sh[t] = v;
+  __syncthreads();
+#pragma unroll
+  for (int s = kCgThreads / 2; s > 0; s >>= 1) {
+    if (t < s) sh[t] += sh[t + s];
+    __syncthreads();
+  }
+  return sh[0];
+}
+
+template <typename T>
+__device__ __forceinline__ T block_max(T v, T *sh) {
+  const int t = threadIdx.x;
+  __syncthreads();
+  sh[t] = v;
+  __syncthreads();
+#pragma unroll
+  for (int s = kCgThreads / 2; s > 0; s >>= 1) {
+    if (t < s) sh[t] = fmax(sh[t], sh[t + s]);
+    __syncthreads();
+  }
+  return sh[0];
+}
+
+// Sum (or max) of the kCgBlocks partials, identical in every block.
+template <typename T>
+__device__ __forceinline__ T parts_sum(const T *part, T *sh) {
+  return block_sum(part[threadIdx.x] + part[threadIdx.x + kCgThreads], sh);
+}
+template <typename T>
+__device__ __forceinline__ T parts_max(const T *part, T *sh) {
+  return block_max(fmax(part[threadIdx.x], part[threadIdx.x + kCgThreads]), sh);
+}
+
+#define CG_LOOP(i, n) for (int i = blockIdx.x * kCgThreads + threadIdx.x; i < (n); i += kCgBlocks * kCgThreads)
+
+// r = b - ax (ax may be null: x = 0), p = r; partials of r.r, max|r|, max|b|.
+template <typename T>
+__global__ void __launch_bounds__(kCgThreads) k_init(int n, const T *__restrict__ b, const T *__restrict__ ax,
+                                                     T *__restrict__ r, T *__restrict__ p, T *__restrict__ part) {
+  __shared__ T sh[kCgThreads];
+  T rr = 0, mr = 0, mb = 0;
+  CG_LOOP(i, n) {
+    const T ri = ax ? b[i] - ax[i] : b[i];
+    r[i] = ri;
+    p[i] = ri;
+    rr += ri * ri;
+    mr = fmax(mr, fabs(ri));
+    mb = fmax(mb, fabs(b[i]));
+  }
+  rr = block_sum(rr, sh);
+  mr = block_max(mr, sh);
+  mb = block_max(mb, sh);
+  if (threadIdx.x == 0) {
+    part[blockIdx.x] = rr;
+    part[kCgBlocks + blockIdx.x] = mr;
+    part[2 * kCgBlocks + blockIdx.x] = mb;
+  }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kCgThreads) k_init_fin(const T *__restrict__ part, T *__restrict__ scal, double tol,
+                                                         int *__restrict__ flag) {
+  __shared__ T sh[kCgThreads];
+  const T rr = parts_sum(part, sh);
+  const T mr = parts_max(part + kCgBlocks, sh);
+  T mb = parts_max(part + 2 * kCgBlocks, sh);
+  if (threadIdx.x == 0) {
+    if (mb == T(0)) mb = T(1);
+    scal[S_RR0] = rr;
+    scal[S_NORMB] = mb;
+    scal[S_RESID] = mr / mb;
+    flag[0] = (double)(mr / mb) <= tol ? 1 : 0;
+    flag[1] = 0;
+  }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kCgThreads) k_dot(int n, const T *__restrict__ a, const T *__restrict__ b,
+                                                    T *__restrict__ part, const int *__restrict__ flag) {
+  if (flag[0]) return;
+  __shared__ T sh[kCgThreads];
+  T acc = 0;
+  CG_LOOP(i, n) acc += a[i] * b[i];
+  acc = block_sum(acc, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = acc;
+}
+
+// alpha = rr / (p.Ap); x += alpha p; r -= alpha Ap; partials of r.r and max|r|.
+template <typename T>
+__global__ void __launch_bounds__(kCgThreads) k_update_xr(int n, T *__restrict__ x, T *__restrict__ r,
+                                                          const T *__restrict__ p, const T *__restrict__ ap,
+                                                          const T *__restrict__ part_pap, const T *__restrict__ scal,
+                                                          int par, T *__restrict__ part, const int *__restrict__ flag) {
+  if (flag[0]) return;
+  __shared__ T sh[kCgThreads];
+  const T alpha = scal[S_RR0 + par] / parts_sum(part_pap, sh);
+  T rr = 0, mr = 0;
+  CG_LOOP(i, n) {
+    x[i] += alpha * p[i];
+    const T ri = r[i] - alpha * ap[i];
+    r[i] = ri;
+    rr += ri * ri;
+    mr = fmax(mr, fabs(ri));
+  }
+  rr = block_sum(rr, sh);
+  mr = block_max(mr, sh);
+  if (threadIdx.x == 0) {
+    part[blockIdx.x] = rr;
+    part[kCgBlocks + blockIdx.x] = mr;
+  }
+}
+
+// Convergence test, then beta = r1.r1 / r0.r0 and p = r + beta p.  Block 0
+// publishes the new r.r (into the other parity slot), the residual, the
+// iteration count and the converged flag.
+template <typename T>
+__global__ void __launch_bounds__(kCgThreads) k_update_p(int n, T *__restrict__ p, const T *__restrict__ r,
+                                                         const T *__restrict__ part, T *__restrict__ scal, int par,
+                                                         double tol, int *__restrict__ flag) {
+  if (flag[0]) return;
+  __shared__ T sh[kCgThreads];
+  const T rr1 = parts_sum(part, sh);
+  const T resid = parts_max(part + kCgBlocks, sh) / scal[S_NORMB];
+  const bool done = (double)resid <= tol;
+  if (!done) {
+    const T beta = rr1 / scal[S_RR0 + par];
+    CG_LOOP(i, n) p[i] = r[i] + beta * p[i];
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    scal[S_RR0 + (par ^ 1)] = rr1;
+    scal[S_RESID] = resid;
+    flag[1] += 1;
+    if (done) flag[0] = 1;
+  }
+}
+
+// FP64 residual r = b - ax and its partial max norms (outer refinement step).
+__global__ void __launch_bounds__(kCgThreads) k_resid64(int n, const double *__restrict__ b,
+                                                        const double *__restrict__ ax, double *__restrict__ r,
+                                                        double *__restrict__ part) {
+  __shared__ double sh[kCgThreads];
+  double mr = 0, mb = 0;
+  CG_LOOP(i, n) {
+    const double ri = b[i] - ax[i];
+    r[i] = ri;
+    mr = fmax(mr, fabs(ri));
+    mb = fmax(mb, fabs(b[i]));
+  }
+  mr = block_max(mr, sh);
+  mb = block_max(mb, sh);
+  if (threadIdx.x == 0) {
+    part[blockIdx.x] = mr;
+    part[kCgBlocks + blockIdx.x] = mb;
+  }
+}
+
+// scal[S_RESID] = max|r| / max|b| (max|b| = 1 if b = 0); scal[S_SCALE] = max|r|.
+__global__ void __launch_bounds__(kCgThreads) k_resid64_fin(const double *__restrict__ part, double *__restrict__ scal) {
+  __shared__ double sh[kCgThreads];
+  const double mr = parts_max(part, sh);
+  double mb = parts_max(part + kCgBlocks, sh);
+  if (threadIdx.x == 0) {
+    if (mb == 0.0) mb = 1.0;
+    scal[S_NORMB] = mb;
+    scal[S_RESID] = mr / mb;
+    scal[S_SCALE] = mr;
+  }
+}
+
+// b32 = r / s (s = max|r|, read on the device).
+__global__ void __launch_bounds__(kCgThreads) k_to32(int n, const double *__restrict__ r, const double *__restrict__ scal,
+                                                     float *__restrict__ b32, float *__restrict__ x32) {
+  const double s = scal[S_SCALE];
+  const double inv = s > 0.0 ? 1.0 / s : 0.0;
+  CG_LOOP(i, n) {
+    b32[i] = (float)(r[i] * inv);
+    x32[i] = 0.0f;
+  }
+}
+
+// x += s d32.
+__global__ void __launch_bounds__(kCgThreads) k_add32(int n, double *__restrict__ x, const float *__restrict__ d32,
+                                                      const double *__restrict__ scal) {
+  const double s = scal[S_SCALE];
+  CG_LOOP(i, n) x[i] += s * (double)d32[i];
+}
+
+template <typename T>
+struct CgVectors {
+  T *r, *p, *ap, *part, *scal;
+};
+
+// Runs up to max_iter CG iterations on A x = b in precision T from the current x.
+// Returns the iterations performed; the final relative residual is scal[S_RESID].
+template <typename T>
+int cg_run(dkt_plan *pl, int dtype, const T *b, T *x, bool x_is_zero, int max_iter, double tol, bool masked,
+           const CgVectors<T> &v, int *flag, int *host, hipStream_t st, int *matvecs) {
+  const int n = (int)pl->N;
+  const dim3 grid(kCgBlocks), blk(kCgThreads);
+  if (!x_is_zero) {
+    plan_apply(pl, dtype, x, v.ap, 1, masked, st, nullptr);
+    (*matvecs)++;
+  }
+  hipLaunchKernelGGL((k_init<T>), grid, blk, 0, st, n, b, x_is_zero ? nullptr : v.ap, v.r, v.p, v.part);
+  hipLaunchKernelGGL((k_init_fin<T>), dim3(1), blk, 0, st, v.part, v.scal, tol, flag);
+  HIP_OK(hipGetLastError());
+  int it = 0;
+  HIP_OK(hipMemcpyAsync(host, flag, 2 * sizeof(int), hipMemcpyDeviceToHost, st));
+  HIP_OK(hipStreamSynchronize(st));
+  if (host[0]) return 0;
+  while (it < max_iter) {
+    const int chunk = std::min(kCgChunk, max_iter - it);
+    for (int c = 0; c < chunk; c++) {
+      const int par = (it + c) & 1;
+      plan_apply(pl, dtype, v.p, v.ap, 1, masked, st, flag);
+      hipLaunchKernelGGL((k_dot<T>), grid, blk, 0, st, n, v.p, v.ap, v.part + 2 * kCgBlocks, flag);
+      hipLaunchKernelGGL((k_update_xr<T>), grid, blk, 0, st, n, x, v.r, v.p, v.ap, v.part + 2 * kCgBlocks, v.scal,
+                         par, v.part, flag);
+      hipLaunchKernelGGL((k_update_p<T>), grid, blk, 0, st, n, v.p, v.r, v.part, v.scal, par, tol, flag);
+      HIP_OK(hipGetLastError());
+    }
+    it += chunk;
+    HIP_OK(hipMemcpyAsync(host, flag, 2 * sizeof(int), hipMemcpyDeviceToHost, st));
+    HIP_OK(hipStreamSynchronize(st));
+    if (host[0]) break;
+  }
+  *matvecs += host[1];
+  return host[1];
+}
+
+struct PinnedInts {
+  int *p = nullptr;
+  PinnedInts() { HIP_OK(hipHostMalloc((void **)&p, 4 * sizeof(int), hipHostMallocDefault)); }
+  ~PinnedInts() {
+    if (p) (void)hipHostFree(p);
+  }
+};
+
+template <typename T>
+T read_scalar(const T *dev, int slot, hipStream_t st) {
+  T v;
+  HIP_OK(hipMemcpyAsync(&v, dev + slot, sizeof(T), hipMemcpyDeviceToHost, st));
+  HIP_OK(hipStreamSynchronize(st));
+  return v;
+}
+
+void ensure_workspace(dkt_plan *pl, bool mixed) {
+  CgWorkspace &w = pl->cg;
+  const size_t n = pl->N;
+  if (w.n != n) {
+    w.release();
+    w.n = n;
+  }
+  if (!w.r.p) {
+    w.r.alloc(n);
+    w.p.alloc(n);
+    w.ap.alloc(n);
+    w.part64.alloc(4 * kCgBlocks);
+    w.scal.alloc(S_COUNT);
+    w.flag.alloc(4);
+  }
+  if (mixed && !w.r32.p) {
+    w.d64.alloc(n);
+    w.r32.alloc(n);
+    w.p32.alloc(n);
+    w.ap32.alloc(n);
+    w.x32.alloc(n);
+    w.b32.alloc(n);
+    w.part32.alloc(4 * kCgBlocks);
+    w.scal32.alloc(S_COUNT);
+  }
+}
+
+}  // namespace
+}  // namespace dkt
+
+using namespace dkt;
+
+extern "C" int dkt_cg_solve(dkt_plan *pl, int mode, const double *b, double *x, int max_iter, double tol, int masked,
+                            double *stats, void *stream) {
+  return guarded([&] {
+    if (!pl || !b || !x) throw std::invalid_argument("null pointer");
+    if (pl->dry) throw std::invalid_argument("a dry-run plan cannot solve");
+    if (max_iter < 0 || !(tol >= 0.0)) throw std::invalid_argument("max_iter >= 0 and tol >= 0 required");
+    if (mode != DKT_CG_F64 && mode != DKT_CG_MIXED) throw std::invalid_argument("unknown CG mode");
+    if ((const void *)b == (const void *)x) throw std::invalid_argument("b and x must not alias");
+    hipStream_t st = (hipStream_t)stream;
+    const bool mixed = mode == DKT_CG_MIXED;
+    ensure_workspace(pl, mixed);
+    CgWorkspace &w = pl->cg;
+    PinnedInts host;
+    int matvecs = 0, iters = 0, outer = 0;
+    double resid = 0.0;
+    if (!mixed) {
+      CgVectors<double> v{w.r.p, w.p.p, w.ap.p, w.part64.p, w.scal.p};
+      iters = cg_run<double>(pl, 0, b, x, false, max_iter, tol, masked != 0, v, w.flag.p, host.p, st, &matvecs);
+      resid = read_scalar(w.scal.p, S_RESID, st);
+    } else {
+      const int n = (int)pl->N;
+      CgVectors<float> v{w.r32.p, w.p32.p, w.ap32.p, w.part32.p, w.scal32.p};
+      while (true) {
+        plan_apply(pl, 0, x, w.ap.p, 1, masked != 0, st, nullptr);
+        matvecs++;
+        hipLaunchKernelGGL(k_resid64, dim3(kCgBlocks), dim3(kCgThreads), 0, st, n, b, w.ap.p, w.d64.p, w.part64.p);
+        hipLaunchKernelGGL(k_resid64_fin, dim3(1), dim3(kCgThreads), 0, st, w.part64.p, w.scal.p);
+        HIP_OK(hipGetLastError());
+        resid = read_scalar(w.scal.p, S_RESID, st);
+        if (resid <= tol || iters >= max_iter || !std::isfinite(resid)) break;
+        hipLaunchKernelGGL(k_to32, dim3(kCgBlocks), dim3(kCgThreads), 0, st, n, w.d64.p, w.scal.p, w.b32.p, w.x32.p);
+        // Inner tolerance: kInnerTol, loosened when the outer target is already close.
+        const double inner_tol = std::min(0.5, std::max(kInnerTol, 0.5 * tol / resid));
+        const int inner = cg_run<float>(pl, 1, w.b32.p, w.x32.p, true, max_iter - iters, inner_tol, masked != 0, v,
+                                        w.flag.p, host.p, st, &matvecs);
+        hipLaunchKernelGGL(k_add32, dim3(kCgBlocks), dim3(kCgThreads), 0, st, n, x, w.x32.p, w.scal.p);
+        HIP_OK(hipGetLastError());
+        iters += inner;
+        outer++;
+        if (inner == 0 || outer >= kMaxOuter) break;  // no progress possible: report the last residual
+      }
+    }
+    if (stats) {
+      stats[0] = iters;
+      stats[1] = resid;
+      stats[2] = outer;
+      stats[3] = matvecs;
+    }
+  });
+}

@@ -25,6 +25,7 @@
 #include "capi_common.h"
 #include "mesh.h"
 #include "plan_common.h"
+#include "plan_impl.h"
 #include "tile_kernel.h"
 
 namespace dkt {
@@ -58,10 +59,10 @@ __global__ void __launch_bounds__(256) leaf_kernel(
     const int32_t *__restrict__ hang_of, const uint8_t *__restrict__ child, const int32_t *__restrict__ pslot,
     const int32_t *__restrict__ hc_off, const uint8_t *__restrict__ hc_rank, const int32_t *__restrict__ hc_node,
     const T *__restrict__ hc_w, const T *__restrict__ kmat, int kmat_per_level, const uint8_t *__restrict__ dirichlet,
-    int hp, T *__restrict__ vout, T *__restrict__ wout) {
+    int hp, T *__restrict__ vout, T *__restrict__ wout, const int *__restrict__ cancel) {
   constexpr int NPE = Pow<P + 1, DIM>::v;
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= L) return;
+  if (e >= L || (cancel && *cancel)) return;
   int32_t s[NPE];
 #pragma unroll
   for (int r = 0; r < NPE; r++) s[r] = slot[(size_t)e * NPE + r];
@@ -118,9 +119,10 @@ __global__ void __launch_bounds__(256) leaf_kernel(
 template <typename T>
 __global__ void __launch_bounds__(256) node_kernel(int N, int nrhs, const int64_t *__restrict__ off,
                                                    const int32_t *__restrict__ src, const T *__restrict__ buf,
-                                                   const uint8_t *__restrict__ dirichlet, T *__restrict__ y) {
+                                                   const uint8_t *__restrict__ dirichlet, T *__restrict__ y,
+                                                   const int *__restrict__ cancel) {
   const int n = blockIdx.x * blockDim.x + threadIdx.x;
-  if (n >= N) return;
+  if (n >= N || (cancel && *cancel)) return;
   const int64_t a = off[n], b = off[n + 1];
   const bool zero = dirichlet && dirichlet[n];
   for (int q = 0; q < nrhs; q++) {
@@ -177,56 +179,6 @@ std::vector<double> ref_matrix(int dim, int p, int op) {
 }
 
 }  // namespace
-
-struct dkt_plan {
-  int dim = 0, order = 0, npe = 0, op = 0, hp = 0, maxd = 0;
-  size_t N = 0, L = 0, H = 0;
-  bool per_level = false, simple = false, dry = false;
-  DevBuf<uint8_t> dirichlet;
-  DevBuf<double> kmat64;
-  DevBuf<float> kmat32;
-  // simple path
-  DevBuf<int32_t> slot, hang_of, pslot, hc_off, hc_node, csr_src;
-  DevBuf<uint8_t> level, child, hc_rank;
-  DevBuf<double> hc_w64;
-  DevBuf<float> hc_w32;
-  DevBuf<int64_t> csr_off;
-  size_t nbuf = 0;
-  // tiled path
-  int tile_leaves = kTileThreads;
-  size_t ntiles = 0, nshared = 0, npartial = 0;
-  int max_nodes = 0, max_grp = 0, max_he = 0, max_rd = 0, max_cr = 0;
-  DevBuf<TileHdr> tiles;
-  DevBuf<int32_t> tile_nodes, tile_out, snode, soff;
-  DevBuf<uint16_t> leaf_loc, group_ploc, readers;
-  DevBuf<uint32_t> leaf_aux, carry_key;
-  DevBuf<uint64_t> hentries, blocks;
-  DevBuf<double> carry_w64;
-  DevBuf<float> carry_w32;
-  // scratch
-  DevBuf<double> scratch64;
-  DevBuf<float> scratch32;
-  double map_bytes = 0;
-  // Tile statistics: mean readers, max readers, mean carry terms, max carry terms,
-  // mean groups, mean nodes, hanging-transpose entries, mean cone length,
-  // mean and max longest replay of tiles with readers.
-  std::vector<double> stats = std::vector<double>(10, 0.0);
-  // Optional event timing (dkt_plan_profile): (start, after tile kernel, end) per launch.
-  bool profiling = false;
-  std::vector<hipEvent_t> ev_pool;
-  size_t ev_used = 0;
-  hipEvent_t next_event() {
-    if (ev_used == ev_pool.size()) {
-      hipEvent_t e;
-      HIP_OK(hipEventCreate(&e));
-      ev_pool.push_back(e);
-    }
-    return ev_pool[ev_used++];
-  }
-  ~dkt_plan() {
-    for (hipEvent_t e : ev_pool) (void)hipEventDestroy(e);
-  }
-};
 
 namespace {
 
@@ -620,7 +572,7 @@ void build_tiled(dkt_plan *pl, const MeshParams &mp, const Traversal &tr) {
 // Launch.
 // ---------------------------------------------------------------------------
 template <typename T, int DIM, int P>
-void run_simple(dkt_plan *pl, int nrhs, const T *x, T *y, bool masked, hipStream_t st) {
+void run_simple(dkt_plan *pl, int nrhs, const T *x, T *y, bool masked, hipStream_t st, const int *cancel) {
   constexpr int NPE = Pow<P + 1, DIM>::v;
   const bool f64 = sizeof(T) == 8;
   const T *kmat = f64 ? (const T *)pl->kmat64.p : (const T *)pl->kmat32.p;
@@ -640,16 +592,16 @@ void run_simple(dkt_plan *pl, int nrhs, const T *x, T *y, bool masked, hipStream
     hipLaunchKernelGGL((leaf_kernel<T, DIM, P>), dim3((pl->L + blk - 1) / blk), dim3(blk), 0, st, (int)pl->L, nrhs, x,
                        pl->slot.p, pl->level.p, pl->hang_of.p, pl->child.p, pl->pslot.p, pl->hc_off.p,
                        pl->hc_rank.p, pl->hc_node.p, hcw, kmat, pl->per_level ? 1 : 0, dir, pl->hp, buf,
-                       buf + pl->L * (size_t)NPE * nrhs);
+                       buf + pl->L * (size_t)NPE * nrhs, cancel);
     HIP_OK(hipGetLastError());
   }
   hipLaunchKernelGGL((node_kernel<T>), dim3((pl->N + blk - 1) / blk), dim3(blk), 0, st, (int)pl->N, nrhs,
-                     pl->csr_off.p, pl->csr_src.p, buf, dir, y);
+                     pl->csr_off.p, pl->csr_src.p, buf, dir, y, cancel);
   HIP_OK(hipGetLastError());
 }
 
 template <typename T, int DIM, int P>
-void run_tiled(dkt_plan *pl, int nrhs, const T *x, T *y, bool masked, hipStream_t st) {
+void run_tiled(dkt_plan *pl, int nrhs, const T *x, T *y, bool masked, hipStream_t st, const int *cancel) {
   constexpr int NPE = Pow<P + 1, DIM>::v;
   const bool f64 = sizeof(T) == 8;
   const T *kmat = f64 ? (const T *)pl->kmat64.p : (const T *)pl->kmat32.p;
@@ -682,6 +634,7 @@ void run_tiled(dkt_plan *pl, int nrhs, const T *x, T *y, bool masked, hipStream_
   A.readers = pl->readers.p;
   A.carry_key = pl->carry_key.p;
   A.dirichlet = masked ? pl->dirichlet.p : nullptr;
+  A.cancel = cancel;
   A.hp = pl->hp;
   A.ld = nrhs;
   A.blocks = pl->blocks.p;
@@ -699,7 +652,7 @@ void run_tiled(dkt_plan *pl, int nrhs, const T *x, T *y, bool masked, hipStream_
     if (pl->nshared) {
       const int blk = 256;
       hipLaunchKernelGGL((fixup_kernel<T>), dim3((pl->nshared + blk - 1) / blk), dim3(blk), 0, st,
-                         (int)pl->nshared, pl->snode.p, pl->soff.p, part, A.dirichlet, nq, nrhs, y + q0);
+                         (int)pl->nshared, pl->snode.p, pl->soff.p, part, A.dirichlet, nq, nrhs, y + q0, cancel);
       HIP_OK(hipGetLastError());
     }
     if (pl->profiling) HIP_OK(hipEventRecord(pl->next_event(), st));
@@ -707,14 +660,14 @@ void run_tiled(dkt_plan *pl, int nrhs, const T *x, T *y, bool masked, hipStream_
 }
 
 template <typename T>
-void run_matvec(dkt_plan *pl, int nrhs, const T *x, T *y, bool masked, hipStream_t st) {
+void run_matvec(dkt_plan *pl, int nrhs, const T *x, T *y, bool masked, hipStream_t st, const int *cancel) {
   const bool simple = pl->simple || pl->per_level;
 #define DKT_DISPATCH(D, P)                             \
   if (pl->dim == D && pl->order == P) {                \
     if (simple)                                        \
-      run_simple<T, D, P>(pl, nrhs, x, y, masked, st); \
+      run_simple<T, D, P>(pl, nrhs, x, y, masked, st, cancel); \
     else                                               \
-      run_tiled<T, D, P>(pl, nrhs, x, y, masked, st);  \
+      run_tiled<T, D, P>(pl, nrhs, x, y, masked, st, cancel);  \
     return;                                            \
   }
   DKT_DISPATCH(2, 1)
@@ -727,6 +680,18 @@ void run_matvec(dkt_plan *pl, int nrhs, const T *x, T *y, bool masked, hipStream
 }
 
 }  // namespace
+
+namespace dkt {
+void plan_apply(dkt_plan *pl, int dtype, const void *x, void *y, int nrhs, bool masked, hipStream_t st,
+                const int *cancel) {
+  if (dtype == 0)
+    run_matvec<double>(pl, nrhs, (const double *)x, (double *)y, masked, st, cancel);
+  else if (dtype == 1)
+    run_matvec<float>(pl, nrhs, (const float *)x, (float *)y, masked, st, cancel);
+  else
+    throw std::invalid_argument("dtype must be 0 (f64) or 1 (f32)");
+}
+}  // namespace dkt
 
 extern "C" int dkt_plan_create(const dkt_da *da, int op, const double *op_data, int flags, dkt_plan **out) {
   return guarded([&] {
@@ -817,13 +782,7 @@ extern "C" int dkt_plan_matvec(dkt_plan *pl, int dtype, const void *x, void *y, 
     if (pl->dry) throw std::invalid_argument("a dry-run plan cannot run a matvec");
     if (nrhs < 1 || nrhs > 64) throw std::invalid_argument("nrhs must be 1..64");
     if (x == y) throw std::invalid_argument("x and y must not alias");
-    hipStream_t st = (hipStream_t)stream;
-    if (dtype == 0)
-      run_matvec<double>(pl, nrhs, (const double *)x, (double *)y, masked != 0, st);
-    else if (dtype == 1)
-      run_matvec<float>(pl, nrhs, (const float *)x, (float *)y, masked != 0, st);
-    else
-      throw std::invalid_argument("dtype must be 0 (f64) or 1 (f32)");
+    plan_apply(pl, dtype, x, y, nrhs, masked != 0, (hipStream_t)stream, nullptr);
   });
 }
 

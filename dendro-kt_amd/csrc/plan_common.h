@@ -146,6 +146,7 @@ struct TileArgs {
   const uint64_t *blocks;
   const uint32_t *carry_key;  // buffer rank (8) << 16 | local node (16)
   const uint8_t *dirichlet;
+  const int *cancel;  // non-null: the tile returns at once when *cancel != 0
   int hp, nq, ld;
   TileDims dims;
 };

@@ -1,0 +1,79 @@
+// The device plan object shared by the matvec (plan.hip) and the solvers (cg.hip).
+#pragma once
+#include <memory>
+#include <vector>
+
+#include "../../include/dkt.h"
+#include "plan_common.h"
+
+// Solver workspace: CG vectors and reduction partials, allocated on first use.
+struct CgWorkspace {
+  size_t n = 0;
+  dkt::DevBuf<double> r, p, ap, d64, part64, scal;
+  dkt::DevBuf<float> r32, p32, ap32, x32, b32, part32, scal32;
+  dkt::DevBuf<int> flag;  // [0] cancel (converged), [1] iterations done
+  void release() {
+    for (auto *b : {&r, &p, &ap, &d64, &part64, &scal}) b->release();
+    for (auto *b : {&r32, &p32, &ap32, &x32, &b32, &part32, &scal32}) b->release();
+    flag.release();
+    n = 0;
+  }
+};
+
+struct dkt_plan {
+  int dim = 0, order = 0, npe = 0, op = 0, hp = 0, maxd = 0;
+  size_t N = 0, L = 0, H = 0;
+  bool per_level = false, simple = false, dry = false;
+  dkt::DevBuf<uint8_t> dirichlet;
+  dkt::DevBuf<double> kmat64;
+  dkt::DevBuf<float> kmat32;
+  // simple path
+  dkt::DevBuf<int32_t> slot, hang_of, pslot, hc_off, hc_node, csr_src;
+  dkt::DevBuf<uint8_t> level, child, hc_rank;
+  dkt::DevBuf<double> hc_w64;
+  dkt::DevBuf<float> hc_w32;
+  dkt::DevBuf<int64_t> csr_off;
+  size_t nbuf = 0;
+  // tiled path
+  int tile_leaves = dkt::kTileThreads;
+  size_t ntiles = 0, nshared = 0, npartial = 0;
+  int max_nodes = 0, max_grp = 0, max_he = 0, max_rd = 0, max_cr = 0;
+  dkt::DevBuf<dkt::TileHdr> tiles;
+  dkt::DevBuf<int32_t> tile_nodes, tile_out, snode, soff;
+  dkt::DevBuf<uint16_t> leaf_loc, group_ploc, readers;
+  dkt::DevBuf<uint32_t> leaf_aux, carry_key;
+  dkt::DevBuf<uint64_t> hentries, blocks;
+  dkt::DevBuf<double> carry_w64;
+  dkt::DevBuf<float> carry_w32;
+  // scratch
+  dkt::DevBuf<double> scratch64;
+  dkt::DevBuf<float> scratch32;
+  double map_bytes = 0;
+  // Tile statistics: mean readers, max readers, mean carry terms, max carry terms,
+  // mean groups, mean nodes, hanging-transpose entries, mean cone length,
+  // mean and max longest replay of tiles with readers.
+  std::vector<double> stats = std::vector<double>(10, 0.0);
+  // Optional event timing (dkt_plan_profile): (start, after tile kernel, end) per launch.
+  bool profiling = false;
+  std::vector<hipEvent_t> ev_pool;
+  size_t ev_used = 0;
+  hipEvent_t next_event() {
+    if (ev_used == ev_pool.size()) {
+      hipEvent_t e;
+      HIP_OK(hipEventCreate(&e));
+      ev_pool.push_back(e);
+    }
+    return ev_pool[ev_used++];
+  }
+  CgWorkspace cg;
+  ~dkt_plan() {
+    for (hipEvent_t e : ev_pool) (void)hipEventDestroy(e);
+  }
+};
+
+namespace dkt {
+// y = A x on `st` (dtype 0 f64, 1 f32).  When `cancel` is non-null and *cancel
+// is nonzero at launch time, the kernels return without touching y.
+void plan_apply(dkt_plan *pl, int dtype, const void *x, void *y, int nrhs, bool masked, hipStream_t st,
+                const int *cancel);
+}  // namespace dkt

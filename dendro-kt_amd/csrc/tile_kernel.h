@@ -111,6 +111,7 @@ __global__ void __launch_bounds__(kTileThreads) tile_kernel(TileArgs A, const T 
   constexpr int NPE = Pow<P + 1, DIM>::v;
   constexpr int G = Group<NPE>::v;
   extern __shared__ __align__(16) unsigned char smem_raw[];
+  if (A.cancel && *A.cancel) return;
   const TileHdr th = A.tiles[blockIdx.x];
   const TileDims D = A.dims;
   const int nq = A.nq;
@@ -319,9 +320,9 @@ template <typename T>
 __global__ void __launch_bounds__(256) fixup_kernel(int ns, const int32_t *__restrict__ snode,
                                                     const int32_t *__restrict__ soff, const T *__restrict__ partial,
                                                     const uint8_t *__restrict__ dirichlet, int nq, int ld,
-                                                    T *__restrict__ y) {
+                                                    T *__restrict__ y, const int *__restrict__ cancel) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= ns) return;
+  if (i >= ns || (cancel && *cancel)) return;
   const int n = snode[i];
   const int a = soff[i], b = soff[i + 1];
   const bool z = dirichlet && dirichlet[n];

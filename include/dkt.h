@@ -135,6 +135,25 @@ int dkt_plan_destroy(dkt_plan *plan);
  * boundary entries of x are read as zero and boundary entries of y are zero. */
 int dkt_plan_matvec(dkt_plan *plan, int dtype, const void *x, void *y, int nrhs, int masked, void *stream);
 
+/* ---- Solver ------------------------------------------------------------ */
+/* Conjugate gradients on A x = b with the plan's operator, as
+ * HeatMat::cgSolve (FEM/examples/src/heatMat.cpp:165-325): unpreconditioned,
+ * one matvec per iteration, stop when ||b - A x||_inf / ||b||_inf <= tol
+ * (||b||_inf = 1 if b = 0) or after max_iter iterations.  masked = 1 uses the
+ * Dirichlet-masked operator of feMatrix::matVec (as cgSolve does through
+ * HeatMat's pre/post hooks); then b should vanish on boundary nodes.
+ * mode DKT_CG_F64: all FP64.  DKT_CG_MIXED: iterative refinement with FP64
+ * residuals and updates around FP32 inner CG solves (FP32 matvec and dots).
+ * b, x: device double [N]; x holds the initial guess on entry.  Reductions
+ * run in a fixed order (bitwise reproducible).  stats (may be NULL): [0]
+ * CG iterations (inner iterations summed for DKT_CG_MIXED), [1] final relative
+ * residual, [2] refinement steps, [3] matvecs.  Returns 0 also when max_iter
+ * is reached; compare stats[1] with tol. */
+#define DKT_CG_F64 0
+#define DKT_CG_MIXED 1
+int dkt_cg_solve(dkt_plan *plan, int mode, const double *b, double *x, int max_iter, double tol, int masked,
+                 double *stats, void *stream);
+
 /* info[0..8] = nodes, leaves, hanging leaves, nodes per element, device map
  * bytes, tiles, shared nodes, partial slots, max nodes per tile;
  * info[9..18] = tile statistics of the tiled path: mean and max stale-buffer
