@@ -124,6 +124,46 @@ def cpu_baseline_port(dkt, npoints, maxd, seed, reps):
                       f"{reps} FP64 matvecs of the oracle's C restatement, 1 thread", "ms_per_matvec": ms}
 
 
+def serve_leg(dkt, nreq):
+    """Served CG: nreq requests over nreq/2 distinct 4D meshes (1e3-3e5 elements),
+    4 HIP streams, tol 1e-8 (FP64 CG).  e2e latency includes the host mesh build and
+    plan upload of a request whose mesh is not cached; solve latency runs from the
+    plan being available (rhs upload, CG, solution download)."""
+    import importlib.util as ilu
+    spec = ilu.spec_from_file_location("dkt_serve", os.path.join(HERE, "dendro-kt_amd", "serve.py"))
+    sv = ilu.module_from_spec(spec)
+    spec.loader.exec_module(sv)
+    rng = np.random.default_rng(0)
+    sizes = np.geomspace(150, 30000, max(1, nreq // 2)).astype(int)
+    meshes = [sv.MeshSpec(4, 9, int(n), 100 + i) for i, n in enumerate(sizes)]
+    reqs = [sv.Request(m, rhs_seed=k) for m in meshes for k in (1, 2)]
+    order = rng.permutation(len(reqs))
+    srv = sv.Server(dkt, n_streams=4)
+    log(f"serving {len(reqs)} CG requests")
+    t0 = time.perf_counter()
+    try:
+        for i in order:
+            srv.submit(reqs[i])
+        done = srv.drain(len(reqs), timeout=600)
+    finally:
+        srv.close()
+    wall = time.perf_counter() - t0
+    errors = [r.error for r in done if r.error]
+    if errors:
+        return {"errors": errors[:3]}
+    e2e = [(r.finished - r.submitted) * 1e3 for r in done]
+    solve = [(r.finished - r.solve_started) * 1e3 for r in done]
+    elems = []
+    for m in meshes:
+        pts = dkt.gauss_points(m.dim, m.maxdepth, m.npoints, m.ncomp, m.sigma, m.seed)
+        elems.append(len(dkt.balance_tree(pts, 1)))
+    return {"requests": len(done), "streams": 4, "tol": 1e-8, "mode": "f64",
+            "elements_min": int(min(elems)), "elements_max": int(max(elems)),
+            "e2e_ms": {k: round(v, 1) for k, v in sv.percentiles(e2e).items()},
+            "solve_ms": {k: round(v, 1) for k, v in sv.percentiles(solve).items()},
+            "max_resid": max(r.stats["resid"] for r in done), "wall_s": round(wall, 2)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -135,6 +175,8 @@ def main():
     ap.add_argument("--cpu-points", type=int, default=5000)
     ap.add_argument("--cpu-reps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-serve", action="store_true", help="skip the served-CG leg")
+    ap.add_argument("--serve-requests", type=int, default=16)
     ap.add_argument("--calib", action="store_true",
                     help="run one 256 MiB elementwise copy (torch.neg) first: the reference dispatch that scales "
                          "FETCH_SIZE / WRITE_SIZE counters to bytes")
@@ -203,6 +245,22 @@ def main():
     gdof64 = N / (ms64 * 1e-3) / 1e9
     gdof32 = N / (ms32 * 1e-3) / 1e9
 
+    # CG on the workload mesh (Dirichlet-masked operator, b keyed by node, zero on the boundary).
+    b = dkt.keyed_uniform(1, da.nodes())
+    b[da.boundary()] = 0.0
+    bd = torch.tensor(b, device="cuda")
+    cg = {}
+    for mode, name in ((dkt.CG_F64, "f64"), (dkt.CG_MIXED, "mixed_f32_f64")):
+        xc = torch.zeros_like(bd)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        st = plan.cg_solve_torch(bd, xc, mode, 20000, 1e-8)
+        torch.cuda.synchronize()
+        cg[name] = dict(st, ms=round((time.perf_counter() - t) * 1e3, 1))
+        log(f"CG {name}: {cg[name]}")
+    del bd, xc
+    serve = None if args.no_serve else serve_leg(dkt, args.serve_requests)
+
     cpu = None
     if not args.no_cpu:
         log("CPU baseline (reference traversal on a sample)")
@@ -235,6 +293,8 @@ def main():
         "hbm_gbs": round(gbs64, 1),
         "fp32": {"gdof_s": round(gdof32, 4), "ms_per_step": round(ms32, 4), "hbm_gbs": round(gbs32, 1),
                  "rel_l2_vs_fp64": rel32},
+        "cg_1e-8": cg,
+        "serve": serve,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

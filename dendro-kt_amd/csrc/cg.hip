@@ -272,14 +272,6 @@ int cg_run(dkt_plan *pl, int dtype, const T *b, T *x, bool x_is_zero, int max_it
   return host[1];
 }
 
-struct PinnedInts {
-  int *p = nullptr;
-  PinnedInts() { HIP_OK(hipHostMalloc((void **)&p, 4 * sizeof(int), hipHostMallocDefault)); }
-  ~PinnedInts() {
-    if (p) (void)hipHostFree(p);
-  }
-};
-
 template <typename T>
 T read_scalar(const T *dev, int slot, hipStream_t st) {
   T v;
@@ -302,6 +294,7 @@ void ensure_workspace(dkt_plan *pl, bool mixed) {
     w.part64.alloc(4 * kCgBlocks);
     w.scal.alloc(S_COUNT);
     w.flag.alloc(4);
+    HIP_OK(hipHostMalloc((void **)&w.host, 4 * sizeof(int), hipHostMallocDefault));
   }
   if (mixed && !w.r32.p) {
     w.d64.alloc(n);
@@ -332,12 +325,11 @@ extern "C" int dkt_cg_solve(dkt_plan *pl, int mode, const double *b, double *x, 
     const bool mixed = mode == DKT_CG_MIXED;
     ensure_workspace(pl, mixed);
     CgWorkspace &w = pl->cg;
-    PinnedInts host;
     int matvecs = 0, iters = 0, outer = 0;
     double resid = 0.0;
     if (!mixed) {
       CgVectors<double> v{w.r.p, w.p.p, w.ap.p, w.part64.p, w.scal.p};
-      iters = cg_run<double>(pl, 0, b, x, false, max_iter, tol, masked != 0, v, w.flag.p, host.p, st, &matvecs);
+      iters = cg_run<double>(pl, 0, b, x, false, max_iter, tol, masked != 0, v, w.flag.p, w.host, st, &matvecs);
       resid = read_scalar(w.scal.p, S_RESID, st);
     } else {
       const int n = (int)pl->N;
@@ -354,7 +346,7 @@ extern "C" int dkt_cg_solve(dkt_plan *pl, int mode, const double *b, double *x, 
         // Inner tolerance: kInnerTol, loosened when the outer target is already close.
         const double inner_tol = std::min(0.5, std::max(kInnerTol, 0.5 * tol / resid));
         const int inner = cg_run<float>(pl, 1, w.b32.p, w.x32.p, true, max_iter - iters, inner_tol, masked != 0, v,
-                                        w.flag.p, host.p, st, &matvecs);
+                                        w.flag.p, w.host, st, &matvecs);
         hipLaunchKernelGGL(k_add32, dim3(kCgBlocks), dim3(kCgThreads), 0, st, n, x, w.x32.p, w.scal.p);
         HIP_OK(hipGetLastError());
         iters += inner;

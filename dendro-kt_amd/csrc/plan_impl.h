@@ -12,12 +12,19 @@ struct CgWorkspace {
   dkt::DevBuf<double> r, p, ap, d64, part64, scal;
   dkt::DevBuf<float> r32, p32, ap32, x32, b32, part32, scal32;
   dkt::DevBuf<int> flag;  // [0] cancel (converged), [1] iterations done
+  int *host = nullptr;    // pinned copy of flag
   void release() {
     for (auto *b : {&r, &p, &ap, &d64, &part64, &scal}) b->release();
     for (auto *b : {&r32, &p32, &ap32, &x32, &b32, &part32, &scal32}) b->release();
     flag.release();
+    if (host) (void)hipHostFree(host);
+    host = nullptr;
     n = 0;
   }
+  CgWorkspace() = default;
+  CgWorkspace(const CgWorkspace &) = delete;
+  CgWorkspace &operator=(const CgWorkspace &) = delete;
+  ~CgWorkspace() { release(); }
 };
 
 struct dkt_plan {

@@ -1,0 +1,69 @@
+"""Serving harness: concurrent CG requests over cached plans on several HIP
+streams.  A request's answer must be bitwise the same whether it is served
+alone or in a mix (fixed-order device reductions), every request must reach
+its tolerance, and a plan larger than the HBM budget is refused."""
+import importlib.util
+import os
+import sys
+
+import numpy as np
+import pytest
+
+REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _serve_mod():
+    if "dkt_serve" in sys.modules:
+        return sys.modules["dkt_serve"]
+    spec = importlib.util.spec_from_file_location("dkt_serve", os.path.join(REPO, "dendro-kt_amd", "serve.py"))
+    m = importlib.util.module_from_spec(spec)
+    sys.modules["dkt_serve"] = m
+    spec.loader.exec_module(m)
+    return m
+
+
+def _torch():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no HIP device")
+    return torch
+
+
+@pytest.mark.gpu
+def test_alone_equals_in_mix(dkt):
+    _torch()
+    sv = _serve_mod()
+    meshes = [sv.MeshSpec(4, 8, n, s) for n, s in ((300, 1), (800, 2), (1500, 3))]
+    reqs = [sv.Request(m, rhs_seed=k) for m in meshes for k in (1, 2)]
+    srv = sv.Server(dkt, n_streams=3)
+    try:
+        for r in reqs:
+            srv.submit(r)
+        done = srv.drain(len(reqs), timeout=300)
+    finally:
+        srv.close()
+    assert all(not r.error for r in done), [r.error for r in done]
+    assert all(r.stats["resid"] <= r.tol for r in done)
+    alone = sv.Server(dkt, n_streams=1)
+    try:
+        for r in reqs:
+            solo = sv.Request(r.mesh, r.rhs_seed)
+            alone.submit(solo)
+            out = alone.drain(reqs.index(r) + 1, timeout=300)[-1]
+            assert not out.error
+            assert np.array_equal(out.x, r.x)
+    finally:
+        alone.close()
+
+
+@pytest.mark.gpu
+def test_admission_refuses_plans_over_budget(dkt):
+    _torch()
+    sv = _serve_mod()
+    srv = sv.Server(dkt, n_streams=1, hbm_budget_bytes=1e4)
+    try:
+        srv.submit(sv.Request(sv.MeshSpec(3, 8, 500, 4), rhs_seed=1))
+        (r,) = srv.drain(1, timeout=120)
+    finally:
+        srv.close()
+    assert r.error.startswith("MemoryError")
