@@ -15,7 +15,8 @@
 //
 // Mixed precision (DKT_CG_MIXED): iterative refinement.  The outer loop keeps
 // x, the residual r = b - A x and its norm in FP64 (FP64 matvec); each inner
-// solve runs the same CG in FP32 (FP32 matvec and accumulation) on
+// solve runs the same CG with FP32 vectors and matvecs (FP32 accumulation
+// inside the matvec) and FP64 dot products and scalars on
 // A d = r / ||r||_inf to a relative residual of kInnerTol (or looser when
 // tol / ||r|| is larger), then x += ||r|| d.
 #include <algorithm>
@@ -66,12 +67,10 @@ __device__ __forceinline__ T block_max(T v, T *sh) {
 }
 
 // Sum (or max) of the kCgBlocks partials, identical in every block.
-template <typename T>
-__device__ __forceinline__ T parts_sum(const T *part, T *sh) {
+__device__ __forceinline__ double parts_sum(const double *part, double *sh) {
   return block_sum(part[threadIdx.x] + part[threadIdx.x + kCgThreads], sh);
 }
-template <typename T>
-__device__ __forceinline__ T parts_max(const T *part, T *sh) {
+__device__ __forceinline__ double parts_max(const double *part, double *sh) {
   return block_max(fmax(part[threadIdx.x], part[threadIdx.x + kCgThreads]), sh);
 }
 
@@ -80,16 +79,16 @@ __device__ __forceinline__ T parts_max(const T *part, T *sh) {
 // r = b - ax (ax may be null: x = 0), p = r; partials of r.r, max|r|, max|b|.
 template <typename T>
 __global__ void __launch_bounds__(kCgThreads) k_init(int n, const T *__restrict__ b, const T *__restrict__ ax,
-                                                     T *__restrict__ r, T *__restrict__ p, T *__restrict__ part) {
-  __shared__ T sh[kCgThreads];
-  T rr = 0, mr = 0, mb = 0;
+                                                     T *__restrict__ r, T *__restrict__ p, double *__restrict__ part) {
+  __shared__ double sh[kCgThreads];
+  double rr = 0, mr = 0, mb = 0;
   CG_LOOP(i, n) {
     const T ri = ax ? b[i] - ax[i] : b[i];
     r[i] = ri;
     p[i] = ri;
-    rr += ri * ri;
-    mr = fmax(mr, fabs(ri));
-    mb = fmax(mb, fabs(b[i]));
+    rr += (double)ri * (double)ri;
+    mr = fmax(mr, (double)fabs(ri));
+    mb = fmax(mb, (double)fabs(b[i]));
   }
   rr = block_sum(rr, sh);
   mr = block_max(mr, sh);
@@ -101,30 +100,29 @@ __global__ void __launch_bounds__(kCgThreads) k_init(int n, const T *__restrict_
   }
 }
 
-template <typename T>
-__global__ void __launch_bounds__(kCgThreads) k_init_fin(const T *__restrict__ part, T *__restrict__ scal, double tol,
-                                                         int *__restrict__ flag) {
-  __shared__ T sh[kCgThreads];
-  const T rr = parts_sum(part, sh);
-  const T mr = parts_max(part + kCgBlocks, sh);
-  T mb = parts_max(part + 2 * kCgBlocks, sh);
+__global__ void __launch_bounds__(kCgThreads) k_init_fin(const double *__restrict__ part, double *__restrict__ scal,
+                                                         double tol, int *__restrict__ flag) {
+  __shared__ double sh[kCgThreads];
+  const double rr = parts_sum(part, sh);
+  const double mr = parts_max(part + kCgBlocks, sh);
+  double mb = parts_max(part + 2 * kCgBlocks, sh);
   if (threadIdx.x == 0) {
-    if (mb == T(0)) mb = T(1);
+    if (mb == 0.0) mb = 1.0;
     scal[S_RR0] = rr;
     scal[S_NORMB] = mb;
     scal[S_RESID] = mr / mb;
-    flag[0] = (double)(mr / mb) <= tol ? 1 : 0;
+    flag[0] = mr / mb <= tol ? 1 : 0;
     flag[1] = 0;
   }
 }
 
 template <typename T>
 __global__ void __launch_bounds__(kCgThreads) k_dot(int n, const T *__restrict__ a, const T *__restrict__ b,
-                                                    T *__restrict__ part, const int *__restrict__ flag) {
+                                                    double *__restrict__ part, const int *__restrict__ flag) {
   if (flag[0]) return;
-  __shared__ T sh[kCgThreads];
-  T acc = 0;
-  CG_LOOP(i, n) acc += a[i] * b[i];
+  __shared__ double sh[kCgThreads];
+  double acc = 0;
+  CG_LOOP(i, n) acc += (double)a[i] * (double)b[i];
   acc = block_sum(acc, sh);
   if (threadIdx.x == 0) part[blockIdx.x] = acc;
 }
@@ -133,18 +131,19 @@ __global__ void __launch_bounds__(kCgThreads) k_dot(int n, const T *__restrict__
 template <typename T>
 __global__ void __launch_bounds__(kCgThreads) k_update_xr(int n, T *__restrict__ x, T *__restrict__ r,
                                                           const T *__restrict__ p, const T *__restrict__ ap,
-                                                          const T *__restrict__ part_pap, const T *__restrict__ scal,
-                                                          int par, T *__restrict__ part, const int *__restrict__ flag) {
+                                                          const double *__restrict__ part_pap,
+                                                          const double *__restrict__ scal, int par,
+                                                          double *__restrict__ part, const int *__restrict__ flag) {
   if (flag[0]) return;
-  __shared__ T sh[kCgThreads];
-  const T alpha = scal[S_RR0 + par] / parts_sum(part_pap, sh);
-  T rr = 0, mr = 0;
+  __shared__ double sh[kCgThreads];
+  const T alpha = (T)(scal[S_RR0 + par] / parts_sum(part_pap, sh));
+  double rr = 0, mr = 0;
   CG_LOOP(i, n) {
     x[i] += alpha * p[i];
     const T ri = r[i] - alpha * ap[i];
     r[i] = ri;
-    rr += ri * ri;
-    mr = fmax(mr, fabs(ri));
+    rr += (double)ri * (double)ri;
+    mr = fmax(mr, (double)fabs(ri));
   }
   rr = block_sum(rr, sh);
   mr = block_max(mr, sh);
@@ -159,15 +158,15 @@ __global__ void __launch_bounds__(kCgThreads) k_update_xr(int n, T *__restrict__
 // iteration count and the converged flag.
 template <typename T>
 __global__ void __launch_bounds__(kCgThreads) k_update_p(int n, T *__restrict__ p, const T *__restrict__ r,
-                                                         const T *__restrict__ part, T *__restrict__ scal, int par,
-                                                         double tol, int *__restrict__ flag) {
+                                                         const double *__restrict__ part, double *__restrict__ scal,
+                                                         int par, double tol, int *__restrict__ flag) {
   if (flag[0]) return;
-  __shared__ T sh[kCgThreads];
-  const T rr1 = parts_sum(part, sh);
-  const T resid = parts_max(part + kCgBlocks, sh) / scal[S_NORMB];
-  const bool done = (double)resid <= tol;
+  __shared__ double sh[kCgThreads];
+  const double rr1 = parts_sum(part, sh);
+  const double resid = parts_max(part + kCgBlocks, sh) / scal[S_NORMB];
+  const bool done = resid <= tol;
   if (!done) {
-    const T beta = rr1 / scal[S_RR0 + par];
+    const T beta = (T)(rr1 / scal[S_RR0 + par]);
     CG_LOOP(i, n) p[i] = r[i] + beta * p[i];
   }
   if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -231,7 +230,8 @@ __global__ void __launch_bounds__(kCgThreads) k_add32(int n, double *__restrict_
 
 template <typename T>
 struct CgVectors {
-  T *r, *p, *ap, *part, *scal;
+  T *r, *p, *ap;
+  double *part, *scal;
 };
 
 // Runs up to max_iter CG iterations on A x = b in precision T from the current x.
@@ -246,7 +246,7 @@ int cg_run(dkt_plan *pl, int dtype, const T *b, T *x, bool x_is_zero, int max_it
     (*matvecs)++;
   }
   hipLaunchKernelGGL((k_init<T>), grid, blk, 0, st, n, b, x_is_zero ? nullptr : v.ap, v.r, v.p, v.part);
-  hipLaunchKernelGGL((k_init_fin<T>), dim3(1), blk, 0, st, v.part, v.scal, tol, flag);
+  hipLaunchKernelGGL(k_init_fin, dim3(1), blk, 0, st, v.part, v.scal, tol, flag);
   HIP_OK(hipGetLastError());
   int it = 0;
   HIP_OK(hipMemcpyAsync(host, flag, 2 * sizeof(int), hipMemcpyDeviceToHost, st));
@@ -333,7 +333,7 @@ extern "C" int dkt_cg_solve(dkt_plan *pl, int mode, const double *b, double *x, 
       resid = read_scalar(w.scal.p, S_RESID, st);
     } else {
       const int n = (int)pl->N;
-      CgVectors<float> v{w.r32.p, w.p32.p, w.ap32.p, w.part32.p, w.scal32.p};
+      CgVectors<float> v{w.r32.p, w.p32.p, w.ap32.p, w.part32.p, w.scal32.p};  // FP64 reductions
       while (true) {
         plan_apply(pl, 0, x, w.ap.p, 1, masked != 0, st, nullptr);
         matvecs++;

@@ -143,7 +143,8 @@ int dkt_plan_matvec(dkt_plan *plan, int dtype, const void *x, void *y, int nrhs,
  * Dirichlet-masked operator of feMatrix::matVec (as cgSolve does through
  * HeatMat's pre/post hooks); then b should vanish on boundary nodes.
  * mode DKT_CG_F64: all FP64.  DKT_CG_MIXED: iterative refinement with FP64
- * residuals and updates around FP32 inner CG solves (FP32 matvec and dots).
+ * residuals and updates around inner CG solves with FP32 vectors and matvecs
+ * (FP32 accumulation) and FP64 dot products.
  * b, x: device double [N]; x holds the initial guess on entry.  Reductions
  * run in a fixed order (bitwise reproducible).  stats (may be NULL): [0]
  * CG iterations (inner iterations summed for DKT_CG_MIXED), [1] final relative
