@@ -29,6 +29,7 @@ OP_MASS = 1
 OP_IDENTITY = 2
 OP_CAPTURED = 3
 PLAN_SIMPLE = 1
+PLAN_TILE_SHIFT = 8
 CG_F64 = 0
 CG_MIXED = 1
 
@@ -262,7 +263,7 @@ class Plan:
     pointers; ``matvec_torch`` is a convenience for torch tensors (plumbing only).
     """
 
-    def __init__(self, da, op=OP_STIFFNESS, op_data=None, simple=False):
+    def __init__(self, da, op=OP_STIFFNESS, op_data=None, simple=False, tile_leaves=0):
         L = load()
         if not hasattr(L, "dkt_plan_create"):
             raise DktError("libdkt.so was built without the device plan")
@@ -273,7 +274,10 @@ class Plan:
         if op_data is not None:
             self._op_data = np.ascontiguousarray(op_data, dtype=np.float64)
             ptr = self._op_data.ctypes.data_as(ctypes.c_void_p)
-        _chk(L.dkt_plan_create(da.h, op, ptr, PLAN_SIMPLE if simple else 0, ctypes.byref(h)))
+        flags = PLAN_SIMPLE if simple else 0
+        if tile_leaves:
+            flags |= (int(tile_leaves).bit_length() - 1) << PLAN_TILE_SHIFT
+        _chk(L.dkt_plan_create(da.h, op, ptr, flags, ctypes.byref(h)))
         self.h = h
 
     def info(self):

@@ -518,11 +518,12 @@ size_t lds_f64(const TiledHost &o, int T) {
   return tile_lds_bytes<double, NPE>(TileDims{o.max_nodes, o.max_grp, o.max_he, o.max_rd, o.max_cr, T}, 1);
 }
 
-void build_tiled(dkt_plan *pl, const MeshParams &mp, const Traversal &tr) {
+void build_tiled(dkt_plan *pl, const MeshParams &mp, const Traversal &tr, int requested) {
   const int npe = pl->npe;
   const int G = npe <= 4 ? 4 : (npe <= 8 ? 8 : (npe <= 16 ? 16 : 32));
-  // Leaves per tile: as many as fit the LDS budget (FP64, one right-hand side).
-  int T = kTileThreads;
+  // Leaves per tile: the requested size, else as many as fit the LDS budget
+  // (FP64, one right-hand side), halving from kTileThreads.
+  int T = requested ? requested : kTileThreads;
   TiledHost o;
   while (true) {
     o = build_tiled_host(mp, tr, pl->N, T, std::max(1, T / G));
@@ -536,7 +537,7 @@ void build_tiled(dkt_plan *pl, const MeshParams &mp, const Traversal &tr) {
       default: throw std::invalid_argument("unsupported nodes per element");
     }
     if (lds <= kLdsLimit) break;
-    if (T <= 64) throw std::runtime_error("tile does not fit in LDS");
+    if (requested || T <= 64) throw std::runtime_error("tile does not fit in LDS");
     T /= 2;
   }
   pl->tile_leaves = T;
@@ -635,6 +636,7 @@ void run_tiled(dkt_plan *pl, int nrhs, const T *x, T *y, bool masked, hipStream_
   A.carry_key = pl->carry_key.p;
   A.dirichlet = masked ? pl->dirichlet.p : nullptr;
   A.cancel = cancel;
+  A.op = pl->op;
   A.hp = pl->hp;
   A.ld = nrhs;
   A.blocks = pl->blocks.p;
@@ -734,10 +736,12 @@ extern "C" int dkt_plan_create(const dkt_da *da, int op, const double *op_data, 
     std::vector<uint8_t> dir(pl->N, 0);
     for (u32 b : ns.bdy) dir[b] = 1;
     pl->dirichlet.upload(dir);
+    const int tshift = (flags >> DKT_PLAN_TILE_SHIFT) & 15;
+    if (tshift && (tshift < 6 || tshift > 8)) throw std::invalid_argument("tile size must be 64, 128 or 256 leaves");
     if (pl->simple || pl->per_level)
       build_simple(pl.get(), mp, tr);
     else
-      build_tiled(pl.get(), mp, tr);
+      build_tiled(pl.get(), mp, tr, tshift ? 1 << tshift : 0);
     *out = pl.release();
   });
 }

@@ -104,6 +104,54 @@ __device__ __forceinline__ void interp(T *a, int c) {
   }
 }
 
+// Element operators of Q1 (p = 1) elements on the unit cube, applied in the
+// Walsh-Hadamard basis.  With the 1D matrices M1 = [[1/3,1/6],[1/6,1/3]] and
+// K1 = [[1,-1],[-1,1]], the mass matrix M1 x ... x M1 and the stiffness
+// sum_d M1 x .. K1 (axis d) .. x M1 depend on two ranks only through the
+// axes on which they differ, so H = [[1,1],[1,-1]]^(x dim) diagonalises them:
+// A = H diag(lambda) H / 2^dim, with, for a frequency f of Hamming weight k,
+//   mass:      lambda = (1/6)^k (1/2)^(dim-k)
+//   stiffness: lambda = 2 k (1/6)^(k-1) (1/2)^(dim-k).
+// H u costs dim 2^dim additions, against 4^dim multiply-adds for the dense matrix.
+__host__ __device__ constexpr double q1_lambda(int dim, int op, int k) {
+  double v = op == 0 ? 2.0 * k : 1.0;
+  for (int i = 0; i < dim; i++) {
+    const bool diff = i < k;
+    if (op == 0 && diff && i == 0) continue;  // the 2 k factor replaces one 1/6 for the axis carrying K1
+    v *= diff ? 1.0 / 6.0 : 0.5;
+  }
+  return v;
+}
+
+template <typename T, int DIM>
+__device__ __forceinline__ void fwht(T *a) {
+  constexpr int NPE = 1 << DIM;
+#pragma unroll
+  for (int d = 0; d < DIM; d++)
+#pragma unroll
+    for (int i = 0; i < NPE; i++)
+      if (!((i >> d) & 1)) {
+        const T x = a[i], y = a[i | (1 << d)];
+        a[i] = x + y;
+        a[i | (1 << d)] = x - y;
+      }
+}
+
+// v = scale * A u for the Q1 operator op (0 stiffness, 1 mass).
+template <typename T, int DIM, int OP>
+__device__ __forceinline__ void q1_apply(const T *u, T *v, T scale) {
+  constexpr int NPE = 1 << DIM;
+  T a[NPE];
+#pragma unroll
+  for (int i = 0; i < NPE; i++) a[i] = u[i];
+  fwht<T, DIM>(a);
+#pragma unroll
+  for (int f = 0; f < NPE; f++) a[f] *= (T)(q1_lambda(DIM, OP, __builtin_popcount(f)) / NPE);
+  fwht<T, DIM>(a);
+#pragma unroll
+  for (int i = 0; i < NPE; i++) v[i] = scale * a[i];
+}
+
 // One tile = one workgroup = up to kTileThreads SFC-consecutive leaves.
 struct TileHdr {
   int32_t leaf0, nleaf;  // leaves
@@ -147,6 +195,7 @@ struct TileArgs {
   const uint32_t *carry_key;  // buffer rank (8) << 16 | local node (16)
   const uint8_t *dirichlet;
   const int *cancel;  // non-null: the tile returns at once when *cancel != 0
+  int op;  // DKT_OP_STIFFNESS, DKT_OP_MASS or DKT_OP_IDENTITY
   int hp, nq, ld;
   TileDims dims;
 };

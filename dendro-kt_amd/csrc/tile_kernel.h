@@ -14,8 +14,9 @@
 //      recurrence B_k = F_k + W_ck (Q_k B_{k-1}) forward from there (exact on
 //      the slots the reader needs), starting from host-computed carry terms
 //      when the cone begins before the tile;
-//   4. v = 2^(-level*hp) K_ref u in registers;
-//   5. direct outputs are added into the LDS accumulator y_loc in nPe phases,
+//   4. v = 2^(-level*hp) K_ref u in registers (p = 1: in the Walsh-Hadamard
+//      basis that diagonalises K_ref; p = 2: dense, K_ref staged in LDS);
+//   5. direct outputs are added into a per-wave LDS accumulator in nPe phases,
 //      one per rank (two disjoint leaves never share a node at the same rank,
 //      so a phase has no write conflicts);
 //   6. the transposed interpolation of missing ranks (matvec.h:482-522) is
@@ -125,7 +126,9 @@ __global__ void __launch_bounds__(kTileThreads) tile_kernel(TileArgs A, const T 
     return p;
   };
   T *x_loc = reinterpret_cast<T *>(carve((size_t)D.max_nodes * nq * sizeof(T)));
-  T *y_loc = reinterpret_cast<T *>(carve((size_t)D.max_nodes * nq * sizeof(T)));
+  const int NW = (TL + 63) / 64;  // waves per tile: each accumulates its direct outputs in its own copy
+  const size_t ystride = (size_t)D.max_nodes * nq;
+  T *y_loc = reinterpret_cast<T *>(carve(NW * ystride * sizeof(T)));
   T *fbuf = reinterpret_cast<T *>(carve((size_t)FS * NPE * sizeof(T)));  // [slot][hanging index]
   T *ubuf = reinterpret_cast<T *>(carve((size_t)ulen * sizeof(T)));      // [reader][slot], then group sums
   T *cst = reinterpret_cast<T *>(carve((size_t)NPE * nq * sizeof(T)));   // parent buffer at the tile start
@@ -136,8 +139,12 @@ __global__ void __launch_bounds__(kTileThreads) tile_kernel(TileArgs A, const T 
   uint16_t *gploc = reinterpret_cast<uint16_t *>(carve((size_t)D.max_grp * NPE * 2));
   uint16_t *ridx_s = reinterpret_cast<uint16_t *>(carve((size_t)TL * 2));
   uint16_t *rdl_s = reinterpret_cast<uint16_t *>(carve((size_t)D.max_rd * 2));
+  // p >= 2: the reference element matrix, read by all lanes from LDS
+  T *k_s = P == 1 ? nullptr : reinterpret_cast<T *>(carve((size_t)NPE * NPE * sizeof(T)));
   const int tid = threadIdx.x;
   const int nthr = blockDim.x;
+  if constexpr (P != 1)
+    for (int k = tid; k < NPE * NPE; k += nthr) k_s[k] = kref[k];
 
   // 1. Stage.
   const bool active = tid < th.nleaf;
@@ -158,7 +165,7 @@ __global__ void __launch_bounds__(kTileThreads) tile_kernel(TileArgs A, const T 
     const bool z = A.dirichlet && A.dirichlet[g];
     for (int q = 0; q < nq; q++) {
       x_loc[j * nq + q] = z ? T(0) : x[(size_t)g * A.ld + q];
-      y_loc[j * nq + q] = T(0);
+      for (int w = 0; w < NW; w++) y_loc[w * ystride + j * nq + q] = T(0);
     }
   }
   for (int k = tid; k < th.ngrp * NPE; k += nthr) gploc[k] = A.group_ploc[(size_t)th.grp0 * NPE + k];
@@ -258,19 +265,32 @@ __global__ void __launch_bounds__(kTileThreads) tile_kernel(TileArgs A, const T 
         if (hang) hv = fbuf[r * FS + hidx] + (ri != kNone ? ubuf[ri * NPE + r] : T(0));
         u[r] = loc[r] != kNone ? x_loc[loc[r] * nq + q] : hv;
       }
+      if constexpr (P == 1) {
+        if (A.op == 0)
+          q1_apply<T, DIM, 0>(u, v, scale);
+        else if (A.op == 1)
+          q1_apply<T, DIM, 1>(u, v, scale);
+        else
 #pragma unroll
-      for (int i = 0; i < NPE; i++) {
-        T acc = T(0);
+          for (int i = 0; i < NPE; i++) v[i] = scale * u[i];
+      } else {
 #pragma unroll
-        for (int j = 0; j < NPE; j++) acc += kref[i * NPE + j] * u[j];
-        v[i] = scale * acc;
+        for (int i = 0; i < NPE; i++) {
+          T acc = T(0);
+#pragma unroll
+          for (int j = 0; j < NPE; j++) acc += k_s[i * NPE + j] * u[j];
+          v[i] = scale * acc;
+        }
       }
     }
-    // 5. Direct outputs, one phase per rank.
+    // 5. Direct outputs, one phase per rank, into the wave's own accumulator:
+    //    no two leaves share a node at one rank, and a wave's LDS operations
+    //    execute in program order, so the phases need no barrier.
+    {
+      T *yw = y_loc + (size_t)(tid >> 6) * ystride;
 #pragma unroll
-    for (int r = 0; r < NPE; r++) {
-      __syncthreads();
-      if (loc[r] != kNone) y_loc[loc[r] * nq + q] += v[r];
+      for (int r = 0; r < NPE; r++)
+        if (loc[r] != kNone) yw[loc[r] * nq + q] += v[r];
     }
     // 6. Transposed interpolation of missing ranks.
     if (th.nhe > 0) {
@@ -302,16 +322,19 @@ __global__ void __launch_bounds__(kTileThreads) tile_kernel(TileArgs A, const T 
     }
     __syncthreads();
   }
-  // 7. Write out: owned nodes to y, shared nodes to their partial slot.
+  // 7. Write out (wave accumulators summed in wave order): owned nodes to y,
+  //    shared nodes to their partial slot.
   for (int j = tid; j < th.nnode; j += nthr) {
     const int o = A.tile_out[th.node0 + j];
     if (o == INT32_MIN) continue;  // gathered only
-    if (o >= 0) {
-      const bool z = A.dirichlet && A.dirichlet[o];
-      for (int q = 0; q < nq; q++) y[(size_t)o * A.ld + q] = z ? T(0) : y_loc[j * nq + q];
-    } else {
-      const size_t slot = (size_t)(-o - 1);
-      for (int q = 0; q < nq; q++) partial[slot * nq + q] = y_loc[j * nq + q];
+    const bool z = o >= 0 && A.dirichlet && A.dirichlet[o];
+    for (int q = 0; q < nq; q++) {
+      T acc = y_loc[j * nq + q];
+      for (int w = 1; w < NW; w++) acc += y_loc[w * ystride + j * nq + q];
+      if (o >= 0)
+        y[(size_t)o * A.ld + q] = z ? T(0) : acc;
+      else
+        partial[(size_t)(-o - 1) * nq + q] = acc;
     }
   }
 }
@@ -339,10 +362,12 @@ inline size_t tile_lds_bytes(const TileDims &D, int nq) {
   const size_t ulen = (size_t)(D.max_he > D.max_rd * NPE ? D.max_he : D.max_rd * NPE);
   auto r16 = [](size_t b) { return (b + 15) & ~(size_t)15; };
   const size_t TL = (size_t)D.tile_leaves;
-  return 2 * r16((size_t)D.max_nodes * nq * sizeof(T)) + r16((TL + 1) * NPE * sizeof(T)) + r16(ulen * sizeof(T)) +
+  const size_t NW = (TL + 63) / 64;
+  return r16((size_t)D.max_nodes * nq * sizeof(T)) + r16(NW * D.max_nodes * nq * sizeof(T)) +
+         r16((TL + 1) * NPE * sizeof(T)) + r16(ulen * sizeof(T)) +
          r16((size_t)NPE * nq * sizeof(T)) + r16((size_t)D.max_cr * sizeof(T)) + r16((size_t)D.max_he * 8) +
          r16(TL * 8) + r16((size_t)D.max_cr * 4) + r16((size_t)D.max_grp * NPE * 2) + r16(TL * 2) +
-         r16((size_t)D.max_rd * 2);
+         r16((size_t)D.max_rd * 2) + ((NPE & (NPE - 1)) ? r16((size_t)NPE * NPE * sizeof(T)) : 0);
 }
 
 }  // namespace dkt

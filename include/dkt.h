@@ -43,6 +43,9 @@ extern "C" {
  * run a matvec. */
 #define DKT_PLAN_SIMPLE 1
 #define DKT_PLAN_DRYRUN 2
+/* Leaves per tile of the tiled path: flags |= log2(leaves) << DKT_PLAN_TILE_SHIFT
+ * with leaves in {64, 128, 256}; 0 = the largest size whose LDS footprint fits. */
+#define DKT_PLAN_TILE_SHIFT 8
 
 typedef struct dkt_octs dkt_octs; /* host list of cells/points */
 typedef struct dkt_da dkt_da;     /* host mesh: tree + CG nodes + traversal */
