@@ -251,8 +251,33 @@ TiledHost build_tiled_host(const MeshParams &mp, const Traversal &tr, size_t N, 
   const int npe = tr.nPe;
   const int dim = mp.dim;
   const size_t L = tr.leaves.size();
-  const size_t ntiles = (L + T - 1) / T;
   StaleBuffer sb(mp, tr);
+  // Tiles: runs of SFC-consecutive leaves, at most T leaves and kMaxReaders
+  // stale-buffer readers each (a reader is a hanging leaf with a missing rank
+  // that interpolates from a parent rank with no parent-level node).
+  std::vector<size_t> bounds{0};
+  {
+    int nrd = 0;
+    for (size_t e = 0; e < L; e++) {
+      bool reader = false;
+      const int h = tr.hang_of[e];
+      if (h >= 0) {
+        const int32_t *sl = &tr.slot[e * npe];
+        const int32_t *ps = &tr.pslot[(size_t)h * npe];
+        for (int r = 0; r < npe && !reader; r++)
+          if (sl[r] < 0)
+            for (const auto &tw : sb.weights(tr.child[e], r))
+              if (ps[tw.first] < 0) reader = true;
+      }
+      if (e - bounds.back() == (size_t)T || (reader && nrd == kMaxReaders)) {
+        bounds.push_back(e);
+        nrd = 0;
+      }
+      nrd += reader ? 1 : 0;
+    }
+    if (bounds.back() != L) bounds.push_back(L);
+  }
+  const size_t ntiles = bounds.size() - 1;
   TiledHost o;
   o.hdr.resize(ntiles);
   o.lloc.assign(L * (size_t)npe, kNone);
@@ -261,7 +286,7 @@ TiledHost build_tiled_host(const MeshParams &mp, const Traversal &tr, size_t N, 
 
   // Pass 1: tiles that write each node (as a direct or hanging-transpose target).
   for (size_t t = 0; t < ntiles; t++) {
-    const size_t l0 = t * T, l1 = std::min(L, l0 + T);
+    const size_t l0 = bounds[t], l1 = bounds[t + 1];
     auto mark = [&](int32_t n) {
       if (last_tile[n] != (int32_t)t) {
         last_tile[n] = (int32_t)t;
@@ -300,7 +325,7 @@ TiledHost build_tiled_host(const MeshParams &mp, const Traversal &tr, size_t N, 
   double s_rd = 0, m_rd = 0, s_cr = 0, m_cr = 0, s_gr = 0, s_nn = 0, s_cone = 0, n_cone = 0;
   double s_span = 0, m_span = 0, n_rtiles = 0;
   for (size_t t = 0; t < ntiles; t++) {
-    const size_t l0 = t * T, l1 = std::min(L, l0 + T);
+    const size_t l0 = bounds[t], l1 = bounds[t + 1];
     TileHdr &H = o.hdr[t];
     H.leaf0 = (int32_t)l0;
     H.nleaf = (int32_t)(l1 - l0);
@@ -521,9 +546,10 @@ size_t lds_f64(const TiledHost &o, int T) {
 void build_tiled(dkt_plan *pl, const MeshParams &mp, const Traversal &tr, int requested) {
   const int npe = pl->npe;
   const int G = npe <= 4 ? 4 : (npe <= 8 ? 8 : (npe <= 16 ? 16 : 32));
-  // Leaves per tile: the requested size, else as many as fit the LDS budget
-  // (FP64, one right-hand side), halving from kTileThreads.
-  int T = requested ? requested : kTileThreads;
+  // Leaves per tile: the requested size, else kDefaultTile (the fastest size for
+  // the 4D p = 1 workload on MI355X; dendro-kt_amd/autotune.py times the others),
+  // halved until the LDS footprint (FP64, one right-hand side) fits.
+  int T = requested ? requested : kDefaultTile;
   TiledHost o;
   while (true) {
     o = build_tiled_host(mp, tr, pl->N, T, std::max(1, T / G));

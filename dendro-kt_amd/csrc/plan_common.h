@@ -19,6 +19,8 @@ namespace dkt {
 
 constexpr uint16_t kNone = 0xFFFF;
 constexpr int kTileThreads = 256;
+constexpr int kDefaultTile = 128;
+constexpr int kMaxReaders = 32;  // stale-buffer readers per tile (bounds the replay outputs in LDS)
 constexpr size_t kLdsLimit = 64 * 1024;
 
 // Host-only plan construction (DKT_PLAN_DRYRUN): sizes are recorded, nothing is

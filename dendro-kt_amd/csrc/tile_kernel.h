@@ -16,7 +16,7 @@
 //      when the cone begins before the tile;
 //   4. v = 2^(-level*hp) K_ref u in registers (p = 1: in the Walsh-Hadamard
 //      basis that diagonalises K_ref; p = 2: dense, K_ref staged in LDS);
-//   5. direct outputs are added into a per-wave LDS accumulator in nPe phases,
+//   5. direct outputs are added into the LDS accumulator y_loc in nPe phases,
 //      one per rank (two disjoint leaves never share a node at the same rank,
 //      so a phase has no write conflicts);
 //   6. the transposed interpolation of missing ranks (matvec.h:482-522) is
@@ -30,6 +30,36 @@
 #include "plan_common.h"
 
 namespace dkt {
+
+// Nodes staged or written per work-item with all their loads in flight.
+constexpr int kNB = 4;
+
+// One leaf's rank-to-local-node row, with 16- or 8-byte loads where the row
+// size allows (rows are nPe uint16 from a 256-byte aligned base).
+template <int NPE>
+__device__ __forceinline__ void load_row(const uint16_t *p, uint16_t *loc) {
+  if constexpr (NPE % 8 == 0) {
+#pragma unroll
+    for (int c = 0; c < NPE / 8; c++) {
+      const uint4 w = reinterpret_cast<const uint4 *>(p)[c];
+      const uint32_t ws[4] = {w.x, w.y, w.z, w.w};
+#pragma unroll
+      for (int k = 0; k < 4; k++) {
+        loc[8 * c + 2 * k] = (uint16_t)(ws[k] & 0xFFFFu);
+        loc[8 * c + 2 * k + 1] = (uint16_t)(ws[k] >> 16);
+      }
+    }
+  } else if constexpr (NPE == 4) {
+    const uint2 w = *reinterpret_cast<const uint2 *>(p);
+    loc[0] = (uint16_t)(w.x & 0xFFFFu);
+    loc[1] = (uint16_t)(w.x >> 16);
+    loc[2] = (uint16_t)(w.y & 0xFFFFu);
+    loc[3] = (uint16_t)(w.y >> 16);
+  } else {
+#pragma unroll
+    for (int r = 0; r < NPE; r++) loc[r] = p[r];
+  }
+}
 
 // Lanes that replay one stale-buffer reader together: one lane per buffer slot.
 template <int NPE>
@@ -126,9 +156,7 @@ __global__ void __launch_bounds__(kTileThreads) tile_kernel(TileArgs A, const T 
     return p;
   };
   T *x_loc = reinterpret_cast<T *>(carve((size_t)D.max_nodes * nq * sizeof(T)));
-  const int NW = (TL + 63) / 64;  // waves per tile: each accumulates its direct outputs in its own copy
-  const size_t ystride = (size_t)D.max_nodes * nq;
-  T *y_loc = reinterpret_cast<T *>(carve(NW * ystride * sizeof(T)));
+  T *y_loc = reinterpret_cast<T *>(carve((size_t)D.max_nodes * nq * sizeof(T)));
   T *fbuf = reinterpret_cast<T *>(carve((size_t)FS * NPE * sizeof(T)));  // [slot][hanging index]
   T *ubuf = reinterpret_cast<T *>(carve((size_t)ulen * sizeof(T)));      // [reader][slot], then group sums
   T *cst = reinterpret_cast<T *>(carve((size_t)NPE * nq * sizeof(T)));   // parent buffer at the tile start
@@ -151,21 +179,36 @@ __global__ void __launch_bounds__(kTileThreads) tile_kernel(TileArgs A, const T 
   uint16_t loc[NPE];
   uint32_t aux = 0;
   if (active) {
-    const uint16_t *lp = A.leaf_loc + (size_t)(th.leaf0 + tid) * NPE;
-#pragma unroll
-    for (int r = 0; r < NPE; r++) loc[r] = lp[r];
+    load_row<NPE>(A.leaf_loc + (size_t)(th.leaf0 + tid) * NPE, loc);
     aux = A.leaf_aux[th.leaf0 + tid];
   } else {
 #pragma unroll
     for (int r = 0; r < NPE; r++) loc[r] = kNone;
   }
   if (tid < TL) ridx_s[tid] = kNone;
-  for (int j = tid; j < th.nnode; j += nthr) {
-    const int g = A.tile_nodes[th.node0 + j];
-    const bool z = A.dirichlet && A.dirichlet[g];
+  // Node gathers in batches of kNB per work-item, all loads issued before use.
+  for (int j0 = tid; j0 < th.nnode; j0 += kNB * nthr) {
+    int g[kNB];
+#pragma unroll
+    for (int b = 0; b < kNB; b++) {
+      const int j = j0 + b * nthr;
+      g[b] = j < th.nnode ? A.tile_nodes[th.node0 + j] : -1;
+    }
+    bool z[kNB];
+#pragma unroll
+    for (int b = 0; b < kNB; b++) z[b] = g[b] >= 0 && A.dirichlet && A.dirichlet[g[b]];
     for (int q = 0; q < nq; q++) {
-      x_loc[j * nq + q] = z ? T(0) : x[(size_t)g * A.ld + q];
-      for (int w = 0; w < NW; w++) y_loc[w * ystride + j * nq + q] = T(0);
+      T xv[kNB];
+#pragma unroll
+      for (int b = 0; b < kNB; b++) xv[b] = g[b] >= 0 ? x[(size_t)g[b] * A.ld + q] : T(0);
+#pragma unroll
+      for (int b = 0; b < kNB; b++) {
+        const int j = j0 + b * nthr;
+        if (g[b] >= 0) {
+          x_loc[j * nq + q] = z[b] ? T(0) : xv[b];
+          y_loc[j * nq + q] = T(0);
+        }
+      }
     }
   }
   for (int k = tid; k < th.ngrp * NPE; k += nthr) gploc[k] = A.group_ploc[(size_t)th.grp0 * NPE + k];
@@ -283,14 +326,11 @@ __global__ void __launch_bounds__(kTileThreads) tile_kernel(TileArgs A, const T 
         }
       }
     }
-    // 5. Direct outputs, one phase per rank, into the wave's own accumulator:
-    //    no two leaves share a node at one rank, and a wave's LDS operations
-    //    execute in program order, so the phases need no barrier.
-    {
-      T *yw = y_loc + (size_t)(tid >> 6) * ystride;
+    // 5. Direct outputs, one phase per rank.
 #pragma unroll
-      for (int r = 0; r < NPE; r++)
-        if (loc[r] != kNone) yw[loc[r] * nq + q] += v[r];
+    for (int r = 0; r < NPE; r++) {
+      __syncthreads();
+      if (loc[r] != kNone) y_loc[loc[r] * nq + q] += v[r];
     }
     // 6. Transposed interpolation of missing ranks.
     if (th.nhe > 0) {
@@ -322,19 +362,28 @@ __global__ void __launch_bounds__(kTileThreads) tile_kernel(TileArgs A, const T 
     }
     __syncthreads();
   }
-  // 7. Write out (wave accumulators summed in wave order): owned nodes to y,
-  //    shared nodes to their partial slot.
-  for (int j = tid; j < th.nnode; j += nthr) {
-    const int o = A.tile_out[th.node0 + j];
-    if (o == INT32_MIN) continue;  // gathered only
-    const bool z = o >= 0 && A.dirichlet && A.dirichlet[o];
-    for (int q = 0; q < nq; q++) {
-      T acc = y_loc[j * nq + q];
-      for (int w = 1; w < NW; w++) acc += y_loc[w * ystride + j * nq + q];
-      if (o >= 0)
-        y[(size_t)o * A.ld + q] = z ? T(0) : acc;
-      else
-        partial[(size_t)(-o - 1) * nq + q] = acc;
+  // 7. Write out: owned nodes to y, shared nodes to their partial slot.
+  for (int j0 = tid; j0 < th.nnode; j0 += kNB * nthr) {
+    int o[kNB];
+#pragma unroll
+    for (int b = 0; b < kNB; b++) {
+      const int j = j0 + b * nthr;
+      o[b] = j < th.nnode ? A.tile_out[th.node0 + j] : INT32_MIN;  // INT32_MIN: gathered only
+    }
+    bool z[kNB];
+#pragma unroll
+    for (int b = 0; b < kNB; b++) z[b] = o[b] >= 0 && A.dirichlet && A.dirichlet[o[b]];
+#pragma unroll
+    for (int b = 0; b < kNB; b++) {
+      const int j = j0 + b * nthr;
+      if (o[b] == INT32_MIN) continue;
+      for (int q = 0; q < nq; q++) {
+        const T acc = y_loc[j * nq + q];
+        if (o[b] >= 0)
+          y[(size_t)o[b] * A.ld + q] = z[b] ? T(0) : acc;
+        else
+          partial[(size_t)(-o[b] - 1) * nq + q] = acc;
+      }
     }
   }
 }
@@ -362,8 +411,7 @@ inline size_t tile_lds_bytes(const TileDims &D, int nq) {
   const size_t ulen = (size_t)(D.max_he > D.max_rd * NPE ? D.max_he : D.max_rd * NPE);
   auto r16 = [](size_t b) { return (b + 15) & ~(size_t)15; };
   const size_t TL = (size_t)D.tile_leaves;
-  const size_t NW = (TL + 63) / 64;
-  return r16((size_t)D.max_nodes * nq * sizeof(T)) + r16(NW * D.max_nodes * nq * sizeof(T)) +
+  return 2 * r16((size_t)D.max_nodes * nq * sizeof(T)) +
          r16((TL + 1) * NPE * sizeof(T)) + r16(ulen * sizeof(T)) +
          r16((size_t)NPE * nq * sizeof(T)) + r16((size_t)D.max_cr * sizeof(T)) + r16((size_t)D.max_he * 8) +
          r16(TL * 8) + r16((size_t)D.max_cr * 4) + r16((size_t)D.max_grp * NPE * 2) + r16(TL * 2) +

@@ -44,7 +44,8 @@ extern "C" {
 #define DKT_PLAN_SIMPLE 1
 #define DKT_PLAN_DRYRUN 2
 /* Leaves per tile of the tiled path: flags |= log2(leaves) << DKT_PLAN_TILE_SHIFT
- * with leaves in {64, 128, 256}; 0 = the largest size whose LDS footprint fits. */
+ * with leaves in {64, 128, 256}; 0 = 128, halved while the LDS footprint does
+ * not fit.  Tiles also end early at 32 stale-buffer readers. */
 #define DKT_PLAN_TILE_SHIFT 8
 
 typedef struct dkt_octs dkt_octs; /* host list of cells/points */

@@ -64,8 +64,8 @@ def _dry_plan(dkt, da, op=None):
 @pytest.mark.parametrize("dim,p", [(2, 1), (3, 1), (4, 1), (2, 2), (3, 2)])
 def test_dry_run_plan_layout(dkt, dim, p):
     """Host-side tiling without a device: counts agree with the mesh, tiles cover the
-    leaves, every shared node has at least two partial slots, and a dry plan refuses
-    to run a matvec."""
+    leaves, readers per tile are capped, every shared node has at least two partial
+    slots, and a dry plan refuses to run a matvec."""
     import gen_inputs as gi
     maxd = 8 if p == 1 else 10
     if p == 1:
@@ -78,9 +78,10 @@ def test_dry_run_plan_layout(dkt, dim, p):
     try:
         assert info["nodes"] == da.n_nodes and info["leaves"] == da.count(3) and info["hanging"] == da.count(4)
         assert info["npe"] == (p + 1) ** dim
-        assert info["tiles"] * info["tile_leaves"] >= info["leaves"] > (info["tiles"] - 1) * info["tile_leaves"]
+        # variable-length tiles: at most tile_leaves leaves and 32 stale-buffer readers each
+        assert info["tiles"] * info["tile_leaves"] >= info["leaves"]
         assert info["partials"] >= 2 * info["shared_nodes"]
-        assert info["readers_max"] <= info["tile_leaves"]
+        assert info["readers_max"] <= min(32, info["tile_leaves"])
         x = (ctypes.c_double * 1)()
         rc = L.dkt_plan_matvec(h, 0, ctypes.cast(x, ctypes.c_void_p), ctypes.cast(x, ctypes.c_void_p), 1, 0, None)
         assert rc != 0
