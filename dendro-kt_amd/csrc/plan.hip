@@ -491,6 +491,32 @@ TiledHost build_tiled_host(const MeshParams &mp, const Traversal &tr, size_t N, 
     H.nrd = (int32_t)(o.rdr.size() - H.rd0);
     H.nbk = (int32_t)(o.blk.size() - H.bk0);
     H.ncr = (int32_t)(o.ckey.size() - H.cr0);
+    // Renumber the tile's local nodes in global node order, so that the x
+    // gathers, the y writes and the partial-slot writes of consecutive
+    // work-items go to increasing addresses.
+    {
+      const size_t nn = touched.size();
+      std::vector<uint32_t> ord(nn);
+      for (size_t k = 0; k < nn; k++) ord[k] = (uint32_t)k;
+      std::sort(ord.begin(), ord.end(), [&](uint32_t a, uint32_t b) { return touched[a] < touched[b]; });
+      std::vector<uint16_t> renum(nn);
+      for (size_t k = 0; k < nn; k++) renum[ord[k]] = (uint16_t)k;
+      std::vector<int32_t> t2(nn);
+      std::vector<uint8_t> it2(nn);
+      for (size_t k = 0; k < nn; k++) {
+        t2[k] = touched[ord[k]];
+        it2[k] = is_target[ord[k]];
+        local[t2[k]] = (int32_t)k;
+      }
+      touched.swap(t2);
+      is_target.swap(it2);
+      for (size_t i = l0 * npe; i < l1 * npe; i++)
+        if (o.lloc[i] != kNone) o.lloc[i] = renum[o.lloc[i]];
+      for (size_t i = (size_t)H.grp0 * npe; i < o.gploc.size(); i++)
+        if (o.gploc[i] != kNone) o.gploc[i] = renum[o.gploc[i]];
+      for (size_t i = H.cr0; i < o.ckey.size(); i++)
+        o.ckey[i] = (o.ckey[i] & 0xFFFF0000u) | renum[o.ckey[i] & 0xFFFFu];
+    }
     // Hanging-transpose entries, sorted by target node for the segmented sum.
     std::sort(gl.begin(), gl.end());
     gl.erase(std::unique(gl.begin(), gl.end()), gl.end());
