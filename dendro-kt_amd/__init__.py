@@ -32,6 +32,7 @@ PLAN_SIMPLE = 1
 PLAN_TILE_SHIFT = 8
 CG_F64 = 0
 CG_MIXED = 1
+BATCH_GRAPH = 1
 
 _lib = None
 
@@ -106,6 +107,10 @@ def _bind_device(L):
     L.dkt_plan_profile.argtypes = [P, c.c_int]
     L.dkt_plan_profile_read.argtypes = [P, c.POINTER(c.c_double), c.POINTER(c.c_double), c.POINTER(c.c_int)]
     L.dkt_cg_solve.argtypes = [P, c.c_int, P, P, c.c_int, c.c_double, c.c_int, c.POINTER(c.c_double), P]
+    L.dkt_batch_create.argtypes = [c.POINTER(P), c.c_int, c.c_int, c.c_int, c.POINTER(P), c.POINTER(P), c.c_int,
+                                   c.c_int, c.POINTER(P)]
+    L.dkt_batch_run.argtypes = [P, P]
+    L.dkt_batch_destroy.argtypes = [P]
 
 
 def _chk(rc):
@@ -335,4 +340,48 @@ class Plan:
     def __del__(self):
         if getattr(self, "h", None) and _lib is not None:
             _lib.dkt_plan_destroy(self.h)
+            self.h = None
+
+
+class Batch:
+    """Batched matvec over many plans: one launch per (dim, p) (dkt_batch_*).
+
+    ``xs``/``ys`` are torch device tensors [N_k] or [N_k, nrhs] (one dtype);
+    with ``graph=True`` the launches are captured into a hipGraph on the first
+    run (needs a non-default stream) and replayed afterwards.
+    """
+
+    def __init__(self, plans, xs, ys, masked=False, graph=False):
+        import torch
+        L = load()
+        n = len(plans)
+        if not (n == len(xs) == len(ys)) or n == 0:
+            raise DktError("plans, xs and ys must have the same nonzero length")
+        dt = xs[0].dtype
+        if dt not in (torch.float64, torch.float32):
+            raise DktError("x must be float64 or float32")
+        nrhs = 1 if xs[0].dim() == 1 else xs[0].shape[1]
+        for p, x, y in zip(plans, xs, ys):
+            if x.dtype != dt or y.dtype != dt or x.shape != y.shape or x.shape[0] != p.n_nodes:
+                raise DktError("vector shape or dtype does not match")
+            if (1 if x.dim() == 1 else x.shape[1]) != nrhs or not (x.is_contiguous() and y.is_contiguous()):
+                raise DktError("all vectors need the same nrhs and contiguous storage")
+        P = ctypes.c_void_p
+        self._keep = (list(plans), list(xs), list(ys))
+        ph = (P * n)(*[p.h.value for p in plans])
+        xp = (P * n)(*[x.data_ptr() for x in xs])
+        yp = (P * n)(*[y.data_ptr() for y in ys])
+        h = P()
+        _chk(L.dkt_batch_create(ph, n, 0 if dt == torch.float64 else 1, nrhs, xp, yp, 1 if masked else 0,
+                                BATCH_GRAPH if graph else 0, ctypes.byref(h)))
+        self.h = h
+
+    def run(self, stream=None):
+        import torch
+        s = stream if stream is not None else torch.cuda.current_stream().cuda_stream
+        _chk(load().dkt_batch_run(self.h, ctypes.c_void_p(s)))
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.dkt_batch_destroy(self.h)
             self.h = None

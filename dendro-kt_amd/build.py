@@ -14,7 +14,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("DKT_OFFLOAD_ARCH", "gfx950")
 
 HOST_SRCS = ["mesh.cpp", "capi_mesh.cpp"]
-HIP_SRCS = ["plan.hip", "cg.hip"]
+HIP_SRCS = ["plan.hip", "cg.hip", "batch.hip"]
 
 HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-fopenmp", "-ffp-contract=off", "-Wall", "-Wno-unused-variable",
               "-Wno-sign-compare"]

@@ -1,0 +1,209 @@
+// Batched matvec over many independent meshes (no counterpart in the
+// reference, which runs one fem::matvec per call, FEM/include/matvec.h:231).
+//
+// Plans of the same (dim, p) run as one launch: workgroup b finds its plan by
+// a binary search over the plans' tile offsets and runs that plan's tile with
+// the plan's maps and vectors (a device table of BatchPlan records), followed
+// by one batched fix-up launch (per chunk of right-hand sides that fits the
+// tiles' LDS).  With DKT_BATCH_GRAPH the launches of every
+// (dim, p) group are captured once into a hipGraph and replayed.  Each plan's
+// result is bitwise the same as its own dkt_plan_matvec: the tile work and
+// the fix-up order are the plan's.
+#include <algorithm>
+#include <map>
+#include <memory>
+#include <vector>
+
+#include "../../include/dkt.h"
+#include "capi_common.h"
+#include "plan_common.h"
+#include "plan_impl.h"
+#include "tile_kernel.h"
+
+using namespace dkt;
+
+namespace {
+
+struct GroupBase {
+  virtual ~GroupBase() = default;
+  virtual void launch(hipStream_t st) = 0;
+};
+
+template <typename T, int DIM, int P>
+struct Group : GroupBase {
+  // One table of plan records per column chunk (columns [q0, q0 + nq)).
+  std::vector<std::unique_ptr<DevBuf<BatchPlan<T>>>> tables;
+  DevBuf<int> tile_start, block_start;
+  DevBuf<T> partial;
+  int nplans = 0, ntiles = 0, nblocks = 0, threads = 0;
+  std::vector<size_t> lds;
+
+  Group(const std::vector<dkt_plan *> &pls, const std::vector<const void *> &xs, const std::vector<void *> &ys,
+        int nrhs, bool masked) {
+    constexpr int NPE = Pow<P + 1, DIM>::v;
+    const bool f64 = sizeof(T) == 8;
+    auto dims_of = [](const dkt_plan *pl) {
+      return TileDims{pl->max_nodes, pl->max_grp, pl->max_he, pl->max_rd, pl->max_cr, pl->tile_leaves};
+    };
+    // Columns per launch: the most every plan's tiles fit in LDS.
+    int chunk = nrhs;
+    for (dkt_plan *pl : pls) {
+      if (tile_lds_bytes<T, NPE>(dims_of(pl), 1) > kLdsLimit) throw std::invalid_argument("tile does not fit in LDS");
+      while (chunk > 1 && tile_lds_bytes<T, NPE>(dims_of(pl), chunk) > kLdsLimit) chunk--;
+    }
+    std::vector<int> ts{0}, bs{0};
+    std::vector<size_t> poff;
+    size_t ptot = 0;
+    for (dkt_plan *pl : pls) {
+      threads = std::max(threads, pl->tile_leaves);
+      poff.push_back(ptot);
+      ptot += pl->npartial * (size_t)chunk;
+      ts.push_back(ts.back() + (int)pl->ntiles);
+      bs.push_back(bs.back() + (int)((pl->nshared + 255) / 256));
+    }
+    partial.alloc(std::max<size_t>(1, ptot));
+    for (int q0 = 0; q0 < nrhs; q0 += chunk) {
+      const int nq = std::min(chunk, nrhs - q0);
+      std::vector<BatchPlan<T>> hp(pls.size());
+      size_t need = 0;
+      for (size_t k = 0; k < pls.size(); k++) {
+        dkt_plan *pl = pls[k];
+        need = std::max(need, tile_lds_bytes<T, NPE>(dims_of(pl), nq));
+        TileArgs &A = hp[k].A;
+        A.tiles = pl->tiles.p;
+        A.tile_nodes = pl->tile_nodes.p;
+        A.tile_out = pl->tile_out.p;
+        A.leaf_loc = pl->leaf_loc.p;
+        A.leaf_aux = pl->leaf_aux.p;
+        A.group_ploc = pl->group_ploc.p;
+        A.hentries = pl->hentries.p;
+        A.readers = pl->readers.p;
+        A.blocks = pl->blocks.p;
+        A.carry_key = pl->carry_key.p;
+        A.dirichlet = masked ? pl->dirichlet.p : nullptr;
+        A.cancel = nullptr;
+        A.op = pl->op;
+        A.hp = pl->hp;
+        A.nq = nq;
+        A.ld = nrhs;
+        A.dims = dims_of(pl);
+        hp[k].kref = f64 ? (const T *)pl->kmat64.p : (const T *)pl->kmat32.p;
+        hp[k].carry_w = f64 ? (const T *)pl->carry_w64.p : (const T *)pl->carry_w32.p;
+        hp[k].x = (const T *)xs[k] + q0;
+        hp[k].y = (T *)ys[k] + q0;
+        hp[k].partial = partial.p + poff[k];
+        hp[k].snode = pl->snode.p;
+        hp[k].soff = pl->soff.p;
+        hp[k].nshared = (int)pl->nshared;
+      }
+      tables.emplace_back(new DevBuf<BatchPlan<T>>());
+      tables.back()->upload(hp);
+      lds.push_back(need);
+    }
+    nplans = (int)pls.size();
+    ntiles = ts.back();
+    nblocks = bs.back();
+    tile_start.upload(ts);
+    block_start.upload(bs);
+  }
+
+  void launch(hipStream_t st) override {
+    for (size_t c = 0; c < tables.size(); c++) {
+      if (ntiles)
+        hipLaunchKernelGGL((tile_kernel_batch<T, DIM, P>), dim3(ntiles), dim3(threads), lds[c], st, tables[c]->p,
+                           tile_start.p, nplans);
+      if (nblocks)
+        hipLaunchKernelGGL((fixup_kernel_batch<T>), dim3(nblocks), dim3(256), 0, st, tables[c]->p, block_start.p,
+                           nplans);
+      HIP_OK(hipGetLastError());
+    }
+  }
+};
+
+template <typename T>
+GroupBase *make_group(int dim, int p, const std::vector<dkt_plan *> &pls, const std::vector<const void *> &xs,
+                      const std::vector<void *> &ys, int nrhs, bool masked) {
+  if (dim == 2 && p == 1) return new Group<T, 2, 1>(pls, xs, ys, nrhs, masked);
+  if (dim == 3 && p == 1) return new Group<T, 3, 1>(pls, xs, ys, nrhs, masked);
+  if (dim == 4 && p == 1) return new Group<T, 4, 1>(pls, xs, ys, nrhs, masked);
+  if (dim == 2 && p == 2) return new Group<T, 2, 2>(pls, xs, ys, nrhs, masked);
+  if (dim == 3 && p == 2) return new Group<T, 3, 2>(pls, xs, ys, nrhs, masked);
+  throw std::invalid_argument("unsupported (dim, order)");
+}
+
+}  // namespace
+
+struct dkt_batch {
+  std::vector<std::unique_ptr<GroupBase>> groups;
+  bool use_graph = false;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  ~dkt_batch() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+  }
+};
+
+extern "C" int dkt_batch_create(dkt_plan *const *plans, int nplans, int dtype, int nrhs, const void *const *x,
+                                void *const *y, int masked, int flags, dkt_batch **out) {
+  return guarded([&] {
+    if (!plans || !x || !y || !out || nplans < 1) throw std::invalid_argument("null pointer or empty batch");
+    if (dtype != 0 && dtype != 1) throw std::invalid_argument("dtype must be 0 (f64) or 1 (f32)");
+    if (nrhs < 1 || nrhs > 64) throw std::invalid_argument("nrhs must be 1..64");
+    std::map<std::pair<int, int>, std::vector<int>> by_kind;
+    for (int k = 0; k < nplans; k++) {
+      dkt_plan *pl = plans[k];
+      if (!pl || !x[k] || !y[k]) throw std::invalid_argument("null plan or vector");
+      if (pl->dry || pl->simple || pl->per_level) throw std::invalid_argument("batch plans must be tiled device plans");
+      if (x[k] == (const void *)y[k]) throw std::invalid_argument("x and y must not alias");
+      by_kind[{pl->dim, pl->order}].push_back(k);
+    }
+    auto b = std::make_unique<dkt_batch>();
+    for (const auto &kv : by_kind) {
+      std::vector<dkt_plan *> pls;
+      std::vector<const void *> xs;
+      std::vector<void *> ys;
+      for (int k : kv.second) {
+        pls.push_back(plans[k]);
+        xs.push_back(x[k]);
+        ys.push_back(y[k]);
+      }
+      GroupBase *g = dtype == 0 ? make_group<double>(kv.first.first, kv.first.second, pls, xs, ys, nrhs, masked != 0)
+                                : make_group<float>(kv.first.first, kv.first.second, pls, xs, ys, nrhs, masked != 0);
+      b->groups.emplace_back(g);
+    }
+    b->use_graph = (flags & DKT_BATCH_GRAPH) != 0;
+    *out = b.release();
+  });
+}
+
+extern "C" int dkt_batch_run(dkt_batch *b, void *stream) {
+  return guarded([&] {
+    if (!b) throw std::invalid_argument("null pointer");
+    hipStream_t st = (hipStream_t)stream;
+    if (!b->use_graph) {
+      for (auto &g : b->groups) g->launch(st);
+      return;
+    }
+    if (!st) throw std::invalid_argument("graph replay needs a non-default stream");
+    if (!b->exec) {
+      HIP_OK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+      try {
+        for (auto &g : b->groups) g->launch(st);
+      } catch (...) {
+        hipGraph_t dropped = nullptr;
+        (void)hipStreamEndCapture(st, &dropped);
+        if (dropped) (void)hipGraphDestroy(dropped);
+        throw;
+      }
+      HIP_OK(hipStreamEndCapture(st, &b->graph));
+      HIP_OK(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+    }
+    HIP_OK(hipGraphLaunch(b->exec, st));
+  });
+}
+
+extern "C" int dkt_batch_destroy(dkt_batch *b) {
+  delete b;
+  return 0;
+}

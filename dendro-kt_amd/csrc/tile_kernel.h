@@ -135,15 +135,16 @@ __device__ __forceinline__ T interp_lanes(T a, int lane, int c) {
   return a;
 }
 
+// One tile of one plan (the workgroup may have more work-items than the
+// plan's tile_leaves; they idle in the leaf phases and help in the loops).
 template <typename T, int DIM, int P>
-__global__ void __launch_bounds__(kTileThreads) tile_kernel(TileArgs A, const T *__restrict__ kref,
-                                                            const T *__restrict__ carry_w, const T *__restrict__ x,
-                                                            T *__restrict__ y, T *__restrict__ partial) {
+__device__ __forceinline__ void tile_body(const TileArgs &A, int tile, const T *__restrict__ kref,
+                                         const T *__restrict__ carry_w, const T *__restrict__ x, T *__restrict__ y,
+                                         T *__restrict__ partial) {
   constexpr int NPE = Pow<P + 1, DIM>::v;
   constexpr int G = Group<NPE>::v;
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  if (A.cancel && *A.cancel) return;
-  const TileHdr th = A.tiles[blockIdx.x];
+  const TileHdr th = A.tiles[tile];
   const TileDims D = A.dims;
   const int nq = A.nq;
   const int TL = D.tile_leaves;
@@ -388,6 +389,44 @@ __global__ void __launch_bounds__(kTileThreads) tile_kernel(TileArgs A, const T 
   }
 }
 
+template <typename T, int DIM, int P>
+__global__ void __launch_bounds__(kTileThreads) tile_kernel(TileArgs A, const T *__restrict__ kref,
+                                                            const T *__restrict__ carry_w, const T *__restrict__ x,
+                                                            T *__restrict__ y, T *__restrict__ partial) {
+  if (A.cancel && *A.cancel) return;
+  tile_body<T, DIM, P>(A, blockIdx.x, kref, carry_w, x, y, partial);
+}
+
+// A plan of a batch: its maps (with nq and ld set for the batch) and vectors.
+template <typename T>
+struct BatchPlan {
+  TileArgs A;
+  const T *kref, *carry_w, *x;
+  T *y, *partial;
+  const int32_t *snode, *soff;
+  int nshared;
+};
+
+// Index of the last start <= v (starts ascending, starts[0] = 0).
+__device__ __forceinline__ int find_range(const int *__restrict__ starts, int n, int v) {
+  int lo = 0, hi = n;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (starts[mid] <= v) lo = mid; else hi = mid;
+  }
+  return lo;
+}
+
+// All tiles of a batch of same-(dim, p) plans in one launch: workgroup b runs
+// tile b - tile_start[k] of plan k.
+template <typename T, int DIM, int P>
+__global__ void __launch_bounds__(kTileThreads) tile_kernel_batch(const BatchPlan<T> *__restrict__ plans,
+                                                                  const int *__restrict__ tile_start, int nplans) {
+  const int k = find_range(tile_start, nplans, blockIdx.x);
+  const BatchPlan<T> &bp = plans[k];
+  tile_body<T, DIM, P>(bp.A, blockIdx.x - tile_start[k], bp.kref, bp.carry_w, bp.x, bp.y, bp.partial);
+}
+
 template <typename T>
 __global__ void __launch_bounds__(256) fixup_kernel(int ns, const int32_t *__restrict__ snode,
                                                     const int32_t *__restrict__ soff, const T *__restrict__ partial,
@@ -402,6 +441,25 @@ __global__ void __launch_bounds__(256) fixup_kernel(int ns, const int32_t *__res
     T acc = T(0);
     for (int k = a; k < b; k++) acc += partial[(size_t)k * nq + q];
     y[(size_t)n * ld + q] = z ? T(0) : acc;
+  }
+}
+
+// Shared nodes of a batch: workgroup b sums block b - block_start[k] of plan k.
+template <typename T>
+__global__ void __launch_bounds__(256) fixup_kernel_batch(const BatchPlan<T> *__restrict__ plans,
+                                                          const int *__restrict__ block_start, int nplans) {
+  const int k = find_range(block_start, nplans, blockIdx.x);
+  const BatchPlan<T> &bp = plans[k];
+  const int i = (blockIdx.x - block_start[k]) * 256 + threadIdx.x;
+  if (i >= bp.nshared) return;
+  const int nq = bp.A.nq, ld = bp.A.ld;
+  const int n = bp.snode[i];
+  const int a = bp.soff[i], b = bp.soff[i + 1];
+  const bool z = bp.A.dirichlet && bp.A.dirichlet[n];
+  for (int q = 0; q < nq; q++) {
+    T acc = T(0);
+    for (int s = a; s < b; s++) acc += bp.partial[(size_t)s * nq + q];
+    bp.y[(size_t)n * ld + q] = z ? T(0) : acc;
   }
 }
 

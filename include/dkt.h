@@ -159,6 +159,22 @@ int dkt_plan_matvec(dkt_plan *plan, int dtype, const void *x, void *y, int nrhs,
 int dkt_cg_solve(dkt_plan *plan, int mode, const double *b, double *x, int max_iter, double tol, int masked,
                  double *stats, void *stream);
 
+/* ---- Batches ----------------------------------------------------------- */
+/* One launch per (dim, p) over the tiles of many plans (independent meshes),
+ * each with its own device vectors x[k], y[k] of [N_k][nrhs] (dtype 0 f64,
+ * 1 f32), plus one batched fix-up launch.  flags DKT_BATCH_GRAPH: the launches
+ * are captured into a hipGraph on the first dkt_batch_run and replayed after
+ * (the stream must not be the default stream).  Plans and vectors are bound at
+ * creation and must outlive the batch.  Every y[k] equals dkt_plan_matvec of
+ * plan k bit for bit.  No counterpart in the reference (one fem::matvec per
+ * call, FEM/include/matvec.h:231-240). */
+typedef struct dkt_batch dkt_batch;
+#define DKT_BATCH_GRAPH 1
+int dkt_batch_create(dkt_plan *const *plans, int nplans, int dtype, int nrhs, const void *const *x, void *const *y,
+                     int masked, int flags, dkt_batch **out);
+int dkt_batch_run(dkt_batch *batch, void *stream);
+int dkt_batch_destroy(dkt_batch *batch);
+
 /* info[0..8] = nodes, leaves, hanging leaves, nodes per element, device map
  * bytes, tiles, shared nodes, partial slots, max nodes per tile;
  * info[9..18] = tile statistics of the tiled path: mean and max stale-buffer

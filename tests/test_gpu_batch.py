@@ -1,0 +1,82 @@
+"""Batched matvec (dkt_batch_*): many meshes of mixed (dim, p) in one launch per
+(dim, p), FP32 and FP64, several right-hand sides.  Every plan's result must be
+bitwise equal to its own dkt_plan_matvec; a captured hipGraph is replayed
+several times with eager work in between, and every replay is checked."""
+import numpy as np
+import pytest
+
+
+def _torch():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no HIP device")
+    return torch
+
+
+def _meshes(dkt, count, seed):
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(count):
+        dim = 3 if k % 4 else 2
+        c = rng.uniform(0.2, 0.8, dim)
+        r = rng.uniform(0.1, 0.3)
+        lev = int(rng.integers(3, 6))
+        p = 1 if k % 3 else 2
+        tree = dkt.sphere_tree(dim, 30, 2, lev, tuple(c), r)
+        da = dkt.DA(dkt.balance_tree(tree, 1), p)
+        out.append(da)
+    pts = dkt.gauss_points(4, 7, 300, 3, 0.08, seed)
+    out.append(dkt.DA(dkt.balance_tree(pts, 1), 1))
+    return out
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("dtype,nrhs", [("f32", 8), ("f64", 3)])
+def test_batch_matches_per_plan_matvec(dkt, dtype, nrhs):
+    torch = _torch()
+    dt = torch.float32 if dtype == "f32" else torch.float64
+    das = _meshes(dkt, 12, 5)
+    plans = [dkt.Plan(da, dkt.OP_STIFFNESS) for da in das]
+    xs, ys, refs = [], [], []
+    for k, (da, plan) in enumerate(zip(das, plans)):
+        x = torch.tensor(np.stack([dkt.keyed_uniform(k * 16 + q, da.nodes()) for q in range(nrhs)], axis=1),
+                         dtype=dt, device="cuda").contiguous()
+        ref = torch.empty_like(x)
+        plan.matvec_torch(x, ref, masked=True)
+        xs.append(x)
+        ys.append(torch.zeros_like(x))
+        refs.append(ref)
+    b = dkt.Batch(plans, xs, ys, masked=True)
+    b.run()
+    torch.cuda.synchronize()
+    for y, ref in zip(ys, refs):
+        assert torch.equal(y, ref)
+
+
+@pytest.mark.gpu
+def test_batch_graph_replays(dkt):
+    torch = _torch()
+    das = _meshes(dkt, 8, 9)
+    plans = [dkt.Plan(da, dkt.OP_STIFFNESS) for da in das]
+    xs = [torch.tensor(dkt.keyed_uniform(k, da.nodes()), dtype=torch.float32, device="cuda")
+          for k, da in enumerate(das)]
+    refs = []
+    for p, x in zip(plans, xs):
+        r = torch.empty_like(x)
+        p.matvec_torch(x, r)
+        refs.append(r)
+    ys = [torch.zeros_like(x) for x in xs]
+    s = torch.cuda.Stream()
+    b = dkt.Batch(plans, xs, ys, graph=True)
+    for rep in range(4):
+        with torch.cuda.stream(s):
+            for y in ys:
+                y.fill_(-1.0)
+            b.run(s.cuda_stream)
+        s.synchronize()
+        for y, ref in zip(ys, refs):
+            assert torch.equal(y, ref), rep
+        # eager work between replays
+        for p, x, y in zip(plans, xs, ys):
+            p.matvec_torch(x, y)
+        torch.cuda.synchronize()
