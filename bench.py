@@ -156,6 +156,52 @@ def cg_leg(dkt, torch, npoints, maxd):
     return out
 
 
+def batch_leg(dkt, torch, ntrees):
+    """Batched matvec: ntrees 3D trees refined around random spheres (centre U[0.2,0.8]^3,
+    radius U[0.1,0.3], max level 4-7: 1e3-1e5 elements), p = 1 and 2 alternating, 8
+    right-hand sides each, FP32, one hipGraph replay per step against the same plans
+    launched one by one."""
+    rng = np.random.default_rng(3)
+    das, elems = [], 0
+    t0 = time.time()
+    for k in range(ntrees):
+        c = tuple(rng.uniform(0.2, 0.8, 3))
+        r = float(rng.uniform(0.1, 0.3))
+        lev = int(rng.integers(4, 8))
+        p = 1 if k % 2 == 0 else 2
+        da = dkt.DA(dkt.balance_tree(dkt.sphere_tree(3, 30, 2, min(lev, 6 if p == 2 else 7), c, r), 1), p)
+        das.append(da)
+        elems += da.n_elements
+    plans = [dkt.Plan(da, dkt.OP_STIFFNESS) for da in das]
+    log(f"batch: {ntrees} trees, {elems} elements, built in {time.time() - t0:.1f}s")
+    nrhs = 8
+    xs = [torch.rand(da.n_nodes, nrhs, dtype=torch.float32, device="cuda") * 2 - 1 for da in das]
+    ys = [torch.empty_like(x) for x in xs]
+    dofs = sum(da.n_nodes for da in das) * nrhs
+    s = torch.cuda.Stream()
+    b = dkt.Batch(plans, xs, ys, graph=True)
+
+    def timed(fn, reps=20):
+        with torch.cuda.stream(s):
+            for _ in range(3):
+                fn()
+            s.synchronize()
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(s)
+            for _ in range(reps):
+                fn()
+            e1.record(s)
+        e1.synchronize()
+        return e0.elapsed_time(e1) / reps
+
+    ms_graph = timed(lambda: b.run(s.cuda_stream))
+    ms_each = timed(lambda: [p.matvec_torch(x, y, stream=s.cuda_stream) for p, x, y in zip(plans, xs, ys)])
+    return {"trees": ntrees, "elements": elems, "nodes": sum(da.n_nodes for da in das), "nrhs": nrhs, "dtype": "f32",
+            "ms_graph": round(ms_graph, 4), "ms_one_launch_per_tree": round(ms_each, 4),
+            "gdof_s_graph": round(dofs / (ms_graph * 1e-3) / 1e9, 3),
+            "gdof_s_one_launch_per_tree": round(dofs / (ms_each * 1e-3) / 1e9, 3)}
+
+
 def serve_leg(dkt, nreq):
     """Served CG: nreq requests over nreq/2 distinct 4D meshes (1e3-3e5 elements),
     4 HIP streams, tol 1e-8 (FP64 CG).  e2e latency includes the host mesh build and
@@ -210,6 +256,8 @@ def main():
     ap.add_argument("--no-serve", action="store_true", help="skip the served-CG leg")
     ap.add_argument("--no-cg", action="store_true", help="skip the FP64 vs mixed CG leg")
     ap.add_argument("--cg-points", type=int, default=20000)
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched multi-tree leg")
+    ap.add_argument("--batch-trees", type=int, default=64)
     ap.add_argument("--serve-requests", type=int, default=16)
     ap.add_argument("--calib", action="store_true",
                     help="run one 256 MiB elementwise copy (torch.neg) first: the reference dispatch that scales "
@@ -284,6 +332,7 @@ def main():
 
     cg = None if args.no_cg else cg_leg(dkt, torch, args.cg_points, args.maxdepth)
     serve = None if args.no_serve else serve_leg(dkt, args.serve_requests)
+    batch = None if args.no_batch else batch_leg(dkt, torch, args.batch_trees)
 
     cpu = None
     if not args.no_cpu:
@@ -319,6 +368,7 @@ def main():
                  "rel_l2_vs_fp64": rel32},
         "cg_1e-8": cg,
         "serve": serve,
+        "batch": batch,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
