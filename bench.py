@@ -203,43 +203,52 @@ def batch_leg(dkt, torch, ntrees):
 
 
 def serve_leg(dkt, nreq):
-    """Served CG: nreq requests over nreq/2 distinct 4D meshes (1e3-3e5 elements),
-    4 HIP streams, tol 1e-8 (FP64 CG).  e2e latency includes the host mesh build and
-    plan upload of a request whose mesh is not cached; solve latency runs from the
-    plan being available (rhs upload, CG, solution download)."""
+    """Served CG under Poisson arrivals: a pool of 10 meshes (70% 3D trees refined
+    around random spheres, 30% 4D Gaussian-mixture trees; p = 1; 1e3-5e5 elements),
+    requests alternating Poisson and heat-equation kinds (heat: b = M f with HeatVec's
+    hooks, then CG), tol 1e-8, 4 HIP streams, plans built before the timed runs.
+    Mixed-precision CG at 50/200/800 requests/s and FP64 CG at 200 requests/s;
+    latency runs from arrival to solution on the host."""
     import importlib.util as ilu
     spec = ilu.spec_from_file_location("dkt_serve", os.path.join(HERE, "dendro-kt_amd", "serve.py"))
     sv = ilu.module_from_spec(spec)
     spec.loader.exec_module(sv)
     rng = np.random.default_rng(0)
-    sizes = np.geomspace(150, 30000, max(1, nreq // 2)).astype(int)
-    meshes = [sv.MeshSpec(4, 9, int(n), 100 + i) for i, n in enumerate(sizes)]
-    reqs = [sv.Request(m, rhs_seed=k) for m in meshes for k in (1, 2)]
-    order = rng.permutation(len(reqs))
+    pool = []
+    for i in range(10):
+        if i % 10 < 7:
+            pool.append(sv.MeshSpec(3, 30, shape="sphere", center=tuple(float(c) for c in rng.uniform(0.2, 0.8, 3)),
+                                    radius=float(rng.uniform(0.1, 0.3)), max_level=int(rng.integers(5, 9))))
+        else:
+            pool.append(sv.MeshSpec(4, 9, int(np.exp(rng.uniform(np.log(200), np.log(60000)))), 100 + i))
     srv = sv.Server(dkt, n_streams=4)
-    log(f"serving {len(reqs)} CG requests")
-    t0 = time.perf_counter()
+    out = {"pool_meshes": len(pool), "streams": 4, "tol": 1e-8, "requests_per_rate": nreq}
     try:
-        for i in order:
-            srv.submit(reqs[i])
-        done = srv.drain(len(reqs), timeout=600)
+        t0 = time.perf_counter()
+        elems = []
+        for m in pool:
+            da, _, _, _ = srv.cache.get(m)
+            elems.append(da.n_elements)
+        out["plan_build_s"] = round(time.perf_counter() - t0, 2)
+        out["elements_min"], out["elements_max"] = int(min(elems)), int(max(elems))
+        runs = [(50, 1, "mixed"), (200, 1, "mixed"), (800, 1, "mixed"), (200, 0, "f64")]
+        for rate, mode, name in runs:
+            reqs = [sv.Request(pool[int(rng.integers(len(pool)))], rhs_seed=int(rng.integers(1 << 30)), mode=mode,
+                               kind="heat" if k % 2 else "poisson") for k in range(nreq)]
+            log(f"serving {nreq} requests at {rate}/s ({name})")
+            done, wall = sv.run_open_loop(srv, reqs, rate, seed=rate)
+            errors = [r.error for r in done if r.error]
+            if errors:
+                out[f"{name}_{rate}"] = {"errors": errors[:3]}
+                continue
+            e2e = [(r.finished - r.submitted) * 1e3 for r in done]
+            out[f"{name}_{rate}_per_s"] = {
+                "solves_per_s": round(len(done) / wall, 2),
+                "e2e_ms": {k: round(v, 1) for k, v in sv.percentiles(e2e).items()},
+                "max_resid": max(r.stats["resid"] for r in done)}
     finally:
         srv.close()
-    wall = time.perf_counter() - t0
-    errors = [r.error for r in done if r.error]
-    if errors:
-        return {"errors": errors[:3]}
-    e2e = [(r.finished - r.submitted) * 1e3 for r in done]
-    solve = [(r.finished - r.solve_started) * 1e3 for r in done]
-    elems = []
-    for m in meshes:
-        pts = dkt.gauss_points(m.dim, m.maxdepth, m.npoints, m.ncomp, m.sigma, m.seed)
-        elems.append(len(dkt.balance_tree(pts, 1)))
-    return {"requests": len(done), "streams": 4, "tol": 1e-8, "mode": "f64",
-            "elements_min": int(min(elems)), "elements_max": int(max(elems)),
-            "e2e_ms": {k: round(v, 1) for k, v in sv.percentiles(e2e).items()},
-            "solve_ms": {k: round(v, 1) for k, v in sv.percentiles(solve).items()},
-            "max_resid": max(r.stats["resid"] for r in done), "wall_s": round(wall, 2)}
+    return out
 
 
 def main():
@@ -258,7 +267,7 @@ def main():
     ap.add_argument("--cg-points", type=int, default=20000)
     ap.add_argument("--no-batch", action="store_true", help="skip the batched multi-tree leg")
     ap.add_argument("--batch-trees", type=int, default=64)
-    ap.add_argument("--serve-requests", type=int, default=16)
+    ap.add_argument("--serve-requests", type=int, default=40)
     ap.add_argument("--calib", action="store_true",
                     help="run one 256 MiB elementwise copy (torch.neg) first: the reference dispatch that scales "
                          "FETCH_SIZE / WRITE_SIZE counters to bytes")

@@ -266,6 +266,7 @@ class Plan:
 
     ``matvec(x_ptr, y_ptr, nrhs, dtype, masked, stream)`` takes raw device
     pointers; ``matvec_torch`` is a convenience for torch tensors (plumbing only).
+    masked: 0 raw, 1 HeatMat's Dirichlet hooks (x and y), 2 HeatVec's (y only).
     """
 
     def __init__(self, da, op=OP_STIFFNESS, op_data=None, simple=False, tile_leaves=0):
@@ -306,7 +307,7 @@ class Plan:
 
     def matvec(self, x_ptr, y_ptr, nrhs=1, dtype=0, masked=False, stream=None):
         _chk(load().dkt_plan_matvec(self.h, dtype, ctypes.c_void_p(x_ptr), ctypes.c_void_p(y_ptr), nrhs,
-                                    1 if masked else 0, ctypes.c_void_p(stream or 0)))
+                                    int(masked), ctypes.c_void_p(stream or 0)))
 
     def matvec_torch(self, x, y, masked=False, stream=None):
         import torch
@@ -372,7 +373,7 @@ class Batch:
         xp = (P * n)(*[x.data_ptr() for x in xs])
         yp = (P * n)(*[y.data_ptr() for y in ys])
         h = P()
-        _chk(L.dkt_batch_create(ph, n, 0 if dt == torch.float64 else 1, nrhs, xp, yp, 1 if masked else 0,
+        _chk(L.dkt_batch_create(ph, n, 0 if dt == torch.float64 else 1, nrhs, xp, yp, int(masked),
                                 BATCH_GRAPH if graph else 0, ctypes.byref(h)))
         self.h = h
 

@@ -39,7 +39,7 @@ struct Group : GroupBase {
   std::vector<size_t> lds;
 
   Group(const std::vector<dkt_plan *> &pls, const std::vector<const void *> &xs, const std::vector<void *> &ys,
-        int nrhs, bool masked) {
+        int nrhs, int masked) {
     constexpr int NPE = Pow<P + 1, DIM>::v;
     const bool f64 = sizeof(T) == 8;
     auto dims_of = [](const dkt_plan *pl) {
@@ -80,7 +80,8 @@ struct Group : GroupBase {
         A.readers = pl->readers.p;
         A.blocks = pl->blocks.p;
         A.carry_key = pl->carry_key.p;
-        A.dirichlet = masked ? pl->dirichlet.p : nullptr;
+        A.dir_in = masked == 1 ? pl->dirichlet.p : nullptr;
+        A.dir_out = masked ? pl->dirichlet.p : nullptr;
         A.cancel = nullptr;
         A.op = pl->op;
         A.hp = pl->hp;
@@ -122,7 +123,7 @@ struct Group : GroupBase {
 
 template <typename T>
 GroupBase *make_group(int dim, int p, const std::vector<dkt_plan *> &pls, const std::vector<const void *> &xs,
-                      const std::vector<void *> &ys, int nrhs, bool masked) {
+                      const std::vector<void *> &ys, int nrhs, int masked) {
   if (dim == 2 && p == 1) return new Group<T, 2, 1>(pls, xs, ys, nrhs, masked);
   if (dim == 3 && p == 1) return new Group<T, 3, 1>(pls, xs, ys, nrhs, masked);
   if (dim == 4 && p == 1) return new Group<T, 4, 1>(pls, xs, ys, nrhs, masked);
@@ -150,6 +151,7 @@ extern "C" int dkt_batch_create(dkt_plan *const *plans, int nplans, int dtype, i
     if (!plans || !x || !y || !out || nplans < 1) throw std::invalid_argument("null pointer or empty batch");
     if (dtype != 0 && dtype != 1) throw std::invalid_argument("dtype must be 0 (f64) or 1 (f32)");
     if (nrhs < 1 || nrhs > 64) throw std::invalid_argument("nrhs must be 1..64");
+    if (masked < 0 || masked > 2) throw std::invalid_argument("masked must be 0, 1 or 2");
     std::map<std::pair<int, int>, std::vector<int>> by_kind;
     for (int k = 0; k < nplans; k++) {
       dkt_plan *pl = plans[k];
@@ -168,8 +170,8 @@ extern "C" int dkt_batch_create(dkt_plan *const *plans, int nplans, int dtype, i
         xs.push_back(x[k]);
         ys.push_back(y[k]);
       }
-      GroupBase *g = dtype == 0 ? make_group<double>(kv.first.first, kv.first.second, pls, xs, ys, nrhs, masked != 0)
-                                : make_group<float>(kv.first.first, kv.first.second, pls, xs, ys, nrhs, masked != 0);
+      GroupBase *g = dtype == 0 ? make_group<double>(kv.first.first, kv.first.second, pls, xs, ys, nrhs, masked)
+                                : make_group<float>(kv.first.first, kv.first.second, pls, xs, ys, nrhs, masked);
       b->groups.emplace_back(g);
     }
     b->use_graph = (flags & DKT_BATCH_GRAPH) != 0;

@@ -237,7 +237,7 @@ struct CgVectors {
 // Runs up to max_iter CG iterations on A x = b in precision T from the current x.
 // Returns the iterations performed; the final relative residual is scal[S_RESID].
 template <typename T>
-int cg_run(dkt_plan *pl, int dtype, const T *b, T *x, bool x_is_zero, int max_iter, double tol, bool masked,
+int cg_run(dkt_plan *pl, int dtype, const T *b, T *x, bool x_is_zero, int max_iter, double tol, int masked,
            const CgVectors<T> &v, int *flag, int *host, hipStream_t st, int *matvecs) {
   const int n = (int)pl->N;
   const dim3 grid(kCgBlocks), blk(kCgThreads);
@@ -329,13 +329,13 @@ extern "C" int dkt_cg_solve(dkt_plan *pl, int mode, const double *b, double *x, 
     double resid = 0.0;
     if (!mixed) {
       CgVectors<double> v{w.r.p, w.p.p, w.ap.p, w.part64.p, w.scal.p};
-      iters = cg_run<double>(pl, 0, b, x, false, max_iter, tol, masked != 0, v, w.flag.p, w.host, st, &matvecs);
+      iters = cg_run<double>(pl, 0, b, x, false, max_iter, tol, masked ? 1 : 0, v, w.flag.p, w.host, st, &matvecs);
       resid = read_scalar(w.scal.p, S_RESID, st);
     } else {
       const int n = (int)pl->N;
       CgVectors<float> v{w.r32.p, w.p32.p, w.ap32.p, w.part32.p, w.scal32.p};  // FP64 reductions
       while (true) {
-        plan_apply(pl, 0, x, w.ap.p, 1, masked != 0, st, nullptr);
+        plan_apply(pl, 0, x, w.ap.p, 1, masked ? 1 : 0, st, nullptr);
         matvecs++;
         hipLaunchKernelGGL(k_resid64, dim3(kCgBlocks), dim3(kCgThreads), 0, st, n, b, w.ap.p, w.d64.p, w.part64.p);
         hipLaunchKernelGGL(k_resid64_fin, dim3(1), dim3(kCgThreads), 0, st, w.part64.p, w.scal.p);
@@ -345,7 +345,7 @@ extern "C" int dkt_cg_solve(dkt_plan *pl, int mode, const double *b, double *x, 
         hipLaunchKernelGGL(k_to32, dim3(kCgBlocks), dim3(kCgThreads), 0, st, n, w.d64.p, w.scal.p, w.b32.p, w.x32.p);
         // Inner tolerance: kInnerTol, loosened when the outer target is already close.
         const double inner_tol = std::min(0.5, std::max(kInnerTol, 0.5 * tol / resid));
-        const int inner = cg_run<float>(pl, 1, w.b32.p, w.x32.p, true, max_iter - iters, inner_tol, masked != 0, v,
+        const int inner = cg_run<float>(pl, 1, w.b32.p, w.x32.p, true, max_iter - iters, inner_tol, masked ? 1 : 0, v,
                                         w.flag.p, w.host, st, &matvecs);
         hipLaunchKernelGGL(k_add32, dim3(kCgBlocks), dim3(kCgThreads), 0, st, n, x, w.x32.p, w.scal.p);
         HIP_OK(hipGetLastError());

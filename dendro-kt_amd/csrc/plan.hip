@@ -58,7 +58,7 @@ __global__ void __launch_bounds__(256) leaf_kernel(
     int L, int nrhs, const T *__restrict__ x, const int32_t *__restrict__ slot, const uint8_t *__restrict__ level,
     const int32_t *__restrict__ hang_of, const uint8_t *__restrict__ child, const int32_t *__restrict__ pslot,
     const int32_t *__restrict__ hc_off, const uint8_t *__restrict__ hc_rank, const int32_t *__restrict__ hc_node,
-    const T *__restrict__ hc_w, const T *__restrict__ kmat, int kmat_per_level, const uint8_t *__restrict__ dirichlet,
+    const T *__restrict__ hc_w, const T *__restrict__ kmat, int kmat_per_level, const uint8_t *__restrict__ dir_in,
     int hp, T *__restrict__ vout, T *__restrict__ wout, const int *__restrict__ cancel) {
   constexpr int NPE = Pow<P + 1, DIM>::v;
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
@@ -71,7 +71,7 @@ __global__ void __launch_bounds__(256) leaf_kernel(
   const int c = child[e];
   const T scale = kmat_per_level ? T(1) : (T)ldexp(1.0, -lev * hp);
   const T *K = kmat + (kmat_per_level ? (size_t)lev * NPE * NPE : 0);
-  auto xv = [&](int n, int q) -> T { return (dirichlet && dirichlet[n]) ? T(0) : x[(size_t)n * nrhs + q]; };
+  auto xv = [&](int n, int q) -> T { return (dir_in && dir_in[n]) ? T(0) : x[(size_t)n * nrhs + q]; };
   for (int q = 0; q < nrhs; q++) {
     T u[NPE];
     if (h >= 0) {
@@ -625,7 +625,7 @@ void build_tiled(dkt_plan *pl, const MeshParams &mp, const Traversal &tr, int re
 // Launch.
 // ---------------------------------------------------------------------------
 template <typename T, int DIM, int P>
-void run_simple(dkt_plan *pl, int nrhs, const T *x, T *y, bool masked, hipStream_t st, const int *cancel) {
+void run_simple(dkt_plan *pl, int nrhs, const T *x, T *y, int mask, hipStream_t st, const int *cancel) {
   constexpr int NPE = Pow<P + 1, DIM>::v;
   const bool f64 = sizeof(T) == 8;
   const T *kmat = f64 ? (const T *)pl->kmat64.p : (const T *)pl->kmat32.p;
@@ -639,22 +639,23 @@ void run_simple(dkt_plan *pl, int nrhs, const T *x, T *y, bool masked, hipStream
     if (pl->scratch32.n < need) pl->scratch32.alloc(need);
     buf = (T *)pl->scratch32.p;
   }
-  const uint8_t *dir = masked ? pl->dirichlet.p : nullptr;
+  const uint8_t *dir_in = mask == 1 ? pl->dirichlet.p : nullptr;
+  const uint8_t *dir_out = mask ? pl->dirichlet.p : nullptr;
   const int blk = 256;
   if (pl->L) {
     hipLaunchKernelGGL((leaf_kernel<T, DIM, P>), dim3((pl->L + blk - 1) / blk), dim3(blk), 0, st, (int)pl->L, nrhs, x,
                        pl->slot.p, pl->level.p, pl->hang_of.p, pl->child.p, pl->pslot.p, pl->hc_off.p,
-                       pl->hc_rank.p, pl->hc_node.p, hcw, kmat, pl->per_level ? 1 : 0, dir, pl->hp, buf,
+                       pl->hc_rank.p, pl->hc_node.p, hcw, kmat, pl->per_level ? 1 : 0, dir_in, pl->hp, buf,
                        buf + pl->L * (size_t)NPE * nrhs, cancel);
     HIP_OK(hipGetLastError());
   }
   hipLaunchKernelGGL((node_kernel<T>), dim3((pl->N + blk - 1) / blk), dim3(blk), 0, st, (int)pl->N, nrhs,
-                     pl->csr_off.p, pl->csr_src.p, buf, dir, y, cancel);
+                     pl->csr_off.p, pl->csr_src.p, buf, dir_out, y, cancel);
   HIP_OK(hipGetLastError());
 }
 
 template <typename T, int DIM, int P>
-void run_tiled(dkt_plan *pl, int nrhs, const T *x, T *y, bool masked, hipStream_t st, const int *cancel) {
+void run_tiled(dkt_plan *pl, int nrhs, const T *x, T *y, int mask, hipStream_t st, const int *cancel) {
   constexpr int NPE = Pow<P + 1, DIM>::v;
   const bool f64 = sizeof(T) == 8;
   const T *kmat = f64 ? (const T *)pl->kmat64.p : (const T *)pl->kmat32.p;
@@ -686,7 +687,8 @@ void run_tiled(dkt_plan *pl, int nrhs, const T *x, T *y, bool masked, hipStream_
   A.hentries = pl->hentries.p;
   A.readers = pl->readers.p;
   A.carry_key = pl->carry_key.p;
-  A.dirichlet = masked ? pl->dirichlet.p : nullptr;
+  A.dir_in = mask == 1 ? pl->dirichlet.p : nullptr;
+  A.dir_out = mask ? pl->dirichlet.p : nullptr;
   A.cancel = cancel;
   A.op = pl->op;
   A.hp = pl->hp;
@@ -706,7 +708,7 @@ void run_tiled(dkt_plan *pl, int nrhs, const T *x, T *y, bool masked, hipStream_
     if (pl->nshared) {
       const int blk = 256;
       hipLaunchKernelGGL((fixup_kernel<T>), dim3((pl->nshared + blk - 1) / blk), dim3(blk), 0, st,
-                         (int)pl->nshared, pl->snode.p, pl->soff.p, part, A.dirichlet, nq, nrhs, y + q0, cancel);
+                         (int)pl->nshared, pl->snode.p, pl->soff.p, part, A.dir_out, nq, nrhs, y + q0, cancel);
       HIP_OK(hipGetLastError());
     }
     if (pl->profiling) HIP_OK(hipEventRecord(pl->next_event(), st));
@@ -714,14 +716,14 @@ void run_tiled(dkt_plan *pl, int nrhs, const T *x, T *y, bool masked, hipStream_
 }
 
 template <typename T>
-void run_matvec(dkt_plan *pl, int nrhs, const T *x, T *y, bool masked, hipStream_t st, const int *cancel) {
+void run_matvec(dkt_plan *pl, int nrhs, const T *x, T *y, int mask, hipStream_t st, const int *cancel) {
   const bool simple = pl->simple || pl->per_level;
 #define DKT_DISPATCH(D, P)                             \
   if (pl->dim == D && pl->order == P) {                \
     if (simple)                                        \
-      run_simple<T, D, P>(pl, nrhs, x, y, masked, st, cancel); \
+      run_simple<T, D, P>(pl, nrhs, x, y, mask, st, cancel); \
     else                                               \
-      run_tiled<T, D, P>(pl, nrhs, x, y, masked, st, cancel);  \
+      run_tiled<T, D, P>(pl, nrhs, x, y, mask, st, cancel);  \
     return;                                            \
   }
   DKT_DISPATCH(2, 1)
@@ -736,12 +738,13 @@ void run_matvec(dkt_plan *pl, int nrhs, const T *x, T *y, bool masked, hipStream
 }  // namespace
 
 namespace dkt {
-void plan_apply(dkt_plan *pl, int dtype, const void *x, void *y, int nrhs, bool masked, hipStream_t st,
+void plan_apply(dkt_plan *pl, int dtype, const void *x, void *y, int nrhs, int mask, hipStream_t st,
                 const int *cancel) {
+  if (mask < 0 || mask > 2) throw std::invalid_argument("masked must be 0, 1 or 2");
   if (dtype == 0)
-    run_matvec<double>(pl, nrhs, (const double *)x, (double *)y, masked, st, cancel);
+    run_matvec<double>(pl, nrhs, (const double *)x, (double *)y, mask, st, cancel);
   else if (dtype == 1)
-    run_matvec<float>(pl, nrhs, (const float *)x, (float *)y, masked, st, cancel);
+    run_matvec<float>(pl, nrhs, (const float *)x, (float *)y, mask, st, cancel);
   else
     throw std::invalid_argument("dtype must be 0 (f64) or 1 (f32)");
 }
@@ -838,7 +841,7 @@ extern "C" int dkt_plan_matvec(dkt_plan *pl, int dtype, const void *x, void *y, 
     if (pl->dry) throw std::invalid_argument("a dry-run plan cannot run a matvec");
     if (nrhs < 1 || nrhs > 64) throw std::invalid_argument("nrhs must be 1..64");
     if (x == y) throw std::invalid_argument("x and y must not alias");
-    plan_apply(pl, dtype, x, y, nrhs, masked != 0, (hipStream_t)stream, nullptr);
+    plan_apply(pl, dtype, x, y, nrhs, masked, (hipStream_t)stream, nullptr);
   });
 }
 

@@ -195,7 +195,8 @@ struct TileArgs {
   const uint16_t *readers;
   const uint64_t *blocks;
   const uint32_t *carry_key;  // buffer rank (8) << 16 | local node (16)
-  const uint8_t *dirichlet;
+  const uint8_t *dir_in;   // non-null: x reads as zero at boundary nodes
+  const uint8_t *dir_out;  // non-null: y is zero at boundary nodes
   const int *cancel;  // non-null: the tile returns at once when *cancel != 0
   int op;  // DKT_OP_STIFFNESS, DKT_OP_MASS or DKT_OP_IDENTITY
   int hp, nq, ld;

@@ -79,8 +79,8 @@ struct dkt_plan {
 };
 
 namespace dkt {
-// y = A x on `st` (dtype 0 f64, 1 f32).  When `cancel` is non-null and *cancel
+// y = A x on `st` (dtype 0 f64, 1 f32; mask as dkt_plan_matvec's `masked`).  When `cancel` is non-null and *cancel
 // is nonzero at launch time, the kernels return without touching y.
-void plan_apply(dkt_plan *pl, int dtype, const void *x, void *y, int nrhs, bool masked, hipStream_t st,
+void plan_apply(dkt_plan *pl, int dtype, const void *x, void *y, int nrhs, int mask, hipStream_t st,
                 const int *cancel);
 }  // namespace dkt

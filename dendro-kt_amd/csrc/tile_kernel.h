@@ -197,7 +197,7 @@ __device__ __forceinline__ void tile_body(const TileArgs &A, int tile, const T *
     }
     bool z[kNB];
 #pragma unroll
-    for (int b = 0; b < kNB; b++) z[b] = g[b] >= 0 && A.dirichlet && A.dirichlet[g[b]];
+    for (int b = 0; b < kNB; b++) z[b] = g[b] >= 0 && A.dir_in && A.dir_in[g[b]];
     for (int q = 0; q < nq; q++) {
       T xv[kNB];
 #pragma unroll
@@ -373,7 +373,7 @@ __device__ __forceinline__ void tile_body(const TileArgs &A, int tile, const T *
     }
     bool z[kNB];
 #pragma unroll
-    for (int b = 0; b < kNB; b++) z[b] = o[b] >= 0 && A.dirichlet && A.dirichlet[o[b]];
+    for (int b = 0; b < kNB; b++) z[b] = o[b] >= 0 && A.dir_out && A.dir_out[o[b]];
 #pragma unroll
     for (int b = 0; b < kNB; b++) {
       const int j = j0 + b * nthr;
@@ -455,7 +455,7 @@ __global__ void __launch_bounds__(256) fixup_kernel_batch(const BatchPlan<T> *__
   const int nq = bp.A.nq, ld = bp.A.ld;
   const int n = bp.snode[i];
   const int a = bp.soff[i], b = bp.soff[i + 1];
-  const bool z = bp.A.dirichlet && bp.A.dirichlet[n];
+  const bool z = bp.A.dir_out && bp.A.dir_out[n];
   for (int q = 0; q < nq; q++) {
     T acc = T(0);
     for (int s = a; s < b; s++) acc += bp.partial[(size_t)s * nq + q];

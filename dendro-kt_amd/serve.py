@@ -1,9 +1,16 @@
 """Request serving for the device CG (no counterpart in the reference, whose
 solves are single-shot calls of HeatMat::cgSolve, FEM/examples/src/heatMat.cpp:165).
 
-A request names a mesh (a seeded point cloud that the host refines, balances
-and enumerates, as ot::DA(tree, ...) does) and a right-hand side; the server
-answers with x = A^-1 b to a relative L_inf residual.  Design:
+A request names a mesh (a seeded Gaussian point cloud, or a tree refined around
+a sphere, that the host refines, balances and enumerates, as ot::DA(tree, ...)
+does), a kind and a right-hand side; the server answers with x = A^-1 b to a
+relative L_inf residual.  Kinds:
+* "poisson": b is given per node (zero on the boundary), A the
+  Dirichlet-masked stiffness (HeatMat::cgSolve);
+* "heat": the reference's heat-equation driver (FEM/examples/src/heatEq.cpp):
+  b = M f by the mass operator with HeatVec's hooks (feVector::computeVec,
+  FEM/include/feVector.h:122-168), then the same CG on the stiffness.
+Design:
 
 * plan cache: mesh -> (DA, device plan), keyed by the mesh spec, with LRU
   eviction when the estimated device footprint exceeds the HBM budget
@@ -28,13 +35,27 @@ import numpy as np
 
 @dataclass(frozen=True)
 class MeshSpec:
+    """shape "gauss": npoints Gaussian-mixture points (seed), one per leaf, 2:1 balanced;
+    shape "sphere": cells above level 2 split everywhere and up to max_level where they
+    meet the sphere (center, radius), 2:1 balanced."""
     dim: int
     maxdepth: int
-    npoints: int
-    seed: int
+    npoints: int = 0
+    seed: int = 0
     order: int = 1
     ncomp: int = 4
     sigma: float = 0.05
+    shape: str = "gauss"
+    center: tuple = ()
+    radius: float = 0.0
+    max_level: int = 0
+
+    def tree(self, dkt):
+        if self.shape == "sphere":
+            return dkt.balance_tree(dkt.sphere_tree(self.dim, self.maxdepth, 2, self.max_level, self.center,
+                                                    self.radius), 1)
+        pts = dkt.gauss_points(self.dim, self.maxdepth, self.npoints, self.ncomp, self.sigma, self.seed)
+        return dkt.balance_tree(pts, 1)
 
 
 @dataclass
@@ -44,6 +65,7 @@ class Request:
     tol: float = 1e-8
     mode: int = 0
     max_iter: int = 100000
+    kind: str = "poisson"
     # filled in by the server
     submitted: float = 0.0
     started: float = 0.0
@@ -60,23 +82,24 @@ class PlanCache:
         self.dkt = dkt
         self.budget = budget_bytes
         self.lock = threading.Lock()
-        self.entries = collections.OrderedDict()  # spec -> (da, plan, bytes, lock)
+        self.entries = collections.OrderedDict()  # spec -> (da, plans, bytes, lock)
         self.building = {}
 
     @staticmethod
-    def footprint(plan, n_nodes):
-        # device maps + CG workspace (FP64 and FP32 vectors) + the request's b and x
-        return plan.info()["map_bytes"] + n_nodes * (8 * 6 + 4 * 5)
+    def footprint(plans, n_nodes):
+        # device maps (stiffness and mass plans) + CG workspace (FP64 and FP32
+        # vectors) + the request's f, b and x
+        return sum(p.info()["map_bytes"] for p in plans) + n_nodes * (8 * 7 + 4 * 5)
 
     def get(self, spec):
-        """(da, plan, plan_lock, built_now).  Builds on a miss; concurrent misses on
-        one spec wait for a single build."""
+        """(da, (stiffness plan, mass plan), plan_lock, built_now).  Builds on a miss;
+        concurrent misses on one spec wait for a single build."""
         while True:
             with self.lock:
                 if spec in self.entries:
                     self.entries.move_to_end(spec)
-                    da, plan, _, lk = self.entries[spec]
-                    return da, plan, lk, False
+                    da, plans, _, lk = self.entries[spec]
+                    return da, plans, lk, False
                 ev = self.building.get(spec)
                 if ev is None:
                     ev = threading.Event()
@@ -89,10 +112,9 @@ class PlanCache:
                 continue
             try:
                 dkt = self.dkt
-                pts = dkt.gauss_points(spec.dim, spec.maxdepth, spec.npoints, spec.ncomp, spec.sigma, spec.seed)
-                da = dkt.DA(dkt.balance_tree(pts, 1), spec.order)
-                plan = dkt.Plan(da, dkt.OP_STIFFNESS)
-                nbytes = self.footprint(plan, da.n_nodes)
+                da = dkt.DA(spec.tree(dkt), spec.order)
+                plans = (dkt.Plan(da, dkt.OP_STIFFNESS), dkt.Plan(da, dkt.OP_MASS))
+                nbytes = self.footprint(plans, da.n_nodes)
                 if nbytes > self.budget:
                     raise MemoryError(f"plan needs {nbytes / 1e9:.1f} GB, budget {self.budget / 1e9:.1f} GB")
                 with self.lock:
@@ -100,8 +122,8 @@ class PlanCache:
                     while self.entries and used + nbytes > self.budget:
                         _, old = self.entries.popitem(last=False)
                         used -= old[2]
-                    self.entries[spec] = (da, plan, nbytes, threading.Lock())
-                    return da, plan, self.entries[spec][3], True
+                    self.entries[spec] = (da, plans, nbytes, threading.Lock())
+                    return da, plans, self.entries[spec][3], True
             finally:
                 with self.lock:
                     self.building.pop(spec, None)
@@ -135,15 +157,23 @@ class Server:
                 return
             try:
                 req.started = time.perf_counter()
-                da, plan, lk, built = self.cache.get(req.mesh)
+                da, (plan, mass), lk, built = self.cache.get(req.mesh)
                 req.plan_built = built
                 req.solve_started = time.perf_counter()
-                b = self.dkt.keyed_uniform(req.rhs_seed, da.nodes())
-                b[da.boundary()] = 0.0
+                f = self.dkt.keyed_uniform(req.rhs_seed, da.nodes())
+                if req.kind == "poisson":
+                    f[da.boundary()] = 0.0
+                elif req.kind != "heat":
+                    raise ValueError(f"unknown request kind {req.kind}")
                 with torch.cuda.stream(stream):
-                    bd = torch.tensor(b, device="cuda")
-                    xd = torch.zeros_like(bd)
+                    fd = torch.tensor(f, device="cuda")
+                    xd = torch.zeros_like(fd)
                     with lk:
+                        if req.kind == "heat":
+                            bd = torch.empty_like(fd)
+                            mass.matvec_torch(fd, bd, masked=2, stream=stream.cuda_stream)
+                        else:
+                            bd = fd
                         req.stats = plan.cg_solve_torch(bd, xd, req.mode, req.max_iter, req.tol,
                                                         stream=stream.cuda_stream)
                     req.x = xd.cpu().numpy()
@@ -169,6 +199,30 @@ class Server:
             self.q.put(None)
         for t in self.workers:
             t.join()
+
+
+def run_open_loop(server, reqs, rate, seed=0, timeout=900):
+    """Submits reqs at Poisson arrival times of the given rate (requests/s) from a
+    separate thread while the server works; returns (done requests, wall seconds
+    from the first arrival to the last completion)."""
+    rng = np.random.default_rng(seed)
+    gaps = rng.exponential(1.0 / rate, len(reqs))
+    n0 = len(server.done)
+    t0 = time.perf_counter()
+
+    def feed():
+        t = t0
+        for r, g in zip(reqs, gaps):
+            t += g
+            d = t - time.perf_counter()
+            if d > 0:
+                time.sleep(d)
+            server.submit(r)
+    th = threading.Thread(target=feed, daemon=True)
+    th.start()
+    done = server.drain(n0 + len(reqs), timeout=timeout)[n0:]
+    th.join()
+    return done, max(r.finished for r in done) - t0
 
 
 def percentiles(values, ps=(50, 99)):

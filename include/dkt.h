@@ -136,7 +136,10 @@ int dkt_plan_destroy(dkt_plan *plan);
  * aliased.  masked = 0: fem::matvec (FEM/include/matvec.h:231-240);
  * masked = 1: feMatrix::matVec with HeatMat's Dirichlet hooks
  * (FEM/include/feMatrix.h:189-259, FEM/examples/src/heatMat.cpp:120-139):
- * boundary entries of x are read as zero and boundary entries of y are zero. */
+ * boundary entries of x are read as zero and boundary entries of y are zero;
+ * masked = 2: feVector::computeVec with HeatVec's hooks
+ * (FEM/include/feVector.h:122-168, FEM/examples/src/heatVec.cpp:78-97):
+ * boundary entries of y are zero (x is read as given). */
 int dkt_plan_matvec(dkt_plan *plan, int dtype, const void *x, void *y, int nrhs, int masked, void *stream);
 
 /* ---- Solver ------------------------------------------------------------ */

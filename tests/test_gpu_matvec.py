@@ -83,3 +83,23 @@ def test_repeated_matvec_is_deterministic(dkt):
         ys.append(yd)
     torch.cuda.synchronize()
     assert torch.equal(ys[0], ys[1]) and torch.equal(ys[1], ys[2])
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", ["ex1_2d_p1", "gauss4d_small"])
+def test_output_only_mask_is_heatvec_compute_vec(dkt, name):
+    """masked = 2 (HeatVec::computeVec's hooks, FEM/examples/src/heatVec.cpp:78-97): the
+    raw mass matvec with the boundary rows zeroed, against the reference's raw y."""
+    torch = _torch()
+    da, z, meta = fx.build_da(dkt, name)
+    nodes = da.nodes()
+    x = fx.make_x("hash1", nodes, meta["dim"])
+    ref = z["y_mass_hash1_raw"].copy()
+    ref[da.boundary()] = 0.0
+    plan = dkt.Plan(da, dkt.OP_MASS)
+    xd = torch.tensor(x, device="cuda")
+    yd = torch.empty_like(xd)
+    plan.matvec_torch(xd, yd, masked=2)
+    torch.cuda.synchronize()
+    y = yd.cpu().numpy()
+    assert np.linalg.norm(y - ref) <= 1e-12 * np.linalg.norm(ref)

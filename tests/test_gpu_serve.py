@@ -1,7 +1,9 @@
-"""Serving harness: concurrent CG requests over cached plans on several HIP
-streams.  A request's answer must be bitwise the same whether it is served
-alone or in a mix (fixed-order device reductions), every request must reach
-its tolerance, and a plan larger than the HBM budget is refused."""
+"""Serving harness: concurrent CG requests (Poisson and heat-equation kinds) over
+cached plans on several HIP streams.  A request's answer must be bitwise the
+same whether it is served alone or in a mix (fixed-order device reductions),
+every request must reach its tolerance, a heat request's right-hand side must
+be the output-masked mass matvec, and a plan larger than the HBM budget is
+refused."""
 import importlib.util
 import os
 import sys
@@ -34,7 +36,8 @@ def test_alone_equals_in_mix(dkt):
     _torch()
     sv = _serve_mod()
     meshes = [sv.MeshSpec(4, 8, n, s) for n, s in ((300, 1), (800, 2), (1500, 3))]
-    reqs = [sv.Request(m, rhs_seed=k) for m in meshes for k in (1, 2)]
+    meshes.append(sv.MeshSpec(3, 30, shape="sphere", center=(0.4, 0.5, 0.6), radius=0.2, max_level=5))
+    reqs = [sv.Request(m, rhs_seed=k, kind=kind) for m in meshes for k, kind in ((1, "poisson"), (2, "heat"))]
     srv = sv.Server(dkt, n_streams=3)
     try:
         for r in reqs:
@@ -47,7 +50,7 @@ def test_alone_equals_in_mix(dkt):
     alone = sv.Server(dkt, n_streams=1)
     try:
         for r in reqs:
-            solo = sv.Request(r.mesh, r.rhs_seed)
+            solo = sv.Request(r.mesh, r.rhs_seed, kind=r.kind)
             alone.submit(solo)
             out = alone.drain(reqs.index(r) + 1, timeout=300)[-1]
             assert not out.error
@@ -67,3 +70,30 @@ def test_admission_refuses_plans_over_budget(dkt):
     finally:
         srv.close()
     assert r.error.startswith("MemoryError")
+
+
+@pytest.mark.gpu
+def test_heat_request_solves_mass_rhs(dkt):
+    """A heat request solves K x = b with b = M f (boundary rows zeroed): check the
+    residual with the device operators, both themselves checked against the reference."""
+    torch = _torch()
+    sv = _serve_mod()
+    spec = sv.MeshSpec(3, 30, shape="sphere", center=(0.5, 0.5, 0.5), radius=0.25, max_level=5)
+    srv = sv.Server(dkt, n_streams=2)
+    try:
+        r = sv.Request(spec, rhs_seed=7, kind="heat")
+        srv.submit(r)
+        (r,) = srv.drain(1, timeout=300)
+    finally:
+        srv.close()
+    assert not r.error, r.error
+    da = dkt.DA(spec.tree(dkt), 1)
+    k_plan, m_plan = dkt.Plan(da, dkt.OP_STIFFNESS), dkt.Plan(da, dkt.OP_MASS)
+    f = torch.tensor(dkt.keyed_uniform(7, da.nodes()), device="cuda")
+    b = torch.empty_like(f)
+    m_plan.matvec_torch(f, b, masked=2)
+    x = torch.tensor(r.x, device="cuda")
+    ax = torch.empty_like(x)
+    k_plan.matvec_torch(x, ax, masked=1)
+    torch.cuda.synchronize()
+    assert float((b - ax).abs().max() / b.abs().max()) <= 1.01 * r.tol
