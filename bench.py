@@ -156,6 +156,34 @@ def cg_leg(dkt, torch, npoints, maxd):
     return out
 
 
+def cg_p2_leg(dkt, torch, level):
+    """3D p=2 Poisson CG (Dirichlet zero, b keyed uniform seed 1) on the sphere tree
+    (r = 0.3 at the centre, base level 3, refined to `level`), FP64 CG vs
+    mixed-precision refinement, both to 1e-8 relative L_inf residual; error of the
+    mixed solution against the FP64 one."""
+    t0 = time.time()
+    da = dkt.DA(dkt.balance_tree(dkt.sphere_tree(3, 30, 3, level, (0.5, 0.5, 0.5), 0.3), 1), 2)
+    plan = dkt.Plan(da, dkt.OP_STIFFNESS)
+    build = time.time() - t0
+    b = dkt.keyed_uniform(1, da.nodes())
+    b[da.boundary()] = 0.0
+    bd = torch.tensor(b, device="cuda")
+    out = {"level": level, "elements": da.n_elements, "nodes": da.n_nodes, "mesh_build_s": round(build, 1)}
+    xs = {}
+    for mode, name in ((dkt.CG_F64, "f64"), (dkt.CG_MIXED, "mixed_f32_f64")):
+        xc = torch.zeros_like(bd)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        st = plan.cg_solve_torch(bd, xc, mode, 100000, 1e-8)
+        torch.cuda.synchronize()
+        out[name] = dict(st, ms=round((time.perf_counter() - t) * 1e3, 1))
+        xs[name] = xc
+        log(f"CG p=2 {name}: {out[name]}")
+    out["mixed_vs_f64_rel_l2"] = float(torch.linalg.norm(xs["mixed_f32_f64"] - xs["f64"]) / torch.linalg.norm(xs["f64"]))
+    out["speedup_mixed"] = round(out["f64"]["ms"] / out["mixed_f32_f64"]["ms"], 3)
+    return out
+
+
 def batch_leg(dkt, torch, ntrees):
     """Batched matvec: ntrees 3D trees refined around random spheres (centre U[0.2,0.8]^3,
     radius U[0.1,0.3], max level 4-7: 1e3-1e5 elements), p = 1 and 2 alternating, 8
@@ -265,6 +293,8 @@ def main():
     ap.add_argument("--no-serve", action="store_true", help="skip the served-CG leg")
     ap.add_argument("--no-cg", action="store_true", help="skip the FP64 vs mixed CG leg")
     ap.add_argument("--cg-points", type=int, default=20000)
+    ap.add_argument("--p2-level", type=int, default=7,
+                    help="max level of the 3D p=2 sphere CG leg (the full workload uses 9)")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched multi-tree leg")
     ap.add_argument("--batch-trees", type=int, default=64)
     ap.add_argument("--serve-requests", type=int, default=40)
@@ -340,6 +370,7 @@ def main():
     gdof32 = N / (ms32 * 1e-3) / 1e9
 
     cg = None if args.no_cg else cg_leg(dkt, torch, args.cg_points, args.maxdepth)
+    cg_p2 = None if args.no_cg else cg_p2_leg(dkt, torch, args.p2_level)
     serve = None if args.no_serve else serve_leg(dkt, args.serve_requests)
     batch = None if args.no_batch else batch_leg(dkt, torch, args.batch_trees)
 
@@ -376,6 +407,7 @@ def main():
         "fp32": {"gdof_s": round(gdof32, 4), "ms_per_step": round(ms32, 4), "hbm_gbs": round(gbs32, 1),
                  "rel_l2_vs_fp64": rel32},
         "cg_1e-8": cg,
+        "cg_p2_1e-8": cg_p2,
         "serve": serve,
         "batch": batch,
         "cpu_baseline": cpu,

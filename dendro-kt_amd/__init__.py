@@ -29,6 +29,7 @@ OP_MASS = 1
 OP_IDENTITY = 2
 OP_CAPTURED = 3
 PLAN_SIMPLE = 1
+PLAN_CONFORMING = 4
 PLAN_TILE_SHIFT = 8
 CG_F64 = 0
 CG_MIXED = 1
@@ -90,6 +91,8 @@ def load(path=None):
                                    c.POINTER(c.c_int32), c.POINTER(c.c_int32)]
     L.dkt_da_stale_terms.argtypes = [P, c.POINTER(c.c_int64), c.POINTER(c.c_uint8), c.POINTER(c.c_int32),
                                      c.POINTER(c.c_double)]
+    L.dkt_da_conforming_terms.argtypes = [P, c.POINTER(c.c_int64), c.POINTER(c.c_uint8), c.POINTER(c.c_int32),
+                                          c.POINTER(c.c_double)]
     _bind_device(L)
     _lib = L
     return L
@@ -255,6 +258,21 @@ class DA:
         return dict(cells=cells, child=child, slot=slot, hang_of=hang, pslot=pslot,
                     hc_off=off, hc_rank=hrank, hc_node=hnode, hc_w=hw)
 
+    def conforming_terms(self):
+        """Conforming constraints: dict(off, rank, node, w), one row of off per mesh element
+        (in element order) that has ranks at points which are not nodes."""
+        c = ctypes
+        H, nt = self.count(9), self.count(8)
+        off = np.empty(H + 1, dtype=np.int64)
+        rank = np.empty(nt, dtype=np.uint8)
+        node = np.empty(nt, dtype=np.int32)
+        w = np.empty(nt, dtype=np.float64)
+        _chk(load().dkt_da_conforming_terms(self.h, off.ctypes.data_as(c.POINTER(c.c_int64)),
+                                            rank.ctypes.data_as(c.POINTER(c.c_uint8)),
+                                            node.ctypes.data_as(c.POINTER(c.c_int32)),
+                                            w.ctypes.data_as(c.POINTER(c.c_double))))
+        return dict(off=off, rank=rank, node=node, w=w)
+
     def __del__(self):
         if getattr(self, "h", None) and _lib is not None:
             _lib.dkt_da_destroy(self.h)
@@ -269,7 +287,7 @@ class Plan:
     masked: 0 raw, 1 HeatMat's Dirichlet hooks (x and y), 2 HeatVec's (y only).
     """
 
-    def __init__(self, da, op=OP_STIFFNESS, op_data=None, simple=False, tile_leaves=0):
+    def __init__(self, da, op=OP_STIFFNESS, op_data=None, simple=False, tile_leaves=0, conforming=False):
         L = load()
         if not hasattr(L, "dkt_plan_create"):
             raise DktError("libdkt.so was built without the device plan")
@@ -280,7 +298,7 @@ class Plan:
         if op_data is not None:
             self._op_data = np.ascontiguousarray(op_data, dtype=np.float64)
             ptr = self._op_data.ctypes.data_as(ctypes.c_void_p)
-        flags = PLAN_SIMPLE if simple else 0
+        flags = (PLAN_SIMPLE if simple else 0) | (PLAN_CONFORMING if conforming else 0)
         if tile_leaves:
             flags |= (int(tile_leaves).bit_length() - 1) << PLAN_TILE_SHIFT
         _chk(L.dkt_plan_create(da.h, op, ptr, flags, ctypes.byref(h)))

@@ -156,7 +156,8 @@ extern "C" int dkt_batch_create(dkt_plan *const *plans, int nplans, int dtype, i
     for (int k = 0; k < nplans; k++) {
       dkt_plan *pl = plans[k];
       if (!pl || !x[k] || !y[k]) throw std::invalid_argument("null plan or vector");
-      if (pl->dry || pl->simple || pl->per_level) throw std::invalid_argument("batch plans must be tiled device plans");
+      if (pl->dry || pl->simple || pl->per_level || pl->conforming)
+        throw std::invalid_argument("batch plans must be tiled device plans");
       if (x[k] == (const void *)y[k]) throw std::invalid_argument("x and y must not alias");
       by_kind[{pl->dim, pl->order}].push_back(k);
     }

@@ -1,5 +1,6 @@
 // C ABI over the host mesh layer (include/dkt.h).
 #include <cstring>
+#include <memory>
 #include <exception>
 #include <string>
 
@@ -25,6 +26,11 @@ struct dkt_da {
   std::vector<Oct> elems;
   NodeSet ns;
   Traversal tr;
+  std::unique_ptr<Traversal> conf;  // conforming element records, built on first use
+  const Traversal &conforming() const {
+    if (!conf) const_cast<dkt_da *>(this)->conf = std::make_unique<Traversal>(build_conforming(mp, elems, ns));
+    return *conf;
+  }
 };
 
 static int check_dims(int dim, int maxdepth) {
@@ -184,6 +190,8 @@ extern "C" size_t dkt_da_count(const dkt_da *da, int what) {
     case 5: return (size_t)da->tr.nPe;
     case 6: compute_stale_terms(da->mp, const_cast<dkt_da *>(da)->tr); return da->tr.stale_needed;
     case 7: compute_stale_terms(da->mp, const_cast<dkt_da *>(da)->tr); return da->tr.hc_node.size();
+    case 8: return da->conforming().cf_node.size();
+    case 9: return da->conforming().cf_off.size() - 1;
     default: return 0;
   }
 }
@@ -238,10 +246,22 @@ extern "C" int dkt_da_stale_terms(const dkt_da *da, int64_t *off, uint8_t *rank,
   });
 }
 
+extern "C" int dkt_da_conforming_terms(const dkt_da *da, int64_t *off, uint8_t *rank, int32_t *node, double *w) {
+  return guarded([&] {
+    if (!da) throw std::invalid_argument("null pointer");
+    const Traversal &t = da->conforming();
+    if (off) std::memcpy(off, t.cf_off.data(), t.cf_off.size() * sizeof(int64_t));
+    if (rank) std::memcpy(rank, t.cf_rank.data(), t.cf_rank.size());
+    if (node) std::memcpy(node, t.cf_node.data(), t.cf_node.size() * sizeof(int32_t));
+    if (w) std::memcpy(w, t.cf_w.data(), t.cf_w.size() * sizeof(double));
+  });
+}
+
 // Internal accessors for the device plan (plan.cpp).
 namespace dkt {
 const MeshParams &da_params(const dkt_da *da) { return da->mp; }
 const NodeSet &da_nodes(const dkt_da *da) { return da->ns; }
 const Traversal &da_traversal(const dkt_da *da) { return da->tr; }
 Traversal &da_traversal_mut(const dkt_da *da) { return const_cast<dkt_da *>(da)->tr; }
+Traversal &da_conforming_mut(const dkt_da *da) { return const_cast<Traversal &>(da->conforming()); }
 }  // namespace dkt

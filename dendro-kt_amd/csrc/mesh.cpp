@@ -799,6 +799,118 @@ void compute_stale_terms(const MeshParams &mp, Traversal &tr) {
   tr.hc_ready = true;
 }
 
+Traversal build_conforming(const MeshParams &mp, const std::vector<Oct> &elems, const NodeSet &ns) {
+  const int dim = mp.dim, p = mp.order, n1 = p + 1;
+  int npe = 1;
+  for (int d = 0; d < dim; d++) npe *= n1;
+  Traversal tr;
+  tr.nPe = npe;
+  using Key = std::array<u32, 5>;  // coordinates, level (level 0xFFFFFFFF: node lookup)
+  struct KeyHash {
+    size_t operator()(const Key &k) const {
+      u64 h = 0x9E3779B97F4A7C15ull;
+      for (u32 v : k) h = (h ^ v) * 0xBF58476D1CE4E5B9ull;
+      return (size_t)(h ^ (h >> 31));
+    }
+  };
+  std::unordered_map<Key, int32_t, KeyHash> at;
+  at.reserve(ns.nodes.size() * 2);
+  for (size_t i = 0; i < ns.nodes.size(); i++) {
+    Key k{0, 0, 0, 0, 0xFFFFFFFFu};
+    for (int d = 0; d < dim; d++) k[d] = ns.nodes[i].x[d];
+    at.emplace(k, (int32_t)i);
+  }
+  double ip[2][9];
+  interp_1d(p, 0, ip[0]);
+  interp_1d(p, 1, ip[1]);
+  std::unordered_map<Key, Combo, KeyHash> memo;
+  // Value at point x, a node position of the cell (anchor a, level lev).
+  std::function<const Combo &(const u32 *, const u32 *, int)> resolve = [&](const u32 *x, const u32 *a,
+                                                                            int lev) -> const Combo & {
+    Key key{0, 0, 0, 0, (u32)lev};
+    for (int d = 0; d < dim; d++) key[d] = x[d];
+    auto m = memo.find(key);
+    if (m != memo.end()) return m->second;
+    Combo v;
+    Key nk = key;
+    nk[4] = 0xFFFFFFFFu;
+    auto f = at.find(nk);
+    if (f != at.end()) {
+      v.push_back({f->second, 1.0});
+    } else {
+      if (lev == 0) throw std::runtime_error("conforming constraint: a root node is missing");
+      const u32 len = 1u << (mp.maxd - lev), step = len / (u32)p;
+      const u32 plen = len << 1, pstep = plen / (u32)p;
+      u32 pa[4] = {0, 0, 0, 0};
+      int c[4] = {0, 0, 0, 0}, t[4] = {0, 0, 0, 0};
+      for (int d = 0; d < dim; d++) {
+        pa[d] = a[d] & ~(plen - 1);
+        c[d] = (int)((a[d] - pa[d]) / len);
+        t[d] = (int)((x[d] - a[d]) / step);
+      }
+      int pn = 1;
+      for (int d = 0; d < dim; d++) pn *= n1;
+      for (int u = 0; u < pn; u++) {
+        double w = 1.0;
+        u32 y[4] = {0, 0, 0, 0};
+        int uu = u;
+        for (int d = 0; d < dim; d++) {
+          const int ud = uu % n1;
+          uu /= n1;
+          w *= ip[c[d]][ud * n1 + t[d]];
+          y[d] = pa[d] + (u32)ud * pstep;
+        }
+        if (w == 0.0) continue;
+        add_scaled(v, resolve(y, pa, lev - 1), w);
+      }
+      compact(v);
+    }
+    return memo.emplace(key, std::move(v)).first->second;
+  };
+  // One record per mesh element (SFC order): the node at each rank where the
+  // point is a node, else the constraint terms.
+  tr.leaves = elems;
+  const size_t E = elems.size();
+  tr.child.assign(E, 0);
+  tr.slot.assign(E * npe, -1);
+  tr.hang_of.assign(E, -1);
+  tr.cf_off.assign(1, 0);
+  for (size_t e = 0; e < E; e++) {
+    const Oct &cell = elems[e];
+    const u32 len = 1u << (mp.maxd - cell.lev), step = len / (u32)p;
+    int c = 0;
+    for (int d = 0; d < dim; d++) c |= (int)((cell.x[d] / len) & 1u) << d;
+    tr.child[e] = (uint8_t)c;
+    bool missing = false;
+    for (int r = 0; r < npe; r++) {
+      Key k{0, 0, 0, 0, 0xFFFFFFFFu};
+      int rr = r;
+      for (int d = 0; d < dim; d++) {
+        k[d] = cell.x[d] + (u32)(rr % n1) * step;
+        rr /= n1;
+      }
+      auto f = at.find(k);
+      if (f != at.end()) {
+        tr.slot[e * npe + r] = f->second;
+        continue;
+      }
+      missing = true;
+      for (const auto &tw : resolve(k.data(), cell.x, (int)cell.lev)) {
+        tr.cf_rank.push_back((uint8_t)r);
+        tr.cf_node.push_back(tw.first);
+        tr.cf_w.push_back(tw.second);
+      }
+    }
+    if (missing) {
+      tr.hang_of[e] = (int32_t)(tr.cf_off.size() - 1);
+      tr.cf_off.push_back((int64_t)tr.cf_node.size());
+    }
+  }
+  tr.pslot.assign((tr.cf_off.size() - 1) * (size_t)npe, -1);
+  tr.cf_ready = true;
+  return tr;
+}
+
 Traversal flatten_traversal(const MeshParams &mp, const std::vector<Oct> &nodes) {
   Traversal tr;
   int npe = 1;

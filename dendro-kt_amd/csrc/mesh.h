@@ -87,6 +87,13 @@ struct Traversal {
   std::vector<double> hc_w;
   size_t stale_needed = 0;        // hanging leaves that read a stale parent-buffer entry
   bool hc_ready = false;          // hc_* computed (compute_stale_terms)
+  // Conforming constraints (build_conforming): for record h and k in
+  // [cf_off[h], cf_off[h+1]), missing rank cf_rank[k] takes cf_w[k] * x[cf_node[k]].
+  std::vector<int64_t> cf_off;
+  std::vector<uint8_t> cf_rank;
+  std::vector<int32_t> cf_node;
+  std::vector<double> cf_w;
+  bool cf_ready = false;
 };
 Traversal flatten_traversal(const MeshParams &mp, const std::vector<Oct> &nodes);
 
@@ -113,6 +120,17 @@ class StaleBuffer {
 
 // Per-leaf stale terms hc_* (all hanging leaves); computed on demand.
 void compute_stale_terms(const MeshParams &mp, Traversal &tr);
+
+// The conforming element list: one record per mesh element (not per reference
+// traversal leaf: the reference traversal leaves part of the domain
+// unvisited), with the node at each rank where the point is a node and, for
+// the other ranks, constraint terms cf_*: the value at a point that is not a
+// node is the parent cell's interpolant there, taken recursively through
+// coarser cells until every point is a node (the continuous piecewise-
+// polynomial space on the 2:1 balanced tree).  The operator
+// sum_e C_e^T K_e C_e is symmetric.  Not the reference's semantics (its
+// static parent buffer, FEM/include/matvec.h:403-457); used for solves at scale.
+Traversal build_conforming(const MeshParams &mp, const std::vector<Oct> &elems, const NodeSet &ns);
 
 // Integer-exact synthetic inputs (also restated in numpy by the tests).
 std::vector<Oct> sphere_tree(const MeshParams &mp, int base_level, int max_level, const double *center,

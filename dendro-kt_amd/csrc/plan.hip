@@ -32,6 +32,7 @@ namespace dkt {
 const MeshParams &da_params(const dkt_da *da);
 const NodeSet &da_nodes(const dkt_da *da);
 Traversal &da_traversal_mut(const dkt_da *da);
+Traversal &da_conforming_mut(const dkt_da *da);
 thread_local bool g_dry_run = false;
 }  // namespace dkt
 
@@ -59,7 +60,7 @@ __global__ void __launch_bounds__(256) leaf_kernel(
     const int32_t *__restrict__ hang_of, const uint8_t *__restrict__ child, const int32_t *__restrict__ pslot,
     const int32_t *__restrict__ hc_off, const uint8_t *__restrict__ hc_rank, const int32_t *__restrict__ hc_node,
     const T *__restrict__ hc_w, const T *__restrict__ kmat, int kmat_per_level, const uint8_t *__restrict__ dir_in,
-    int hp, T *__restrict__ vout, T *__restrict__ wout, const int *__restrict__ cancel) {
+    int hp, int conforming, T *__restrict__ vout, T *__restrict__ wout, const int *__restrict__ cancel) {
   constexpr int NPE = Pow<P + 1, DIM>::v;
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= L || (cancel && *cancel)) return;
@@ -105,7 +106,16 @@ __global__ void __launch_bounds__(256) leaf_kernel(
     }
 #pragma unroll
     for (int r = 0; r < NPE; r++) vout[((size_t)e * NPE + r) * nrhs + q] = v[r];
-    if (h >= 0) {
+    if (h >= 0 && conforming) {  // transposed constraint terms
+      for (int k = hc_off[h]; k < hc_off[h + 1]; k++) {
+        const int r = hc_rank[k];
+        T vr = T(0);
+#pragma unroll
+        for (int rr = 0; rr < NPE; rr++)
+          if (rr == r) vr = v[rr];
+        wout[(size_t)k * nrhs + q] = hc_w[k] * vr;
+      }
+    } else if (h >= 0) {
       T w[NPE];
 #pragma unroll
       for (int r = 0; r < NPE; r++) w[r] = s[r] < 0 ? v[r] : T(0);
@@ -185,32 +195,49 @@ namespace {
 // ---------------------------------------------------------------------------
 // Simple path construction.
 // ---------------------------------------------------------------------------
+// tr: the reference traversal, or for conforming plans the conforming element
+// records (build_conforming).
 void build_simple(dkt_plan *pl, const MeshParams &mp, Traversal &tr) {
   const int npe = pl->npe;
-  compute_stale_terms(mp, tr);
+  const bool conf = pl->conforming;
+  if (!conf) compute_stale_terms(mp, tr);
+  const std::vector<int64_t> &t_off = conf ? tr.cf_off : tr.hc_off;
+  const std::vector<uint8_t> &t_rank = conf ? tr.cf_rank : tr.hc_rank;
+  const std::vector<int32_t> &t_node = conf ? tr.cf_node : tr.hc_node;
+  const std::vector<double> &t_w = conf ? tr.cf_w : tr.hc_w;
   std::vector<uint8_t> lev(pl->L);
   for (size_t e = 0; e < pl->L; e++) lev[e] = (uint8_t)tr.leaves[e].lev;
   pl->slot.upload(tr.slot);
   pl->level.upload(lev);
   pl->hang_of.upload(tr.hang_of);
   pl->child.upload(tr.child);
-  pl->pslot.upload(tr.pslot.empty() ? std::vector<int32_t>(npe, -1) : tr.pslot);
-  pl->hc_off.upload(std::vector<int32_t>(tr.hc_off.begin(), tr.hc_off.end()));
-  pl->hc_rank.upload(nonempty(tr.hc_rank));
-  pl->hc_node.upload(nonempty(tr.hc_node));
-  pl->hc_w64.upload(nonempty(tr.hc_w));
-  pl->hc_w32.upload(cast<float>(nonempty(tr.hc_w)));
+  // Conforming: the constraint terms carry the whole value of a missing node,
+  // so no parent-level interpolation is added.
+  pl->pslot.upload(tr.pslot.empty() || conf ? std::vector<int32_t>(std::max<size_t>(tr.pslot.size(), npe), -1)
+                                            : tr.pslot);
+  pl->hc_off.upload(std::vector<int32_t>(t_off.begin(), t_off.end()));
+  pl->hc_rank.upload(nonempty(t_rank));
+  pl->hc_node.upload(nonempty(t_node));
+  pl->hc_w64.upload(nonempty(t_w));
+  pl->hc_w32.upload(cast<float>(nonempty(t_w)));
+  if ((pl->L + pl->H) * (size_t)npe + t_node.size() >= (size_t)INT32_MAX)
+    throw std::invalid_argument("mesh too large for 32-bit contribution indices");
   // Node-side CSR over contribution positions, in leaf SFC order then rank:
-  // direct outputs at e*nPe+r, hanging transposes at L*nPe + h*nPe + s.
+  // direct outputs at e*nPe+r; hanging transposes at L*nPe + h*nPe + s, or
+  // (conforming) the transposed constraint term k at L*nPe + k.
   std::vector<int64_t> cnt(pl->N + 1, 0);
   for (size_t e = 0; e < pl->L; e++) {
     const int32_t *sl = &tr.slot[e * npe];
     for (int r = 0; r < npe; r++)
       if (sl[r] >= 0) cnt[sl[r] + 1]++;
     const int h = tr.hang_of[e];
-    if (h >= 0)
+    if (h < 0) continue;
+    if (conf) {
+      for (int64_t k = t_off[h]; k < t_off[h + 1]; k++) cnt[t_node[k] + 1]++;
+    } else {
       for (int s = 0; s < npe; s++)
         if (tr.pslot[(size_t)h * npe + s] >= 0 && sl[s] < 0) cnt[tr.pslot[(size_t)h * npe + s] + 1]++;
+    }
   }
   for (size_t n = 0; n < pl->N; n++) cnt[n + 1] += cnt[n];
   std::vector<int32_t> src(cnt[pl->N]);
@@ -220,15 +247,19 @@ void build_simple(dkt_plan *pl, const MeshParams &mp, Traversal &tr) {
     for (int r = 0; r < npe; r++)
       if (sl[r] >= 0) src[fill[sl[r]]++] = (int32_t)(e * npe + r);
     const int h = tr.hang_of[e];
-    if (h >= 0)
+    if (h < 0) continue;
+    if (conf) {
+      for (int64_t k = t_off[h]; k < t_off[h + 1]; k++) src[fill[t_node[k]]++] = (int32_t)(pl->L * npe + k);
+    } else {
       for (int s = 0; s < npe; s++) {
         const int32_t n = tr.pslot[(size_t)h * npe + s];
         if (n >= 0 && sl[s] < 0) src[fill[n]++] = (int32_t)(pl->L * npe + (size_t)h * npe + s);
       }
+    }
   }
   pl->csr_off.upload(cnt);
   pl->csr_src.upload(src);
-  pl->nbuf = (pl->L + pl->H) * (size_t)npe;
+  pl->nbuf = conf ? pl->L * (size_t)npe + t_node.size() : (pl->L + pl->H) * (size_t)npe;
 }
 
 // ---------------------------------------------------------------------------
@@ -645,7 +676,8 @@ void run_simple(dkt_plan *pl, int nrhs, const T *x, T *y, int mask, hipStream_t 
   if (pl->L) {
     hipLaunchKernelGGL((leaf_kernel<T, DIM, P>), dim3((pl->L + blk - 1) / blk), dim3(blk), 0, st, (int)pl->L, nrhs, x,
                        pl->slot.p, pl->level.p, pl->hang_of.p, pl->child.p, pl->pslot.p, pl->hc_off.p,
-                       pl->hc_rank.p, pl->hc_node.p, hcw, kmat, pl->per_level ? 1 : 0, dir_in, pl->hp, buf,
+                       pl->hc_rank.p, pl->hc_node.p, hcw, kmat, pl->per_level ? 1 : 0, dir_in, pl->hp,
+                       pl->conforming ? 1 : 0, buf,
                        buf + pl->L * (size_t)NPE * nrhs, cancel);
     HIP_OK(hipGetLastError());
   }
@@ -717,7 +749,7 @@ void run_tiled(dkt_plan *pl, int nrhs, const T *x, T *y, int mask, hipStream_t s
 
 template <typename T>
 void run_matvec(dkt_plan *pl, int nrhs, const T *x, T *y, int mask, hipStream_t st, const int *cancel) {
-  const bool simple = pl->simple || pl->per_level;
+  const bool simple = pl->simple || pl->per_level || pl->conforming;
 #define DKT_DISPATCH(D, P)                             \
   if (pl->dim == D && pl->order == P) {                \
     if (simple)                                        \
@@ -755,7 +787,8 @@ extern "C" int dkt_plan_create(const dkt_da *da, int op, const double *op_data, 
     if (!da || !out) throw std::invalid_argument("null pointer");
     const MeshParams &mp = da_params(da);
     const NodeSet &ns = da_nodes(da);
-    Traversal &tr = da_traversal_mut(da);
+    const bool conforming = (flags & DKT_PLAN_CONFORMING) != 0;
+    Traversal &tr = conforming ? da_conforming_mut(da) : da_traversal_mut(da);
     auto pl = std::make_unique<dkt_plan>();
     pl->dim = mp.dim;
     pl->order = mp.order;
@@ -767,6 +800,7 @@ extern "C" int dkt_plan_create(const dkt_da *da, int op, const double *op_data, 
     pl->H = tr.pslot.size() / tr.nPe;
     pl->simple = (flags & DKT_PLAN_SIMPLE) != 0;
     pl->dry = (flags & DKT_PLAN_DRYRUN) != 0;
+    pl->conforming = conforming;
     struct DryGuard {
       bool prev;
       explicit DryGuard(bool on) : prev(g_dry_run) { g_dry_run = on; }
@@ -793,7 +827,7 @@ extern "C" int dkt_plan_create(const dkt_da *da, int op, const double *op_data, 
     pl->dirichlet.upload(dir);
     const int tshift = (flags >> DKT_PLAN_TILE_SHIFT) & 15;
     if (tshift && (tshift < 6 || tshift > 8)) throw std::invalid_argument("tile size must be 64, 128 or 256 leaves");
-    if (pl->simple || pl->per_level)
+    if (pl->simple || pl->per_level || pl->conforming)
       build_simple(pl.get(), mp, tr);
     else
       build_tiled(pl.get(), mp, tr, tshift ? 1 << tshift : 0);

@@ -30,7 +30,7 @@ struct CgWorkspace {
 struct dkt_plan {
   int dim = 0, order = 0, npe = 0, op = 0, hp = 0, maxd = 0;
   size_t N = 0, L = 0, H = 0;
-  bool per_level = false, simple = false, dry = false;
+  bool per_level = false, simple = false, dry = false, conforming = false;
   dkt::DevBuf<uint8_t> dirichlet;
   dkt::DevBuf<double> kmat64;
   dkt::DevBuf<float> kmat32;

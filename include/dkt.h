@@ -43,6 +43,10 @@ extern "C" {
  * run a matvec. */
 #define DKT_PLAN_SIMPLE 1
 #define DKT_PLAN_DRYRUN 2
+/* The conforming operator sum_e C_e^T K_e C_e (symmetric; see
+ * dkt_da_conforming_terms) instead of the reference's traversal semantics;
+ * runs on the per-leaf/per-node kernels. */
+#define DKT_PLAN_CONFORMING 4
 /* Leaves per tile of the tiled path: flags |= log2(leaves) << DKT_PLAN_TILE_SHIFT
  * with leaves in {64, 128, 256}; 0 = 128, halved while the LDS footprint does
  * not fit.  Tiles also end early at 32 stale-buffer readers. */
@@ -94,7 +98,8 @@ int dkt_da_create(const dkt_octs *tree, int order, dkt_da **out);
 int dkt_da_destroy(dkt_da *da);
 /* Counts: 0 elements, 1 nodes (DA::getLocalNodalSz), 2 boundary nodes,
  * 3 traversal leaves, 4 hanging leaves, 5 nodes per element, 6 hanging
- * leaves that read a stale parent-buffer entry, 7 stale-term entries. */
+ * leaves that read a stale parent-buffer entry, 7 stale-term entries,
+ * 8 conforming-constraint entries, 9 elements with constrained ranks. */
 size_t dkt_da_count(const dkt_da *da, int what);
 /* uint32 [N][dim+1] node coords + level (DA::getTNCoords, oda.h:252). */
 int dkt_da_nodes(const dkt_da *da, uint32_t *out);
@@ -117,6 +122,16 @@ int dkt_da_traversal(const dkt_da *da, uint32_t *leaf_cells, uint8_t *child, int
  * and k in [off[h], off[h+1]), missing rank rank[k] adds w[k] * x[node[k]].
  * off: int64 [H+1]; rank: uint8, node: int32, w: double [count 7]. */
 int dkt_da_stale_terms(const dkt_da *da, int64_t *off, uint8_t *rank, int32_t *node, double *w);
+
+/* Conforming hanging-node constraints (this build's operator for solves at
+ * scale; the reference has none -- its static-buffer interpolation above is
+ * neither symmetric nor conforming, and its traversal leaves part of the
+ * domain unvisited).  Over the mesh elements in order: the h-th element that
+ * has ranks at points which are not nodes owns k in [off[h], off[h+1]), where
+ * rank rank[k] takes w[k] * x[node[k]]; the value at such a point is the
+ * parent cell's interpolant there, recursively through coarser cells.
+ * off: int64 [count 9 + 1]; rank, node, w: [count 8]. */
+int dkt_da_conforming_terms(const dkt_da *da, int64_t *off, uint8_t *rank, int32_t *node, double *w);
 
 /* ---- Device plan and matvec -------------------------------------------- */
 /* Uploads the flattened traversal of `da` to the current HIP device.
