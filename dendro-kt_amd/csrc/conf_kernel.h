@@ -1,0 +1,183 @@
+// Tiled matvec of the conforming operator sum_e C_e^T K_e C_e (mesh elements,
+// hanging-node constraint terms from build_conforming; no counterpart in the
+// reference, whose traversal semantics are tile_kernel's).
+//
+// A tile holds SFC-consecutive elements (one per work-item) and two kinds of
+// local points: the mesh nodes its elements touch, and its hanging points
+// (element node positions that are not mesh nodes, one per position and
+// element level), each with its row of constraint terms over the tile's nodes.
+// Every element rank then reads a local point, hanging or not:
+//   1. stage x of the tile's nodes; a hanging point's value is its row applied
+//      to them (terms in row order);
+//   2. per element: u[r] = value at its local point of rank r, v = 2^(-level*hp) K_ref u,
+//      v[r] added to that point's accumulator, one phase per rank;
+//   3. transposed rows: each node adds sum_k w_k * acc[hanging point of term k]
+//      over its terms in an order fixed by the plan (terms sorted by node);
+//   4. write-out as tile_kernel (owned nodes to y, shared ones to partial slots
+//      summed by fixup_kernel).
+// Every sum runs in an order fixed by the plan: bitwise reproducible.
+#pragma once
+#include "plan_common.h"
+
+namespace dkt {
+
+struct ConfTileHdr {
+  int32_t leaf0, nleaf;  // elements
+  int32_t node0, nnode;  // mesh nodes (tile_nodes / tile_out); local points [0, nnode)
+  int32_t v0, nv;        // hanging points (virt_off), local points [nnode, nnode + nv)
+  int32_t t0, nt;        // constraint terms of the hanging points' rows
+};
+
+struct ConfArgs {
+  const ConfTileHdr *tiles;
+  const int32_t *tile_nodes, *tile_out;
+  const uint16_t *leaf_loc;   // [E][nPe] local point of every rank
+  const uint8_t *leaf_lev;    // [E] element level
+  const uint16_t *virt_off;   // [hanging point] tile-relative first term of its row
+  const uint16_t *term_node;  // tile-local node per term
+  const uint16_t *term_virt;  // hanging point (0-based within the tile) per term
+  const uint16_t *term_perm;  // tile-relative term indices sorted by node
+  const uint8_t *dir_in, *dir_out;
+  const int *cancel;
+  int op, hp, nq, ld;
+  int max_points, max_virt, max_terms, tile_leaves;
+};
+
+template <typename T, int DIM, int P>
+__global__ void __launch_bounds__(kTileThreads) conf_tile_kernel(ConfArgs A, const T *__restrict__ kref,
+                                                                 const T *__restrict__ term_w,
+                                                                 const T *__restrict__ x, T *__restrict__ y,
+                                                                 T *__restrict__ partial) {
+  constexpr int NPE = Pow<P + 1, DIM>::v;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  if (A.cancel && *A.cancel) return;
+  const ConfTileHdr th = A.tiles[blockIdx.x];
+  const int nq = A.nq;
+  size_t off = 0;
+  auto carve = [&](size_t bytes) {
+    unsigned char *p = smem_raw + off;
+    off += (bytes + 15) & ~(size_t)15;
+    return p;
+  };
+  T *x_loc = reinterpret_cast<T *>(carve((size_t)A.max_points * nq * sizeof(T)));
+  T *y_loc = reinterpret_cast<T *>(carve((size_t)A.max_points * nq * sizeof(T)));
+  T *tw_s = reinterpret_cast<T *>(carve((size_t)A.max_terms * sizeof(T)));
+  uint16_t *tj_s = reinterpret_cast<uint16_t *>(carve((size_t)A.max_terms * 2));
+  uint16_t *tv_s = reinterpret_cast<uint16_t *>(carve((size_t)A.max_terms * 2));
+  uint16_t *tp_s = reinterpret_cast<uint16_t *>(carve((size_t)A.max_terms * 2));
+  uint16_t *vo_s = reinterpret_cast<uint16_t *>(carve((size_t)(A.max_virt + 1) * 2));
+  T *k_s = P == 1 ? nullptr : reinterpret_cast<T *>(carve((size_t)NPE * NPE * sizeof(T)));
+  const int tid = threadIdx.x, nthr = blockDim.x;
+  if constexpr (P != 1)
+    for (int k = tid; k < NPE * NPE; k += nthr) k_s[k] = kref[k];
+
+  // 1. Stage nodes and rows.
+  const bool active = tid < th.nleaf;
+  uint16_t loc[NPE];
+  int lev = 0;
+  if (active) {
+    const uint16_t *lp = A.leaf_loc + (size_t)(th.leaf0 + tid) * NPE;
+#pragma unroll
+    for (int r = 0; r < NPE; r++) loc[r] = lp[r];
+    lev = A.leaf_lev[th.leaf0 + tid];
+  } else {
+#pragma unroll
+    for (int r = 0; r < NPE; r++) loc[r] = 0;
+  }
+  const int npts = th.nnode + th.nv;
+  for (int j = tid; j < th.nnode; j += nthr) {
+    const int g = A.tile_nodes[th.node0 + j];
+    const bool z = A.dir_in && A.dir_in[g];
+    for (int q = 0; q < nq; q++) x_loc[j * nq + q] = z ? T(0) : x[(size_t)g * A.ld + q];
+  }
+  for (int j = tid; j < npts * nq; j += nthr) y_loc[j] = T(0);
+  for (int k = tid; k < th.nt; k += nthr) {
+    tj_s[k] = A.term_node[th.t0 + k];
+    tv_s[k] = A.term_virt[th.t0 + k];
+    tw_s[k] = term_w[th.t0 + k];
+    tp_s[k] = A.term_perm[th.t0 + k];
+  }
+  for (int v = tid; v < th.nv; v += nthr) vo_s[v] = A.virt_off[th.v0 + v];
+  if (tid == 0) vo_s[th.nv] = (uint16_t)th.nt;
+  __syncthreads();
+  for (int v = tid; v < th.nv; v += nthr) {
+    const int k0 = vo_s[v], k1 = vo_s[v + 1];
+    for (int q = 0; q < nq; q++) {
+      T s = T(0);
+      for (int k = k0; k < k1; k++) s += tw_s[k] * x_loc[tj_s[k] * nq + q];
+      x_loc[(th.nnode + v) * nq + q] = s;
+    }
+  }
+  __syncthreads();
+  const T scale = (T)ldexp(1.0, -lev * A.hp);
+
+  // 2. Elements.
+  for (int q = 0; q < nq; q++) {
+    T v[NPE];
+    if (active) {
+      T u[NPE];
+#pragma unroll
+      for (int r = 0; r < NPE; r++) u[r] = x_loc[loc[r] * nq + q];
+      if constexpr (P == 1) {
+        if (A.op == 0)
+          q1_apply<T, DIM, 0>(u, v, scale);
+        else if (A.op == 1)
+          q1_apply<T, DIM, 1>(u, v, scale);
+        else
+#pragma unroll
+          for (int i = 0; i < NPE; i++) v[i] = scale * u[i];
+      } else {
+#pragma unroll
+        for (int i = 0; i < NPE; i++) {
+          T acc = T(0);
+#pragma unroll
+          for (int j = 0; j < NPE; j++) acc += k_s[i * NPE + j] * u[j];
+          v[i] = scale * acc;
+        }
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < NPE; r++) {
+      if (active) y_loc[loc[r] * nq + q] += v[r];
+      __syncthreads();
+    }
+  }
+
+  // 3. Transposed rows: per node, its terms in sorted order (segment heads).
+  for (int i = tid; i < th.nt; i += nthr) {
+    const int j = tj_s[tp_s[i]];
+    if (i > 0 && tj_s[tp_s[i - 1]] == j) continue;
+    for (int q = 0; q < nq; q++) {
+      T acc = T(0);
+      for (int m = i; m < th.nt && tj_s[tp_s[m]] == j; m++) {
+        const int k = tp_s[m];
+        acc += tw_s[k] * y_loc[(th.nnode + tv_s[k]) * nq + q];
+      }
+      y_loc[j * nq + q] += acc;
+    }
+  }
+  __syncthreads();
+
+  // 4. Write out.
+  for (int j = tid; j < th.nnode; j += nthr) {
+    const int o = A.tile_out[th.node0 + j];
+    if (o >= 0) {
+      const bool z = A.dir_out && A.dir_out[o];
+      for (int q = 0; q < nq; q++) y[(size_t)o * A.ld + q] = z ? T(0) : y_loc[j * nq + q];
+    } else {
+      for (int q = 0; q < nq; q++) partial[(size_t)(-o - 1) * nq + q] = y_loc[j * nq + q];
+    }
+  }
+}
+
+// LDS bytes of conf_tile_kernel for nq right-hand sides (the p = 2 operator
+// is staged too when nPe is not a power of two).
+template <typename T, int NPE>
+inline size_t conf_lds_bytes(int max_points, int max_virt, int max_terms, int nq) {
+  auto r16 = [](size_t b) { return (b + 15) & ~(size_t)15; };
+  return 2 * r16((size_t)max_points * nq * sizeof(T)) + r16((size_t)max_terms * sizeof(T)) +
+         3 * r16((size_t)max_terms * 2) + r16((size_t)(max_virt + 1) * 2) +
+         ((NPE & (NPE - 1)) ? r16((size_t)NPE * NPE * sizeof(T)) : 0);
+}
+
+}  // namespace dkt

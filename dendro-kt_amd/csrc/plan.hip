@@ -14,6 +14,7 @@
 // Multi-RHS vectors are node-major, [node][nrhs] (the reference's
 // dof-interleaved layout, include/oda.h:296-322).
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstring>
 #include <memory>
@@ -27,6 +28,7 @@
 #include "plan_common.h"
 #include "plan_impl.h"
 #include "tile_kernel.h"
+#include "conf_kernel.h"
 
 namespace dkt {
 const MeshParams &da_params(const dkt_da *da);
@@ -653,6 +655,189 @@ void build_tiled(dkt_plan *pl, const MeshParams &mp, const Traversal &tr, int re
 }
 
 // ---------------------------------------------------------------------------
+// Tiled conforming operator construction (conf_kernel.h).
+// ---------------------------------------------------------------------------
+void build_conf_tiled(dkt_plan *pl, const MeshParams &mp, const Traversal &tr, int T) {
+  const int npe = pl->npe, dim = mp.dim, n1 = mp.order + 1;
+  const size_t L = tr.leaves.size(), N = pl->N;
+  // A hanging point: an element node position that is not a mesh node, keyed
+  // by its coordinates and the element's level (the constraint row of a
+  // position depends on the level it is resolved from).
+  using Key = std::array<u32, 5>;
+  struct KeyHash {
+    size_t operator()(const Key &k) const {
+      u64 h = 0x9E3779B97F4A7C15ull;
+      for (u32 v : k) h = (h ^ v) * 0xBF58476D1CE4E5B9ull;
+      return (size_t)(h ^ (h >> 31));
+    }
+  };
+  auto key_of = [&](size_t e, int r) {
+    const Oct &c = tr.leaves[e];
+    const u32 step = (1u << (mp.maxd - c.lev)) / (u32)mp.order;
+    Key k{0, 0, 0, 0, c.lev};
+    for (int d = 0; d < dim; d++, r /= n1) k[d] = c.x[d] + (u32)(r % n1) * step;
+    return k;
+  };
+  // Terms of rank r of element e: cf rows hold each element's ranks in order.
+  std::vector<int64_t> rk0(npe), rk1(npe);
+  auto rank_ranges = [&](size_t e) {
+    std::fill(rk0.begin(), rk0.end(), 0);
+    std::fill(rk1.begin(), rk1.end(), 0);
+    const int h = tr.hang_of[e];
+    if (h < 0) return;
+    for (int64_t k = tr.cf_off[h]; k < tr.cf_off[h + 1]; k++) {
+      const int r = tr.cf_rank[k];
+      if (rk1[r] == 0) rk0[r] = k;
+      rk1[r] = k + 1;
+    }
+  };
+  std::unordered_map<Key, int32_t, KeyHash> vmap;
+  vmap.reserve(4096);
+  // Tile bounds: at most T elements, kMaxConfTerms terms and kMaxConfVirt hanging points.
+  std::vector<size_t> bounds{0};
+  {
+    int64_t nt = 0, nv = 0;
+    std::vector<Key> fresh;
+    for (size_t e = 0; e < L; e++) {
+      rank_ranges(e);
+      for (int pass = 0; pass < 2; pass++) {
+        int64_t add_t = 0;
+        fresh.clear();
+        for (int r = 0; r < npe; r++)
+          if (tr.slot[e * npe + r] < 0) {
+            const Key k = key_of(e, r);
+            if (!vmap.count(k) && std::find(fresh.begin(), fresh.end(), k) == fresh.end()) {
+              fresh.push_back(k);
+              add_t += rk1[r] - rk0[r];
+            }
+          }
+        if (add_t > kMaxConfTerms || (int64_t)fresh.size() > kMaxConfVirt)
+          throw std::runtime_error("element with too many constraint terms");
+        if (pass == 0 && (e - bounds.back() == (size_t)T || nt + add_t > kMaxConfTerms ||
+                          nv + (int64_t)fresh.size() > kMaxConfVirt)) {
+          bounds.push_back(e);
+          vmap.clear();
+          nt = nv = 0;
+          continue;
+        }
+        for (const Key &k : fresh) vmap.emplace(k, 0);
+        nt += add_t;
+        nv += (int64_t)fresh.size();
+        break;
+      }
+    }
+    if (bounds.back() != L) bounds.push_back(L);
+  }
+  const size_t ntiles = bounds.size() - 1;
+  std::vector<int32_t> last_tile(N, -1), local(N, -1);
+  std::vector<ConfTileHdr> hdr(ntiles);
+  std::vector<int32_t> tnodes;
+  std::vector<uint16_t> lloc(L * (size_t)npe, 0), voff, tnode, tvirt, tperm;
+  std::vector<uint8_t> llev(L, 0);
+  std::vector<double> tw;
+  std::vector<std::pair<size_t, int32_t>> vref;
+  int max_points = 0, max_virt = 0, max_terms = 0;
+  for (size_t t = 0; t < ntiles; t++) {
+    ConfTileHdr &H = hdr[t];
+    H.leaf0 = (int32_t)bounds[t];
+    H.nleaf = (int32_t)(bounds[t + 1] - bounds[t]);
+    H.node0 = (int32_t)tnodes.size();
+    H.v0 = (int32_t)voff.size();
+    H.t0 = (int32_t)tnode.size();
+    int32_t nn = 0;
+    auto loc_of = [&](int32_t n) -> uint16_t {
+      if (last_tile[n] != (int32_t)t) {
+        last_tile[n] = (int32_t)t;
+        local[n] = nn++;
+        tnodes.push_back(n);
+      }
+      return (uint16_t)local[n];
+    };
+    vmap.clear();
+    vref.clear();
+    for (size_t e = bounds[t]; e < bounds[t + 1]; e++) {
+      llev[e] = (uint8_t)tr.leaves[e].lev;
+      rank_ranges(e);
+      for (int r = 0; r < npe; r++) {
+        const int32_t n = tr.slot[e * npe + r];
+        if (n >= 0) {
+          lloc[e * npe + r] = loc_of(n);
+          continue;
+        }
+        auto ins = vmap.emplace(key_of(e, r), (int32_t)(voff.size() - H.v0));
+        if (ins.second) {
+          voff.push_back((uint16_t)(tnode.size() - H.t0));
+          for (int64_t k = rk0[r]; k < rk1[r]; k++) {
+            tnode.push_back(loc_of(tr.cf_node[k]));
+            tvirt.push_back((uint16_t)ins.first->second);
+            tw.push_back(tr.cf_w[k]);
+          }
+        }
+        vref.push_back({e * npe + r, ins.first->second});
+      }
+    }
+    H.nnode = nn;
+    H.nv = (int32_t)(voff.size() - H.v0);
+    H.nt = (int32_t)(tnode.size() - H.t0);
+    if (nn + H.nv >= 0xFFFF) throw std::runtime_error("tile has too many points");
+    for (const auto &vr : vref) lloc[vr.first] = (uint16_t)(nn + vr.second);
+    std::vector<uint16_t> perm(H.nt);
+    for (int k = 0; k < H.nt; k++) perm[k] = (uint16_t)k;
+    std::stable_sort(perm.begin(), perm.end(),
+                     [&](uint16_t a, uint16_t b) { return tnode[H.t0 + a] < tnode[H.t0 + b]; });
+    tperm.insert(tperm.end(), perm.begin(), perm.end());
+    max_points = std::max(max_points, nn + H.nv);
+    max_virt = std::max(max_virt, H.nv);
+    max_terms = std::max(max_terms, H.nt);
+  }
+  // Halve the tiles while one right-hand side in FP64 does not fit in LDS.
+  const size_t kbytes = (npe & (npe - 1)) ? (size_t)npe * npe * 8 : 0;
+  if (conf_lds_bytes<double, 4>(max_points, max_virt, max_terms, 1) + kbytes > (size_t)kLdsLimit && T > 16)
+    return build_conf_tiled(pl, mp, tr, T / 2);
+  // Owned nodes go to y; a node in several tiles gets a partial slot per tile
+  // (node-major, tile order), summed by fixup_kernel.
+  std::vector<int32_t> ntarget(N, 0);
+  for (int32_t n : tnodes) ntarget[n]++;
+  std::vector<int32_t> pbase(N, -1), pfill(N, 0), snode, soff{0};
+  int64_t np = 0;
+  for (size_t n = 0; n < N; n++)
+    if (ntarget[n] > 1) {
+      pbase[n] = (int32_t)np;
+      np += ntarget[n];
+      snode.push_back((int32_t)n);
+      soff.push_back((int32_t)np);
+      if (np >= INT32_MAX) throw std::runtime_error("too many partial sums");
+    }
+  std::vector<int32_t> tout(tnodes.size());
+  for (size_t i = 0; i < tnodes.size(); i++) {
+    const int32_t n = tnodes[i];
+    tout[i] = ntarget[n] == 1 ? n : -(pbase[n] + pfill[n]++) - 1;
+  }
+  pl->tile_leaves = T;
+  pl->ntiles = ntiles;
+  pl->nshared = snode.size();
+  pl->npartial = (size_t)np;
+  pl->max_nodes = max_points;
+  pl->max_virt = max_virt;
+  pl->max_terms = max_terms;
+  pl->ctiles.upload(hdr);
+  pl->tile_nodes.upload(nonempty(tnodes));
+  pl->tile_out.upload(nonempty(tout));
+  pl->leaf_loc.upload(nonempty(lloc));
+  pl->leaf_lev.upload(nonempty(llev));
+  pl->virt_off.upload(nonempty(voff));
+  pl->term_node.upload(nonempty(tnode));
+  pl->term_virt.upload(nonempty(tvirt));
+  pl->term_perm.upload(nonempty(tperm));
+  pl->term_w64.upload(nonempty(tw));
+  pl->term_w32.upload(cast<float>(nonempty(tw)));
+  pl->snode.upload(nonempty(snode));
+  pl->soff.upload(soff);
+  pl->map_bytes = (double)(hdr.size() * sizeof(ConfTileHdr) + tnodes.size() * 8 + lloc.size() * 2 + llev.size() +
+                           voff.size() * 2 + tnode.size() * 14 + snode.size() * 8);
+}
+
+// ---------------------------------------------------------------------------
 // Launch.
 // ---------------------------------------------------------------------------
 template <typename T, int DIM, int P>
@@ -747,16 +932,76 @@ void run_tiled(dkt_plan *pl, int nrhs, const T *x, T *y, int mask, hipStream_t s
   }
 }
 
+template <typename T, int DIM, int P>
+void run_conf(dkt_plan *pl, int nrhs, const T *x, T *y, int mask, hipStream_t st, const int *cancel) {
+  constexpr int NPE = Pow<P + 1, DIM>::v;
+  const bool f64 = sizeof(T) == 8;
+  const T *kmat = f64 ? (const T *)pl->kmat64.p : (const T *)pl->kmat32.p;
+  const T *tw = f64 ? (const T *)pl->term_w64.p : (const T *)pl->term_w32.p;
+  auto lds_for = [&](int nq) { return conf_lds_bytes<T, NPE>(pl->max_nodes, pl->max_virt, pl->max_terms, nq); };
+  if (lds_for(1) > kLdsLimit) throw std::invalid_argument("tile does not fit in LDS");
+  int chunk = 1;
+  while (chunk < nrhs && lds_for(chunk + 1) <= kLdsLimit) chunk++;
+  const size_t need = std::max<size_t>(1, pl->npartial * (size_t)chunk);
+  T *part;
+  if (f64) {
+    if (pl->scratch64.n < need) pl->scratch64.alloc(need);
+    part = (T *)pl->scratch64.p;
+  } else {
+    if (pl->scratch32.n < need) pl->scratch32.alloc(need);
+    part = (T *)pl->scratch32.p;
+  }
+  ConfArgs A;
+  A.tiles = pl->ctiles.p;
+  A.tile_nodes = pl->tile_nodes.p;
+  A.tile_out = pl->tile_out.p;
+  A.leaf_loc = pl->leaf_loc.p;
+  A.leaf_lev = pl->leaf_lev.p;
+  A.virt_off = pl->virt_off.p;
+  A.term_node = pl->term_node.p;
+  A.term_virt = pl->term_virt.p;
+  A.term_perm = pl->term_perm.p;
+  A.dir_in = mask == 1 ? pl->dirichlet.p : nullptr;
+  A.dir_out = mask ? pl->dirichlet.p : nullptr;
+  A.cancel = cancel;
+  A.op = pl->op;
+  A.hp = pl->hp;
+  A.ld = nrhs;
+  A.max_points = pl->max_nodes;
+  A.max_virt = pl->max_virt;
+  A.max_terms = pl->max_terms;
+  A.tile_leaves = pl->tile_leaves;
+  for (int q0 = 0; q0 < nrhs; q0 += chunk) {
+    const int nq = std::min(chunk, nrhs - q0);
+    A.nq = nq;
+    if (pl->profiling) HIP_OK(hipEventRecord(pl->next_event(), st));
+    if (pl->ntiles) {
+      hipLaunchKernelGGL((conf_tile_kernel<T, DIM, P>), dim3(pl->ntiles), dim3(pl->tile_leaves), lds_for(nq), st, A,
+                         kmat, tw, x + q0, y + q0, part);
+      HIP_OK(hipGetLastError());
+    }
+    if (pl->profiling) HIP_OK(hipEventRecord(pl->next_event(), st));
+    if (pl->nshared) {
+      hipLaunchKernelGGL((fixup_kernel<T>), dim3((pl->nshared + 255) / 256), dim3(256), 0, st, (int)pl->nshared,
+                         pl->snode.p, pl->soff.p, part, A.dir_out, nq, nrhs, y + q0, cancel);
+      HIP_OK(hipGetLastError());
+    }
+    if (pl->profiling) HIP_OK(hipEventRecord(pl->next_event(), st));
+  }
+}
+
 template <typename T>
 void run_matvec(dkt_plan *pl, int nrhs, const T *x, T *y, int mask, hipStream_t st, const int *cancel) {
-  const bool simple = pl->simple || pl->per_level || pl->conforming;
-#define DKT_DISPATCH(D, P)                             \
-  if (pl->dim == D && pl->order == P) {                \
-    if (simple)                                        \
-      run_simple<T, D, P>(pl, nrhs, x, y, mask, st, cancel); \
-    else                                               \
-      run_tiled<T, D, P>(pl, nrhs, x, y, mask, st, cancel);  \
-    return;                                            \
+  const bool simple = pl->simple || pl->per_level;
+#define DKT_DISPATCH(D, P)                                    \
+  if (pl->dim == D && pl->order == P) {                       \
+    if (simple)                                               \
+      run_simple<T, D, P>(pl, nrhs, x, y, mask, st, cancel);  \
+    else if (pl->conforming)                                  \
+      run_conf<T, D, P>(pl, nrhs, x, y, mask, st, cancel);    \
+    else                                                      \
+      run_tiled<T, D, P>(pl, nrhs, x, y, mask, st, cancel);   \
+    return;                                                   \
   }
   DKT_DISPATCH(2, 1)
   DKT_DISPATCH(3, 1)
@@ -827,8 +1072,10 @@ extern "C" int dkt_plan_create(const dkt_da *da, int op, const double *op_data, 
     pl->dirichlet.upload(dir);
     const int tshift = (flags >> DKT_PLAN_TILE_SHIFT) & 15;
     if (tshift && (tshift < 6 || tshift > 8)) throw std::invalid_argument("tile size must be 64, 128 or 256 leaves");
-    if (pl->simple || pl->per_level || pl->conforming)
+    if (pl->simple || pl->per_level)
       build_simple(pl.get(), mp, tr);
+    else if (pl->conforming)
+      build_conf_tiled(pl.get(), mp, tr, tshift ? 1 << tshift : kDefaultTile);
     else
       build_tiled(pl.get(), mp, tr, tshift ? 1 << tshift : 0);
     *out = pl.release();
@@ -880,7 +1127,8 @@ extern "C" int dkt_plan_matvec(dkt_plan *pl, int dtype, const void *x, void *y, 
 }
 
 // info: nodes, leaves, hanging leaves, nPe, device map bytes, tiles, shared nodes,
-// partial slots, max nodes per tile, the tile statistics, leaves per tile.
+// partial slots, max nodes per tile, the tile statistics, leaves per tile,
+// max constraint terms per conforming tile.
 extern "C" int dkt_plan_info(const dkt_plan *pl, double *info, int n) {
   return guarded([&] {
     if (!pl || !info) throw std::invalid_argument("null pointer");
@@ -889,6 +1137,7 @@ extern "C" int dkt_plan_info(const dkt_plan *pl, double *info, int n) {
                              (double)pl->nshared, (double)pl->npartial, (double)pl->max_nodes};
     v.insert(v.end(), pl->stats.begin(), pl->stats.end());
     v.push_back((double)pl->tile_leaves);
+    v.push_back((double)pl->max_terms);
     for (int i = 0; i < n && i < (int)v.size(); i++) info[i] = v[i];
   });
 }

@@ -20,6 +20,8 @@ namespace dkt {
 constexpr uint16_t kNone = 0xFFFF;
 constexpr int kTileThreads = 256;
 constexpr int kDefaultTile = 128;
+constexpr int kMaxConfTerms = 2048;  // constraint terms per conforming tile
+constexpr int kMaxConfVirt = 1024;   // hanging points per conforming tile
 constexpr int kMaxReaders = 32;  // stale-buffer readers per tile (bounds the replay outputs in LDS)
 constexpr size_t kLdsLimit = 64 * 1024;
 

@@ -5,6 +5,7 @@
 
 #include "../../include/dkt.h"
 #include "plan_common.h"
+#include "conf_kernel.h"
 
 // Solver workspace: CG vectors and reduction partials, allocated on first use.
 struct CgWorkspace {
@@ -44,12 +45,18 @@ struct dkt_plan {
   // tiled path
   int tile_leaves = dkt::kTileThreads;
   size_t ntiles = 0, nshared = 0, npartial = 0;
-  int max_nodes = 0, max_grp = 0, max_he = 0, max_rd = 0, max_cr = 0;
+  int max_nodes = 0, max_grp = 0, max_he = 0, max_rd = 0, max_cr = 0, max_virt = 0, max_terms = 0;
   dkt::DevBuf<dkt::TileHdr> tiles;
   dkt::DevBuf<int32_t> tile_nodes, tile_out, snode, soff;
   dkt::DevBuf<uint16_t> leaf_loc, group_ploc, readers;
   dkt::DevBuf<uint32_t> leaf_aux, carry_key;
   dkt::DevBuf<uint64_t> hentries, blocks;
+  // conforming tiles
+  dkt::DevBuf<dkt::ConfTileHdr> ctiles;
+  dkt::DevBuf<uint16_t> virt_off, term_node, term_virt, term_perm;
+  dkt::DevBuf<uint8_t> leaf_lev;
+  dkt::DevBuf<double> term_w64;
+  dkt::DevBuf<float> term_w32;
   dkt::DevBuf<double> carry_w64;
   dkt::DevBuf<float> carry_w32;
   // scratch

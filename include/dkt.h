@@ -45,7 +45,8 @@ extern "C" {
 #define DKT_PLAN_DRYRUN 2
 /* The conforming operator sum_e C_e^T K_e C_e (symmetric; see
  * dkt_da_conforming_terms) instead of the reference's traversal semantics;
- * runs on the per-leaf/per-node kernels. */
+ * runs on tiles of mesh elements (at most 2048 constraint terms per tile),
+ * or on the per-leaf/per-node kernels with DKT_PLAN_SIMPLE. */
 #define DKT_PLAN_CONFORMING 4
 /* Leaves per tile of the tiled path: flags |= log2(leaves) << DKT_PLAN_TILE_SHIFT
  * with leaves in {64, 128, 256}; 0 = 128, halved while the LDS footprint does
@@ -199,7 +200,8 @@ int dkt_batch_destroy(dkt_batch *batch);
  * readers per tile, mean and max tile-start buffer terms per tile, mean
  * sibling groups, mean nodes per tile, hanging-transpose entries, mean cone
  * length per reader, mean and max longest replay per tile with readers;
- * info[19] = leaves per tile. */
+ * info[19] = leaves per tile; info[20] = max constraint terms per tile of a
+ * tiled conforming plan. */
 int dkt_plan_info(const dkt_plan *plan, double *info, int n);
 
 /* Event timing of subsequent tiled matvecs on their stream (no counterpart in

@@ -123,3 +123,84 @@ def test_conforming_cg_converges_where_reference_operator_stalls(dkt):
     torch.cuda.synchronize()
     assert float((bd - r).abs().max() / bd.abs().max()) <= 1.01e-8
     assert np.linalg.norm(out[dkt.CG_MIXED] - out[dkt.CG_F64]) <= 1e-6 * np.linalg.norm(out[dkt.CG_F64])
+
+
+def _dry_conf_info(dkt, da, tile_leaves=0):
+    import ctypes
+    L = dkt.load()
+    h = ctypes.c_void_p()
+    flags = 2 | dkt.PLAN_CONFORMING
+    if tile_leaves:
+        flags |= (tile_leaves.bit_length() - 1) << dkt.PLAN_TILE_SHIFT
+    assert L.dkt_plan_create(da.h, dkt.OP_STIFFNESS, None, flags, ctypes.byref(h)) == 0, L.dkt_last_error()
+    keys = 21
+    out = (ctypes.c_double * keys)()
+    try:
+        assert L.dkt_plan_info(h, out, keys) == 0
+    finally:
+        L.dkt_plan_destroy(h)
+    return list(out)
+
+
+@pytest.mark.parametrize("dim,p,seed", CASES)
+@pytest.mark.parametrize("tile", [64, 256])
+def test_conforming_tiled_layout(dkt, dim, p, seed, tile):
+    """Host-side conforming tiles (dry run): the tiles cover the mesh elements with at
+    most `tile` elements (fewer where LDS requires) and 2048 constraint terms each, and every node shared by
+    tiles gets one partial slot per tile."""
+    da, _ = _da(dkt, dim, p, seed)
+    info = _dry_conf_info(dkt, da, tile)
+    n_elem = len(da.elements())
+    assert info[0] == da.n_nodes and info[1] == n_elem
+    assert 16 <= info[19] <= tile  # halved while one FP64 column does not fit in LDS
+    assert info[5] * info[19] >= n_elem
+    assert 0 < info[20] <= 2048
+    assert info[7] >= 2 * info[6]
+    assert info[8] <= info[19] * info[3] + 1024  # local points: element nodes and hanging points
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("dim,p,seed", CASES)
+def test_conforming_tiled_matches_simple(dkt, dim, p, seed):
+    """The tiled conforming kernel against the per-element kernels and the assembled
+    operator: FP64 within 1e-13, every masking mode, every tile size; three
+    right-hand sides give columns bitwise equal to single-RHS calls; repeated calls
+    are bitwise identical."""
+    torch = _torch()
+    da, maxd = _da(dkt, dim, p, seed)
+    nodes = da.nodes()
+    n = len(nodes)
+    A = cn.assemble(da.elements(), nodes, dim, p, maxd)
+    m = np.ones(n, bool)
+    m[da.boundary()] = False
+    simple = dkt.Plan(da, dkt.OP_STIFFNESS, conforming=True, simple=True)
+    X = np.stack([gi.keyed_uniform(s, nodes[:, :dim]) for s in (5, 6, 7)], axis=1)
+    for tile in (64, 128, 256):
+        plan = dkt.Plan(da, dkt.OP_STIFFNESS, conforming=True, tile_leaves=tile)
+        for masked in (0, 1, 2):
+            xd = torch.tensor(X[:, 0].copy(), device="cuda")
+            ya, yb, yc = (torch.empty_like(xd) for _ in range(3))
+            plan.matvec_torch(xd, ya, masked=masked)
+            plan.matvec_torch(xd, yb, masked=masked)
+            simple.matvec_torch(xd, yc, masked=masked)
+            torch.cuda.synchronize()
+            ya, yb, yc = ya.cpu().numpy(), yb.cpu().numpy(), yc.cpu().numpy()
+            x = X[:, 0]
+            if masked == 0:
+                ref = A @ x
+            elif masked == 1:
+                ref = np.where(m, A @ np.where(m, x, 0.0), 0.0)
+            else:
+                ref = np.where(m, A @ x, 0.0)
+            assert np.array_equal(ya, yb)
+            assert np.linalg.norm(ya - yc) <= 1e-13 * np.linalg.norm(yc)
+            assert np.linalg.norm(ya - ref) <= 1e-12 * np.linalg.norm(ref)
+        Xd = torch.tensor(X, device="cuda")
+        Y = torch.empty_like(Xd)
+        plan.matvec_torch(Xd, Y, masked=1)
+        torch.cuda.synchronize()
+        for c in range(3):
+            yc = torch.empty(n, dtype=torch.float64, device="cuda")
+            plan.matvec_torch(Xd[:, c].contiguous(), yc, masked=1)
+            torch.cuda.synchronize()
+            assert torch.equal(Y[:, c], yc)
