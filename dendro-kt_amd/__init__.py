@@ -308,7 +308,7 @@ class Plan:
         keys = ["nodes", "leaves", "hanging", "npe", "map_bytes", "tiles", "shared_nodes", "partials",
                 "max_tile_nodes", "readers_mean", "readers_max", "carry_terms_mean", "carry_terms_max",
                 "groups_mean", "tile_nodes_mean", "hanging_entries", "cone_mean", "span_mean", "span_max",
-                "tile_leaves", "max_tile_terms"]
+                "tile_leaves", "max_tile_terms", "max_tile_hanging", "lds_bytes"]
         out = (ctypes.c_double * len(keys))()
         _chk(load().dkt_plan_info(self.h, out, len(keys)))
         return dict(zip(keys, list(out)))

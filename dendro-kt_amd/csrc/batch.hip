@@ -37,13 +37,14 @@ struct Group : GroupBase {
   DevBuf<T> partial;
   int nplans = 0, ntiles = 0, nblocks = 0, threads = 0;
   std::vector<size_t> lds;
+  std::vector<int> nqs;  // columns per chunk
 
   Group(const std::vector<dkt_plan *> &pls, const std::vector<const void *> &xs, const std::vector<void *> &ys,
         int nrhs, int masked) {
     constexpr int NPE = Pow<P + 1, DIM>::v;
     const bool f64 = sizeof(T) == 8;
     auto dims_of = [](const dkt_plan *pl) {
-      return TileDims{pl->max_nodes, pl->max_grp, pl->max_he, pl->max_rd, pl->max_cr, pl->tile_leaves};
+      return TileDims{pl->max_nodes, pl->max_grp, pl->max_he, pl->max_rd, pl->max_cr, pl->tile_leaves, pl->max_hang};
     };
     // Columns per launch: the most every plan's tiles fit in LDS.
     int chunk = nrhs;
@@ -100,6 +101,7 @@ struct Group : GroupBase {
       tables.emplace_back(new DevBuf<BatchPlan<T>>());
       tables.back()->upload(hp);
       lds.push_back(need);
+      nqs.push_back(nq);
     }
     nplans = (int)pls.size();
     ntiles = ts.back();
@@ -110,9 +112,12 @@ struct Group : GroupBase {
 
   void launch(hipStream_t st) override {
     for (size_t c = 0; c < tables.size(); c++) {
-      if (ntiles)
-        hipLaunchKernelGGL((tile_kernel_batch<T, DIM, P>), dim3(ntiles), dim3(threads), lds[c], st, tables[c]->p,
-                           tile_start.p, nplans);
+      if (ntiles && nqs[c] == 1)
+        hipLaunchKernelGGL((tile_kernel_batch<T, DIM, P, true>), dim3(ntiles), dim3(threads), lds[c], st,
+                           tables[c]->p, tile_start.p, nplans);
+      else if (ntiles)
+        hipLaunchKernelGGL((tile_kernel_batch<T, DIM, P, false>), dim3(ntiles), dim3(threads), lds[c], st,
+                           tables[c]->p, tile_start.p, nplans);
       if (nblocks)
         hipLaunchKernelGGL((fixup_kernel_batch<T>), dim3(nblocks), dim3(256), 0, st, tables[c]->p, block_start.p,
                            nplans);

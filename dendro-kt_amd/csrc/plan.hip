@@ -275,7 +275,7 @@ struct TiledHost {
   std::vector<uint64_t> hent, blk;
   std::vector<uint16_t> rdr;
   std::vector<double> cw;
-  int max_nodes = 0, max_grp = 0, max_he = 0, max_rd = 0, max_cr = 0;
+  int max_nodes = 0, max_grp = 0, max_he = 0, max_rd = 0, max_cr = 0, max_hang = 0;
   size_t nshared = 0, npartial = 0;
   std::vector<double> stats;
 };
@@ -403,11 +403,15 @@ TiledHost build_tiled_host(const MeshParams &mp, const Traversal &tr, size_t N, 
       par.lev = c.lev - 1;
       const uint32_t pmask = ~(uint32_t)(((uint64_t)1 << (mp.maxd - (int)par.lev)) - 1);
       for (int d = 0; d < dim; d++) par.x[d] = c.x[d] & pmask;
-      bool same = cur_group >= 0 && par.lev == cur_parent.lev && gcount[cur_group] < 255;
+      // Groups end at 64-leaf boundaries: the kernel sums a group's transposed
+      // values within one wavefront.
+      bool same = cur_group >= 0 && par.lev == cur_parent.lev && (e - l0) % 64 != 0;
       for (int d = 0; same && d < dim; d++) same = par.x[d] == cur_parent.x[d];
       const int32_t *ps = &tr.pslot[(size_t)h * npe];
       if (!same) {
         cur_group = (int)gfirst.size();
+        if (cur_group > 255) throw std::runtime_error("tile has too many sibling groups");
+        aux |= 1u << 26;
         cur_parent = par;
         gfirst.push_back((uint8_t)hang_leaves.size());  // hanging index of the group's first leaf
         gcount.push_back(0);
@@ -556,8 +560,7 @@ TiledHost build_tiled_host(const MeshParams &mp, const Traversal &tr, size_t N, 
     std::vector<std::pair<uint32_t, uint64_t>> ent;
     for (const auto &g : gl) {
       const uint16_t j = o.gploc[(size_t)(H.grp0 + g.first) * npe + g.second];
-      const uint64_t v = (uint64_t)gfirst[g.first] | ((uint64_t)gcount[g.first] << 8) |
-                         ((uint64_t)g.second << 16) | ((uint64_t)j << 24);
+      const uint64_t v = (uint64_t)g.first | ((uint64_t)g.second << 16) | ((uint64_t)j << 24);
       ent.push_back({(uint32_t)j, v});
     }
     std::stable_sort(ent.begin(), ent.end(),
@@ -583,6 +586,7 @@ TiledHost build_tiled_host(const MeshParams &mp, const Traversal &tr, size_t N, 
     o.max_he = std::max(o.max_he, H.nhe);
     o.max_rd = std::max(o.max_rd, H.nrd);
     o.max_cr = std::max(o.max_cr, H.ncr);
+    o.max_hang = std::max(o.max_hang, (int)hang_leaves.size());
     if (H.nrd > 0) n_rtiles += 1;
     s_rd += H.nrd;
     m_rd = std::max<double>(m_rd, H.nrd);
@@ -599,7 +603,7 @@ TiledHost build_tiled_host(const MeshParams &mp, const Traversal &tr, size_t N, 
 
 template <int NPE>
 size_t lds_f64(const TiledHost &o, int T) {
-  return tile_lds_bytes<double, NPE>(TileDims{o.max_nodes, o.max_grp, o.max_he, o.max_rd, o.max_cr, T}, 1);
+  return tile_lds_bytes<double, NPE>(TileDims{o.max_nodes, o.max_grp, o.max_he, o.max_rd, o.max_cr, T, o.max_hang}, 1);
 }
 
 void build_tiled(dkt_plan *pl, const MeshParams &mp, const Traversal &tr, int requested) {
@@ -621,6 +625,7 @@ void build_tiled(dkt_plan *pl, const MeshParams &mp, const Traversal &tr, int re
       case 27: lds = lds_f64<27>(o, T); break;
       default: throw std::invalid_argument("unsupported nodes per element");
     }
+    pl->lds_bytes = lds;
     if (lds <= kLdsLimit) break;
     if (requested || T <= 64) throw std::runtime_error("tile does not fit in LDS");
     T /= 2;
@@ -634,6 +639,7 @@ void build_tiled(dkt_plan *pl, const MeshParams &mp, const Traversal &tr, int re
   pl->max_he = o.max_he;
   pl->max_rd = o.max_rd;
   pl->max_cr = o.max_cr;
+  pl->max_hang = o.max_hang;
   pl->stats = o.stats;
   pl->tiles.upload(o.hdr);
   pl->tile_nodes.upload(o.tnodes);
@@ -792,8 +798,8 @@ void build_conf_tiled(dkt_plan *pl, const MeshParams &mp, const Traversal &tr, i
   }
   // Halve the tiles while one right-hand side in FP64 does not fit in LDS.
   const size_t kbytes = (npe & (npe - 1)) ? (size_t)npe * npe * 8 : 0;
-  if (conf_lds_bytes<double, 4>(max_points, max_virt, max_terms, 1) + kbytes > (size_t)kLdsLimit && T > 16)
-    return build_conf_tiled(pl, mp, tr, T / 2);
+  pl->lds_bytes = conf_lds_bytes<double, 4>(max_points, max_virt, max_terms, 1) + kbytes;
+  if (pl->lds_bytes > (size_t)kLdsLimit && T > 16) return build_conf_tiled(pl, mp, tr, T / 2);
   // Owned nodes go to y; a node in several tiles gets a partial slot per tile
   // (node-major, tile order), summed by fixup_kernel.
   std::vector<int32_t> ntarget(N, 0);
@@ -879,7 +885,7 @@ void run_tiled(dkt_plan *pl, int nrhs, const T *x, T *y, int mask, hipStream_t s
   const T *cw = f64 ? (const T *)pl->carry_w64.p : (const T *)pl->carry_w32.p;
   auto lds_for = [&](int nq) {
     return tile_lds_bytes<T, NPE>(
-        TileDims{pl->max_nodes, pl->max_grp, pl->max_he, pl->max_rd, pl->max_cr, pl->tile_leaves}, nq);
+        TileDims{pl->max_nodes, pl->max_grp, pl->max_he, pl->max_rd, pl->max_cr, pl->tile_leaves, pl->max_hang}, nq);
   };
   // Columns are processed in chunks whose LDS footprint stays within budget.
   if (lds_for(1) > kLdsLimit) throw std::invalid_argument("tile does not fit in LDS");
@@ -911,14 +917,18 @@ void run_tiled(dkt_plan *pl, int nrhs, const T *x, T *y, int mask, hipStream_t s
   A.hp = pl->hp;
   A.ld = nrhs;
   A.blocks = pl->blocks.p;
-  A.dims = TileDims{pl->max_nodes, pl->max_grp, pl->max_he, pl->max_rd, pl->max_cr, pl->tile_leaves};
+  A.dims = TileDims{pl->max_nodes, pl->max_grp, pl->max_he, pl->max_rd, pl->max_cr, pl->tile_leaves, pl->max_hang};
   for (int q0 = 0; q0 < nrhs; q0 += chunk) {
     const int nq = std::min(chunk, nrhs - q0);
     A.nq = nq;
     if (pl->profiling) HIP_OK(hipEventRecord(pl->next_event(), st));
     if (pl->ntiles) {
-      hipLaunchKernelGGL((tile_kernel<T, DIM, P>), dim3(pl->ntiles), dim3(pl->tile_leaves), lds_for(nq), st, A, kmat,
-                         cw, x + q0, y + q0, part);
+      if (nq == 1)
+        hipLaunchKernelGGL((tile_kernel<T, DIM, P, true>), dim3(pl->ntiles), dim3(pl->tile_leaves), lds_for(nq), st, A,
+                           kmat, cw, x + q0, y + q0, part);
+      else
+        hipLaunchKernelGGL((tile_kernel<T, DIM, P, false>), dim3(pl->ntiles), dim3(pl->tile_leaves), lds_for(nq), st,
+                           A, kmat, cw, x + q0, y + q0, part);
       HIP_OK(hipGetLastError());
     }
     if (pl->profiling) HIP_OK(hipEventRecord(pl->next_event(), st));
@@ -1128,7 +1138,8 @@ extern "C" int dkt_plan_matvec(dkt_plan *pl, int dtype, const void *x, void *y, 
 
 // info: nodes, leaves, hanging leaves, nPe, device map bytes, tiles, shared nodes,
 // partial slots, max nodes per tile, the tile statistics, leaves per tile,
-// max constraint terms per conforming tile.
+// max constraint terms per conforming tile, max hanging leaves per tile, LDS
+// bytes per workgroup (FP64, one column).
 extern "C" int dkt_plan_info(const dkt_plan *pl, double *info, int n) {
   return guarded([&] {
     if (!pl || !info) throw std::invalid_argument("null pointer");
@@ -1138,6 +1149,8 @@ extern "C" int dkt_plan_info(const dkt_plan *pl, double *info, int n) {
     v.insert(v.end(), pl->stats.begin(), pl->stats.end());
     v.push_back((double)pl->tile_leaves);
     v.push_back((double)pl->max_terms);
+    v.push_back((double)pl->max_hang);
+    v.push_back((double)pl->lds_bytes);
     for (int i = 0; i < n && i < (int)v.size(); i++) info[i] = v[i];
   });
 }

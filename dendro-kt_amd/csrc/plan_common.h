@@ -168,23 +168,26 @@ struct TileHdr {
 };
 
 // Leaf word: bits 0-4 level, 5-8 child index, 9 hanging, 10-17 hanging index
-// within the tile, 18-31 sibling group (tile-local).
+// within the tile, 18-25 sibling group (tile-local), 26 first leaf of its group.
+// A sibling group is a run of consecutive hanging leaves with one parent that
+// does not cross a 64-leaf (wavefront) boundary of the tile.
 __host__ __device__ inline int aux_level(uint32_t a) { return a & 31u; }
 __host__ __device__ inline int aux_child(uint32_t a) { return (a >> 5) & 15u; }
 __host__ __device__ inline bool aux_hang(uint32_t a) { return (a >> 9) & 1u; }
 __host__ __device__ inline int aux_hidx(uint32_t a) { return (a >> 10) & 255u; }
-__host__ __device__ inline int aux_group(uint32_t a) { return a >> 18; }
-// Hanging-transpose entry: first leaf (8) | count (8) | parent rank (8) | target local node (16).
-__host__ __device__ inline int he_first(uint32_t lo) { return (int)(lo & 255u); }
+__host__ __device__ inline int aux_group(uint32_t a) { return (a >> 18) & 255u; }
+__host__ __device__ inline bool aux_ghead(uint32_t a) { return (a >> 26) & 1u; }
+// Hanging-transpose entry (uint64): sibling group (16) | parent rank (8) << 16 |
+// target local node (16) << 24.
 // Reader (uint16): hanging index (8) | tile-local leaf (8) << 8.
 // Replay block (uint64): start hanging index (16, 0xFFFF = from the carried
 // tile-start state) | last hanging index (16) << 16 | first reader (16) << 32 |
 // reader count (16) << 48.
-// Chain word (in LDS, per hanging index): child (8) | absent parent ranks (32) << 32.
+// Chain word (in LDS, per hanging index): child (8) | sibling group (16) << 8.
 
 // Per-tile maxima that size the LDS carve.
 struct TileDims {
-  int max_nodes, max_grp, max_he, max_rd, max_cr, tile_leaves;
+  int max_nodes, max_grp, max_he, max_rd, max_cr, tile_leaves, max_hang;
 };
 
 struct TileArgs {

@@ -45,7 +45,8 @@ struct dkt_plan {
   // tiled path
   int tile_leaves = dkt::kTileThreads;
   size_t ntiles = 0, nshared = 0, npartial = 0;
-  int max_nodes = 0, max_grp = 0, max_he = 0, max_rd = 0, max_cr = 0, max_virt = 0, max_terms = 0;
+  int max_nodes = 0, max_grp = 0, max_he = 0, max_rd = 0, max_cr = 0, max_virt = 0, max_terms = 0, max_hang = 0;
+  size_t lds_bytes = 0;  // LDS per workgroup of the tiled kernel, FP64, one column
   dkt::DevBuf<dkt::TileHdr> tiles;
   dkt::DevBuf<int32_t> tile_nodes, tile_out, snode, soff;
   dkt::DevBuf<uint16_t> leaf_loc, group_ploc, readers;

@@ -4,24 +4,26 @@
 // Per tile:
 //   1. stage in LDS: the tile's distinct nodes from x (x_loc), the parent-node
 //      maps of its sibling groups, its hanging-transpose entries, leaf words;
-//   2. each hanging leaf interpolates its present parent nodes in registers
-//      (IKD_Parent2Child, FEM/include/refel.h:214-240) into fbuf;
+//   2. the parent-buffer state at the tile start, from host-expanded terms;
 //   3. stale parent-buffer reads: the reference interpolates in a static buffer
 //      in place and reads absent parent ranks from what the previous hanging
 //      leaf left there (FEM/include/matvec.h:283,411-456).  For every leaf that
 //      reads such a value ("reader"), the host gives the start of its
 //      dependency cone; a group of lanes (one per buffer slot) replays the
-//      recurrence B_k = F_k + W_ck (Q_k B_{k-1}) forward from there (exact on
-//      the slots the reader needs), starting from host-computed carry terms
-//      when the cone begins before the tile;
-//   4. v = 2^(-level*hp) K_ref u in registers (p = 1: in the Walsh-Hadamard
-//      basis that diagonalises K_ref; p = 2: dense, K_ref staged in LDS);
+//      recurrence B_k = W_ck (parent nodes where present, else B_(k-1))
+//      forward from there, exact on the slots the reader needs;
+//   4. each leaf forms its element vector (hanging leaves: parent nodes
+//      interpolated in registers, IKD_Parent2Child, FEM/include/refel.h:214-240,
+//      or the replayed buffer for readers) and applies
+//      v = 2^(-level*hp) K_ref u (p = 1: in the Walsh-Hadamard basis that
+//      diagonalises K_ref; p = 2: dense, K_ref staged in LDS);
 //   5. direct outputs are added into the LDS accumulator y_loc in nPe phases,
 //      one per rank (two disjoint leaves never share a node at the same rank,
 //      so a phase has no write conflicts);
 //   6. the transposed interpolation of missing ranks (matvec.h:482-522) is
-//      summed per (sibling group, parent rank), then per target node in a
-//      segmented sum over entries sorted by target (no conflicts);
+//      summed over each sibling group with cross-lane moves inside the
+//      wavefront, then per target node in a segmented sum over the (group,
+//      parent rank) entries sorted by target (no conflicts);
 //   7. nodes touched by this tile only are written to y; nodes shared with
 //      other tiles go to partial slots, summed in tile order by fixup_kernel.
 // All reductions run in an order fixed by the plan: bitwise reproducible,
@@ -66,6 +68,18 @@ template <int NPE>
 struct Group {
   static constexpr int v = NPE <= 4 ? 4 : (NPE <= 8 ? 8 : (NPE <= 16 ? 16 : 32));
 };
+
+// Bytes of the tile kernel's shared LDS region: carry terms (values, then
+// keys), reader values or per-group transposed sums, whichever is largest.
+template <typename T, int NPE>
+__host__ __device__ inline size_t tile_region_bytes(const TileDims &D) {
+  const size_t carry = (((size_t)D.max_cr * sizeof(T) + 15) & ~(size_t)15) + (size_t)D.max_cr * 4;
+  const size_t readers = (size_t)D.max_rd * NPE * sizeof(T);
+  const size_t groups = (size_t)D.max_grp * NPE * sizeof(T);
+  size_t m = carry > readers ? carry : readers;
+  m = m > groups ? m : groups;
+  return m > 16 ? m : 16;
+}
 
 // Cross-lane moves inside a row of 16 lanes (DPP, a VALU operand modifier;
 // no LDS round trip).  Controls: quad_perm [1,0,3,2] = 0xB1, [2,3,0,1] = 0x4E,
@@ -137,7 +151,9 @@ __device__ __forceinline__ T interp_lanes(T a, int lane, int c) {
 
 // One tile of one plan (the workgroup may have more work-items than the
 // plan's tile_leaves; they idle in the leaf phases and help in the loops).
-template <typename T, int DIM, int P>
+// ONE: a single right-hand side per launch (nq = 1 at compile time, which
+// keeps the leaf state out of a runtime column loop and halves the VGPRs).
+template <typename T, int DIM, int P, bool ONE>
 __device__ __forceinline__ void tile_body(const TileArgs &A, int tile, const T *__restrict__ kref,
                                          const T *__restrict__ carry_w, const T *__restrict__ x, T *__restrict__ y,
                                          T *__restrict__ partial) {
@@ -146,10 +162,8 @@ __device__ __forceinline__ void tile_body(const TileArgs &A, int tile, const T *
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const TileHdr th = A.tiles[tile];
   const TileDims D = A.dims;
-  const int nq = A.nq;
+  const int nq = ONE ? 1 : A.nq;
   const int TL = D.tile_leaves;
-  const int FS = TL + 1;  // odd row stride of fbuf: a lane group reading one leaf's slots hits distinct banks
-  const int ulen = D.max_he > D.max_rd * NPE ? D.max_he : D.max_rd * NPE;
   size_t off = 0;
   auto carve = [&](size_t bytes) {
     unsigned char *p = smem_raw + off;
@@ -158,13 +172,16 @@ __device__ __forceinline__ void tile_body(const TileArgs &A, int tile, const T *
   };
   T *x_loc = reinterpret_cast<T *>(carve((size_t)D.max_nodes * nq * sizeof(T)));
   T *y_loc = reinterpret_cast<T *>(carve((size_t)D.max_nodes * nq * sizeof(T)));
-  T *fbuf = reinterpret_cast<T *>(carve((size_t)FS * NPE * sizeof(T)));  // [slot][hanging index]
-  T *ubuf = reinterpret_cast<T *>(carve((size_t)ulen * sizeof(T)));      // [reader][slot], then group sums
-  T *cst = reinterpret_cast<T *>(carve((size_t)NPE * nq * sizeof(T)));   // parent buffer at the tile start
-  T *cwt_s = reinterpret_cast<T *>(carve((size_t)D.max_cr * sizeof(T)));
+  // Shared region: carry terms (until the tile-start state is formed), then
+  // reader values [reader][slot] (replay to element phase), then group sums
+  // [group][slot] (transposed phase).
+  unsigned char *region = carve(tile_region_bytes<T, NPE>(D));
+  T *ubuf = reinterpret_cast<T *>(region);
+  T *cwt_s = reinterpret_cast<T *>(region);
+  uint32_t *ckey_s = reinterpret_cast<uint32_t *>(region + (((size_t)D.max_cr * sizeof(T) + 15) & ~(size_t)15));
+  T *cst = reinterpret_cast<T *>(carve((size_t)NPE * nq * sizeof(T)));  // parent buffer at the tile start
   uint64_t *hent = reinterpret_cast<uint64_t *>(carve((size_t)D.max_he * 8));
-  uint64_t *cw_s = reinterpret_cast<uint64_t *>(carve((size_t)TL * 8));  // chain word per hanging index
-  uint32_t *ckey_s = reinterpret_cast<uint32_t *>(carve((size_t)D.max_cr * 4));
+  uint32_t *cw_s = reinterpret_cast<uint32_t *>(carve((size_t)(D.max_hang + 1) * 4));  // chain word per hanging index
   uint16_t *gploc = reinterpret_cast<uint16_t *>(carve((size_t)D.max_grp * NPE * 2));
   uint16_t *ridx_s = reinterpret_cast<uint16_t *>(carve((size_t)TL * 2));
   uint16_t *rdl_s = reinterpret_cast<uint16_t *>(carve((size_t)D.max_rd * 2));
@@ -219,46 +236,44 @@ __device__ __forceinline__ void tile_body(const TileArgs &A, int tile, const T *
     ckey_s[k] = A.carry_key[th.cr0 + k];
     cwt_s[k] = carry_w[th.cr0 + k];
   }
-  __syncthreads();
-
   const bool hang = active && aux_hang(aux);
   const int child = aux_child(aux);
   const int hidx = aux_hidx(aux);
+  const int grp_id = aux_group(aux);
   const T scale = (T)ldexp(1.0, -aux_level(aux) * A.hp);
-  const uint16_t *ploc = gploc + (size_t)aux_group(aux) * NPE;
-  if (hang) {
-    uint64_t absent = 0;
-#pragma unroll
-    for (int s = 0; s < NPE; s++)
-      if (ploc[s] == kNone) absent |= (uint64_t)1 << s;
-    cw_s[hidx] = (uint64_t)child | (absent << 32);
-  }
-  const int lane = tid % G, grp = tid / G, ngroups = nthr / G;
-  const int sl = lane < NPE ? lane : 0;
+  if (hang) cw_s[hidx] = (uint32_t)child | ((uint32_t)grp_id << 8);
+  __syncthreads();
 
-  for (int q = 0; q < nq; q++) {
-    // 2. Interpolation of the present parent nodes; parent-buffer state at the
-    //    tile start (terms sorted by slot).
-    if (hang) {
-      T f[NPE];
-#pragma unroll
-      for (int s = 0; s < NPE; s++) f[s] = ploc[s] != kNone ? x_loc[ploc[s] * nq + q] : T(0);
-      interp<T, DIM, P, false>(f, child);
-#pragma unroll
-      for (int r = 0; r < NPE; r++) fbuf[r * FS + hidx] = f[r];
+  // 2. Parent-buffer state at the tile start, every column: slot s sums its
+  //    terms (sorted by slot) in plan order.
+  if (tid < NPE) {
+    int lo = 0, hi = th.ncr;
+    while (lo < hi) {
+      const int mid = (lo + hi) >> 1;
+      if ((int)(ckey_s[mid] >> 16) < tid) lo = mid + 1; else hi = mid;
     }
-    if (tid < NPE) {
+    for (int q = 0; q < nq; q++) {
       T acc = T(0);
-      for (int c = 0; c < th.ncr; c++) {
-        const uint32_t key = ckey_s[c];
-        if ((int)(key >> 16) == tid) acc += cwt_s[c] * x_loc[(key & 0xFFFFu) * nq + q];
-      }
+      for (int c = lo; c < th.ncr && (int)(ckey_s[c] >> 16) == tid; c++)
+        acc += cwt_s[c] * x_loc[(ckey_s[c] & 0xFFFFu) * nq + q];
       cst[tid * nq + q] = acc;
     }
-    __syncthreads();
-    // 3. Stale parent-buffer reads: a lane group per replay block runs
-    //    B_k = F_k + W_ck (Q_k B_(k-1)) over the block's hanging leaves and
-    //    keeps W_ck (Q_k B_(k-1)) of each reader it passes.
+  }
+  __syncthreads();
+  const uint16_t *ploc = gploc + (size_t)grp_id * NPE;
+  const int lane = tid % G, grp = tid / G, ngroups = nthr / G;
+  const int sl = lane < NPE ? lane : 0;
+  const int wl = tid & 63;  // lane in the wavefront
+  // Sibling-group runs in this wavefront: a run ends at the next group head or
+  // non-hanging work-item.
+  const uint64_t starts = __ballot(!hang || aux_ghead(aux));
+  const uint64_t later = wl == 63 ? 0 : (starts >> (wl + 1));
+  const int run_end = later ? wl + 1 + __builtin_ctzll(later) : 64;
+
+  for (int q = 0; q < nq; q++) {
+    // 3. Stale parent-buffer reads: a lane group per replay block runs the
+    //    recurrence over the block's hanging leaves, B_k = W_ck (parent nodes
+    //    where present, else B_(k-1)), and stores B_k of each reader it passes.
     for (int b0 = 0; b0 < th.nbk; b0 += ngroups) {
       const int bi = b0 + grp;
       if (bi >= th.nbk) continue;
@@ -267,47 +282,64 @@ __device__ __forceinline__ void tile_body(const TileArgs &A, int tile, const T *
       int p = (int)((blk >> 32) & 0xFFFFu);
       const int pend = p + (int)(blk >> 48);
       int nxt = rdl_s[p] & 255;
+      // Parent value of this lane's slot for hanging index k (kNone: absent).
+      auto parent = [&](uint32_t cw, T &xp) {
+        const uint16_t pl = gploc[(size_t)(cw >> 8) * NPE + sl];
+        xp = (lane < NPE && pl != kNone) ? x_loc[pl * nq + q] : T(0);
+        return lane < NPE && pl == kNone;
+      };
       T B;
       int k;
       if (S == 0xFFFF) {
         B = cst[sl * nq + q];
         k = 0;
       } else {
-        B = fbuf[sl * FS + S];
+        const uint32_t cw0 = cw_s[S];
+        T xp;
+        parent(cw0, xp);
+        B = interp_lanes<T, DIM, P>(xp, lane, (int)(cw0 & 255u));
         k = S + 1;
       }
-      // Chain words and F values do not depend on B: loaded one step ahead.
-      uint64_t cw = cw_s[k];
-      T Fk = fbuf[sl * FS + k];
+      // Chain words and parent values do not depend on B: loaded one step ahead.
+      uint32_t cw = cw_s[k];
+      T xp;
+      bool ab = parent(cw, xp);
       for (;; k++) {
         const bool last = k >= E;
-        const uint64_t cwn = last ? cw : cw_s[k + 1];
-        const T Fn = fbuf[sl * FS + (last ? k : k + 1)];
-        const bool absent = lane < NPE && ((cw >> (32 + lane)) & 1u);
-        const T b = interp_lanes<T, DIM, P>(absent ? B : T(0), lane, (int)(cw & 15u));
+        const uint32_t cwn = last ? cw : cw_s[k + 1];
+        T xpn;
+        const bool abn = parent(cwn, xpn);
+        B = interp_lanes<T, DIM, P>(ab ? B : xp, lane, (int)(cw & 255u));
         if (k == nxt) {
-          if (lane < NPE) ubuf[p * NPE + lane] = b;
+          if (lane < NPE) ubuf[p * NPE + lane] = B;
           if (lane == 0) ridx_s[rdl_s[p] >> 8] = (uint16_t)p;
           p++;
           nxt = p < pend ? (rdl_s[p] & 255) : -1;
         }
         if (last) break;
-        B = Fk + b;
         cw = cwn;
-        Fk = Fn;
+        xp = xpn;
+        ab = abn;
       }
     }
     __syncthreads();
-    // 4. Element vector and elemental operator.
+    // 4. Element vector and elemental operator: a hanging leaf's missing ranks
+    //    interpolate its parent nodes (readers: the replayed buffer B_k).
     T v[NPE];
     if (active) {
       T u[NPE];
       const int ri = ridx_s[tid];
+      if (hang) {
+        T f[NPE];
 #pragma unroll
-      for (int r = 0; r < NPE; r++) {
-        T hv = T(0);
-        if (hang) hv = fbuf[r * FS + hidx] + (ri != kNone ? ubuf[ri * NPE + r] : T(0));
-        u[r] = loc[r] != kNone ? x_loc[loc[r] * nq + q] : hv;
+        for (int s = 0; s < NPE; s++) f[s] = ploc[s] != kNone ? x_loc[ploc[s] * nq + q] : T(0);
+        interp<T, DIM, P, false>(f, child);
+#pragma unroll
+        for (int r = 0; r < NPE; r++)
+          u[r] = loc[r] != kNone ? x_loc[loc[r] * nq + q] : (ri != kNone ? ubuf[ri * NPE + r] : f[r]);
+      } else {
+#pragma unroll
+        for (int r = 0; r < NPE; r++) u[r] = loc[r] != kNone ? x_loc[loc[r] * nq + q] : T(0);
       }
       if constexpr (P == 1) {
         if (A.op == 0)
@@ -333,31 +365,36 @@ __device__ __forceinline__ void tile_body(const TileArgs &A, int tile, const T *
       __syncthreads();
       if (loc[r] != kNone) y_loc[loc[r] * nq + q] += v[r];
     }
-    // 6. Transposed interpolation of missing ranks.
+    // 6. Transposed interpolation of missing ranks, summed over each sibling
+    //    group inside the wavefront (pairwise, fixed order), then per target
+    //    node over its (group, parent rank) entries sorted by target.
     if (th.nhe > 0) {
-      if (active) {
-        T w[NPE];
+      T w[NPE];
 #pragma unroll
-        for (int r = 0; r < NPE; r++) w[r] = (hang && loc[r] == kNone) ? v[r] : T(0);
-        interp<T, DIM, P, true>(w, child);
+      for (int r = 0; r < NPE; r++) w[r] = (hang && loc[r] == kNone) ? v[r] : T(0);
+      interp<T, DIM, P, true>(w, child);
 #pragma unroll
-        for (int s = 0; s < NPE; s++)
-          if (hang) fbuf[s * FS + hidx] = loc[s] == kNone ? w[s] : T(0);
+      for (int s = 0; s < NPE; s++) w[s] = (hang && loc[s] == kNone) ? w[s] : T(0);
+#pragma unroll
+      for (int d = 1; d < (1 << DIM); d <<= 1) {
+#pragma unroll
+        for (int s = 0; s < NPE; s++) {
+          const T o = __shfl_down(w[s], d, 64);
+          if (wl + d < run_end) w[s] += o;
+        }
       }
-      __syncthreads();
-      for (int k = tid; k < th.nhe; k += nthr) {
-        const uint64_t e = hent[k];
-        const int f0 = (int)(e & 255u), cnt = (int)((e >> 8) & 255u), s = (int)((e >> 16) & 255u);
-        T acc = T(0);
-        for (int l = 0; l < cnt; l++) acc += fbuf[s * FS + f0 + l];
-        ubuf[k] = acc;
-      }
+      if (hang && aux_ghead(aux))
+#pragma unroll
+        for (int s = 0; s < NPE; s++) ubuf[grp_id * NPE + s] = w[s];
       __syncthreads();
       for (int k = tid; k < th.nhe; k += nthr) {
         const int j = (int)((hent[k] >> 24) & 0xFFFFu);
         if (k > 0 && (int)((hent[k - 1] >> 24) & 0xFFFFu) == j) continue;  // not a segment head
         T acc = T(0);
-        for (int m = k; m < th.nhe && (int)((hent[m] >> 24) & 0xFFFFu) == j; m++) acc += ubuf[m];
+        for (int m = k; m < th.nhe && (int)((hent[m] >> 24) & 0xFFFFu) == j; m++) {
+          const uint64_t e = hent[m];
+          acc += ubuf[(int)(e & 0xFFFFu) * NPE + (int)((e >> 16) & 255u)];
+        }
         y_loc[j * nq + q] += acc;
       }
     }
@@ -389,12 +426,12 @@ __device__ __forceinline__ void tile_body(const TileArgs &A, int tile, const T *
   }
 }
 
-template <typename T, int DIM, int P>
+template <typename T, int DIM, int P, bool ONE>
 __global__ void __launch_bounds__(kTileThreads) tile_kernel(TileArgs A, const T *__restrict__ kref,
                                                             const T *__restrict__ carry_w, const T *__restrict__ x,
                                                             T *__restrict__ y, T *__restrict__ partial) {
   if (A.cancel && *A.cancel) return;
-  tile_body<T, DIM, P>(A, blockIdx.x, kref, carry_w, x, y, partial);
+  tile_body<T, DIM, P, ONE>(A, blockIdx.x, kref, carry_w, x, y, partial);
 }
 
 // A plan of a batch: its maps (with nq and ld set for the batch) and vectors.
@@ -419,12 +456,12 @@ __device__ __forceinline__ int find_range(const int *__restrict__ starts, int n,
 
 // All tiles of a batch of same-(dim, p) plans in one launch: workgroup b runs
 // tile b - tile_start[k] of plan k.
-template <typename T, int DIM, int P>
+template <typename T, int DIM, int P, bool ONE>
 __global__ void __launch_bounds__(kTileThreads) tile_kernel_batch(const BatchPlan<T> *__restrict__ plans,
                                                                   const int *__restrict__ tile_start, int nplans) {
   const int k = find_range(tile_start, nplans, blockIdx.x);
   const BatchPlan<T> &bp = plans[k];
-  tile_body<T, DIM, P>(bp.A, blockIdx.x - tile_start[k], bp.kref, bp.carry_w, bp.x, bp.y, bp.partial);
+  tile_body<T, DIM, P, ONE>(bp.A, blockIdx.x - tile_start[k], bp.kref, bp.carry_w, bp.x, bp.y, bp.partial);
 }
 
 template <typename T>
@@ -466,14 +503,11 @@ __global__ void __launch_bounds__(256) fixup_kernel_batch(const BatchPlan<T> *__
 // LDS bytes of one tile for nq right-hand sides (layout of tile_kernel).
 template <typename T, int NPE>
 inline size_t tile_lds_bytes(const TileDims &D, int nq) {
-  const size_t ulen = (size_t)(D.max_he > D.max_rd * NPE ? D.max_he : D.max_rd * NPE);
   auto r16 = [](size_t b) { return (b + 15) & ~(size_t)15; };
-  const size_t TL = (size_t)D.tile_leaves;
-  return 2 * r16((size_t)D.max_nodes * nq * sizeof(T)) +
-         r16((TL + 1) * NPE * sizeof(T)) + r16(ulen * sizeof(T)) +
-         r16((size_t)NPE * nq * sizeof(T)) + r16((size_t)D.max_cr * sizeof(T)) + r16((size_t)D.max_he * 8) +
-         r16(TL * 8) + r16((size_t)D.max_cr * 4) + r16((size_t)D.max_grp * NPE * 2) + r16(TL * 2) +
-         r16((size_t)D.max_rd * 2) + ((NPE & (NPE - 1)) ? r16((size_t)NPE * NPE * sizeof(T)) : 0);
+  return 2 * r16((size_t)D.max_nodes * nq * sizeof(T)) + r16(tile_region_bytes<T, NPE>(D)) +
+         r16((size_t)NPE * nq * sizeof(T)) + r16((size_t)D.max_he * 8) + r16((size_t)(D.max_hang + 1) * 4) +
+         r16((size_t)D.max_grp * NPE * 2) + r16((size_t)D.tile_leaves * 2) + r16((size_t)D.max_rd * 2) +
+         ((NPE & (NPE - 1)) ? r16((size_t)NPE * NPE * sizeof(T)) : 0);
 }
 
 }  // namespace dkt

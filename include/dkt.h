@@ -201,7 +201,8 @@ int dkt_batch_destroy(dkt_batch *batch);
  * sibling groups, mean nodes per tile, hanging-transpose entries, mean cone
  * length per reader, mean and max longest replay per tile with readers;
  * info[19] = leaves per tile; info[20] = max constraint terms per tile of a
- * tiled conforming plan. */
+ * tiled conforming plan; info[21] = max hanging leaves per tile; info[22] =
+ * LDS bytes per workgroup of the tiled kernel (FP64, one column). */
 int dkt_plan_info(const dkt_plan *plan, double *info, int n);
 
 /* Event timing of subsequent tiled matvecs on their stream (no counterpart in
