@@ -272,7 +272,8 @@ struct TiledHost {
   std::vector<int32_t> tnodes, tout, snode, soff;
   std::vector<uint16_t> lloc, gploc;
   std::vector<uint32_t> laux, ckey;
-  std::vector<uint64_t> hent, blk;
+  std::vector<uint32_t> hent;
+  std::vector<uint64_t> blk;
   std::vector<uint16_t> rdr;
   std::vector<double> cw;
   int max_nodes = 0, max_grp = 0, max_he = 0, max_rd = 0, max_cr = 0, max_hang = 0;
@@ -285,15 +286,33 @@ TiledHost build_tiled_host(const MeshParams &mp, const Traversal &tr, size_t N, 
   const int dim = mp.dim;
   const size_t L = tr.leaves.size();
   StaleBuffer sb(mp, tr);
-  // Tiles: runs of SFC-consecutive leaves, at most T leaves and kMaxReaders
-  // stale-buffer readers each (a reader is a hanging leaf with a missing rank
-  // that interpolates from a parent rank with no parent-level node).
+  // Tiles: runs of SFC-consecutive leaves, at most T leaves, kMaxReaders
+  // stale-buffer readers (a reader is a hanging leaf with a missing rank that
+  // interpolates from a parent rank with no parent-level node) and T / 4
+  // sibling groups each.
+  auto parent_of = [&](size_t e) {
+    Oct par = tr.leaves[e];
+    par.lev -= 1;
+    const uint32_t pmask = ~(uint32_t)(((uint64_t)1 << (mp.maxd - (int)par.lev)) - 1);
+    for (int d = 0; d < dim; d++) par.x[d] &= pmask;
+    return par;
+  };
+  auto same_oct = [&](const Oct &a, const Oct &b) {
+    bool same = a.lev == b.lev;
+    for (int d = 0; same && d < dim; d++) same = a.x[d] == b.x[d];
+    return same;
+  };
+  const int max_groups = std::max(1, T / 4);
   std::vector<size_t> bounds{0};
   {
-    int nrd = 0;
+    int nrd = 0, ngrp = 0;
+    bool prev_hang = false;
+    Oct prev_par{};
     for (size_t e = 0; e < L; e++) {
       bool reader = false;
       const int h = tr.hang_of[e];
+      Oct par{};
+      bool new_group = false;
       if (h >= 0) {
         const int32_t *sl = &tr.slot[e * npe];
         const int32_t *ps = &tr.pslot[(size_t)h * npe];
@@ -301,12 +320,18 @@ TiledHost build_tiled_host(const MeshParams &mp, const Traversal &tr, size_t N, 
           if (sl[r] < 0)
             for (const auto &tw : sb.weights(tr.child[e], r))
               if (ps[tw.first] < 0) reader = true;
+        par = parent_of(e);
+        new_group = !(prev_hang && same_oct(par, prev_par) && (e - bounds.back()) % 64 != 0);
       }
-      if (e - bounds.back() == (size_t)T || (reader && nrd == kMaxReaders)) {
+      if (e - bounds.back() == (size_t)T || (reader && nrd == kMaxReaders) || (new_group && ngrp == max_groups)) {
         bounds.push_back(e);
         nrd = 0;
+        ngrp = 0;
       }
       nrd += reader ? 1 : 0;
+      ngrp += new_group ? 1 : 0;
+      prev_hang = h >= 0;
+      prev_par = par;
     }
     if (bounds.back() != L) bounds.push_back(L);
   }
@@ -474,16 +499,33 @@ TiledHost build_tiled_host(const MeshParams &mp, const Traversal &tr, size_t N, 
       s_cone += (double)hi - start;
       n_cone += 1;
     }
-    // Buffer state at the tile start on the slots carried readers need.
+    // Buffer state at the tile start on the slots carried readers need: a
+    // header of nPe + 1 slot offsets, then the terms (slot << 16 | local node)
+    // sorted by slot; the weights run parallel (zero under the header).
+    size_t carry_terms = o.ckey.size();
     if (!entry.empty() && !hang_leaves.empty()) {
       const int before = tr.hang_of[hang_leaves[0]] - 1;  // last hanging leaf before the tile
+      std::vector<uint32_t> ck;
+      std::vector<double> cwl;
       if (before >= 0)
         for (int s : entry)
           for (const auto &e2 : sb.after((size_t)before, s)) {
             if (e2.second == 0.0) continue;
-            o.ckey.push_back(((uint32_t)s << 16) | loc_of(e2.first, false));
-            o.cw.push_back(e2.second);
+            ck.push_back(((uint32_t)s << 16) | loc_of(e2.first, false));
+            cwl.push_back(e2.second);
           }
+      if (!ck.empty()) {
+        std::vector<uint32_t> hdr_off(npe + 1, 0);
+        for (uint32_t k : ck) hdr_off[(k >> 16) + 1]++;
+        for (int s = 0; s < npe; s++) hdr_off[s + 1] += hdr_off[s];
+        for (int s = 0; s <= npe; s++) {
+          o.ckey.push_back(hdr_off[s] + (uint32_t)npe + 1);
+          o.cw.push_back(0.0);
+        }
+        carry_terms = o.ckey.size();
+        o.ckey.insert(o.ckey.end(), ck.begin(), ck.end());
+        o.cw.insert(o.cw.end(), cwl.begin(), cwl.end());
+      }
     }
     // Blocks of consecutive readers (by own index) with the smallest maximum
     // replay length, at most one block per lane group.
@@ -551,20 +593,20 @@ TiledHost build_tiled_host(const MeshParams &mp, const Traversal &tr, size_t N, 
         if (o.lloc[i] != kNone) o.lloc[i] = renum[o.lloc[i]];
       for (size_t i = (size_t)H.grp0 * npe; i < o.gploc.size(); i++)
         if (o.gploc[i] != kNone) o.gploc[i] = renum[o.gploc[i]];
-      for (size_t i = H.cr0; i < o.ckey.size(); i++)
+      for (size_t i = carry_terms; i < o.ckey.size(); i++)
         o.ckey[i] = (o.ckey[i] & 0xFFFF0000u) | renum[o.ckey[i] & 0xFFFFu];
     }
     // Hanging-transpose entries, sorted by target node for the segmented sum.
     std::sort(gl.begin(), gl.end());
     gl.erase(std::unique(gl.begin(), gl.end()), gl.end());
-    std::vector<std::pair<uint32_t, uint64_t>> ent;
+    std::vector<std::pair<uint32_t, uint32_t>> ent;
     for (const auto &g : gl) {
       const uint16_t j = o.gploc[(size_t)(H.grp0 + g.first) * npe + g.second];
-      const uint64_t v = (uint64_t)g.first | ((uint64_t)g.second << 16) | ((uint64_t)j << 24);
+      const uint32_t v = (uint32_t)g.first | ((uint32_t)g.second << 8) | ((uint32_t)j << 16);
       ent.push_back({(uint32_t)j, v});
     }
     std::stable_sort(ent.begin(), ent.end(),
-                     [](const std::pair<uint32_t, uint64_t> &a, const std::pair<uint32_t, uint64_t> &b) {
+                     [](const std::pair<uint32_t, uint32_t> &a, const std::pair<uint32_t, uint32_t> &b) {
                        return a.first < b.first;
                      });
     for (const auto &e : ent) o.hent.push_back(e.second);
@@ -590,8 +632,9 @@ TiledHost build_tiled_host(const MeshParams &mp, const Traversal &tr, size_t N, 
     if (H.nrd > 0) n_rtiles += 1;
     s_rd += H.nrd;
     m_rd = std::max<double>(m_rd, H.nrd);
-    s_cr += H.ncr;
-    m_cr = std::max<double>(m_cr, H.ncr);
+    const double nct = H.ncr ? H.ncr - (npe + 1) : 0;  // carry terms without the header
+    s_cr += nct;
+    m_cr = std::max<double>(m_cr, nct);
     s_gr += H.ngrp;
     s_nn += H.nnode;
   }
@@ -656,7 +699,7 @@ void build_tiled(dkt_plan *pl, const MeshParams &mp, const Traversal &tr, int re
   pl->snode.upload(nonempty(o.snode));
   pl->soff.upload(o.soff);
   pl->map_bytes = (double)(o.hdr.size() * sizeof(TileHdr) + o.tnodes.size() * 8 + o.lloc.size() * 2 +
-                           o.laux.size() * 4 + o.gploc.size() * 2 + o.hent.size() * 8 + o.rdr.size() * 2 +
+                           o.laux.size() * 4 + o.gploc.size() * 2 + o.hent.size() * 4 + o.rdr.size() * 2 +
                            o.blk.size() * 8 + o.ckey.size() * 12 + o.snode.size() * 8);
 }
 

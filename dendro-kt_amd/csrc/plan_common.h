@@ -87,6 +87,30 @@ struct Ip<2> {
 template <typename T, int DIM, int P, bool BWD>
 __device__ __forceinline__ void interp(T *a, int c) {
   constexpr int n1 = P + 1, NPE = Pow<P + 1, DIM>::v;
+  if constexpr (P == 1) {
+    // In place, one axis at a time over the pairs (a0, a1) of that axis:
+    // forward, the child's node at the child bit keeps the parent value and the
+    // other is the midpoint; backward is the transpose.
+#pragma unroll
+    for (int d = 0; d < DIM; d++) {
+      const bool cb = (c >> d) & 1;
+#pragma unroll
+      for (int r = 0; r < NPE; r++) {
+        if ((r >> d) & 1) continue;
+        const T a0 = a[r], a1 = a[r | (1 << d)];
+        if constexpr (!BWD) {
+          const T m = T(0.5) * (a0 + a1);
+          a[r] = cb ? m : a0;
+          a[r | (1 << d)] = cb ? a1 : m;
+        } else {
+          const T h = T(0.5) * (cb ? a0 : a1);
+          a[r] = cb ? h : a0 + h;
+          a[r | (1 << d)] = cb ? a1 + h : h;
+        }
+      }
+    }
+    return;
+  }
   T b[NPE];
 #pragma unroll
   for (int d = 0; d < DIM; d++) {
@@ -177,8 +201,8 @@ __host__ __device__ inline bool aux_hang(uint32_t a) { return (a >> 9) & 1u; }
 __host__ __device__ inline int aux_hidx(uint32_t a) { return (a >> 10) & 255u; }
 __host__ __device__ inline int aux_group(uint32_t a) { return (a >> 18) & 255u; }
 __host__ __device__ inline bool aux_ghead(uint32_t a) { return (a >> 26) & 1u; }
-// Hanging-transpose entry (uint64): sibling group (16) | parent rank (8) << 16 |
-// target local node (16) << 24.
+// Hanging-transpose entry (uint32): sibling group (8) | parent rank (8) << 8 |
+// target local node (16) << 16.
 // Reader (uint16): hanging index (8) | tile-local leaf (8) << 8.
 // Replay block (uint64): start hanging index (16, 0xFFFF = from the carried
 // tile-start state) | last hanging index (16) << 16 | first reader (16) << 32 |
@@ -196,10 +220,10 @@ struct TileArgs {
   const uint16_t *leaf_loc;
   const uint32_t *leaf_aux;
   const uint16_t *group_ploc;
-  const uint64_t *hentries;
+  const uint32_t *hentries;
   const uint16_t *readers;
   const uint64_t *blocks;
-  const uint32_t *carry_key;  // buffer rank (8) << 16 | local node (16)
+  const uint32_t *carry_key;  // per tile: nPe + 1 slot offsets, then buffer rank (8) << 16 | local node (16)
   const uint8_t *dir_in;   // non-null: x reads as zero at boundary nodes
   const uint8_t *dir_out;  // non-null: y is zero at boundary nodes
   const int *cancel;  // non-null: the tile returns at once when *cancel != 0

@@ -51,7 +51,8 @@ struct dkt_plan {
   dkt::DevBuf<int32_t> tile_nodes, tile_out, snode, soff;
   dkt::DevBuf<uint16_t> leaf_loc, group_ploc, readers;
   dkt::DevBuf<uint32_t> leaf_aux, carry_key;
-  dkt::DevBuf<uint64_t> hentries, blocks;
+  dkt::DevBuf<uint32_t> hentries;
+  dkt::DevBuf<uint64_t> blocks;
   // conforming tiles
   dkt::DevBuf<dkt::ConfTileHdr> ctiles;
   dkt::DevBuf<uint16_t> virt_off, term_node, term_virt, term_perm;

@@ -69,15 +69,21 @@ struct Group {
   static constexpr int v = NPE <= 4 ? 4 : (NPE <= 8 ? 8 : (NPE <= 16 ? 16 : 32));
 };
 
-// Bytes of the tile kernel's shared LDS region: carry terms (values, then
-// keys), reader values or per-group transposed sums, whichever is largest.
+// Elements of the tile kernel's y_loc: the accumulator, and with one column
+// (the single-column kernel) the groups' parent values before it.
+template <int NPE>
+__host__ __device__ inline size_t tile_yloc_elems(const TileDims &D, int nq) {
+  const size_t acc = (size_t)D.max_nodes * nq, par = (size_t)D.max_grp * NPE;
+  return nq == 1 && par > acc ? par : acc;
+}
+
+// Bytes of the tile kernel's shared LDS region: reader values or per-group
+// transposed sums, whichever is larger.
 template <typename T, int NPE>
 __host__ __device__ inline size_t tile_region_bytes(const TileDims &D) {
-  const size_t carry = (((size_t)D.max_cr * sizeof(T) + 15) & ~(size_t)15) + (size_t)D.max_cr * 4;
   const size_t readers = (size_t)D.max_rd * NPE * sizeof(T);
   const size_t groups = (size_t)D.max_grp * NPE * sizeof(T);
-  size_t m = carry > readers ? carry : readers;
-  m = m > groups ? m : groups;
+  const size_t m = readers > groups ? readers : groups;
   return m > 16 ? m : 16;
 }
 
@@ -171,16 +177,15 @@ __device__ __forceinline__ void tile_body(const TileArgs &A, int tile, const T *
     return p;
   };
   T *x_loc = reinterpret_cast<T *>(carve((size_t)D.max_nodes * nq * sizeof(T)));
-  T *y_loc = reinterpret_cast<T *>(carve((size_t)D.max_nodes * nq * sizeof(T)));
-  // Shared region: carry terms (until the tile-start state is formed), then
-  // reader values [reader][slot] (replay to element phase), then group sums
-  // [group][slot] (transposed phase).
-  unsigned char *region = carve(tile_region_bytes<T, NPE>(D));
-  T *ubuf = reinterpret_cast<T *>(region);
-  T *cwt_s = reinterpret_cast<T *>(region);
-  uint32_t *ckey_s = reinterpret_cast<uint32_t *>(region + (((size_t)D.max_cr * sizeof(T) + 15) & ~(size_t)15));
+  // ONE: y_loc first holds the parent values of the sibling groups [group][slot]
+  // (replay and element phases), then the accumulator.
+  T *y_loc = reinterpret_cast<T *>(carve(tile_yloc_elems<NPE>(D, nq) * sizeof(T)));
+  T *pval = y_loc;
+  // Shared region: reader values [reader][slot] (replay to element phase),
+  // then group sums [group][slot] (transposed phase).
+  T *ubuf = reinterpret_cast<T *>(carve(tile_region_bytes<T, NPE>(D)));
   T *cst = reinterpret_cast<T *>(carve((size_t)NPE * nq * sizeof(T)));  // parent buffer at the tile start
-  uint64_t *hent = reinterpret_cast<uint64_t *>(carve((size_t)D.max_he * 8));
+  uint32_t *hent = reinterpret_cast<uint32_t *>(carve((size_t)D.max_he * 4));
   uint32_t *cw_s = reinterpret_cast<uint32_t *>(carve((size_t)(D.max_hang + 1) * 4));  // chain word per hanging index
   uint16_t *gploc = reinterpret_cast<uint16_t *>(carve((size_t)D.max_grp * NPE * 2));
   uint16_t *ridx_s = reinterpret_cast<uint16_t *>(carve((size_t)TL * 2));
@@ -224,7 +229,7 @@ __device__ __forceinline__ void tile_body(const TileArgs &A, int tile, const T *
         const int j = j0 + b * nthr;
         if (g[b] >= 0) {
           x_loc[j * nq + q] = z[b] ? T(0) : xv[b];
-          y_loc[j * nq + q] = T(0);
+          if constexpr (!ONE) y_loc[j * nq + q] = T(0);
         }
       }
     }
@@ -232,10 +237,6 @@ __device__ __forceinline__ void tile_body(const TileArgs &A, int tile, const T *
   for (int k = tid; k < th.ngrp * NPE; k += nthr) gploc[k] = A.group_ploc[(size_t)th.grp0 * NPE + k];
   for (int k = tid; k < th.nhe; k += nthr) hent[k] = A.hentries[th.he0 + k];
   for (int k = tid; k < th.nrd; k += nthr) rdl_s[k] = A.readers[th.rd0 + k];
-  for (int k = tid; k < th.ncr; k += nthr) {
-    ckey_s[k] = A.carry_key[th.cr0 + k];
-    cwt_s[k] = carry_w[th.cr0 + k];
-  }
   const bool hang = active && aux_hang(aux);
   const int child = aux_child(aux);
   const int hidx = aux_hidx(aux);
@@ -245,20 +246,19 @@ __device__ __forceinline__ void tile_body(const TileArgs &A, int tile, const T *
   __syncthreads();
 
   // 2. Parent-buffer state at the tile start, every column: slot s sums its
-  //    terms (sorted by slot) in plan order.
+  //    terms (a header of slot offsets, then the terms by slot) in plan order.
   if (tid < NPE) {
-    int lo = 0, hi = th.ncr;
-    while (lo < hi) {
-      const int mid = (lo + hi) >> 1;
-      if ((int)(ckey_s[mid] >> 16) < tid) lo = mid + 1; else hi = mid;
-    }
+    const uint32_t *ck = A.carry_key + th.cr0;
+    const T *cwp = carry_w + th.cr0;
+    const int lo = th.ncr > 0 ? (int)ck[tid] : 0, hi = th.ncr > 0 ? (int)ck[tid + 1] : 0;
     for (int q = 0; q < nq; q++) {
       T acc = T(0);
-      for (int c = lo; c < th.ncr && (int)(ckey_s[c] >> 16) == tid; c++)
-        acc += cwt_s[c] * x_loc[(ckey_s[c] & 0xFFFFu) * nq + q];
+      for (int c = lo; c < hi; c++) acc += cwp[c] * x_loc[(ck[c] & 0xFFFFu) * nq + q];
       cst[tid * nq + q] = acc;
     }
   }
+  if constexpr (ONE)
+    for (int k = tid; k < th.ngrp * NPE; k += nthr) pval[k] = gploc[k] != kNone ? x_loc[gploc[k]] : T(0);
   __syncthreads();
   const uint16_t *ploc = gploc + (size_t)grp_id * NPE;
   const int lane = tid % G, grp = tid / G, ngroups = nthr / G;
@@ -284,8 +284,12 @@ __device__ __forceinline__ void tile_body(const TileArgs &A, int tile, const T *
       int nxt = rdl_s[p] & 255;
       // Parent value of this lane's slot for hanging index k (kNone: absent).
       auto parent = [&](uint32_t cw, T &xp) {
-        const uint16_t pl = gploc[(size_t)(cw >> 8) * NPE + sl];
-        xp = (lane < NPE && pl != kNone) ? x_loc[pl * nq + q] : T(0);
+        const size_t gi = (size_t)(cw >> 8) * NPE + sl;
+        const uint16_t pl = gploc[gi];
+        if constexpr (ONE)
+          xp = lane < NPE ? pval[gi] : T(0);
+        else
+          xp = (lane < NPE && pl != kNone) ? x_loc[pl * nq + q] : T(0);
         return lane < NPE && pl == kNone;
       };
       T B;
@@ -331,8 +335,13 @@ __device__ __forceinline__ void tile_body(const TileArgs &A, int tile, const T *
       const int ri = ridx_s[tid];
       if (hang) {
         T f[NPE];
+        if constexpr (ONE) {
 #pragma unroll
-        for (int s = 0; s < NPE; s++) f[s] = ploc[s] != kNone ? x_loc[ploc[s] * nq + q] : T(0);
+          for (int s = 0; s < NPE; s++) f[s] = pval[grp_id * NPE + s];
+        } else {
+#pragma unroll
+          for (int s = 0; s < NPE; s++) f[s] = ploc[s] != kNone ? x_loc[ploc[s] * nq + q] : T(0);
+        }
         interp<T, DIM, P, false>(f, child);
 #pragma unroll
         for (int r = 0; r < NPE; r++)
@@ -358,6 +367,11 @@ __device__ __forceinline__ void tile_body(const TileArgs &A, int tile, const T *
           v[i] = scale * acc;
         }
       }
+    }
+    if constexpr (ONE) {
+      // parent values are consumed: y_loc becomes the accumulator
+      __syncthreads();
+      for (int j = tid; j < th.nnode; j += nthr) y_loc[j] = T(0);
     }
     // 5. Direct outputs, one phase per rank.
 #pragma unroll
@@ -388,12 +402,12 @@ __device__ __forceinline__ void tile_body(const TileArgs &A, int tile, const T *
         for (int s = 0; s < NPE; s++) ubuf[grp_id * NPE + s] = w[s];
       __syncthreads();
       for (int k = tid; k < th.nhe; k += nthr) {
-        const int j = (int)((hent[k] >> 24) & 0xFFFFu);
-        if (k > 0 && (int)((hent[k - 1] >> 24) & 0xFFFFu) == j) continue;  // not a segment head
+        const int j = (int)(hent[k] >> 16);
+        if (k > 0 && (int)(hent[k - 1] >> 16) == j) continue;  // not a segment head
         T acc = T(0);
-        for (int m = k; m < th.nhe && (int)((hent[m] >> 24) & 0xFFFFu) == j; m++) {
-          const uint64_t e = hent[m];
-          acc += ubuf[(int)(e & 0xFFFFu) * NPE + (int)((e >> 16) & 255u)];
+        for (int m = k; m < th.nhe && (int)(hent[m] >> 16) == j; m++) {
+          const uint32_t e = hent[m];
+          acc += ubuf[(int)(e & 255u) * NPE + (int)((e >> 8) & 255u)];
         }
         y_loc[j * nq + q] += acc;
       }
@@ -504,8 +518,9 @@ __global__ void __launch_bounds__(256) fixup_kernel_batch(const BatchPlan<T> *__
 template <typename T, int NPE>
 inline size_t tile_lds_bytes(const TileDims &D, int nq) {
   auto r16 = [](size_t b) { return (b + 15) & ~(size_t)15; };
-  return 2 * r16((size_t)D.max_nodes * nq * sizeof(T)) + r16(tile_region_bytes<T, NPE>(D)) +
-         r16((size_t)NPE * nq * sizeof(T)) + r16((size_t)D.max_he * 8) + r16((size_t)(D.max_hang + 1) * 4) +
+  return r16((size_t)D.max_nodes * nq * sizeof(T)) + r16(tile_yloc_elems<NPE>(D, nq) * sizeof(T)) +
+         r16(tile_region_bytes<T, NPE>(D)) +
+         r16((size_t)NPE * nq * sizeof(T)) + r16((size_t)D.max_he * 4) + r16((size_t)(D.max_hang + 1) * 4) +
          r16((size_t)D.max_grp * NPE * 2) + r16((size_t)D.tile_leaves * 2) + r16((size_t)D.max_rd * 2) +
          ((NPE & (NPE - 1)) ? r16((size_t)NPE * NPE * sizeof(T)) : 0);
 }
