@@ -90,10 +90,15 @@ __device__ __forceinline__ void interp(T *a, int c) {
   if constexpr (P == 1) {
     // In place, one axis at a time over the pairs (a0, a1) of that axis:
     // forward, the child's node at the child bit keeps the parent value and the
-    // other is the midpoint; backward is the transpose.
+    // other is the midpoint; backward is the transpose,
+    //   cb = 0: (a0 + a1/2, a1/2),  cb = 1: (a0/2, a1 + a0/2),
+    // written with 0/1 factors: every product with them is exact, so each
+    // fma adds an exact zero or rounds once like the plain sum.
 #pragma unroll
     for (int d = 0; d < DIM; d++) {
       const bool cb = (c >> d) & 1;
+      const T one_c = cb ? T(1) : T(0), one_n = T(1) - one_c;
+      const T half_c = T(0.5) * one_c, half_n = T(0.5) * one_n;
 #pragma unroll
       for (int r = 0; r < NPE; r++) {
         if ((r >> d) & 1) continue;
@@ -103,9 +108,9 @@ __device__ __forceinline__ void interp(T *a, int c) {
           a[r] = cb ? m : a0;
           a[r | (1 << d)] = cb ? a1 : m;
         } else {
-          const T h = T(0.5) * (cb ? a0 : a1);
-          a[r] = cb ? h : a0 + h;
-          a[r | (1 << d)] = cb ? a1 + h : h;
+          const T h = fma(a0, half_c, a1 * half_n);
+          a[r] = fma(a0, one_n, h);
+          a[r | (1 << d)] = fma(a1, one_c, h);
         }
       }
     }

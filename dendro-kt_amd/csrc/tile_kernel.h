@@ -391,11 +391,9 @@ __device__ __forceinline__ void tile_body(const TileArgs &A, int tile, const T *
       for (int s = 0; s < NPE; s++) w[s] = (hang && loc[s] == kNone) ? w[s] : T(0);
 #pragma unroll
       for (int d = 1; d < (1 << DIM); d <<= 1) {
+        const T in_run = wl + d < run_end ? T(1) : T(0);  // o * 0 is an exact zero
 #pragma unroll
-        for (int s = 0; s < NPE; s++) {
-          const T o = __shfl_down(w[s], d, 64);
-          if (wl + d < run_end) w[s] += o;
-        }
+        for (int s = 0; s < NPE; s++) w[s] = fma(__shfl_down(w[s], d, 64), in_run, w[s]);
       }
       if (hang && aux_ghead(aux))
 #pragma unroll
