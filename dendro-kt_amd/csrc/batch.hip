@@ -89,7 +89,8 @@ struct Group : GroupBase {
         A.nq = nq;
         A.ld = nrhs;
         A.dims = dims_of(pl);
-        hp[k].kref = f64 ? (const T *)pl->kmat64.p : (const T *)pl->kmat32.p;
+        hp[k].kref = P == 2 ? (f64 ? (const T *)pl->kfac64.p : (const T *)pl->kfac32.p)
+                            : (f64 ? (const T *)pl->kmat64.p : (const T *)pl->kmat32.p);
         hp[k].carry_w = f64 ? (const T *)pl->carry_w64.p : (const T *)pl->carry_w32.p;
         hp[k].x = (const T *)xs[k] + q0;
         hp[k].y = (T *)ys[k] + q0;

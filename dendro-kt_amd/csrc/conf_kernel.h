@@ -60,16 +60,14 @@ __global__ void __launch_bounds__(kTileThreads) conf_tile_kernel(ConfArgs A, con
     return p;
   };
   T *x_loc = reinterpret_cast<T *>(carve((size_t)A.max_points * nq * sizeof(T)));
-  T *y_loc = reinterpret_cast<T *>(carve((size_t)A.max_points * nq * sizeof(T)));
+  // One column: the accumulator reuses x_loc once every element has read it.
+  T *y_loc = nq == 1 ? x_loc : reinterpret_cast<T *>(carve((size_t)A.max_points * nq * sizeof(T)));
   T *tw_s = reinterpret_cast<T *>(carve((size_t)A.max_terms * sizeof(T)));
   uint16_t *tj_s = reinterpret_cast<uint16_t *>(carve((size_t)A.max_terms * 2));
   uint16_t *tv_s = reinterpret_cast<uint16_t *>(carve((size_t)A.max_terms * 2));
   uint16_t *tp_s = reinterpret_cast<uint16_t *>(carve((size_t)A.max_terms * 2));
   uint16_t *vo_s = reinterpret_cast<uint16_t *>(carve((size_t)(A.max_virt + 1) * 2));
-  T *k_s = P == 1 ? nullptr : reinterpret_cast<T *>(carve((size_t)NPE * NPE * sizeof(T)));
   const int tid = threadIdx.x, nthr = blockDim.x;
-  if constexpr (P != 1)
-    for (int k = tid; k < NPE * NPE; k += nthr) k_s[k] = kref[k];
 
   // 1. Stage nodes and rows.
   const bool active = tid < th.nleaf;
@@ -90,7 +88,8 @@ __global__ void __launch_bounds__(kTileThreads) conf_tile_kernel(ConfArgs A, con
     const bool z = A.dir_in && A.dir_in[g];
     for (int q = 0; q < nq; q++) x_loc[j * nq + q] = z ? T(0) : x[(size_t)g * A.ld + q];
   }
-  for (int j = tid; j < npts * nq; j += nthr) y_loc[j] = T(0);
+  if (nq > 1)
+    for (int j = tid; j < npts * nq; j += nthr) y_loc[j] = T(0);
   for (int k = tid; k < th.nt; k += nthr) {
     tj_s[k] = A.term_node[th.t0 + k];
     tv_s[k] = A.term_virt[th.t0 + k];
@@ -127,14 +126,15 @@ __global__ void __launch_bounds__(kTileThreads) conf_tile_kernel(ConfArgs A, con
 #pragma unroll
           for (int i = 0; i < NPE; i++) v[i] = scale * u[i];
       } else {
+        q2_apply<T, DIM>(u, kref, A.op, scale);
 #pragma unroll
-        for (int i = 0; i < NPE; i++) {
-          T acc = T(0);
-#pragma unroll
-          for (int j = 0; j < NPE; j++) acc += k_s[i * NPE + j] * u[j];
-          v[i] = scale * acc;
-        }
+        for (int i = 0; i < NPE; i++) v[i] = u[i];
       }
+    }
+    if (nq == 1) {
+      __syncthreads();
+      for (int j = tid; j < npts; j += nthr) y_loc[j] = T(0);
+      __syncthreads();
     }
 #pragma unroll
     for (int r = 0; r < NPE; r++) {
@@ -170,14 +170,12 @@ __global__ void __launch_bounds__(kTileThreads) conf_tile_kernel(ConfArgs A, con
   }
 }
 
-// LDS bytes of conf_tile_kernel for nq right-hand sides (the p = 2 operator
-// is staged too when nPe is not a power of two).
+// LDS bytes of conf_tile_kernel for nq right-hand sides.
 template <typename T, int NPE>
 inline size_t conf_lds_bytes(int max_points, int max_virt, int max_terms, int nq) {
   auto r16 = [](size_t b) { return (b + 15) & ~(size_t)15; };
-  return 2 * r16((size_t)max_points * nq * sizeof(T)) + r16((size_t)max_terms * sizeof(T)) +
-         3 * r16((size_t)max_terms * 2) + r16((size_t)(max_virt + 1) * 2) +
-         ((NPE & (NPE - 1)) ? r16((size_t)NPE * NPE * sizeof(T)) : 0);
+  return (nq == 1 ? 1 : 2) * r16((size_t)max_points * nq * sizeof(T)) + r16((size_t)max_terms * sizeof(T)) +
+         3 * r16((size_t)max_terms * 2) + r16((size_t)(max_virt + 1) * 2);
 }
 
 }  // namespace dkt

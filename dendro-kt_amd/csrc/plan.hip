@@ -157,6 +157,83 @@ void one_d(int p, std::vector<double> &K, std::vector<double> &M) {
   }
 }
 
+// Simultaneous diagonalisation of the 1D p = 2 matrices: B (3 x 3) and lambda
+// with M1 = B^T B and K1 = B^T diag(lambda) B, so that the element mass is
+// (B^T)^(x dim) (B)^(x dim) and the stiffness (B^T)^(x dim) diag(sum_d lambda)
+// B^(x dim).  M1 = L L^T (Cholesky), C = L^-1 K1 L^-T = Q diag(lambda) Q^T
+// (Jacobi rotations), B = Q^T L^T.  Returns B row-major, then lambda.
+std::vector<double> q2_factors() {
+  std::vector<double> K1, M1;
+  one_d(2, K1, M1);
+  double L[3][3] = {};
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j <= i; j++) {
+      double s = M1[i * 3 + j];
+      for (int k = 0; k < j; k++) s -= L[i][k] * L[j][k];
+      L[i][j] = i == j ? std::sqrt(s) : s / L[j][j];
+    }
+  // Li = L^-1 (lower triangular)
+  double Li[3][3] = {};
+  for (int i = 0; i < 3; i++) {
+    Li[i][i] = 1.0 / L[i][i];
+    for (int j = 0; j < i; j++) {
+      double s = 0.0;
+      for (int k = j; k < i; k++) s -= L[i][k] * Li[k][j];
+      Li[i][j] = s / L[i][i];
+    }
+  }
+  double C[3][3] = {}, Q[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++)
+      for (int a = 0; a < 3; a++)
+        for (int b = 0; b < 3; b++) C[i][j] += Li[i][a] * K1[a * 3 + b] * Li[j][b];
+  for (int sweep = 0; sweep < 50; sweep++) {
+    double off = 0.0;
+    for (int p = 0; p < 3; p++)
+      for (int q = p + 1; q < 3; q++) off += C[p][q] * C[p][q];
+    if (off < 1e-40) break;
+    for (int p = 0; p < 3; p++)
+      for (int q = p + 1; q < 3; q++) {
+        if (std::fabs(C[p][q]) < 1e-300) continue;
+        const double th = 0.5 * (C[q][q] - C[p][p]) / C[p][q];
+        const double t = (th >= 0 ? 1.0 : -1.0) / (std::fabs(th) + std::sqrt(th * th + 1.0));
+        const double c = 1.0 / std::sqrt(t * t + 1.0), s = t * c;
+        for (int k = 0; k < 3; k++) {  // C <- J^T C J, Q <- Q J
+          const double ckp = C[k][p], ckq = C[k][q];
+          C[k][p] = c * ckp - s * ckq;
+          C[k][q] = s * ckp + c * ckq;
+        }
+        for (int k = 0; k < 3; k++) {
+          const double cpk = C[p][k], cqk = C[q][k];
+          C[p][k] = c * cpk - s * cqk;
+          C[q][k] = s * cpk + c * cqk;
+        }
+        for (int k = 0; k < 3; k++) {
+          const double qkp = Q[k][p], qkq = Q[k][q];
+          Q[k][p] = c * qkp - s * qkq;
+          Q[k][q] = s * qkp + c * qkq;
+        }
+      }
+  }
+  std::vector<double> out(12, 0.0);
+  for (int i = 0; i < 3; i++)
+    for (int k = 0; k < 3; k++)
+      for (int a = 0; a < 3; a++) out[i * 3 + k] += Q[a][i] * L[k][a];  // (Q^T L^T)[i][k]
+  for (int i = 0; i < 3; i++) out[9 + i] = C[i][i];
+  // Check the factorisation against the 1D matrices.
+  for (int a = 0; a < 3; a++)
+    for (int b = 0; b < 3; b++) {
+      double m = 0.0, k = 0.0;
+      for (int i = 0; i < 3; i++) {
+        m += out[i * 3 + a] * out[i * 3 + b];
+        k += out[i * 3 + a] * out[9 + i] * out[i * 3 + b];
+      }
+      if (std::fabs(m - M1[a * 3 + b]) > 1e-14 || std::fabs(k - K1[a * 3 + b]) > 1e-13)
+        throw std::runtime_error("p = 2 sum factorisation check failed");
+    }
+  return out;
+}
+
 // Stiffness = sum over axes t of tensor products with the 1D stiffness on t and
 // the 1D mass elsewhere; mass = tensor product of 1D masses.
 std::vector<double> ref_matrix(int dim, int p, int op) {
@@ -840,8 +917,7 @@ void build_conf_tiled(dkt_plan *pl, const MeshParams &mp, const Traversal &tr, i
     max_terms = std::max(max_terms, H.nt);
   }
   // Halve the tiles while one right-hand side in FP64 does not fit in LDS.
-  const size_t kbytes = (npe & (npe - 1)) ? (size_t)npe * npe * 8 : 0;
-  pl->lds_bytes = conf_lds_bytes<double, 4>(max_points, max_virt, max_terms, 1) + kbytes;
+  pl->lds_bytes = conf_lds_bytes<double, 4>(max_points, max_virt, max_terms, 1);
   if (pl->lds_bytes > (size_t)kLdsLimit && T > 16) return build_conf_tiled(pl, mp, tr, T / 2);
   // Owned nodes go to y; a node in several tiles gets a partial slot per tile
   // (node-major, tile order), summed by fixup_kernel.
@@ -924,7 +1000,8 @@ template <typename T, int DIM, int P>
 void run_tiled(dkt_plan *pl, int nrhs, const T *x, T *y, int mask, hipStream_t st, const int *cancel) {
   constexpr int NPE = Pow<P + 1, DIM>::v;
   const bool f64 = sizeof(T) == 8;
-  const T *kmat = f64 ? (const T *)pl->kmat64.p : (const T *)pl->kmat32.p;
+  const T *kmat = P == 2 ? (f64 ? (const T *)pl->kfac64.p : (const T *)pl->kfac32.p)
+                         : (f64 ? (const T *)pl->kmat64.p : (const T *)pl->kmat32.p);
   const T *cw = f64 ? (const T *)pl->carry_w64.p : (const T *)pl->carry_w32.p;
   auto lds_for = [&](int nq) {
     return tile_lds_bytes<T, NPE>(
@@ -989,7 +1066,8 @@ template <typename T, int DIM, int P>
 void run_conf(dkt_plan *pl, int nrhs, const T *x, T *y, int mask, hipStream_t st, const int *cancel) {
   constexpr int NPE = Pow<P + 1, DIM>::v;
   const bool f64 = sizeof(T) == 8;
-  const T *kmat = f64 ? (const T *)pl->kmat64.p : (const T *)pl->kmat32.p;
+  const T *kmat = P == 2 ? (f64 ? (const T *)pl->kfac64.p : (const T *)pl->kfac32.p)
+                         : (f64 ? (const T *)pl->kmat64.p : (const T *)pl->kmat32.p);
   const T *tw = f64 ? (const T *)pl->term_w64.p : (const T *)pl->term_w32.p;
   auto lds_for = [&](int nq) { return conf_lds_bytes<T, NPE>(pl->max_nodes, pl->max_virt, pl->max_terms, nq); };
   if (lds_for(1) > kLdsLimit) throw std::invalid_argument("tile does not fit in LDS");
@@ -1120,6 +1198,11 @@ extern "C" int dkt_plan_create(const dkt_da *da, int op, const double *op_data, 
     }
     pl->kmat64.upload(km);
     pl->kmat32.upload(cast<float>(km));
+    if (mp.order == 2) {
+      const std::vector<double> kf = q2_factors();
+      pl->kfac64.upload(kf);
+      pl->kfac32.upload(cast<float>(kf));
+    }
     std::vector<uint8_t> dir(pl->N, 0);
     for (u32 b : ns.bdy) dir[b] = 1;
     pl->dirichlet.upload(dir);
@@ -1128,7 +1211,7 @@ extern "C" int dkt_plan_create(const dkt_da *da, int op, const double *op_data, 
     if (pl->simple || pl->per_level)
       build_simple(pl.get(), mp, tr);
     else if (pl->conforming)
-      build_conf_tiled(pl.get(), mp, tr, tshift ? 1 << tshift : kDefaultTile);
+      build_conf_tiled(pl.get(), mp, tr, tshift ? 1 << tshift : 256);
     else
       build_tiled(pl.get(), mp, tr, tshift ? 1 << tshift : 0);
     *out = pl.release();

@@ -116,6 +116,35 @@ __device__ __forceinline__ void interp(T *a, int c) {
     }
     return;
   }
+  if constexpr (P == 2) {
+    // In place, one axis line (3 values) at a time.
+#pragma unroll
+    for (int d = 0; d < DIM; d++) {
+      const int cb = (c >> d) & 1;
+      constexpr int strides[4] = {1, 3, 9, 27};
+      const int st = strides[d];
+      T w[3][3];  // w[k][i]: parent node k -> child node i (forward)
+#pragma unroll
+      for (int k = 0; k < 3; k++)
+#pragma unroll
+        for (int i = 0; i < 3; i++) w[k][i] = cb ? (T)Ip<2>::w(1, k, i) : (T)Ip<2>::w(0, k, i);
+#pragma unroll
+      for (int r = 0; r < NPE; r++) {
+        if ((r / st) % 3 != 0) continue;
+        const T a0 = a[r], a1 = a[r + st], a2 = a[r + 2 * st];
+        if constexpr (!BWD) {
+          a[r] = w[0][0] * a0 + w[1][0] * a1 + w[2][0] * a2;
+          a[r + st] = w[0][1] * a0 + w[1][1] * a1 + w[2][1] * a2;
+          a[r + 2 * st] = w[0][2] * a0 + w[1][2] * a1 + w[2][2] * a2;
+        } else {
+          a[r] = w[0][0] * a0 + w[0][1] * a1 + w[0][2] * a2;
+          a[r + st] = w[1][0] * a0 + w[1][1] * a1 + w[1][2] * a2;
+          a[r + 2 * st] = w[2][0] * a0 + w[2][1] * a1 + w[2][2] * a2;
+        }
+      }
+    }
+    return;
+  }
   T b[NPE];
 #pragma unroll
   for (int d = 0; d < DIM; d++) {
@@ -183,6 +212,58 @@ __device__ __forceinline__ void q1_apply(const T *u, T *v, T scale) {
   fwht<T, DIM>(a);
 #pragma unroll
   for (int i = 0; i < NPE; i++) v[i] = scale * a[i];
+}
+
+// p = 2 element operators by sum factorisation, in place on u:
+// f = [B (3 x 3, row-major), lambda (3)] with M1 = B^T B, K1 = B^T diag(lambda) B;
+// u <- scale (B^T)^(x DIM) D B^(x DIM) u, D = sum_d lambda[i_d] (stiffness,
+// op 0) or 1 (mass, op 1); op 2 (identity) scales only.
+template <typename T, int DIM>
+__device__ __forceinline__ void q2_apply(T *u, const T *__restrict__ f, int op, T scale) {
+  constexpr int NPE = Pow<3, DIM>::v;
+  if (op == 2) {
+#pragma unroll
+    for (int i = 0; i < NPE; i++) u[i] *= scale;
+    return;
+  }
+  T B[9], lam[3];
+#pragma unroll
+  for (int k = 0; k < 9; k++) B[k] = f[k];
+#pragma unroll
+  for (int k = 0; k < 3; k++) lam[k] = f[9 + k];
+  constexpr int strides[4] = {1, 3, 9, 27};
+#pragma unroll
+  for (int d = 0; d < DIM; d++) {
+    const int st = strides[d];
+#pragma unroll
+    for (int r = 0; r < NPE; r++) {
+      if ((r / st) % 3 != 0) continue;
+      const T a0 = u[r], a1 = u[r + st], a2 = u[r + 2 * st];
+      u[r] = B[0] * a0 + B[1] * a1 + B[2] * a2;
+      u[r + st] = B[3] * a0 + B[4] * a1 + B[5] * a2;
+      u[r + 2 * st] = B[6] * a0 + B[7] * a1 + B[8] * a2;
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < NPE; r++) {
+    T dsum = T(0);
+    int rr = r;
+#pragma unroll
+    for (int d = 0; d < DIM; d++, rr /= 3) dsum += lam[rr % 3];
+    u[r] *= (op == 0 ? dsum : T(1)) * scale;
+  }
+#pragma unroll
+  for (int d = 0; d < DIM; d++) {
+    const int st = strides[d];
+#pragma unroll
+    for (int r = 0; r < NPE; r++) {
+      if ((r / st) % 3 != 0) continue;
+      const T a0 = u[r], a1 = u[r + st], a2 = u[r + 2 * st];
+      u[r] = B[0] * a0 + B[3] * a1 + B[6] * a2;
+      u[r + st] = B[1] * a0 + B[4] * a1 + B[7] * a2;
+      u[r + 2 * st] = B[2] * a0 + B[5] * a1 + B[8] * a2;
+    }
+  }
 }
 
 // One tile = one workgroup = up to kTileThreads SFC-consecutive leaves.

@@ -35,6 +35,9 @@ struct dkt_plan {
   dkt::DevBuf<uint8_t> dirichlet;
   dkt::DevBuf<double> kmat64;
   dkt::DevBuf<float> kmat32;
+  // p = 2: B (3 x 3) and lambda of the sum-factorised element operators (tile kernels)
+  dkt::DevBuf<double> kfac64;
+  dkt::DevBuf<float> kfac32;
   // simple path
   dkt::DevBuf<int32_t> slot, hang_of, pslot, hc_off, hc_node, csr_src;
   dkt::DevBuf<uint8_t> level, child, hc_rank;

@@ -190,12 +190,8 @@ __device__ __forceinline__ void tile_body(const TileArgs &A, int tile, const T *
   uint16_t *gploc = reinterpret_cast<uint16_t *>(carve((size_t)D.max_grp * NPE * 2));
   uint16_t *ridx_s = reinterpret_cast<uint16_t *>(carve((size_t)TL * 2));
   uint16_t *rdl_s = reinterpret_cast<uint16_t *>(carve((size_t)D.max_rd * 2));
-  // p >= 2: the reference element matrix, read by all lanes from LDS
-  T *k_s = P == 1 ? nullptr : reinterpret_cast<T *>(carve((size_t)NPE * NPE * sizeof(T)));
   const int tid = threadIdx.x;
   const int nthr = blockDim.x;
-  if constexpr (P != 1)
-    for (int k = tid; k < NPE * NPE; k += nthr) k_s[k] = kref[k];
 
   // 1. Stage.
   const bool active = tid < th.nleaf;
@@ -359,13 +355,9 @@ __device__ __forceinline__ void tile_body(const TileArgs &A, int tile, const T *
 #pragma unroll
           for (int i = 0; i < NPE; i++) v[i] = scale * u[i];
       } else {
+        q2_apply<T, DIM>(u, kref, A.op, scale);
 #pragma unroll
-        for (int i = 0; i < NPE; i++) {
-          T acc = T(0);
-#pragma unroll
-          for (int j = 0; j < NPE; j++) acc += k_s[i * NPE + j] * u[j];
-          v[i] = scale * acc;
-        }
+        for (int i = 0; i < NPE; i++) v[i] = u[i];
       }
     }
     if constexpr (ONE) {
@@ -383,21 +375,23 @@ __device__ __forceinline__ void tile_body(const TileArgs &A, int tile, const T *
     //    group inside the wavefront (pairwise, fixed order), then per target
     //    node over its (group, parent rank) entries sorted by target.
     if (th.nhe > 0) {
-      T w[NPE];
+      if (__ballot(hang)) {  // wavefronts without hanging leaves have no groups
+        T w[NPE];
 #pragma unroll
-      for (int r = 0; r < NPE; r++) w[r] = (hang && loc[r] == kNone) ? v[r] : T(0);
-      interp<T, DIM, P, true>(w, child);
+        for (int r = 0; r < NPE; r++) w[r] = (hang && loc[r] == kNone) ? v[r] : T(0);
+        interp<T, DIM, P, true>(w, child);
 #pragma unroll
-      for (int s = 0; s < NPE; s++) w[s] = (hang && loc[s] == kNone) ? w[s] : T(0);
+        for (int s = 0; s < NPE; s++) w[s] = (hang && loc[s] == kNone) ? w[s] : T(0);
 #pragma unroll
-      for (int d = 1; d < (1 << DIM); d <<= 1) {
-        const T in_run = wl + d < run_end ? T(1) : T(0);  // o * 0 is an exact zero
+        for (int d = 1; d < (1 << DIM); d <<= 1) {
+          const T in_run = wl + d < run_end ? T(1) : T(0);
 #pragma unroll
-        for (int s = 0; s < NPE; s++) w[s] = fma(__shfl_down(w[s], d, 64), in_run, w[s]);
+          for (int s = 0; s < NPE; s++) w[s] = fma(__shfl_down(w[s], d, 64), in_run, w[s]);
+        }
+        if (hang && aux_ghead(aux))
+#pragma unroll
+          for (int s = 0; s < NPE; s++) ubuf[grp_id * NPE + s] = w[s];
       }
-      if (hang && aux_ghead(aux))
-#pragma unroll
-        for (int s = 0; s < NPE; s++) ubuf[grp_id * NPE + s] = w[s];
       __syncthreads();
       for (int k = tid; k < th.nhe; k += nthr) {
         const int j = (int)(hent[k] >> 16);
@@ -519,8 +513,7 @@ inline size_t tile_lds_bytes(const TileDims &D, int nq) {
   return r16((size_t)D.max_nodes * nq * sizeof(T)) + r16(tile_yloc_elems<NPE>(D, nq) * sizeof(T)) +
          r16(tile_region_bytes<T, NPE>(D)) +
          r16((size_t)NPE * nq * sizeof(T)) + r16((size_t)D.max_he * 4) + r16((size_t)(D.max_hang + 1) * 4) +
-         r16((size_t)D.max_grp * NPE * 2) + r16((size_t)D.tile_leaves * 2) + r16((size_t)D.max_rd * 2) +
-         ((NPE & (NPE - 1)) ? r16((size_t)NPE * NPE * sizeof(T)) : 0);
+         r16((size_t)D.max_grp * NPE * 2) + r16((size_t)D.tile_leaves * 2) + r16((size_t)D.max_rd * 2);
 }
 
 }  // namespace dkt

@@ -45,8 +45,10 @@ extern "C" {
 #define DKT_PLAN_DRYRUN 2
 /* The conforming operator sum_e C_e^T K_e C_e (symmetric; see
  * dkt_da_conforming_terms) instead of the reference's traversal semantics;
- * runs on tiles of mesh elements (at most 2048 constraint terms per tile),
- * or on the per-leaf/per-node kernels with DKT_PLAN_SIMPLE. */
+ * runs on tiles of mesh elements (default 256, halved while the LDS
+ * footprint does not fit; at most 2048
+ * constraint terms per tile), or on the per-leaf/per-node kernels with
+ * DKT_PLAN_SIMPLE. */
 #define DKT_PLAN_CONFORMING 4
 /* Leaves per tile of the tiled path: flags |= log2(leaves) << DKT_PLAN_TILE_SHIFT
  * with leaves in {64, 128, 256}; 0 = 128, halved while the LDS footprint does
