@@ -33,6 +33,8 @@ PLAN_CONFORMING = 4
 PLAN_TILE_SHIFT = 8
 CG_F64 = 0
 CG_MIXED = 1
+CG_MIXED_RESTART = 2
+CG_MIXED_RELIABLE = 3
 BATCH_GRAPH = 1
 
 _lib = None

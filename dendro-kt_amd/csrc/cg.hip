@@ -13,12 +13,19 @@
 // return at once, and the host only checks the flag every kCgChunk
 // iterations.
 //
-// Mixed precision (DKT_CG_MIXED): iterative refinement.  The outer loop keeps
-// x, the residual r = b - A x and its norm in FP64 (FP64 matvec); each inner
-// solve runs the same CG with FP32 vectors and matvecs (FP32 accumulation
-// inside the matvec) and FP64 dot products and scalars on
-// A d = r / ||r||_inf to a relative residual of kInnerTol (or looser when
-// tol / ||r|| is larger), then x += ||r|| d.
+// Mixed precision, DKT_CG_MIXED_RELIABLE: CG with reliable updates.  The iteration
+// runs with FP32 vectors and matvecs (FP32 accumulation inside the matvec) and
+// FP64 dot products and scalars, accumulating its update of x in FP32.  When
+// the recursive FP32 residual has dropped by kReliableDrop since the last
+// update (or reaches tol), the update is added to x in FP64, the true residual
+// b - A x is formed with an FP64 matvec and replaces the FP32 one, and the
+// iteration continues with the same search direction (beta from the replaced
+// residual), so no Krylov information is discarded.  The solve stops on the
+// true FP64 residual.  A replacement that does not reduce the true residual
+// restarts the search direction.
+// DKT_CG_MIXED_RESTART is iterative refinement: each inner solve runs the same
+// FP32 CG from zero on A d = r / ||r||_inf to a relative residual of kInnerTol
+// (or looser when tol / ||r|| is larger), then x += ||r|| d.
 #include <algorithm>
 #include <cmath>
 
@@ -35,8 +42,9 @@ constexpr int kCgThreads = 256;
 constexpr int kCgChunk = 8;
 constexpr double kInnerTol = 1e-4;
 constexpr int kMaxOuter = 64;
+constexpr double kReliableDrop = 0.1;
 // Scalar slots.
-enum { S_RR0 = 0, S_RR1 = 1, S_NORMB = 2, S_RESID = 3, S_SCALE = 4, S_COUNT = 8 };
+enum { S_RR0 = 0, S_RR1 = 1, S_NORMB = 2, S_RESID = 3, S_SCALE = 4, S_BETA = 5, S_COUNT = 8 };
 
 template <typename T>
 __device__ __forceinline__ T block_sum(T v, T *sh) {
@@ -228,6 +236,57 @@ __global__ void __launch_bounds__(kCgThreads) k_add32(int n, double *__restrict_
   CG_LOOP(i, n) x[i] += s * (double)d32[i];
 }
 
+// Reliable-update kernels (FP32 iteration, FP64 x and residual).
+// x += x32, x32 = 0.
+__global__ void __launch_bounds__(kCgThreads) k_add32u(int n, double *__restrict__ x, float *__restrict__ x32) {
+  CG_LOOP(i, n) {
+    x[i] += (double)x32[i];
+    x32[i] = 0.0f;
+  }
+}
+
+// r32 = r64 (restart: p32 = r32 and x32 = 0 too); partials of r32.r32.
+__global__ void __launch_bounds__(kCgThreads) k_rel32(int n, const double *__restrict__ r64, float *__restrict__ r32,
+                                                      float *__restrict__ p32, float *__restrict__ x32, int restart,
+                                                      double *__restrict__ part) {
+  __shared__ double sh[kCgThreads];
+  double rr = 0;
+  CG_LOOP(i, n) {
+    const float ri = (float)r64[i];
+    r32[i] = ri;
+    if (restart) {
+      p32[i] = ri;
+      x32[i] = 0.0f;
+    }
+    rr += (double)ri * (double)ri;
+  }
+  rr = block_sum(rr, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = rr;
+}
+
+// New r.r into the slot the next iteration reads (par ^ 1), beta = new / old
+// r.r (0 on a restart), ||b|| from the FP64 residual, flags cleared.
+__global__ void __launch_bounds__(kCgThreads) k_rel32_fin(const double *__restrict__ part, double *__restrict__ scal32,
+                                                          const double *__restrict__ scal64, int par, int restart,
+                                                          int *__restrict__ flag) {
+  __shared__ double sh[kCgThreads];
+  const double rr = parts_sum(part, sh);
+  if (threadIdx.x == 0) {
+    scal32[S_BETA] = restart ? 0.0 : rr / scal32[S_RR0 + par];
+    scal32[S_RR0 + (par ^ 1)] = rr;
+    scal32[S_NORMB] = scal64[S_NORMB];
+    flag[0] = 0;
+    if (restart == 2) flag[1] = 0;  // first start: iteration count
+  }
+}
+
+// p = r + beta p (after a residual replacement).
+__global__ void __launch_bounds__(kCgThreads) k_pupd32(int n, float *__restrict__ p32, const float *__restrict__ r32,
+                                                       const double *__restrict__ scal32) {
+  const float beta = (float)scal32[S_BETA];
+  CG_LOOP(i, n) p32[i] = r32[i] + beta * p32[i];
+}
+
 template <typename T>
 struct CgVectors {
   T *r, *p, *ap;
@@ -308,6 +367,69 @@ void ensure_workspace(dkt_plan *pl, bool mixed) {
   }
 }
 
+// Mixed-precision CG with reliable updates (see the file comment).  Returns
+// the FP32 iterations; *resid is the final true relative residual.
+int cg_reliable(dkt_plan *pl, const double *b, double *x, int max_iter, double tol, int masked, hipStream_t st,
+                int *matvecs, int *updates, double *resid) {
+  CgWorkspace &w = pl->cg;
+  const int n = (int)pl->N;
+  const dim3 grid(kCgBlocks), blk(kCgThreads);
+  CgVectors<float> v{w.r32.p, w.p32.p, w.ap32.p, w.part32.p, w.scal32.p};
+  int *flag = w.flag.p, *host = w.host;
+  // True FP64 residual of the current x into d64 and scal[S_RESID].
+  auto true_residual = [&]() {
+    plan_apply(pl, 0, x, w.ap.p, 1, masked, st, nullptr);
+    (*matvecs)++;
+    hipLaunchKernelGGL(k_resid64, grid, blk, 0, st, n, b, w.ap.p, w.d64.p, w.part64.p);
+    hipLaunchKernelGGL(k_resid64_fin, dim3(1), blk, 0, st, w.part64.p, w.scal.p);
+    HIP_OK(hipGetLastError());
+    return read_scalar(w.scal.p, S_RESID, st);
+  };
+  *resid = true_residual();
+  if (*resid <= tol || max_iter == 0 || !std::isfinite(*resid)) return 0;
+  // Start: r32 = p32 = r, x32 = 0 (a restart; the r.r goes to slot 0, flags cleared).
+  hipLaunchKernelGGL(k_rel32, grid, blk, 0, st, n, w.d64.p, v.r, v.p, w.x32.p, 2, v.part);
+  hipLaunchKernelGGL(k_rel32_fin, dim3(1), blk, 0, st, v.part, v.scal, w.scal.p, 1, 2, flag);
+  HIP_OK(hipGetLastError());
+  double last = *resid;
+  int it = 0, stalls = 0;
+  double tol_in = std::max(tol, kReliableDrop * last);
+  while (it < max_iter) {
+    const int chunk = std::min(kCgChunk, max_iter - it);
+    for (int c = 0; c < chunk; c++) {
+      const int par = (it + c) & 1;
+      plan_apply(pl, 1, v.p, v.ap, 1, masked, st, flag);
+      hipLaunchKernelGGL((k_dot<float>), grid, blk, 0, st, n, v.p, v.ap, v.part + 2 * kCgBlocks, flag);
+      hipLaunchKernelGGL((k_update_xr<float>), grid, blk, 0, st, n, w.x32.p, v.r, v.p, v.ap, v.part + 2 * kCgBlocks,
+                         v.scal, par, v.part, flag);
+      hipLaunchKernelGGL((k_update_p<float>), grid, blk, 0, st, n, v.p, v.r, v.part, v.scal, par, tol_in, flag);
+      HIP_OK(hipGetLastError());
+    }
+    HIP_OK(hipMemcpyAsync(host, flag, 2 * sizeof(int), hipMemcpyDeviceToHost, st));
+    HIP_OK(hipStreamSynchronize(st));
+    it = host[1];  // iterations completed (the chunk's launches after a stop return at once)
+    if (!host[0] && it < max_iter) continue;
+    // Reliable update: fold the FP32 update into x, replace the residual.
+    hipLaunchKernelGGL(k_add32u, grid, blk, 0, st, n, x, w.x32.p);
+    *resid = true_residual();
+    (*updates)++;
+    if (*resid <= tol || it >= max_iter || !std::isfinite(*resid)) break;
+    // No progress since the last replacement: restart the search direction.
+    stalls = *resid < 0.9 * last ? 0 : stalls + 1;
+    const int restart = stalls > 0 ? 1 : 0;
+    if (stalls > 8) break;
+    const int par = (it - 1) & 1;  // parity of the iteration that stopped
+    hipLaunchKernelGGL(k_rel32, grid, blk, 0, st, n, w.d64.p, v.r, v.p, w.x32.p, restart, v.part);
+    hipLaunchKernelGGL(k_rel32_fin, dim3(1), blk, 0, st, v.part, v.scal, w.scal.p, par, restart, flag);
+    if (!restart) hipLaunchKernelGGL(k_pupd32, grid, blk, 0, st, n, v.p, v.r, v.scal);
+    HIP_OK(hipGetLastError());
+    last = std::min(last, *resid);
+    tol_in = std::max(tol, kReliableDrop * *resid);
+  }
+  *matvecs += it;
+  return it;
+}
+
 }  // namespace
 }  // namespace dkt
 
@@ -319,10 +441,14 @@ extern "C" int dkt_cg_solve(dkt_plan *pl, int mode, const double *b, double *x, 
     if (!pl || !b || !x) throw std::invalid_argument("null pointer");
     if (pl->dry) throw std::invalid_argument("a dry-run plan cannot solve");
     if (max_iter < 0 || !(tol >= 0.0)) throw std::invalid_argument("max_iter >= 0 and tol >= 0 required");
-    if (mode != DKT_CG_F64 && mode != DKT_CG_MIXED) throw std::invalid_argument("unknown CG mode");
+    if (mode < DKT_CG_F64 || mode > DKT_CG_MIXED_RELIABLE) throw std::invalid_argument("unknown CG mode");
+    // DKT_CG_MIXED: reliable updates on the symmetric (conforming) operator;
+    // restarted refinement on the reference traversal's operator, which is not
+    // symmetric at hanging nodes and on which restarts converge faster.
+    if (mode == DKT_CG_MIXED) mode = pl->conforming ? DKT_CG_MIXED_RELIABLE : DKT_CG_MIXED_RESTART;
     if ((const void *)b == (const void *)x) throw std::invalid_argument("b and x must not alias");
     hipStream_t st = (hipStream_t)stream;
-    const bool mixed = mode == DKT_CG_MIXED;
+    const bool mixed = mode != DKT_CG_F64;
     ensure_workspace(pl, mixed);
     CgWorkspace &w = pl->cg;
     int matvecs = 0, iters = 0, outer = 0;
@@ -331,6 +457,8 @@ extern "C" int dkt_cg_solve(dkt_plan *pl, int mode, const double *b, double *x, 
       CgVectors<double> v{w.r.p, w.p.p, w.ap.p, w.part64.p, w.scal.p};
       iters = cg_run<double>(pl, 0, b, x, false, max_iter, tol, masked ? 1 : 0, v, w.flag.p, w.host, st, &matvecs);
       resid = read_scalar(w.scal.p, S_RESID, st);
+    } else if (mode == DKT_CG_MIXED_RELIABLE) {
+      iters = cg_reliable(pl, b, x, max_iter, tol, masked ? 1 : 0, st, &matvecs, &outer, &resid);
     } else {
       const int n = (int)pl->N;
       CgVectors<float> v{w.r32.p, w.p32.p, w.ap32.p, w.part32.p, w.scal32.p};  // FP64 reductions

@@ -167,16 +167,24 @@ int dkt_plan_matvec(dkt_plan *plan, int dtype, const void *x, void *y, int nrhs,
  * (||b||_inf = 1 if b = 0) or after max_iter iterations.  masked = 1 uses the
  * Dirichlet-masked operator of feMatrix::matVec (as cgSolve does through
  * HeatMat's pre/post hooks); then b should vanish on boundary nodes.
- * mode DKT_CG_F64: all FP64.  DKT_CG_MIXED: iterative refinement with FP64
- * residuals and updates around inner CG solves with FP32 vectors and matvecs
- * (FP32 accumulation) and FP64 dot products.
+ * mode DKT_CG_F64: all FP64.  The mixed modes run CG with FP32 vectors and
+ * matvecs (FP32 accumulation) and FP64 dot products; the stop test uses the
+ * FP64 residual.  DKT_CG_MIXED_RELIABLE: reliable updates, the FP32 update of
+ * x is folded into x in FP64 and the residual replaced by the FP64 b - A x
+ * whenever the FP32 residual has dropped tenfold, keeping the search
+ * direction.  DKT_CG_MIXED_RESTART: iterative refinement, each FP32 inner
+ * solve restarted from zero on the FP64 residual.  DKT_CG_MIXED: reliable
+ * updates on conforming plans (symmetric operator), restarts otherwise (the
+ * reference operator is not symmetric at hanging nodes).
  * b, x: device double [N]; x holds the initial guess on entry.  Reductions
  * run in a fixed order (bitwise reproducible).  stats (may be NULL): [0]
- * CG iterations (inner iterations summed for DKT_CG_MIXED), [1] final relative
- * residual, [2] refinement steps, [3] matvecs.  Returns 0 also when max_iter
- * is reached; compare stats[1] with tol. */
+ * CG iterations (FP32 iterations for the mixed modes), [1] final relative
+ * residual, [2] residual replacements / refinement steps, [3] matvecs.
+ * Returns 0 also when max_iter is reached; compare stats[1] with tol. */
 #define DKT_CG_F64 0
 #define DKT_CG_MIXED 1
+#define DKT_CG_MIXED_RESTART 2
+#define DKT_CG_MIXED_RELIABLE 3
 int dkt_cg_solve(dkt_plan *plan, int mode, const double *b, double *x, int max_iter, double tol, int masked,
                  double *stats, void *stream);
 
