@@ -6,6 +6,7 @@
 #include <array>
 #include <cmath>
 #include <map>
+#include <memory>
 #include <mutex>
 #include <numeric>
 #include <functional>
@@ -395,6 +396,65 @@ struct InteriorKey {
 };
 }  // namespace
 
+// Exterior node instances of every element, sorted by location and level;
+// per location the coarsest instance is kept when the location is on the
+// domain boundary or carries several levels, else when all 2^(dim - cell dim)
+// incident cells produced it (no hanging node).  K holds the key
+// (coordinates with maxd + 1 bits each, then 5 bits of level).
+template <typename K>
+std::vector<Oct> resolve_instances(const MeshParams &mp, const std::vector<Oct> &elems, const std::vector<int> &extIdx) {
+  const int dim = mp.dim, maxd = mp.maxd, p = mp.order, cb = maxd + 1;
+  const int next = (int)extIdx.size();
+  const size_t total = elems.size() * (size_t)next;
+  std::unique_ptr<K[]> keys(new K[total]);  // filled in parallel (first touch)
+#pragma omp parallel for schedule(static)
+  for (size_t e = 0; e < elems.size(); e++) {
+    const Oct &el = elems[e];
+    const u64 len = (u64)1 << (maxd - el.lev);
+    for (int j = 0; j < next; j++) {
+      int rr = extIdx[j];
+      K k = 0;
+      for (int d = 0; d < dim; d++) {
+        const int i = rr % (p + 1);
+        rr /= (p + 1);
+        k = (k << cb) | (K)(u32)(len * (u64)i / (u64)p + el.x[d]);
+      }
+      keys[e * next + j] = (k << 5) | (K)el.lev;
+    }
+  }
+  __gnu_parallel::sort(keys.get(), keys.get() + total);
+  const K cmask = ((K)1 << cb) - 1;
+  auto decode = [&](K k) {
+    Oct o{};
+    o.lev = (u32)(k & 31);
+    k >>= 5;
+    for (int d = dim - 1; d >= 0; d--) {
+      o.x[d] = (u32)(k & cmask);
+      k >>= cb;
+    }
+    return o;
+  };
+  std::vector<Oct> sel;
+  sel.reserve(total / 4);
+  for (size_t b = 0; b < total;) {
+    size_t e = b + 1;
+    const K pos = keys[b] >> 5;
+    while (e < total && (keys[e] >> 5) == pos) e++;
+    const Oct first = decode(keys[b]);  // coarsest instance (sorted by level)
+    const bool mixed = (u32)(keys[e - 1] & 31) != first.lev;
+    bool keep;
+    if (on_boundary(first, dim, maxd) || mixed) {
+      keep = true;
+    } else {
+      const int cd = cell_dim(first, dim, maxd, first.lev);
+      keep = (e - b) == ((size_t)1 << (dim - cd));
+    }
+    if (keep) sel.push_back(first);
+    b = e;
+  }
+  return sel;
+}
+
 NodeSet enumerate_nodes(const MeshParams &mp, const std::vector<Oct> &elems) {
   const int dim = mp.dim, maxd = mp.maxd, p = mp.order;
   if (p < 1) throw std::invalid_argument("order must be >= 1");
@@ -402,8 +462,8 @@ NodeSet enumerate_nodes(const MeshParams &mp, const std::vector<Oct> &elems) {
   for (int d = 0; d < dim; d++) npe *= (p + 1), nint *= (p - 1);
   const int next = npe - nint;
 
-  // 1. Exterior node instances (Element::appendExteriorNodes, nsort.tcc:360-389).
-  std::vector<Oct> inst(elems.size() * (size_t)next);
+  // 1. Exterior node instances (Element::appendExteriorNodes, nsort.tcc:360-389),
+  //    as sortable keys: coordinates (maxd + 1 bits each) then the level.
   std::vector<int> extIdx;
   for (int r = 0; r < npe; r++) {
     bool ext = false;
@@ -415,62 +475,29 @@ NodeSet enumerate_nodes(const MeshParams &mp, const std::vector<Oct> &elems) {
     }
     if (ext) extIdx.push_back(r);
   }
-#pragma omp parallel for schedule(static)
-  for (size_t e = 0; e < elems.size(); e++) {
-    const Oct &el = elems[e];
-    const u64 len = (u64)1 << (maxd - el.lev);
-    for (int j = 0; j < next; j++) {
-      int rr = extIdx[j];
-      Oct o{};
-      o.lev = el.lev;
-      for (int d = 0; d < dim; d++) {
-        const int i = rr % (p + 1);
-        rr /= (p + 1);
-        o.x[d] = (u32)(len * (u64)i / (u64)p + el.x[d]);
-      }
-      inst[e * next + j] = o;
-    }
-  }
-
+  if (p > 2) throw std::invalid_argument("orders above 2 are not supported");
+  const int key_bits = dim * (maxd + 1) + 5;
+  if (key_bits > 128) throw std::invalid_argument("maxdepth too large for the node keys of this dim");
   // 2. Resolve duplicates and hanging nodes per location
   //    (resolveInterface_lowOrder nsort.tcc:1266-1313; boundary: coarsest wins, :913-926).
-  if (p > 2) throw std::invalid_argument("orders above 2 are not supported");
-  std::vector<std::pair<u128, u32>> srt(inst.size());
-#pragma omp parallel for schedule(static)
-  for (size_t i = 0; i < inst.size(); i++) srt[i] = {(pack(inst[i], dim) << 5) | inst[i].lev, (u32)i};
-  __gnu_parallel::sort(srt.begin(), srt.end());
-  std::vector<u32> sel;
-  sel.reserve(inst.size() / 2);
-  for (size_t b = 0; b < srt.size();) {
-    size_t e = b + 1;
-    const u128 pos = srt[b].first >> 5;
-    while (e < srt.size() && (srt[e].first >> 5) == pos) e++;
-    const Oct &first = inst[srt[b].second];  // coarsest instance (sorted by level)
-    const bool mixed = inst[srt[e - 1].second].lev != first.lev;
-    bool keep;
-    if (on_boundary(first, dim, maxd) || mixed) {
-      keep = true;
-    } else {
-      const int cd = cell_dim(first, dim, maxd, first.lev);
-      keep = (e - b) == ((size_t)1 << (dim - cd));
-    }
-    if (keep) sel.push_back(srt[b].second);
-    b = e;
-  }
-  srt.clear();
-  srt.shrink_to_fit();
-
+  const std::vector<Oct> inst = key_bits <= 64 ? resolve_instances<u64>(mp, elems, extIdx)
+                                                : resolve_instances<u128>(mp, elems, extIdx);
   // 3. Order: interior nodes by (container post-order, hyperplane, point key);
   //    boundary nodes after them, by (root bucket, point key).
   const SfcTables &t = sfc_tables(dim, mp.curve);
-  std::vector<InteriorKey> ik;
-  std::vector<std::pair<std::pair<int, u128>, u32>> bk;
-  for (u32 s : sel) {
+  const size_t nsel = inst.size();
+  std::vector<uint8_t> is_bdy(nsel);
+  std::vector<InteriorKey> ik_all(nsel);
+  std::vector<std::pair<std::pair<int, u128>, u32>> bk_all(nsel);
+#pragma omp parallel for schedule(static)
+  for (size_t si = 0; si < nsel; si++) {
+    const u32 s = (u32)si;
     const Oct &o = inst[s];
-    if (on_boundary(o, dim, maxd)) {
+    is_bdy[s] = on_boundary(o, dim, maxd);
+    if (is_bdy[s]) {
       unsigned m0 = 0;
       for (int d = 0; d < dim; d++) m0 |= ((o.x[d] >> maxd) & 1u) << d;
-      bk.push_back({{(int)t.inv[m0], point_key(t, o, dim, maxd)}, s});
+      bk_all[s] = {{(int)t.inv[m0], point_key(t, o, dim, maxd)}, s};
     } else {
       int maxlow = 0;
       for (int d = 0; d < dim; d++) {
@@ -485,9 +512,19 @@ NodeSet enumerate_nodes(const MeshParams &mp, const std::vector<Oct> &elems) {
           hp = d;
           break;
         }
-      ik.push_back({point_key(t, o, dim, maxd), lc, hp, s});
+      ik_all[s] = {point_key(t, o, dim, maxd), lc, hp, s};
     }
   }
+  std::vector<InteriorKey> ik;
+  std::vector<std::pair<std::pair<int, u128>, u32>> bk;
+  for (size_t s = 0; s < nsel; s++) {
+    if (is_bdy[s])
+      bk.push_back(bk_all[s]);
+    else
+      ik.push_back(ik_all[s]);
+  }
+  std::vector<InteriorKey>().swap(ik_all);
+  std::vector<std::pair<std::pair<int, u128>, u32>>().swap(bk_all);
   const int D = dim, MD = maxd;
   __gnu_parallel::sort(ik.begin(), ik.end(), [D, MD](const InteriorKey &a, const InteriorKey &b) {
     const int l = std::min(a.lc, b.lc);
@@ -501,7 +538,7 @@ NodeSet enumerate_nodes(const MeshParams &mp, const std::vector<Oct> &elems) {
   __gnu_parallel::sort(bk.begin(), bk.end());
 
   NodeSet ns;
-  ns.nodes.reserve(sel.size() + elems.size() * (size_t)nint);
+  ns.nodes.reserve(nsel + elems.size() * (size_t)nint);
   for (const auto &k : ik) ns.nodes.push_back(inst[k.idx]);
   for (const auto &k : bk) ns.nodes.push_back(inst[k.second]);
   ns.n_exterior = ns.nodes.size();
@@ -728,7 +765,7 @@ StaleBuffer::StaleBuffer(const MeshParams &mp, const Traversal &tr) : tr_(tr), n
       }
 }
 
-const Combo &StaleBuffer::after(size_t k0, int s0) {
+const Combo &StaleBuffer::after(size_t k0, int s0, std::unordered_map<uint64_t, Combo> &memo_) const {
   auto key = [this](size_t k, int s) { return (uint64_t)k * (uint64_t)npe_ + (uint64_t)s; };
   auto it = memo_.find(key(k0, s0));
   if (it != memo_.end()) return it->second;

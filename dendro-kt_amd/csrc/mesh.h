@@ -105,7 +105,9 @@ using Combo = std::vector<std::pair<int32_t, double>>;
 class StaleBuffer {
  public:
   StaleBuffer(const MeshParams &mp, const Traversal &tr);
-  const Combo &after(size_t k, int s);
+  const Combo &after(size_t k, int s) { return after(k, s, memo_); }
+  // Same, with a caller-owned memo (concurrent callers each pass their own).
+  const Combo &after(size_t k, int s, std::unordered_map<uint64_t, Combo> &memo) const;
   size_t leaf_of(size_t h) const { return leaf_of_[h]; }
   // Nonzero interpolation weights (parent rank s, w) of child rank r, child c.
   const std::vector<std::pair<int, double>> &weights(int c, int r) const { return wnz_[c][r]; }

@@ -14,6 +14,10 @@
 // Multi-RHS vectors are node-major, [node][nrhs] (the reference's
 // dof-interleaved layout, include/oda.h:296-322).
 #include <algorithm>
+#include <atomic>
+#include <exception>
+#include <mutex>
+#include <thread>
 #include <array>
 #include <cmath>
 #include <cstring>
@@ -380,6 +384,11 @@ TiledHost build_tiled_host(const MeshParams &mp, const Traversal &tr, size_t N, 
     return same;
   };
   const int max_groups = std::max(1, T / 4);
+  // wmask[c * nPe + r]: parent slots with a nonzero weight for child c, rank r.
+  std::vector<uint32_t> wmask((size_t)(1 << dim) * npe, 0);
+  for (int c = 0; c < (1 << dim); c++)
+    for (int r = 0; r < npe; r++)
+      for (const auto &tw : sb.weights(c, r)) wmask[(size_t)c * npe + r] |= 1u << tw.first;
   std::vector<size_t> bounds{0};
   {
     int nrd = 0, ngrp = 0;
@@ -393,10 +402,12 @@ TiledHost build_tiled_host(const MeshParams &mp, const Traversal &tr, size_t N, 
       if (h >= 0) {
         const int32_t *sl = &tr.slot[e * npe];
         const int32_t *ps = &tr.pslot[(size_t)h * npe];
-        for (int r = 0; r < npe && !reader; r++)
-          if (sl[r] < 0)
-            for (const auto &tw : sb.weights(tr.child[e], r))
-              if (ps[tw.first] < 0) reader = true;
+        uint32_t need = 0, abs_m = 0;
+        for (int r = 0; r < npe; r++) {
+          if (sl[r] < 0) need |= wmask[(size_t)tr.child[e] * npe + r];
+          if (ps[r] < 0) abs_m |= 1u << r;
+        }
+        reader = (need & abs_m) != 0;
         par = parent_of(e);
         new_group = !(prev_hang && same_oct(par, prev_par) && (e - bounds.back()) % 64 != 0);
       }
@@ -417,7 +428,7 @@ TiledHost build_tiled_host(const MeshParams &mp, const Traversal &tr, size_t N, 
   o.hdr.resize(ntiles);
   o.lloc.assign(L * (size_t)npe, kNone);
   o.laux.assign(L, 0);
-  std::vector<int32_t> last_tile(N, -1), ntarget(N, 0), local(N, -1);
+  std::vector<int32_t> last_tile(N, -1), ntarget(N, 0);
 
   // Pass 1: tiles that write each node (as a direct or hanging-transpose target).
   for (size_t t = 0; t < ntiles; t++) {
@@ -455,37 +466,62 @@ TiledHost build_tiled_host(const MeshParams &mp, const Traversal &tr, size_t N, 
   o.nshared = o.snode.size();
   o.npartial = (size_t)np;
 
-  // Pass 2: tiles.
-  std::fill(last_tile.begin(), last_tile.end(), -1);
-  double s_rd = 0, m_rd = 0, s_cr = 0, m_cr = 0, s_gr = 0, s_nn = 0, s_cone = 0, n_cone = 0;
-  double s_span = 0, m_span = 0, n_rtiles = 0;
-  for (size_t t = 0; t < ntiles; t++) {
-    const size_t l0 = bounds[t], l1 = bounds[t + 1];
-    TileHdr &H = o.hdr[t];
-    H.leaf0 = (int32_t)l0;
-    H.nleaf = (int32_t)(l1 - l0);
-    H.node0 = (int32_t)o.tnodes.size();
-    H.grp0 = (int32_t)(o.gploc.size() / npe);
-    H.he0 = (int32_t)o.hent.size();
-    H.rd0 = (int32_t)o.rdr.size();
-    H.bk0 = (int32_t)o.blk.size();
-    H.cr0 = (int32_t)o.ckey.size();
+  // Pass 2: tiles, built concurrently (each tile writes its own leaf rows and
+  // a TileOut), then concatenated in tile order.
+  struct TileOut {
+    std::vector<uint16_t> gploc, rdr;
+    std::vector<uint32_t> hent, ckey;
+    std::vector<uint64_t> blk;
+    std::vector<double> cw;
     std::vector<int32_t> touched;
     std::vector<uint8_t> is_target;
+    int ngrp = 0, nhang = 0;
+    double cone_sum = 0, cone_n = 0, span = -1;
+  };
+  // Tile-local node numbers: open addressing over global node ids.
+  struct LocalMap {
+    std::vector<int32_t> key, val;
+    std::vector<uint32_t> used;
+    LocalMap() : key(1 << 15, -1), val(1 << 15, 0) {}
+    void clear() {
+      for (uint32_t u : used) key[u] = -1;
+      used.clear();
+    }
+    // Returns the slot of n (key[slot] == n after the call) and whether it was new.
+    uint32_t find(int32_t n, bool *fresh) {
+      const uint32_t mask = (uint32_t)key.size() - 1;
+      uint32_t h = ((uint32_t)n * 2654435761u) & mask;
+      while (key[h] != -1 && key[h] != n) h = (h + 1) & mask;
+      *fresh = key[h] == -1;
+      if (*fresh) {
+        if (used.size() * 2 >= key.size()) throw std::runtime_error("tile has too many nodes");
+        key[h] = n;
+        used.push_back(h);
+      }
+      return h;
+    }
+  };
+  std::vector<TileOut> outs(ntiles);
+  auto build_tile = [&](size_t t, TileOut &to, LocalMap &lm, std::unordered_map<uint64_t, Combo> &memo) {
+    const size_t l0 = bounds[t], l1 = bounds[t + 1];
+    lm.clear();
+    std::vector<int32_t> &touched = to.touched;
+    std::vector<uint8_t> &is_target = to.is_target;
     auto loc_of = [&](int32_t n, bool target) -> uint16_t {
-      if (last_tile[n] != (int32_t)t) {
-        last_tile[n] = (int32_t)t;
-        local[n] = (int32_t)touched.size();
+      bool fresh;
+      const uint32_t h = lm.find(n, &fresh);
+      if (fresh) {
+        lm.val[h] = (int32_t)touched.size();
         touched.push_back(n);
         is_target.push_back(0);
       }
-      if (target) is_target[local[n]] = 1;
-      if (local[n] >= 0xFFFF) throw std::runtime_error("tile has too many nodes");
-      return (uint16_t)local[n];
+      const int32_t l = lm.val[h];
+      if (target) is_target[l] = 1;
+      if (l >= 0xFFFF) throw std::runtime_error("tile has too many nodes");
+      return (uint16_t)l;
     };
-    std::vector<std::pair<int, int>> gl;  // (group, parent rank) hanging-transpose targets
-    std::vector<uint8_t> gfirst, gcount;
-    std::vector<size_t> hang_leaves;      // leaf ids of the tile's hanging leaves, in order
+    std::vector<uint32_t> gl;  // per group: parent ranks that are hanging-transpose targets
+    std::vector<size_t> hang_leaves;  // leaf ids of the tile's hanging leaves, in order
     int cur_group = -1;
     Oct cur_parent{};
     for (size_t e = l0; e < l1; e++) {
@@ -501,31 +537,25 @@ TiledHost build_tiled_host(const MeshParams &mp, const Traversal &tr, size_t N, 
         continue;
       }
       aux |= 1u << 9;
-      Oct par = c;
-      par.lev = c.lev - 1;
-      const uint32_t pmask = ~(uint32_t)(((uint64_t)1 << (mp.maxd - (int)par.lev)) - 1);
-      for (int d = 0; d < dim; d++) par.x[d] = c.x[d] & pmask;
+      const Oct par = parent_of(e);
       // Groups end at 64-leaf boundaries: the kernel sums a group's transposed
       // values within one wavefront.
-      bool same = cur_group >= 0 && par.lev == cur_parent.lev && (e - l0) % 64 != 0;
-      for (int d = 0; same && d < dim; d++) same = par.x[d] == cur_parent.x[d];
+      const bool same = cur_group >= 0 && same_oct(par, cur_parent) && (e - l0) % 64 != 0;
       const int32_t *ps = &tr.pslot[(size_t)h * npe];
       if (!same) {
-        cur_group = (int)gfirst.size();
+        cur_group = (int)gl.size();
         if (cur_group > 255) throw std::runtime_error("tile has too many sibling groups");
         aux |= 1u << 26;
         cur_parent = par;
-        gfirst.push_back((uint8_t)hang_leaves.size());  // hanging index of the group's first leaf
-        gcount.push_back(0);
-        for (int s = 0; s < npe; s++) o.gploc.push_back(ps[s] >= 0 ? loc_of(ps[s], false) : kNone);
+        gl.push_back(0);
+        for (int s = 0; s < npe; s++) to.gploc.push_back(ps[s] >= 0 ? loc_of(ps[s], false) : kNone);
       }
-      gcount[cur_group]++;
       aux |= ((uint32_t)hang_leaves.size() << 10) | ((uint32_t)cur_group << 18);
       o.laux[e] = aux;
       for (int s = 0; s < npe; s++)
         if (ps[s] >= 0 && sl[s] < 0) {
           loc_of(ps[s], true);
-          gl.push_back({cur_group, s});
+          gl[cur_group] |= 1u << s;
         }
       hang_leaves.push_back(e);
     }
@@ -537,71 +567,75 @@ TiledHost build_tiled_host(const MeshParams &mp, const Traversal &tr, size_t N, 
     // as well, so readers are grouped into blocks, one replay per block.
     std::vector<std::pair<int, int>> rd;  // (start or -1, own hanging index)
     std::vector<uint8_t> rleaf;
-    std::set<int> entry;
+    uint32_t entry = 0;  // slot sets as bit masks (nPe <= 27)
+    auto absent = [&](size_t leaf) {
+      const int32_t *pk = &tr.pslot[(size_t)tr.hang_of[leaf] * npe];
+      uint32_t m = 0;
+      for (int s = 0; s < npe; s++)
+        if (pk[s] < 0) m |= 1u << s;
+      return m;
+    };
     for (size_t hi = 0; hi < hang_leaves.size(); hi++) {
       const size_t e = hang_leaves[hi];
-      const int hg = tr.hang_of[e];
       const int c = tr.child[e];
       const int32_t *sl = &tr.slot[e * npe];
-      const int32_t *ps = &tr.pslot[(size_t)hg * npe];
-      std::set<int> cur;
+      uint32_t cur = 0;
       for (int r = 0; r < npe; r++)
-        if (sl[r] < 0)
-          for (const auto &tw : sb.weights(c, r))
-            if (ps[tw.first] < 0) cur.insert(tw.first);
-      if (cur.empty()) continue;
+        if (sl[r] < 0) cur |= wmask[c * npe + r];
+      cur &= absent(e);
+      if (!cur) continue;
       int k = (int)hi - 1;
       int start = -1;
       while (true) {
         if (k < 0) {
-          entry.insert(cur.begin(), cur.end());
+          entry |= cur;
           break;
         }
         const size_t ek = hang_leaves[k];
         const int ck = tr.child[ek];
-        const int32_t *pk = &tr.pslot[(size_t)tr.hang_of[ek] * npe];
-        std::set<int> nxt;
-        for (int s : cur)
-          for (const auto &tw : sb.weights(ck, s))
-            if (pk[tw.first] < 0) nxt.insert(tw.first);
-        if (nxt.empty()) {
+        uint32_t nxt = 0;
+        for (uint32_t m = cur; m; m &= m - 1) nxt |= wmask[ck * npe + __builtin_ctz(m)];
+        nxt &= absent(ek);
+        if (!nxt) {
           start = k;
           break;
         }
-        cur.swap(nxt);
+        cur = nxt;
         k--;
       }
       rd.push_back({start, (int)hi});
       rleaf.push_back((uint8_t)(e - l0));
-      s_cone += (double)hi - start;
-      n_cone += 1;
+      to.cone_sum += (double)hi - start;
+      to.cone_n += 1;
     }
     // Buffer state at the tile start on the slots carried readers need: a
     // header of nPe + 1 slot offsets, then the terms (slot << 16 | local node)
     // sorted by slot; the weights run parallel (zero under the header).
-    size_t carry_terms = o.ckey.size();
-    if (!entry.empty() && !hang_leaves.empty()) {
+    size_t carry_terms = 0;
+    if (entry && !hang_leaves.empty()) {
       const int before = tr.hang_of[hang_leaves[0]] - 1;  // last hanging leaf before the tile
       std::vector<uint32_t> ck;
       std::vector<double> cwl;
       if (before >= 0)
-        for (int s : entry)
-          for (const auto &e2 : sb.after((size_t)before, s)) {
+        for (uint32_t m = entry; m; m &= m - 1) {
+          const int s = __builtin_ctz(m);
+          for (const auto &e2 : sb.after((size_t)before, s, memo)) {
             if (e2.second == 0.0) continue;
             ck.push_back(((uint32_t)s << 16) | loc_of(e2.first, false));
             cwl.push_back(e2.second);
           }
+        }
       if (!ck.empty()) {
         std::vector<uint32_t> hdr_off(npe + 1, 0);
         for (uint32_t k : ck) hdr_off[(k >> 16) + 1]++;
         for (int s = 0; s < npe; s++) hdr_off[s + 1] += hdr_off[s];
         for (int s = 0; s <= npe; s++) {
-          o.ckey.push_back(hdr_off[s] + (uint32_t)npe + 1);
-          o.cw.push_back(0.0);
+          to.ckey.push_back(hdr_off[s] + (uint32_t)npe + 1);
+          to.cw.push_back(0.0);
         }
-        carry_terms = o.ckey.size();
-        o.ckey.insert(o.ckey.end(), ck.begin(), ck.end());
-        o.cw.insert(o.cw.end(), cwl.begin(), cwl.end());
+        carry_terms = to.ckey.size();
+        to.ckey.insert(to.ckey.end(), ck.begin(), ck.end());
+        to.cw.insert(to.cw.end(), cwl.begin(), cwl.end());
       }
     }
     // Blocks of consecutive readers (by own index) with the smallest maximum
@@ -636,17 +670,14 @@ TiledHost build_tiled_host(const MeshParams &mp, const Traversal &tr, size_t N, 
         }
         mspan = std::max(mspan, mx - mn);
         const uint64_t S = mn < 0 ? 0xFFFFu : (uint64_t)mn;
-        o.blk.push_back(S | ((uint64_t)mx << 16) | ((uint64_t)(o.rdr.size() - H.rd0 + b.first) << 32) |
-                        ((uint64_t)(b.second - b.first) << 48));
+        to.blk.push_back(S | ((uint64_t)mx << 16) | ((uint64_t)b.first << 32) |
+                         ((uint64_t)(b.second - b.first) << 48));
       }
-      s_span += mspan;
-      m_span = std::max<double>(m_span, mspan);
-      for (size_t r = 0; r < rd.size(); r++) o.rdr.push_back((uint16_t)(rd[r].second | (rleaf[r] << 8)));
+      to.span = mspan;
+      for (size_t r = 0; r < rd.size(); r++) to.rdr.push_back((uint16_t)(rd[r].second | (rleaf[r] << 8)));
     }
-    H.ngrp = (int32_t)gfirst.size();
-    H.nrd = (int32_t)(o.rdr.size() - H.rd0);
-    H.nbk = (int32_t)(o.blk.size() - H.bk0);
-    H.ncr = (int32_t)(o.ckey.size() - H.cr0);
+    to.ngrp = (int)gl.size();
+    to.nhang = (int)hang_leaves.size();
     // Renumber the tile's local nodes in global node order, so that the x
     // gathers, the y writes and the partial-slot writes of consecutive
     // work-items go to increasing addresses.
@@ -662,38 +693,89 @@ TiledHost build_tiled_host(const MeshParams &mp, const Traversal &tr, size_t N, 
       for (size_t k = 0; k < nn; k++) {
         t2[k] = touched[ord[k]];
         it2[k] = is_target[ord[k]];
-        local[t2[k]] = (int32_t)k;
       }
       touched.swap(t2);
       is_target.swap(it2);
       for (size_t i = l0 * npe; i < l1 * npe; i++)
         if (o.lloc[i] != kNone) o.lloc[i] = renum[o.lloc[i]];
-      for (size_t i = (size_t)H.grp0 * npe; i < o.gploc.size(); i++)
-        if (o.gploc[i] != kNone) o.gploc[i] = renum[o.gploc[i]];
-      for (size_t i = carry_terms; i < o.ckey.size(); i++)
-        o.ckey[i] = (o.ckey[i] & 0xFFFF0000u) | renum[o.ckey[i] & 0xFFFFu];
+      for (uint16_t &g : to.gploc)
+        if (g != kNone) g = renum[g];
+      for (size_t i = carry_terms; i < to.ckey.size(); i++)
+        to.ckey[i] = (to.ckey[i] & 0xFFFF0000u) | renum[to.ckey[i] & 0xFFFFu];
     }
     // Hanging-transpose entries, sorted by target node for the segmented sum.
-    std::sort(gl.begin(), gl.end());
-    gl.erase(std::unique(gl.begin(), gl.end()), gl.end());
     std::vector<std::pair<uint32_t, uint32_t>> ent;
-    for (const auto &g : gl) {
-      const uint16_t j = o.gploc[(size_t)(H.grp0 + g.first) * npe + g.second];
-      const uint32_t v = (uint32_t)g.first | ((uint32_t)g.second << 8) | ((uint32_t)j << 16);
-      ent.push_back({(uint32_t)j, v});
-    }
+    for (size_t g = 0; g < gl.size(); g++)  // by group, then parent rank
+      for (uint32_t m = gl[g]; m; m &= m - 1) {
+        const int s = __builtin_ctz(m);
+        const uint16_t j = to.gploc[g * npe + s];
+        ent.push_back({(uint32_t)j, (uint32_t)g | ((uint32_t)s << 8) | ((uint32_t)j << 16)});
+      }
     std::stable_sort(ent.begin(), ent.end(),
                      [](const std::pair<uint32_t, uint32_t> &a, const std::pair<uint32_t, uint32_t> &b) {
                        return a.first < b.first;
                      });
-    for (const auto &e : ent) o.hent.push_back(e.second);
-    H.nhe = (int32_t)ent.size();
+    for (const auto &e : ent) to.hent.push_back(e.second);
+  };
+  {
+    const unsigned hw = std::thread::hardware_concurrency();
+    const int nthreads = (int)std::max(1u, std::min(16u, hw ? hw : 1u));
+    std::atomic<size_t> next{0};
+    std::exception_ptr err;
+    std::mutex err_mu;
+    auto worker = [&]() {
+      try {
+        LocalMap lm;
+        std::unordered_map<uint64_t, Combo> memo;
+        while (true) {
+          const size_t t0 = next.fetch_add(16);
+          if (t0 >= ntiles) break;
+          if (memo.size() > (1u << 20)) memo.clear();
+          for (size_t t = t0; t < std::min(ntiles, t0 + 16); t++) build_tile(t, outs[t], lm, memo);
+        }
+      } catch (...) {
+        std::lock_guard<std::mutex> g(err_mu);
+        if (!err) err = std::current_exception();
+        next.store(ntiles);
+      }
+    };
+    std::vector<std::thread> pool;
+    for (int i = 1; i < nthreads; i++) pool.emplace_back(worker);
+    worker();
+    for (auto &th : pool) th.join();
+    if (err) std::rethrow_exception(err);
+  }
+  // Concatenation in tile order; shared nodes get their partial slots in tile order.
+  double s_rd = 0, m_rd = 0, s_cr = 0, m_cr = 0, s_gr = 0, s_nn = 0, s_cone = 0, n_cone = 0;
+  double s_span = 0, m_span = 0, n_rtiles = 0;
+  for (size_t t = 0; t < ntiles; t++) {
+    TileOut &to = outs[t];
+    TileHdr &H = o.hdr[t];
+    H.leaf0 = (int32_t)bounds[t];
+    H.nleaf = (int32_t)(bounds[t + 1] - bounds[t]);
+    H.node0 = (int32_t)o.tnodes.size();
+    H.grp0 = (int32_t)(o.gploc.size() / npe);
+    H.he0 = (int32_t)o.hent.size();
+    H.rd0 = (int32_t)o.rdr.size();
+    H.bk0 = (int32_t)o.blk.size();
+    H.cr0 = (int32_t)o.ckey.size();
+    H.ngrp = to.ngrp;
+    H.nrd = (int32_t)to.rdr.size();
+    H.nbk = (int32_t)to.blk.size();
+    H.ncr = (int32_t)to.ckey.size();
+    H.nhe = (int32_t)to.hent.size();
+    H.nnode = (int32_t)to.touched.size();
+    o.gploc.insert(o.gploc.end(), to.gploc.begin(), to.gploc.end());
+    o.hent.insert(o.hent.end(), to.hent.begin(), to.hent.end());
+    o.rdr.insert(o.rdr.end(), to.rdr.begin(), to.rdr.end());
+    o.blk.insert(o.blk.end(), to.blk.begin(), to.blk.end());
+    o.ckey.insert(o.ckey.end(), to.ckey.begin(), to.ckey.end());
+    o.cw.insert(o.cw.end(), to.cw.begin(), to.cw.end());
     // Output classification of the tile's nodes.
-    H.nnode = (int32_t)touched.size();
-    for (size_t j = 0; j < touched.size(); j++) {
-      const int32_t n = touched[j];
+    for (size_t j = 0; j < to.touched.size(); j++) {
+      const int32_t n = to.touched[j];
       o.tnodes.push_back(n);
-      if (!is_target[j])
+      if (!to.is_target[j])
         o.tout.push_back(INT32_MIN);
       else if (ntarget[n] == 1)
         o.tout.push_back(n);
@@ -705,7 +787,7 @@ TiledHost build_tiled_host(const MeshParams &mp, const Traversal &tr, size_t N, 
     o.max_he = std::max(o.max_he, H.nhe);
     o.max_rd = std::max(o.max_rd, H.nrd);
     o.max_cr = std::max(o.max_cr, H.ncr);
-    o.max_hang = std::max(o.max_hang, (int)hang_leaves.size());
+    o.max_hang = std::max(o.max_hang, to.nhang);
     if (H.nrd > 0) n_rtiles += 1;
     s_rd += H.nrd;
     m_rd = std::max<double>(m_rd, H.nrd);
@@ -714,6 +796,14 @@ TiledHost build_tiled_host(const MeshParams &mp, const Traversal &tr, size_t N, 
     m_cr = std::max<double>(m_cr, nct);
     s_gr += H.ngrp;
     s_nn += H.nnode;
+    s_cone += to.cone_sum;
+    n_cone += to.cone_n;
+    if (to.span >= 0) {
+      s_span += to.span;
+      m_span = std::max(m_span, to.span);
+    }
+    TileOut().touched.swap(to.touched);  // release as we go
+    std::vector<uint8_t>().swap(to.is_target);
   }
   const double nt = std::max<double>(1, (double)ntiles);
   o.stats = {s_rd / nt, m_rd, s_cr / nt, m_cr, s_gr / nt, s_nn / nt, (double)o.hent.size(),
