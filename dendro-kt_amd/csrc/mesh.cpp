@@ -280,20 +280,46 @@ std::vector<Oct> construct_tree(const MeshParams &mp, std::vector<Oct> &pts, siz
 }
 
 namespace {
-inline u128 pack(const Oct &o, int dim) {
-  u128 v = 0;
-  for (int d = 0; d < dim; d++) v = (v << 32) | o.x[d];
-  return v;
-}
-void dedupe_level(std::vector<Oct> &v, int dim) {
-  std::vector<std::pair<u128, u32>> k(v.size());
-  for (size_t i = 0; i < v.size(); i++) k[i] = {pack(v[i], dim), (u32)i};
+// Distinct octants of v, all of level lev (any order), through sorted keys of
+// their coordinates (maxd + 1 bits each).  Entries with lev == kSkip are
+// dropped.
+constexpr u32 kSkip = 0xFFFFFFFFu;
+template <typename K>
+void dedupe_octs_t(std::vector<Oct> &v, int dim, int maxd, u32 lev) {
+  const int cb = maxd + 1;
+  std::vector<K> k(v.size());
+#pragma omp parallel for schedule(static)
+  for (size_t i = 0; i < v.size(); i++) {
+    if (v[i].lev == kSkip) {
+      k[i] = ~(K)0;  // above every key: coordinates never have all cb bits set
+      continue;
+    }
+    K key = 0;
+    for (int d = 0; d < dim; d++) key = (key << cb) | (K)v[i].x[d];
+    k[i] = key;
+  }
   __gnu_parallel::sort(k.begin(), k.end());
-  std::vector<Oct> out;
-  out.reserve(v.size());
-  for (size_t i = 0; i < k.size(); i++)
-    if (i == 0 || k[i].first != k[i - 1].first) out.push_back(v[k[i].second]);
-  v.swap(out);
+  k.erase(std::unique(k.begin(), k.end()), k.end());
+  while (!k.empty() && k.back() == ~(K)0) k.pop_back();
+  const K cmask = ((K)1 << cb) - 1;
+  v.resize(k.size());
+#pragma omp parallel for schedule(static)
+  for (size_t i = 0; i < k.size(); i++) {
+    K key = k[i];
+    Oct o{};
+    o.lev = lev;
+    for (int d = dim - 1; d >= 0; d--) {
+      o.x[d] = (u32)(key & cmask);
+      key >>= cb;
+    }
+    v[i] = o;
+  }
+}
+void dedupe_octs(std::vector<Oct> &v, int dim, int maxd, u32 lev) {
+  if (dim * (maxd + 1) <= 64)
+    dedupe_octs_t<u64>(v, dim, maxd, lev);
+  else
+    dedupe_octs_t<u128>(v, dim, maxd, lev);
 }
 }  // namespace
 
@@ -307,19 +333,22 @@ std::vector<Oct> balance_tree(const MeshParams &mp, std::vector<Oct> &pts, size_
   int nbr = 1;
   for (int d = 0; d < mp.dim; d++) nbr *= 3;
   for (int l = mp.maxd; l > 0; l--) {
-    std::vector<Oct> par;
-    par.reserve(lv[l].size());
     const u32 plen = 1u << (mp.maxd - (l - 1));
-    for (const Oct &o : lv[l]) {
-      Oct p = o;
+    std::vector<Oct> par(lv[l].size());
+#pragma omp parallel for schedule(static)
+    for (size_t i = 0; i < lv[l].size(); i++) {
+      Oct p = lv[l][i];
       p.lev = (u32)(l - 1);
-      for (int d = 0; d < mp.dim; d++) p.x[d] = o.x[d] & ~(plen - 1);
-      par.push_back(p);
+      for (int d = 0; d < mp.dim; d++) p.x[d] &= ~(plen - 1);
+      par[i] = p;
     }
-    dedupe_level(par, mp.dim);
+    dedupe_octs(par, mp.dim, mp.maxd, (u32)(l - 1));
     std::vector<Oct> &dst = lv[l - 1];
-    dst.reserve(dst.size() + par.size() * nbr);
-    for (const Oct &p : par) {
+    const size_t base = dst.size();
+    dst.resize(base + par.size() * (size_t)nbr);
+#pragma omp parallel for schedule(static)
+    for (size_t i = 0; i < par.size(); i++) {
+      const Oct &p = par[i];
       for (int code = 0; code < nbr; code++) {
         Oct q = p;
         int c = code;
@@ -335,10 +364,11 @@ std::vector<Oct> balance_tree(const MeshParams &mp, std::vector<Oct> &pts, size_
             else q.x[d] = p.x[d] + plen;
           }
         }
-        if (ok) dst.push_back(q);
+        if (!ok) q.lev = kSkip;
+        dst[base + i * nbr + code] = q;
       }
     }
-    dedupe_level(dst, mp.dim);
+    dedupe_octs(dst, mp.dim, mp.maxd, (u32)(l - 1));
   }
   std::vector<Oct> seeds;
   size_t tot = 0;
@@ -949,20 +979,126 @@ Traversal build_conforming(const MeshParams &mp, const std::vector<Oct> &elems, 
 }
 
 Traversal flatten_traversal(const MeshParams &mp, const std::vector<Oct> &nodes) {
-  Traversal tr;
   int npe = 1;
   for (int d = 0; d < mp.dim; d++) npe *= (mp.order + 1);
-  tr.nPe = npe;
-  Flattener f{mp, sfc_tables(mp.dim, mp.curve), nodes, tr, npe, {}, {}, {}};
-  f.buf.resize(mp.maxd + 2);
-  interp_1d(mp.order, 0, f.ip[0]);
-  interp_1d(mp.order, 1, f.ip[1]);
-  f.init_weights();
-  std::vector<int32_t> all(nodes.size());
-  std::iota(all.begin(), all.end(), 0);
+  const SfcTables &t = sfc_tables(mp.dim, mp.curve);
+  // The first kSplit levels of the recursion are expanded here, in visiting
+  // order, into independent subtree tasks (a subtree's records depend only on
+  // its cell, its node bucket and its parent's bucket); the tasks run
+  // concurrently into separate record lists, concatenated in order.
+  constexpr int kSplit = 2;
+  struct Ref {
+    int vec;
+    size_t off, n;
+  };
+  struct Task {
+    Oct cell, parent;
+    int rot;
+    Ref ids, pids;
+  };
+  std::vector<std::vector<int32_t>> store(1);
+  store[0].resize(nodes.size());
+  std::iota(store[0].begin(), store[0].end(), 0);
+  std::vector<Task> tasks;
+  std::function<void(const Oct &, int, Ref, Ref, const Oct &)> expand = [&](const Oct &cell, int rot, Ref ids, Ref pids,
+                                                                             const Oct &parent) {
+    if (ids.n == 0) return;
+    const int lev = (int)cell.lev;
+    const int32_t *id = store[ids.vec].data() + ids.off;
+    bool isLeaf = true;
+    for (size_t i = 0; i < ids.n && isLeaf; i++) isLeaf = (int)nodes[id[i]].lev <= lev;
+    if (isLeaf || lev >= kSplit) {
+      tasks.push_back({cell, parent, rot, ids, pids});
+      return;
+    }
+    // Same bucketing as Flattener::rec.
+    const int nch = t.nch;
+    const u32 h = 1u << (mp.maxd - lev - 1);
+    size_t cnt[16] = {0}, off[17];
+    std::vector<std::pair<unsigned, unsigned>> cc(ids.n);
+    for (size_t i = 0; i < ids.n; i++) {
+      const Oct &x = nodes[id[i]];
+      unsigned lo = 0, dup = 0;
+      for (int d = 0; d < mp.dim; d++) {
+        const u32 mid = cell.x[d] + h;
+        if (x.x[d] == mid)
+          dup |= 1u << d;
+        else if (x.x[d] > mid)
+          lo |= 1u << d;
+      }
+      cc[i] = {lo, dup};
+      for (unsigned sub = dup;; sub = (sub - 1) & dup) {
+        cnt[t.inv[rot * nch + (lo | sub)]]++;
+        if (sub == 0) break;
+      }
+    }
+    off[0] = 0;
+    for (int c = 0; c < nch; c++) off[c + 1] = off[c] + cnt[c];
+    const int vec = (int)store.size();
+    store.emplace_back(off[nch]);
+    id = store[ids.vec].data() + ids.off;  // store may have moved
+    std::vector<int32_t> &b = store[vec];
+    size_t fill[16];
+    for (int c = 0; c < nch; c++) fill[c] = off[c];
+    for (size_t i = 0; i < ids.n; i++) {
+      const unsigned lo = cc[i].first, dup = cc[i].second;
+      for (unsigned sub = 0;; sub = (sub - dup) & dup) {
+        b[fill[t.inv[rot * nch + (lo | sub)]]++] = id[i];
+        if (sub == dup) break;
+      }
+    }
+    for (int cs = 0; cs < nch; cs++) {
+      const unsigned m = t.perm[rot * nch + cs];
+      Oct ch = cell;
+      ch.lev = (u32)(lev + 1);
+      for (int d = 0; d < mp.dim; d++) ch.x[d] = cell.x[d] + (((m >> d) & 1u) ? h : 0u);
+      expand(ch, t.child[rot * nch + m], Ref{vec, off[cs], cnt[cs]}, ids, cell);
+    }
+  };
   Oct root{};
   root.lev = 0;
-  f.rec(root, 0, all.data(), all.size(), nullptr, 0, root);
+  expand(root, 0, Ref{0, 0, nodes.size()}, Ref{0, 0, 0}, root);
+  std::vector<Traversal> parts(tasks.size());
+  std::vector<std::string> errs(tasks.size());
+#pragma omp parallel for schedule(dynamic, 1)
+  for (size_t k = 0; k < tasks.size(); k++) {
+    try {
+      Traversal &pt = parts[k];
+      pt.nPe = npe;
+      Flattener f{mp, t, nodes, pt, npe, {}, {}, {}};
+      f.buf.resize(mp.maxd + 2);
+      interp_1d(mp.order, 0, f.ip[0]);
+      interp_1d(mp.order, 1, f.ip[1]);
+      const Task &tk = tasks[k];
+      const int32_t *pid = tk.pids.n ? store[tk.pids.vec].data() + tk.pids.off : nullptr;
+      f.rec(tk.cell, tk.rot, store[tk.ids.vec].data() + tk.ids.off, tk.ids.n, pid, tk.pids.n, tk.parent);
+    } catch (const std::exception &e) {
+      errs[k] = e.what();
+    }
+  }
+  for (const auto &e : errs)
+    if (!e.empty()) throw std::runtime_error(e);
+  Traversal tr;
+  tr.nPe = npe;
+  size_t L = 0, H = 0;
+  for (const auto &pt : parts) {
+    L += pt.leaves.size();
+    H += pt.pslot.size() / npe;
+  }
+  tr.leaves.reserve(L);
+  tr.child.reserve(L);
+  tr.slot.reserve(L * npe);
+  tr.hang_of.reserve(L);
+  tr.pslot.reserve(H * npe);
+  for (auto &pt : parts) {
+    const int32_t h0 = (int32_t)(tr.pslot.size() / npe);
+    tr.leaves.insert(tr.leaves.end(), pt.leaves.begin(), pt.leaves.end());
+    tr.child.insert(tr.child.end(), pt.child.begin(), pt.child.end());
+    tr.slot.insert(tr.slot.end(), pt.slot.begin(), pt.slot.end());
+    for (int32_t h : pt.hang_of) tr.hang_of.push_back(h < 0 ? h : h + h0);
+    tr.pslot.insert(tr.pslot.end(), pt.pslot.begin(), pt.pslot.end());
+    pt = Traversal();
+  }
   return tr;
 }
 
