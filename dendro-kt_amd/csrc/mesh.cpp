@@ -890,10 +890,10 @@ Traversal build_conforming(const MeshParams &mp, const std::vector<Oct> &elems, 
   double ip[2][9];
   interp_1d(p, 0, ip[0]);
   interp_1d(p, 1, ip[1]);
-  std::unordered_map<Key, Combo, KeyHash> memo;
+  using Memo = std::unordered_map<Key, Combo, KeyHash>;
   // Value at point x, a node position of the cell (anchor a, level lev).
-  std::function<const Combo &(const u32 *, const u32 *, int)> resolve = [&](const u32 *x, const u32 *a,
-                                                                            int lev) -> const Combo & {
+  std::function<const Combo &(const u32 *, const u32 *, int, Memo &)> resolve = [&](const u32 *x, const u32 *a, int lev,
+                                                                                    Memo &memo) -> const Combo & {
     Key key{0, 0, 0, 0, (u32)lev};
     for (int d = 0; d < dim; d++) key[d] = x[d];
     auto m = memo.find(key);
@@ -928,7 +928,7 @@ Traversal build_conforming(const MeshParams &mp, const std::vector<Oct> &elems, 
           y[d] = pa[d] + (u32)ud * pstep;
         }
         if (w == 0.0) continue;
-        add_scaled(v, resolve(y, pa, lev - 1), w);
+        add_scaled(v, resolve(y, pa, lev - 1, memo), w);
       }
       compact(v);
     }
@@ -942,36 +942,77 @@ Traversal build_conforming(const MeshParams &mp, const std::vector<Oct> &elems, 
   tr.slot.assign(E * npe, -1);
   tr.hang_of.assign(E, -1);
   tr.cf_off.assign(1, 0);
-  for (size_t e = 0; e < E; e++) {
-    const Oct &cell = elems[e];
-    const u32 len = 1u << (mp.maxd - cell.lev), step = len / (u32)p;
-    int c = 0;
-    for (int d = 0; d < dim; d++) c |= (int)((cell.x[d] / len) & 1u) << d;
-    tr.child[e] = (uint8_t)c;
-    bool missing = false;
-    for (int r = 0; r < npe; r++) {
-      Key k{0, 0, 0, 0, 0xFFFFFFFFu};
-      int rr = r;
-      for (int d = 0; d < dim; d++) {
-        k[d] = cell.x[d] + (u32)(rr % n1) * step;
-        rr /= n1;
-      }
-      auto f = at.find(k);
-      if (f != at.end()) {
-        tr.slot[e * npe + r] = f->second;
-        continue;
-      }
-      missing = true;
-      for (const auto &tw : resolve(k.data(), cell.x, (int)cell.lev)) {
-        tr.cf_rank.push_back((uint8_t)r);
-        tr.cf_node.push_back(tw.first);
-        tr.cf_w.push_back(tw.second);
+  // Elements in chunks on worker threads (each with its own memo of resolved
+  // points), terms concatenated in element order.
+  constexpr size_t kChunk = 2048;
+  const size_t nchunks = (E + kChunk - 1) / kChunk;
+  struct ChunkTerms {
+    std::vector<uint8_t> rank;
+    std::vector<int32_t> node;
+    std::vector<double> w;
+    std::vector<uint32_t> count;  // per element of the chunk: terms (0: no constrained rank)
+  };
+  std::vector<ChunkTerms> chunks(nchunks);
+  std::vector<std::string> errs(nchunks);
+#pragma omp parallel
+  {
+    Memo memo;
+#pragma omp for schedule(dynamic, 1)
+    for (size_t ci = 0; ci < nchunks; ci++) {
+      try {
+        if (memo.size() > (1u << 20)) memo.clear();
+        ChunkTerms &ct = chunks[ci];
+        const size_t e0 = ci * kChunk, e1 = std::min(E, e0 + kChunk);
+        ct.count.assign(e1 - e0, 0);
+        for (size_t e = e0; e < e1; e++) {
+          const Oct &cell = elems[e];
+          const u32 len = 1u << (mp.maxd - cell.lev), step = len / (u32)p;
+          int c = 0;
+          for (int d = 0; d < dim; d++) c |= (int)((cell.x[d] / len) & 1u) << d;
+          tr.child[e] = (uint8_t)c;
+          const size_t before = ct.node.size();
+          for (int r = 0; r < npe; r++) {
+            Key k{0, 0, 0, 0, 0xFFFFFFFFu};
+            int rr = r;
+            for (int d = 0; d < dim; d++) {
+              k[d] = cell.x[d] + (u32)(rr % n1) * step;
+              rr /= n1;
+            }
+            auto f = at.find(k);
+            if (f != at.end()) {
+              tr.slot[e * npe + r] = f->second;
+              continue;
+            }
+            for (const auto &tw : resolve(k.data(), cell.x, (int)cell.lev, memo)) {
+              ct.rank.push_back((uint8_t)r);
+              ct.node.push_back(tw.first);
+              ct.w.push_back(tw.second);
+            }
+          }
+          ct.count[e - e0] = (uint32_t)(ct.node.size() - before);
+        }
+      } catch (const std::exception &ex) {
+        errs[ci] = ex.what();
       }
     }
-    if (missing) {
-      tr.hang_of[e] = (int32_t)(tr.cf_off.size() - 1);
+  }
+  for (const auto &er : errs)
+    if (!er.empty()) throw std::runtime_error(er);
+  for (size_t ci = 0; ci < nchunks; ci++) {
+    ChunkTerms &ct = chunks[ci];
+    const size_t e0 = ci * kChunk;
+    size_t k = 0;
+    for (size_t i = 0; i < ct.count.size(); i++) {
+      if (!ct.count[i]) continue;
+      tr.hang_of[e0 + i] = (int32_t)(tr.cf_off.size() - 1);
+      for (uint32_t m = 0; m < ct.count[i]; m++, k++) {
+        tr.cf_rank.push_back(ct.rank[k]);
+        tr.cf_node.push_back(ct.node[k]);
+        tr.cf_w.push_back(ct.w[k]);
+      }
       tr.cf_off.push_back((int64_t)tr.cf_node.size());
     }
+    ct = ChunkTerms();
   }
   tr.pslot.assign((tr.cf_off.size() - 1) * (size_t)npe, -1);
   tr.cf_ready = true;
