@@ -470,6 +470,21 @@ __global__ void __launch_bounds__(kTileThreads) tile_kernel_batch(const BatchPla
   tile_body<T, DIM, P, ONE>(bp.A, blockIdx.x - tile_start[k], bp.kref, bp.carry_w, bp.x, bp.y, bp.partial);
 }
 
+// Sum of the partial slots [a, b) of column q in slot order (a < b); the
+// first 8 slots load together.
+template <typename T>
+__device__ __forceinline__ T sum_partials(const T *__restrict__ partial, int a, int b, int nq, int q) {
+  T v[8];
+#pragma unroll
+  for (int j = 0; j < 8; j++) v[j] = a + j < b ? partial[(size_t)(a + j) * nq + q] : T(0);
+  T acc = v[0];
+#pragma unroll
+  for (int j = 1; j < 8; j++)
+    if (a + j < b) acc += v[j];
+  for (int k = a + 8; k < b; k++) acc += partial[(size_t)k * nq + q];
+  return acc;
+}
+
 template <typename T>
 __global__ void __launch_bounds__(256) fixup_kernel(int ns, const int32_t *__restrict__ snode,
                                                     const int32_t *__restrict__ soff, const T *__restrict__ partial,
@@ -480,11 +495,7 @@ __global__ void __launch_bounds__(256) fixup_kernel(int ns, const int32_t *__res
   const int n = snode[i];
   const int a = soff[i], b = soff[i + 1];
   const bool z = dirichlet && dirichlet[n];
-  for (int q = 0; q < nq; q++) {
-    T acc = T(0);
-    for (int k = a; k < b; k++) acc += partial[(size_t)k * nq + q];
-    y[(size_t)n * ld + q] = z ? T(0) : acc;
-  }
+  for (int q = 0; q < nq; q++) y[(size_t)n * ld + q] = z ? T(0) : sum_partials(partial, a, b, nq, q);
 }
 
 // Shared nodes of a batch: workgroup b sums block b - block_start[k] of plan k.
@@ -499,11 +510,7 @@ __global__ void __launch_bounds__(256) fixup_kernel_batch(const BatchPlan<T> *__
   const int n = bp.snode[i];
   const int a = bp.soff[i], b = bp.soff[i + 1];
   const bool z = bp.A.dir_out && bp.A.dir_out[n];
-  for (int q = 0; q < nq; q++) {
-    T acc = T(0);
-    for (int s = a; s < b; s++) acc += bp.partial[(size_t)s * nq + q];
-    bp.y[(size_t)n * ld + q] = z ? T(0) : acc;
-  }
+  for (int q = 0; q < nq; q++) bp.y[(size_t)n * ld + q] = z ? T(0) : sum_partials(bp.partial, a, b, nq, q);
 }
 
 // LDS bytes of one tile for nq right-hand sides (layout of tile_kernel).
