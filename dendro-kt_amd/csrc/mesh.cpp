@@ -426,6 +426,49 @@ struct InteriorKey {
 };
 }  // namespace
 
+// A 256-bit key for dims whose coordinates and level do not fit 128 bits
+// (4D with maxdepth 30): the operations resolve_instances uses.
+struct WideKey {
+  u128 hi = 0, lo = 0;
+  WideKey() = default;
+  WideKey(u128 v) : hi(0), lo(v) {}
+  friend WideKey operator<<(WideKey a, int s) {
+    if (s == 0) return a;
+    WideKey r;
+    if (s >= 128) {
+      r.hi = a.lo << (s - 128);
+    } else {
+      r.hi = (a.hi << s) | (a.lo >> (128 - s));
+      r.lo = a.lo << s;
+    }
+    return r;
+  }
+  friend WideKey operator>>(WideKey a, int s) {
+    if (s == 0) return a;
+    WideKey r;
+    if (s >= 128) {
+      r.lo = a.hi >> (s - 128);
+    } else {
+      r.lo = (a.lo >> s) | (a.hi << (128 - s));
+      r.hi = a.hi >> s;
+    }
+    return r;
+  }
+  friend WideKey operator|(WideKey a, WideKey b) { a.hi |= b.hi; a.lo |= b.lo; return a; }
+  friend WideKey operator&(WideKey a, WideKey b) { a.hi &= b.hi; a.lo &= b.lo; return a; }
+  friend WideKey operator-(WideKey a, WideKey b) {
+    WideKey r;
+    r.lo = a.lo - b.lo;
+    r.hi = a.hi - b.hi - (a.lo < b.lo ? 1 : 0);
+    return r;
+  }
+  friend bool operator==(const WideKey &a, const WideKey &b) { return a.hi == b.hi && a.lo == b.lo; }
+  friend bool operator!=(const WideKey &a, const WideKey &b) { return !(a == b); }
+  friend bool operator<(const WideKey &a, const WideKey &b) { return a.hi != b.hi ? a.hi < b.hi : a.lo < b.lo; }
+  WideKey &operator>>=(int s) { return *this = *this >> s; }
+  explicit operator u32() const { return (u32)lo; }
+};
+
 // Exterior node instances of every element, sorted by location and level;
 // per location the coarsest instance is kept when the location is on the
 // domain boundary or carries several levels, else when all 2^(dim - cell dim)
@@ -507,11 +550,11 @@ NodeSet enumerate_nodes(const MeshParams &mp, const std::vector<Oct> &elems) {
   }
   if (p > 2) throw std::invalid_argument("orders above 2 are not supported");
   const int key_bits = dim * (maxd + 1) + 5;
-  if (key_bits > 128) throw std::invalid_argument("maxdepth too large for the node keys of this dim");
   // 2. Resolve duplicates and hanging nodes per location
   //    (resolveInterface_lowOrder nsort.tcc:1266-1313; boundary: coarsest wins, :913-926).
-  const std::vector<Oct> inst = key_bits <= 64 ? resolve_instances<u64>(mp, elems, extIdx)
-                                                : resolve_instances<u128>(mp, elems, extIdx);
+  const std::vector<Oct> inst = key_bits <= 64    ? resolve_instances<u64>(mp, elems, extIdx)
+                                : key_bits <= 128 ? resolve_instances<u128>(mp, elems, extIdx)
+                                                  : resolve_instances<WideKey>(mp, elems, extIdx);
   // 3. Order: interior nodes by (container post-order, hyperplane, point key);
   //    boundary nodes after them, by (root bucket, point key).
   const SfcTables &t = sfc_tables(dim, mp.curve);

@@ -183,3 +183,21 @@ def test_traversal_slots_cover_each_leaf_node_once(dkt):
         assert len(np.unique(present)) == len(present)
         if tr["hang_of"][e] < 0:
             assert np.all(s >= 0)
+
+
+def test_node_enumeration_independent_of_maxdepth_4d(dkt):
+    """The same 4D tree expressed at maxdepth 9 and 30 (coordinates scaled by
+    2^21) has the same CG nodes, in the same order, with the same levels.  At
+    maxdepth 30 the 4D location keys (4 x 31 coordinate bits + level) exceed 128
+    bits, so this runs the wide-key path of the enumeration."""
+    pts9 = dkt.gauss_points(4, 9, 3000, 4, 0.05, 3)
+    da9 = dkt.DA(dkt.balance_tree(pts9, 1), 1)
+    arr = pts9.array().copy()
+    arr[:, :4] <<= 21
+    arr[:, 4] += 21
+    da30 = dkt.DA(dkt.balance_tree(dkt.Octs.from_array(arr, 4, 30, dkt.CURVE_HILBERT), 1), 1)
+    n9, n30 = da9.nodes(), da30.nodes()
+    assert len(n9) == len(n30)
+    assert np.array_equal(n9[:, :4].astype(np.uint64) << 21, n30[:, :4].astype(np.uint64))
+    assert np.array_equal(n9[:, 4], n30[:, 4])
+    assert np.array_equal(da9.boundary(), da30.boundary())
