@@ -93,6 +93,7 @@ def load(path=None):
                                    c.POINTER(c.c_int32), c.POINTER(c.c_int32)]
     L.dkt_da_stale_terms.argtypes = [P, c.POINTER(c.c_int64), c.POINTER(c.c_uint8), c.POINTER(c.c_int32),
                                      c.POINTER(c.c_double)]
+    L.dkt_da_write_vtu.argtypes = [P, c.c_char_p, P, c.c_char_p]
     L.dkt_da_conforming_terms.argtypes = [P, c.POINTER(c.c_int64), c.POINTER(c.c_uint8), c.POINTER(c.c_int32),
                                           c.POINTER(c.c_double)]
     _bind_device(L)
@@ -259,6 +260,18 @@ class DA:
                                        hw.ctypes.data_as(c.POINTER(c.c_double))))
         return dict(cells=cells, child=child, slot=slot, hang_of=hang, pslot=pslot,
                     hc_off=off, hc_rank=hrank, hc_node=hnode, hc_w=hw)
+
+    def write_vtu(self, path, values=None, name="u"):
+        """ASCII .vtu of the mesh (dim 2 and 3): one cell per element, cell data
+        "level", and the nodal field `values` (host, length n_nodes) at the
+        corners (conforming interpolant at hanging corners)."""
+        ptr = None
+        if values is not None:
+            values = np.ascontiguousarray(values, dtype=np.float64)
+            if values.shape != (self.n_nodes,):
+                raise DktError("values must have one entry per node")
+            ptr = values.ctypes.data_as(ctypes.c_void_p)
+        _chk(load().dkt_da_write_vtu(self.h, str(path).encode(), ptr, name.encode()))
 
     def conforming_terms(self):
         """Conforming constraints: dict(off, rank, node, w), one row of off per mesh element

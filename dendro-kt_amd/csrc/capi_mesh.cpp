@@ -1,4 +1,5 @@
 // C ABI over the host mesh layer (include/dkt.h).
+#include <cstdio>
 #include <cstring>
 #include <memory>
 #include <exception>
@@ -254,6 +255,79 @@ extern "C" int dkt_da_conforming_terms(const dkt_da *da, int64_t *off, uint8_t *
     if (rank) std::memcpy(rank, t.cf_rank.data(), t.cf_rank.size());
     if (node) std::memcpy(node, t.cf_node.data(), t.cf_node.size() * sizeof(int32_t));
     if (w) std::memcpy(w, t.cf_w.data(), t.cf_w.size() * sizeof(double));
+  });
+}
+
+extern "C" int dkt_da_write_vtu(const dkt_da *da, const char *path, const double *values, const char *name) {
+  return guarded([&] {
+    if (!da || !path) throw std::invalid_argument("null pointer");
+    const int dim = da->mp.dim, p = da->mp.order, n1 = p + 1;
+    if (dim != 2 && dim != 3) throw std::invalid_argument("VTU output supports dim 2 and 3");
+    const Traversal &c = da->conforming();
+    int npe = 1;
+    for (int d = 0; d < dim; d++) npe *= n1;
+    const int ncorner = 1 << dim;
+    // VTK corner order (quad: 0 1 3 2; hexahedron: 0 1 3 2 4 5 7 6) of the
+    // lexicographic corners, and the element rank of each corner.
+    const int order2[4] = {0, 1, 3, 2}, order3[8] = {0, 1, 3, 2, 4, 5, 7, 6};
+    const int *vtk_order = dim == 2 ? order2 : order3;
+    int corner_rank[8];
+    for (int k = 0; k < ncorner; k++) {
+      int r = 0, stride = 1;
+      for (int d = 0; d < dim; d++, stride *= n1) r += ((k >> d) & 1) * p * stride;
+      corner_rank[k] = r;
+    }
+    FILE *f = std::fopen(path, "w");
+    if (!f) throw std::runtime_error(std::string("cannot open ") + path);
+    const size_t E = da->elems.size();
+    const double scale = 1.0 / (double)(1u << da->mp.maxd);
+    std::fprintf(f, "<?xml version=\"1.0\"?>\n<VTKFile type=\"UnstructuredGrid\" version=\"0.1\" byte_order=\"LittleEndian\">\n");
+    std::fprintf(f, "  <UnstructuredGrid>\n    <Piece NumberOfPoints=\"%zu\" NumberOfCells=\"%zu\">\n", E * ncorner, E);
+    std::fprintf(f, "      <Points>\n        <DataArray type=\"Float64\" NumberOfComponents=\"3\" format=\"ascii\">\n");
+    for (size_t e = 0; e < E; e++) {
+      const Oct &o = da->elems[e];
+      const double len = (double)(1u << (da->mp.maxd - o.lev)) * scale;
+      for (int v = 0; v < ncorner; v++) {
+        const int k = vtk_order[v];
+        double x[3] = {0, 0, 0};
+        for (int d = 0; d < dim; d++) x[d] = o.x[d] * scale + ((k >> d) & 1) * len;
+        std::fprintf(f, "%.17g %.17g %.17g\n", x[0], x[1], x[2]);
+      }
+    }
+    std::fprintf(f, "        </DataArray>\n      </Points>\n      <Cells>\n");
+    std::fprintf(f, "        <DataArray type=\"Int64\" Name=\"connectivity\" format=\"ascii\">\n");
+    for (size_t i = 0; i < E * ncorner; i++) std::fprintf(f, "%zu\n", i);
+    std::fprintf(f, "        </DataArray>\n        <DataArray type=\"Int64\" Name=\"offsets\" format=\"ascii\">\n");
+    for (size_t e = 1; e <= E; e++) std::fprintf(f, "%zu\n", e * ncorner);
+    std::fprintf(f, "        </DataArray>\n        <DataArray type=\"UInt8\" Name=\"types\" format=\"ascii\">\n");
+    for (size_t e = 0; e < E; e++) std::fprintf(f, "%d\n", dim == 2 ? 9 : 12);
+    std::fprintf(f, "        </DataArray>\n      </Cells>\n      <CellData Scalars=\"level\">\n");
+    std::fprintf(f, "        <DataArray type=\"Int32\" Name=\"level\" format=\"ascii\">\n");
+    for (size_t e = 0; e < E; e++) std::fprintf(f, "%u\n", da->elems[e].lev);
+    std::fprintf(f, "        </DataArray>\n      </CellData>\n");
+    if (values) {
+      const char *nm = name && *name ? name : "u";
+      std::fprintf(f, "      <PointData Scalars=\"%s\">\n        <DataArray type=\"Float64\" Name=\"%s\" format=\"ascii\">\n",
+                   nm, nm);
+      for (size_t e = 0; e < E; e++) {
+        const int h = c.hang_of[e];
+        for (int v = 0; v < ncorner; v++) {
+          const int r = corner_rank[vtk_order[v]];
+          const int32_t node = c.slot[e * npe + r];
+          double val = 0.0;
+          if (node >= 0) {
+            val = values[node];
+          } else {
+            for (int64_t k = c.cf_off[h]; k < c.cf_off[h + 1]; k++)
+              if (c.cf_rank[k] == r) val += c.cf_w[k] * values[c.cf_node[k]];
+          }
+          std::fprintf(f, "%.17g\n", val);
+        }
+      }
+      std::fprintf(f, "        </DataArray>\n      </PointData>\n");
+    }
+    std::fprintf(f, "    </Piece>\n  </UnstructuredGrid>\n</VTKFile>\n");
+    if (std::fclose(f) != 0) throw std::runtime_error(std::string("write failed: ") + path);
   });
 }
 

@@ -136,6 +136,14 @@ int dkt_da_stale_terms(const dkt_da *da, int64_t *off, uint8_t *rank, int32_t *n
  * off: int64 [count 9 + 1]; rank, node, w: [count 8]. */
 int dkt_da_conforming_terms(const dkt_da *da, int64_t *off, uint8_t *rank, int32_t *node, double *w);
 
+/* Writes the mesh as an ASCII VTK unstructured grid (.vtu) for dim 2 (quads)
+ * and 3 (hexahedra), one cell per element with its own corner points in
+ * [0,1]^dim, cell data "level", and, when values (double [count 1], host) is
+ * not NULL, point data `name` at the corners: the node's value, or at a
+ * hanging corner the conforming interpolant.  Replaces the reference's
+ * oct2vtu (IO/vtk/include/oct2vtk.h:272-273); 4D meshes are refused. */
+int dkt_da_write_vtu(const dkt_da *da, const char *path, const double *values, const char *name);
+
 /* ---- Device plan and matvec -------------------------------------------- */
 /* Uploads the flattened traversal of `da` to the current HIP device.
  * op: DKT_OP_*; op_data: for DKT_OP_CAPTURED, double [(maxdepth+1)][nPe][nPe]

@@ -204,3 +204,39 @@ def test_conforming_tiled_matches_simple(dkt, dim, p, seed):
             plan.matvec_torch(Xd[:, c].contiguous(), yc, masked=1)
             torch.cuda.synchronize()
             assert torch.equal(Y[:, c], yc)
+
+
+@pytest.mark.parametrize("dim,p,seed", [(2, 1, 1), (3, 1, 2), (3, 2, 5)])
+def test_write_vtu_linear_field(dkt, dim, p, seed, tmp_path):
+    """VTU output: one cell per element (quad / hexahedron), corner points in
+    [0,1]^dim, cell levels, and a linear nodal field reproduced exactly at every
+    corner, hanging corners included (the conforming interpolant of a linear
+    function is the function)."""
+    import xml.etree.ElementTree as ET
+    da, maxd = _da(dkt, dim, p, seed)
+    nodes = da.nodes()
+    coef = np.linspace(0.7, -0.4, dim)
+    f = 0.25 + (nodes[:, :dim] / float(1 << maxd)) @ coef
+    path = tmp_path / "mesh.vtu"
+    da.write_vtu(path, f, "phi")
+    root = ET.parse(path).getroot()
+    piece = root.find("UnstructuredGrid/Piece")
+    ne = len(da.elements())
+    nc = 1 << dim
+    assert int(piece.get("NumberOfCells")) == ne and int(piece.get("NumberOfPoints")) == ne * nc
+    arrays = {a.get("Name") or "points": np.array(a.text.split(), dtype=float) for a in piece.iter("DataArray")}
+    pts = arrays["points"].reshape(-1, 3)
+    assert pts.min() >= 0.0 and pts.max() <= 1.0
+    assert np.all(arrays["types"] == (9 if dim == 2 else 12))
+    assert np.array_equal(arrays["level"], da.elements()[:, dim].astype(float))
+    want = 0.25 + pts[:, :dim] @ coef
+    assert np.abs(arrays["phi"] - want).max() <= 1e-13
+    # VTK corner order: the second point of every cell differs from the first along x only
+    d01 = pts[1::nc] - pts[0::nc]
+    assert np.all(d01[:, 0] > 0) and np.all(d01[:, 1:] == 0)
+
+
+def test_write_vtu_refuses_4d(dkt):
+    da, _ = _da(dkt, 4, 1, 3)
+    with pytest.raises(dkt.DktError):
+        da.write_vtu("/dev/null")
