@@ -22,8 +22,9 @@
 //      so a phase has no write conflicts);
 //   6. the transposed interpolation of missing ranks (matvec.h:482-522) is
 //      summed over each sibling group with cross-lane moves inside the
-//      wavefront, then per target node in a segmented sum over the (group,
-//      parent rank) entries sorted by target (no conflicts);
+//      wavefront (DPP row shifts, one permute across rows), then per target
+//      node in a segmented sum over the (group, parent rank) entries sorted
+//      by target (no conflicts);
 //   7. nodes touched by this tile only are written to y; nodes shared with
 //      other tiles go to partial slots, summed in tile order by fixup_kernel.
 // All reductions run in an order fixed by the plan: bitwise reproducible,
@@ -108,6 +109,16 @@ __device__ __forceinline__ T xor_lane(T a) {
   else if constexpr (D == 1) return dpp_mov<0x4E>(a);
   else if constexpr (D == 2) return dpp_mov<0x1B>(dpp_mov<0x141>(a));  // (lane ^ 7) ^ 3
   else return dpp_mov<0x128>(a);
+}
+
+// One step of the sibling-group sum: w[s] += (value of lane wl + D in the same
+// row of 16) when that lane is still in the run.  Products with the 0/1 factor
+// are exact, so the fma adds an exact zero or rounds once like the plain sum.
+template <typename T, int CTRL, int D, int N>
+__device__ __forceinline__ void group_step(T (&w)[N], int wl, int run_end) {
+  const T f = (wl + D < run_end && (wl & 15) + D < 16) ? T(1) : T(0);
+#pragma unroll
+  for (int s = 0; s < N; s++) w[s] = fma(dpp_mov<CTRL>(w[s]), f, w[s]);
 }
 
 // Forward parent-to-child interpolation with the buffer spread over a lane
@@ -382,11 +393,19 @@ __device__ __forceinline__ void tile_body(const TileArgs &A, int tile, const T *
         interp<T, DIM, P, true>(w, child);
 #pragma unroll
         for (int s = 0; s < NPE; s++) w[s] = (hang && loc[s] == kNone) ? w[s] : T(0);
+        // Pairwise sums inside rows of 16 lanes with DPP row shifts (lane i
+        // reads lane i + d of its row; a 0/1 factor keeps the run), then a run
+        // that continues into the next row adds that row's first partial (a
+        // group spans at most two rows).
+        group_step<T, 0x101, 1>(w, wl, run_end);
+        if constexpr (DIM > 1) group_step<T, 0x102, 2>(w, wl, run_end);
+        if constexpr (DIM > 2) group_step<T, 0x104, 4>(w, wl, run_end);
+        if constexpr (DIM > 3) group_step<T, 0x108, 8>(w, wl, run_end);
+        {
+          const int nxt = (wl | 15) + 1;
+          const T fx = nxt < run_end ? T(1) : T(0);
 #pragma unroll
-        for (int d = 1; d < (1 << DIM); d <<= 1) {
-          const T in_run = wl + d < run_end ? T(1) : T(0);
-#pragma unroll
-          for (int s = 0; s < NPE; s++) w[s] = fma(__shfl_down(w[s], d, 64), in_run, w[s]);
+          for (int s = 0; s < NPE; s++) w[s] = fma(__shfl(w[s], nxt & 63, 64), fx, w[s]);
         }
         if (hang && aux_ghead(aux))
 #pragma unroll
