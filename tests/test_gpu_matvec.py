@@ -53,6 +53,38 @@ def test_matvec_matches_reference(dkt, name, simple):
         assert err32 <= TOL32, (name, key, err32)
 
 
+@pytest.mark.parametrize("tile", [64, 256])
+@pytest.mark.parametrize("name", fx.names())
+def test_tile_sizes_match_reference(dkt, name, tile):
+    """The 64- and 256-leaf tilings (one and four wavefronts per tile, sibling
+    groups cut at wavefront boundaries) against the reference's outputs, FP64,
+    every operator and mask of the fixture; one and three columns."""
+    torch = _torch()
+    da, z, meta = fx.build_da(dkt, name)
+    nodes = da.nodes()
+    plans = {}
+    for key, op, xs, masked in fx.y_cases(z):
+        code = fx.op_code(dkt, op)
+        if code not in plans:
+            plans[code] = dkt.Plan(da, code, tile_leaves=tile)
+        plan = plans[code]
+        x = fx.make_x(xs, nodes, meta["dim"])
+        ref = z[key]
+        xd = torch.tensor(x, dtype=torch.float64, device="cuda")
+        yd = torch.full_like(xd, float("nan"))
+        plan.matvec_torch(xd, yd, masked=masked)
+        x3 = torch.stack([xd, 2.0 * xd, -xd], dim=1).contiguous()
+        y3 = torch.full_like(x3, float("nan"))
+        plan.matvec_torch(x3, y3, masked=masked)
+        torch.cuda.synchronize()
+        y = yd.cpu().numpy()
+        err = np.linalg.norm(y - ref) / max(np.linalg.norm(ref), 1e-300)
+        assert err <= TOL64, (name, key, tile, err)
+        y3 = y3.cpu().numpy()
+        assert np.array_equal(y3[:, 0], y), (name, key, tile)
+        assert np.linalg.norm(y3[:, 1] - 2.0 * ref) <= TOL64 * 2.0 * max(np.linalg.norm(ref), 1e-300)
+
+
 @pytest.mark.parametrize("nrhs", [3, 8, 24])
 def test_multi_rhs_columns_match_single(dkt, nrhs):
     torch = _torch()
