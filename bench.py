@@ -201,8 +201,8 @@ def cg_p2_leg(dkt, torch, level):
 def batch_leg(dkt, torch, ntrees):
     """Batched matvec: ntrees 3D trees refined around random spheres (centre U[0.2,0.8]^3,
     radius U[0.1,0.3], max level 4-7: 1e3-1e5 elements), p = 1 and 2 alternating, 8
-    right-hand sides each, FP32, one hipGraph replay per step against the same plans
-    launched one by one."""
+    right-hand sides each, FP32, tile sizes per (dim, p) from the autotuner, one
+    hipGraph replay per step against the same plans launched one by one."""
     rng = np.random.default_rng(3)
     das, elems = [], 0
     t0 = time.time()
@@ -214,9 +214,20 @@ def batch_leg(dkt, torch, ntrees):
         da = dkt.DA(dkt.balance_tree(dkt.sphere_tree(3, 30, 2, min(lev, 6 if p == 2 else 7), c, r), 1), p)
         das.append(da)
         elems += da.n_elements
-    plans = [dkt.Plan(da, dkt.OP_STIFFNESS) for da in das]
     log(f"batch: {ntrees} trees, {elems} elements, built in {time.time() - t0:.1f}s")
     nrhs = 8
+    # Tile size per (dim, p) from the autotuner, timed on the group's largest tree.
+    import importlib.util as ilu
+    spec = ilu.spec_from_file_location("dkt_autotune", os.path.join(HERE, "dendro-kt_amd", "autotune.py"))
+    at = ilu.module_from_spec(spec)
+    spec.loader.exec_module(at)
+    tiles = {}
+    for p in (1, 2):
+        group = [da for da in das if da.order == p]
+        if group:
+            sample = max(group, key=lambda d: d.n_elements)
+            tiles[p] = at.autotune(dkt, sample, nrhs=nrhs)["best"]["f32"]
+    plans = [dkt.Plan(da, dkt.OP_STIFFNESS, tile_leaves=tiles[da.order]) for da in das]
     xs = [torch.rand(da.n_nodes, nrhs, dtype=torch.float32, device="cuda") * 2 - 1 for da in das]
     ys = [torch.empty_like(x) for x in xs]
     dofs = sum(da.n_nodes for da in das) * nrhs
@@ -239,6 +250,7 @@ def batch_leg(dkt, torch, ntrees):
     ms_graph = timed(lambda: b.run(s.cuda_stream))
     ms_each = timed(lambda: [p.matvec_torch(x, y, stream=s.cuda_stream) for p, x, y in zip(plans, xs, ys)])
     return {"trees": ntrees, "elements": elems, "nodes": sum(da.n_nodes for da in das), "nrhs": nrhs, "dtype": "f32",
+            "tile_leaves": {f"3d_p{p}": t for p, t in tiles.items()},
             "ms_graph": round(ms_graph, 4), "ms_one_launch_per_tree": round(ms_each, 4),
             "gdof_s_graph": round(dofs / (ms_graph * 1e-3) / 1e9, 3),
             "gdof_s_one_launch_per_tree": round(dofs / (ms_each * 1e-3) / 1e9, 3)}

@@ -80,3 +80,28 @@ def test_batch_graph_replays(dkt):
         for p, x, y in zip(plans, xs, ys):
             p.matvec_torch(x, y)
         torch.cuda.synchronize()
+
+
+@pytest.mark.gpu
+def test_autotuner_picks_a_candidate_and_caches(dkt):
+    """The autotuner times every candidate tile size that fits, returns the fastest
+    per precision (for one and for eight columns), and answers again from its cache."""
+    import importlib.util as ilu
+    import os
+    torch = _torch()
+    spec = ilu.spec_from_file_location(
+        "dkt_autotune", os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
+                                     "dendro-kt_amd", "autotune.py"))
+    at = ilu.module_from_spec(spec)
+    spec.loader.exec_module(at)
+    da = dkt.DA(dkt.balance_tree(dkt.sphere_tree(3, 30, 2, 6, (0.5, 0.5, 0.5), 0.3), 1), 1)
+    for nrhs in (1, 8):
+        r = at.autotune(dkt, da, reps=3, warmup=1, nrhs=nrhs)
+        assert set(r["ms"]) <= set(at.CANDIDATES) and r["ms"]
+        for prec in ("f64", "f32"):
+            best = r["best"][prec]
+            assert best in r["ms"]
+            assert r["ms"][best][prec] == min(v[prec] for v in r["ms"].values())
+        assert at.autotune(dkt, da, nrhs=nrhs) is r
+        plan = dkt.Plan(da, dkt.OP_STIFFNESS, tile_leaves=r["best"]["f32"])
+        assert plan.info()["tile_leaves"] == r["best"]["f32"]
