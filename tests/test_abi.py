@@ -93,3 +93,22 @@ def test_plan_matvec_argument_checks(dkt):
     L = dkt.load()
     rc = L.dkt_plan_matvec(None, 0, None, None, 1, 0, None)
     assert rc != 0 and b"null" in L.dkt_last_error()
+
+
+def test_plain_c_caller(tmp_path):
+    """The boundary is usable from C alone: a C program that includes dkt.h and
+    links libdkt.so builds the config-0 geometry (82,776 elements, 55,639 nodes)
+    and a host-side plan."""
+    import shutil
+    import subprocess
+    if shutil.which("gcc") is None:
+        pytest.skip("no C compiler")
+    repo = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    libdir = os.path.join(repo, "dendro-kt_amd")
+    exe = tmp_path / "abi_example"
+    subprocess.run(["gcc", "-std=c99", "-Wall", "-Werror", "-I", os.path.join(repo, "include"),
+                    os.path.join(repo, "tests", "c_abi", "abi_example.c"), "-L", libdir, "-ldkt",
+                    "-Wl,-rpath," + libdir, "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], capture_output=True, text=True, timeout=300)
+    assert out.returncode == 0, out.stderr
+    assert "elements 82776 nodes 55639" in out.stdout, out.stdout
