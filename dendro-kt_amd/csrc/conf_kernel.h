@@ -51,7 +51,7 @@ __global__ void __launch_bounds__(kTileThreads) conf_tile_kernel(ConfArgs A, con
   constexpr int NPE = Pow<P + 1, DIM>::v;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   if (A.cancel && *A.cancel) return;
-  const ConfTileHdr th = A.tiles[blockIdx.x];
+  const ConfTileHdr th = A.tiles[xcd_tile(blockIdx.x, gridDim.x)];
   const int nq = A.nq;
   size_t off = 0;
   auto carve = [&](size_t bytes) {

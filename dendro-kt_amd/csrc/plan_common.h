@@ -25,6 +25,17 @@ constexpr int kMaxConfVirt = 1024;   // hanging points per conforming tile
 constexpr int kMaxReaders = 32;  // stale-buffer readers per tile (bounds the replay outputs in LDS)
 constexpr size_t kLdsLimit = 64 * 1024;
 
+// Tile of workgroup b of an n-workgroup launch.  Workgroups go to the eight
+// XCDs round-robin (b to XCD b % 8), each with its own L2; this map hands every
+// XCD one contiguous run of SFC-consecutive tiles, taken in order, so the nodes
+// that neighbouring tiles share are fetched into one L2 instead of eight.  A
+// bijection of [0, n) for every n.
+__host__ __device__ inline int xcd_tile(int b, int n) {
+  constexpr int kXcd = 8;
+  const int q = n / kXcd, r = n % kXcd, x = b % kXcd, i = b / kXcd;
+  return (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + i;
+}
+
 // Host-only plan construction (DKT_PLAN_DRYRUN): sizes are recorded, nothing is
 // allocated on the device.
 extern thread_local bool g_dry_run;

@@ -216,6 +216,16 @@ __device__ __forceinline__ void tile_body(const TileArgs &A, int tile, const T *
     for (int r = 0; r < NPE; r++) loc[r] = kNone;
   }
   if (tid < TL) ridx_s[tid] = kNone;
+  // Output slots of the first write-out batch, loaded with the stage loads so
+  // that the write-out does not wait for them.
+  int o_first[kNB];
+#pragma unroll
+  for (int b = 0; b < kNB; b++) {
+    const int j = tid + b * nthr;
+    o_first[b] = j < th.nnode ? A.tile_out[th.node0 + j] : INT32_MIN;  // INT32_MIN: gathered only
+  }
+  // Replay block of this lane group's first round, likewise.
+  const uint64_t blk_first = (tid / G) < th.nbk ? A.blocks[th.bk0 + tid / G] : 0;
   // Node gathers in batches of kNB per work-item, all loads issued before use.
   for (int j0 = tid; j0 < th.nnode; j0 += kNB * nthr) {
     int g[kNB];
@@ -284,7 +294,7 @@ __device__ __forceinline__ void tile_body(const TileArgs &A, int tile, const T *
     for (int b0 = 0; b0 < th.nbk; b0 += ngroups) {
       const int bi = b0 + grp;
       if (bi >= th.nbk) continue;
-      const uint64_t blk = A.blocks[th.bk0 + bi];
+      const uint64_t blk = b0 == 0 ? blk_first : A.blocks[th.bk0 + bi];
       const int S = (int)(blk & 0xFFFFu), E = (int)((blk >> 16) & 0xFFFFu);
       int p = (int)((blk >> 32) & 0xFFFFu);
       const int pend = p + (int)(blk >> 48);
@@ -431,7 +441,7 @@ __device__ __forceinline__ void tile_body(const TileArgs &A, int tile, const T *
 #pragma unroll
     for (int b = 0; b < kNB; b++) {
       const int j = j0 + b * nthr;
-      o[b] = j < th.nnode ? A.tile_out[th.node0 + j] : INT32_MIN;  // INT32_MIN: gathered only
+      o[b] = j0 == tid ? o_first[b] : (j < th.nnode ? A.tile_out[th.node0 + j] : INT32_MIN);
     }
     bool z[kNB];
 #pragma unroll
@@ -456,7 +466,7 @@ __global__ void __launch_bounds__(kTileThreads) tile_kernel(TileArgs A, const T 
                                                             const T *__restrict__ carry_w, const T *__restrict__ x,
                                                             T *__restrict__ y, T *__restrict__ partial) {
   if (A.cancel && *A.cancel) return;
-  tile_body<T, DIM, P, ONE>(A, blockIdx.x, kref, carry_w, x, y, partial);
+  tile_body<T, DIM, P, ONE>(A, xcd_tile(blockIdx.x, gridDim.x), kref, carry_w, x, y, partial);
 }
 
 // A plan of a batch: its maps (with nq and ld set for the batch) and vectors.
@@ -479,14 +489,15 @@ __device__ __forceinline__ int find_range(const int *__restrict__ starts, int n,
   return lo;
 }
 
-// All tiles of a batch of same-(dim, p) plans in one launch: workgroup b runs
-// tile b - tile_start[k] of plan k.
+// All tiles of a batch of same-(dim, p) plans in one launch: batch tile
+// t = xcd_tile(b) of workgroup b is tile t - tile_start[k] of plan k.
 template <typename T, int DIM, int P, bool ONE>
 __global__ void __launch_bounds__(kTileThreads) tile_kernel_batch(const BatchPlan<T> *__restrict__ plans,
                                                                   const int *__restrict__ tile_start, int nplans) {
-  const int k = find_range(tile_start, nplans, blockIdx.x);
+  const int t = xcd_tile(blockIdx.x, gridDim.x);
+  const int k = find_range(tile_start, nplans, t);
   const BatchPlan<T> &bp = plans[k];
-  tile_body<T, DIM, P, ONE>(bp.A, blockIdx.x - tile_start[k], bp.kref, bp.carry_w, bp.x, bp.y, bp.partial);
+  tile_body<T, DIM, P, ONE>(bp.A, t - tile_start[k], bp.kref, bp.carry_w, bp.x, bp.y, bp.partial);
 }
 
 // Sum of the partial slots [a, b) of column q in slot order (a < b); the
