@@ -226,6 +226,18 @@ __device__ __forceinline__ void tile_body(const TileArgs &A, int tile, const T *
   }
   // Replay block of this lane group's first round, likewise.
   const uint64_t blk_first = (tid / G) < th.nbk ? A.blocks[th.bk0 + tid / G] : 0;
+  // First two batches of the group parent maps and transpose entries, and the
+  // readers' first batch: loads issued here, stored after the node gathers.
+  const int ngp = th.ngrp * NPE;
+  uint16_t gp_first[2];
+  uint32_t he_first[2];
+#pragma unroll
+  for (int b = 0; b < 2; b++) {
+    const int k = tid + b * nthr;
+    gp_first[b] = k < ngp ? A.group_ploc[(size_t)th.grp0 * NPE + k] : kNone;
+    he_first[b] = k < th.nhe ? A.hentries[th.he0 + k] : 0u;
+  }
+  const uint16_t rd_first = tid < th.nrd ? A.readers[th.rd0 + tid] : kNone;
   // Node gathers in batches of kNB per work-item, all loads issued before use.
   for (int j0 = tid; j0 < th.nnode; j0 += kNB * nthr) {
     int g[kNB];
@@ -251,9 +263,16 @@ __device__ __forceinline__ void tile_body(const TileArgs &A, int tile, const T *
       }
     }
   }
-  for (int k = tid; k < th.ngrp * NPE; k += nthr) gploc[k] = A.group_ploc[(size_t)th.grp0 * NPE + k];
-  for (int k = tid; k < th.nhe; k += nthr) hent[k] = A.hentries[th.he0 + k];
-  for (int k = tid; k < th.nrd; k += nthr) rdl_s[k] = A.readers[th.rd0 + k];
+#pragma unroll
+  for (int b = 0; b < 2; b++) {
+    const int k = tid + b * nthr;
+    if (k < ngp) gploc[k] = gp_first[b];
+    if (k < th.nhe) hent[k] = he_first[b];
+  }
+  if (tid < th.nrd) rdl_s[tid] = rd_first;
+  for (int k = tid + 2 * nthr; k < ngp; k += nthr) gploc[k] = A.group_ploc[(size_t)th.grp0 * NPE + k];
+  for (int k = tid + 2 * nthr; k < th.nhe; k += nthr) hent[k] = A.hentries[th.he0 + k];
+  for (int k = tid + nthr; k < th.nrd; k += nthr) rdl_s[k] = A.readers[th.rd0 + k];
   const bool hang = active && aux_hang(aux);
   const int child = aux_child(aux);
   const int hidx = aux_hidx(aux);
@@ -348,24 +367,31 @@ __device__ __forceinline__ void tile_body(const TileArgs &A, int tile, const T *
     //    interpolate its parent nodes (readers: the replayed buffer B_k).
     T v[NPE];
     if (active) {
+      // Missing ranks first (zero for a leaf without hanging nodes), then the
+      // present ranks replace them: one LDS read per rank, no branches.
       T u[NPE];
-      const int ri = ridx_s[tid];
       if (hang) {
-        T f[NPE];
         if constexpr (ONE) {
 #pragma unroll
-          for (int s = 0; s < NPE; s++) f[s] = pval[grp_id * NPE + s];
+          for (int s = 0; s < NPE; s++) u[s] = pval[grp_id * NPE + s];
         } else {
 #pragma unroll
-          for (int s = 0; s < NPE; s++) f[s] = ploc[s] != kNone ? x_loc[ploc[s] * nq + q] : T(0);
+          for (int s = 0; s < NPE; s++) u[s] = ploc[s] != kNone ? x_loc[ploc[s] * nq + q] : T(0);
         }
-        interp<T, DIM, P, false>(f, child);
+        interp<T, DIM, P, false>(u, child);
+        const int ri = ridx_s[tid];
+        if (ri != kNone)
 #pragma unroll
-        for (int r = 0; r < NPE; r++)
-          u[r] = loc[r] != kNone ? x_loc[loc[r] * nq + q] : (ri != kNone ? ubuf[ri * NPE + r] : f[r]);
+          for (int r = 0; r < NPE; r++) u[r] = ubuf[ri * NPE + r];
       } else {
 #pragma unroll
-        for (int r = 0; r < NPE; r++) u[r] = loc[r] != kNone ? x_loc[loc[r] * nq + q] : T(0);
+        for (int r = 0; r < NPE; r++) u[r] = T(0);
+      }
+#pragma unroll
+      for (int r = 0; r < NPE; r++) {
+        const bool present = loc[r] != kNone;
+        const T xv = x_loc[(present ? loc[r] : 0) * nq + q];
+        u[r] = present ? xv : u[r];
       }
       if constexpr (P == 1) {
         if (A.op == 0)
@@ -461,10 +487,19 @@ __device__ __forceinline__ void tile_body(const TileArgs &A, int tile, const T *
   }
 }
 
+// Waves per SIMD the compiler is asked to fit: 8, the most, for the p = 1
+// single-column kernels whose vector registers allow it (their scalar
+// registers, the uniform tile fields, otherwise hold them to 7); 1 leaves the
+// register allocation free.
 template <typename T, int DIM, int P, bool ONE>
-__global__ void __launch_bounds__(kTileThreads) tile_kernel(TileArgs A, const T *__restrict__ kref,
-                                                            const T *__restrict__ carry_w, const T *__restrict__ x,
-                                                            T *__restrict__ y, T *__restrict__ partial) {
+constexpr int tile_waves() {
+  return ONE && P == 1 && (sizeof(T) == 4 || DIM <= 3) ? 8 : 1;
+}
+
+template <typename T, int DIM, int P, bool ONE>
+__global__ void __launch_bounds__(kTileThreads) __attribute__((amdgpu_waves_per_eu(tile_waves<T, DIM, P, ONE>())))
+tile_kernel(TileArgs A, const T *__restrict__ kref, const T *__restrict__ carry_w, const T *__restrict__ x,
+            T *__restrict__ y, T *__restrict__ partial) {
   if (A.cancel && *A.cancel) return;
   tile_body<T, DIM, P, ONE>(A, xcd_tile(blockIdx.x, gridDim.x), kref, carry_w, x, y, partial);
 }
@@ -492,8 +527,8 @@ __device__ __forceinline__ int find_range(const int *__restrict__ starts, int n,
 // All tiles of a batch of same-(dim, p) plans in one launch: batch tile
 // t = xcd_tile(b) of workgroup b is tile t - tile_start[k] of plan k.
 template <typename T, int DIM, int P, bool ONE>
-__global__ void __launch_bounds__(kTileThreads) tile_kernel_batch(const BatchPlan<T> *__restrict__ plans,
-                                                                  const int *__restrict__ tile_start, int nplans) {
+__global__ void __launch_bounds__(kTileThreads) __attribute__((amdgpu_waves_per_eu(tile_waves<T, DIM, P, ONE>())))
+tile_kernel_batch(const BatchPlan<T> *__restrict__ plans, const int *__restrict__ tile_start, int nplans) {
   const int t = xcd_tile(blockIdx.x, gridDim.x);
   const int k = find_range(tile_start, nplans, t);
   const BatchPlan<T> &bp = plans[k];
