@@ -114,9 +114,13 @@ __device__ __forceinline__ T xor_lane(T a) {
 // One step of the sibling-group sum: w[s] += (value of lane wl + D in the same
 // row of 16) when that lane is still in the run.  Products with the 0/1 factor
 // are exact, so the fma adds an exact zero or rounds once like the plain sum.
+// A step that no lane of the wavefront needs (every run within a row is at
+// most D lanes long) is skipped: it would add exact zeros only.
 template <typename T, int CTRL, int D, int N>
 __device__ __forceinline__ void group_step(T (&w)[N], int wl, int run_end) {
-  const T f = (wl + D < run_end && (wl & 15) + D < 16) ? T(1) : T(0);
+  const bool in = wl + D < run_end && (wl & 15) + D < 16;
+  if (!__ballot(in)) return;
+  const T f = in ? T(1) : T(0);
 #pragma unroll
   for (int s = 0; s < N; s++) w[s] = fma(dpp_mov<CTRL>(w[s]), f, w[s]);
 }
@@ -437,8 +441,8 @@ __device__ __forceinline__ void tile_body(const TileArgs &A, int tile, const T *
         if constexpr (DIM > 1) group_step<T, 0x102, 2>(w, wl, run_end);
         if constexpr (DIM > 2) group_step<T, 0x104, 4>(w, wl, run_end);
         if constexpr (DIM > 3) group_step<T, 0x108, 8>(w, wl, run_end);
-        {
-          const int nxt = (wl | 15) + 1;
+        const int nxt = (wl | 15) + 1;
+        if (__ballot(nxt < run_end)) {
           const T fx = nxt < run_end ? T(1) : T(0);
 #pragma unroll
           for (int s = 0; s < NPE; s++) w[s] = fma(__shfl(w[s], nxt & 63, 64), fx, w[s]);
