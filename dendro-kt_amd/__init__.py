@@ -113,6 +113,8 @@ def _bind_device(L):
     L.dkt_plan_profile.argtypes = [P, c.c_int]
     L.dkt_plan_profile_read.argtypes = [P, c.POINTER(c.c_double), c.POINTER(c.c_double), c.POINTER(c.c_int)]
     L.dkt_cg_solve.argtypes = [P, c.c_int, P, P, c.c_int, c.c_double, c.c_int, c.POINTER(c.c_double), P]
+    L.dkt_cg_solve_batch.argtypes = [c.POINTER(P), c.c_int, c.POINTER(P), c.POINTER(P), c.c_int, c.c_double, c.c_int,
+                                     c.POINTER(c.c_double), P]
     L.dkt_batch_create.argtypes = [c.POINTER(P), c.c_int, c.c_int, c.c_int, c.POINTER(P), c.POINTER(P), c.c_int,
                                    c.c_int, c.POINTER(P)]
     L.dkt_batch_run.argtypes = [P, P]
@@ -375,6 +377,29 @@ class Plan:
         if getattr(self, "h", None) and _lib is not None:
             _lib.dkt_plan_destroy(self.h)
             self.h = None
+
+
+def cg_solve_batch_torch(plans, bs, xs, max_iter=10000, tol=1e-8, masked=True, stream=None):
+    """FP64 CG on every (plans[k], bs[k]) in lockstep (dkt_cg_solve_batch): xs[k]
+    is overwritten with the solution from x = 0, bitwise what Plan.cg_solve
+    returns from a zero guess.  Returns [dict(iters, resid)] per solve."""
+    import torch
+    n = len(plans)
+    if n < 1 or len(bs) != n or len(xs) != n:
+        raise DktError("plans, bs and xs must be non-empty lists of one length")
+    for pl, b, x in zip(plans, bs, xs):
+        if b.dtype != torch.float64 or x.dtype != torch.float64 or b.shape != (pl.n_nodes,) or x.shape != b.shape:
+            raise DktError("b and x must be float64 device vectors of the plan's node count")
+        if not (b.is_cuda and x.is_cuda and b.is_contiguous() and x.is_contiguous()):
+            raise DktError("b and x must be contiguous device tensors")
+    V = ctypes.c_void_p
+    hp = (V * n)(*[pl.h.value for pl in plans])
+    hb = (V * n)(*[b.data_ptr() for b in bs])
+    hx = (V * n)(*[x.data_ptr() for x in xs])
+    st = (ctypes.c_double * (2 * n))()
+    s = stream if stream is not None else torch.cuda.current_stream().cuda_stream
+    _chk(load().dkt_cg_solve_batch(hp, n, hb, hx, max_iter, tol, 1 if masked else 0, st, V(s or 0)))
+    return [dict(iters=int(st[2 * k]), resid=st[2 * k + 1]) for k in range(n)]
 
 
 class Batch:

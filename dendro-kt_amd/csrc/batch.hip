@@ -40,7 +40,7 @@ struct Group : GroupBase {
   std::vector<int> nqs;  // columns per chunk
 
   Group(const std::vector<dkt_plan *> &pls, const std::vector<const void *> &xs, const std::vector<void *> &ys,
-        int nrhs, int masked) {
+        int nrhs, int masked, const std::vector<const int *> &cancel) {
     constexpr int NPE = Pow<P + 1, DIM>::v;
     const bool f64 = sizeof(T) == 8;
     auto dims_of = [](const dkt_plan *pl) {
@@ -83,7 +83,7 @@ struct Group : GroupBase {
         A.carry_key = pl->carry_key.p;
         A.dir_in = masked == 1 ? pl->dirichlet.p : nullptr;
         A.dir_out = masked ? pl->dirichlet.p : nullptr;
-        A.cancel = nullptr;
+        A.cancel = cancel.empty() ? nullptr : cancel[k];
         A.op = pl->op;
         A.hp = pl->hp;
         A.nq = nq;
@@ -129,16 +129,57 @@ struct Group : GroupBase {
 
 template <typename T>
 GroupBase *make_group(int dim, int p, const std::vector<dkt_plan *> &pls, const std::vector<const void *> &xs,
-                      const std::vector<void *> &ys, int nrhs, int masked) {
-  if (dim == 2 && p == 1) return new Group<T, 2, 1>(pls, xs, ys, nrhs, masked);
-  if (dim == 3 && p == 1) return new Group<T, 3, 1>(pls, xs, ys, nrhs, masked);
-  if (dim == 4 && p == 1) return new Group<T, 4, 1>(pls, xs, ys, nrhs, masked);
-  if (dim == 2 && p == 2) return new Group<T, 2, 2>(pls, xs, ys, nrhs, masked);
-  if (dim == 3 && p == 2) return new Group<T, 3, 2>(pls, xs, ys, nrhs, masked);
+                      const std::vector<void *> &ys, int nrhs, int masked,
+                      const std::vector<const int *> &cancel = {}) {
+  if (dim == 2 && p == 1) return new Group<T, 2, 1>(pls, xs, ys, nrhs, masked, cancel);
+  if (dim == 3 && p == 1) return new Group<T, 3, 1>(pls, xs, ys, nrhs, masked, cancel);
+  if (dim == 4 && p == 1) return new Group<T, 4, 1>(pls, xs, ys, nrhs, masked, cancel);
+  if (dim == 2 && p == 2) return new Group<T, 2, 2>(pls, xs, ys, nrhs, masked, cancel);
+  if (dim == 3 && p == 2) return new Group<T, 3, 2>(pls, xs, ys, nrhs, masked, cancel);
   throw std::invalid_argument("unsupported (dim, order)");
 }
 
+// Plans grouped by (dim, p): indices into the caller's list.
+std::map<std::pair<int, int>, std::vector<int>> by_kind_of(dkt_plan *const *plans, int nplans) {
+  std::map<std::pair<int, int>, std::vector<int>> by_kind;
+  for (int k = 0; k < nplans; k++) {
+    dkt_plan *pl = plans[k];
+    if (pl->dry || pl->simple || pl->per_level || pl->conforming)
+      throw std::invalid_argument("batch plans must be tiled device plans");
+    by_kind[{pl->dim, pl->order}].push_back(k);
+  }
+  return by_kind;
+}
+
+struct GroupsLauncher : MatvecLauncher {
+  std::vector<std::unique_ptr<GroupBase>> groups;
+  void launch(hipStream_t st) override {
+    for (auto &g : groups) g->launch(st);
+  }
+};
+
 }  // namespace
+
+std::unique_ptr<MatvecLauncher> dkt::batch_matvec_f64(const std::vector<dkt_plan *> &plans,
+                                                      const std::vector<const double *> &x,
+                                                      const std::vector<double *> &y, int mask,
+                                                      const std::vector<const int *> &cancel) {
+  auto out = std::make_unique<GroupsLauncher>();
+  for (const auto &kv : by_kind_of(plans.data(), (int)plans.size())) {
+    std::vector<dkt_plan *> pls;
+    std::vector<const void *> xs;
+    std::vector<void *> ys;
+    std::vector<const int *> cs;
+    for (int k : kv.second) {
+      pls.push_back(plans[k]);
+      xs.push_back(x[k]);
+      ys.push_back(y[k]);
+      cs.push_back(cancel[k]);
+    }
+    out->groups.emplace_back(make_group<double>(kv.first.first, kv.first.second, pls, xs, ys, 1, mask, cs));
+  }
+  return out;
+}
 
 struct dkt_batch {
   std::vector<std::unique_ptr<GroupBase>> groups;
@@ -158,17 +199,12 @@ extern "C" int dkt_batch_create(dkt_plan *const *plans, int nplans, int dtype, i
     if (dtype != 0 && dtype != 1) throw std::invalid_argument("dtype must be 0 (f64) or 1 (f32)");
     if (nrhs < 1 || nrhs > 64) throw std::invalid_argument("nrhs must be 1..64");
     if (masked < 0 || masked > 2) throw std::invalid_argument("masked must be 0, 1 or 2");
-    std::map<std::pair<int, int>, std::vector<int>> by_kind;
     for (int k = 0; k < nplans; k++) {
-      dkt_plan *pl = plans[k];
-      if (!pl || !x[k] || !y[k]) throw std::invalid_argument("null plan or vector");
-      if (pl->dry || pl->simple || pl->per_level || pl->conforming)
-        throw std::invalid_argument("batch plans must be tiled device plans");
+      if (!plans[k] || !x[k] || !y[k]) throw std::invalid_argument("null plan or vector");
       if (x[k] == (const void *)y[k]) throw std::invalid_argument("x and y must not alias");
-      by_kind[{pl->dim, pl->order}].push_back(k);
     }
     auto b = std::make_unique<dkt_batch>();
-    for (const auto &kv : by_kind) {
+    for (const auto &kv : by_kind_of(plans, nplans)) {
       std::vector<dkt_plan *> pls;
       std::vector<const void *> xs;
       std::vector<void *> ys;

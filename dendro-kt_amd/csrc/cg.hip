@@ -84,10 +84,15 @@ __device__ __forceinline__ double parts_max(const double *part, double *sh) {
 
 #define CG_LOOP(i, n) for (int i = blockIdx.x * kCgThreads + threadIdx.x; i < (n); i += kCgBlocks * kCgThreads)
 
+// The bodies of the iteration kernels run one solve's block blockIdx.x of
+// kCgBlocks; the single-solve kernels and the batched ones (one solve per
+// blockIdx.y) share them, so a solve's arithmetic and reduction order do not
+// depend on what it is batched with.
+
 // r = b - ax (ax may be null: x = 0), p = r; partials of r.r, max|r|, max|b|.
 template <typename T>
-__global__ void __launch_bounds__(kCgThreads) k_init(int n, const T *__restrict__ b, const T *__restrict__ ax,
-                                                     T *__restrict__ r, T *__restrict__ p, double *__restrict__ part) {
+__device__ __forceinline__ void init_body(int n, const T *__restrict__ b, const T *__restrict__ ax,
+                                          T *__restrict__ r, T *__restrict__ p, double *__restrict__ part) {
   __shared__ double sh[kCgThreads];
   double rr = 0, mr = 0, mb = 0;
   CG_LOOP(i, n) {
@@ -107,9 +112,14 @@ __global__ void __launch_bounds__(kCgThreads) k_init(int n, const T *__restrict_
     part[2 * kCgBlocks + blockIdx.x] = mb;
   }
 }
+template <typename T>
+__global__ void __launch_bounds__(kCgThreads) k_init(int n, const T *__restrict__ b, const T *__restrict__ ax,
+                                                     T *__restrict__ r, T *__restrict__ p, double *__restrict__ part) {
+  init_body(n, b, ax, r, p, part);
+}
 
-__global__ void __launch_bounds__(kCgThreads) k_init_fin(const double *__restrict__ part, double *__restrict__ scal,
-                                                         double tol, int *__restrict__ flag) {
+__device__ __forceinline__ void init_fin_body(const double *__restrict__ part, double *__restrict__ scal, double tol,
+                                              int *__restrict__ flag) {
   __shared__ double sh[kCgThreads];
   const double rr = parts_sum(part, sh);
   const double mr = parts_max(part + kCgBlocks, sh);
@@ -123,10 +133,14 @@ __global__ void __launch_bounds__(kCgThreads) k_init_fin(const double *__restric
     flag[1] = 0;
   }
 }
+__global__ void __launch_bounds__(kCgThreads) k_init_fin(const double *__restrict__ part, double *__restrict__ scal,
+                                                         double tol, int *__restrict__ flag) {
+  init_fin_body(part, scal, tol, flag);
+}
 
 template <typename T>
-__global__ void __launch_bounds__(kCgThreads) k_dot(int n, const T *__restrict__ a, const T *__restrict__ b,
-                                                    double *__restrict__ part, const int *__restrict__ flag) {
+__device__ __forceinline__ void dot_body(int n, const T *__restrict__ a, const T *__restrict__ b,
+                                         double *__restrict__ part, const int *__restrict__ flag) {
   if (flag[0]) return;
   __shared__ double sh[kCgThreads];
   double acc = 0;
@@ -134,14 +148,18 @@ __global__ void __launch_bounds__(kCgThreads) k_dot(int n, const T *__restrict__
   acc = block_sum(acc, sh);
   if (threadIdx.x == 0) part[blockIdx.x] = acc;
 }
+template <typename T>
+__global__ void __launch_bounds__(kCgThreads) k_dot(int n, const T *__restrict__ a, const T *__restrict__ b,
+                                                    double *__restrict__ part, const int *__restrict__ flag) {
+  dot_body(n, a, b, part, flag);
+}
 
 // alpha = rr / (p.Ap); x += alpha p; r -= alpha Ap; partials of r.r and max|r|.
 template <typename T>
-__global__ void __launch_bounds__(kCgThreads) k_update_xr(int n, T *__restrict__ x, T *__restrict__ r,
-                                                          const T *__restrict__ p, const T *__restrict__ ap,
-                                                          const double *__restrict__ part_pap,
-                                                          const double *__restrict__ scal, int par,
-                                                          double *__restrict__ part, const int *__restrict__ flag) {
+__device__ __forceinline__ void update_xr_body(int n, T *__restrict__ x, T *__restrict__ r, const T *__restrict__ p,
+                                               const T *__restrict__ ap, const double *__restrict__ part_pap,
+                                               const double *__restrict__ scal, int par, double *__restrict__ part,
+                                               const int *__restrict__ flag) {
   if (flag[0]) return;
   __shared__ double sh[kCgThreads];
   const T alpha = (T)(scal[S_RR0 + par] / parts_sum(part_pap, sh));
@@ -160,14 +178,22 @@ __global__ void __launch_bounds__(kCgThreads) k_update_xr(int n, T *__restrict__
     part[kCgBlocks + blockIdx.x] = mr;
   }
 }
+template <typename T>
+__global__ void __launch_bounds__(kCgThreads) k_update_xr(int n, T *__restrict__ x, T *__restrict__ r,
+                                                          const T *__restrict__ p, const T *__restrict__ ap,
+                                                          const double *__restrict__ part_pap,
+                                                          const double *__restrict__ scal, int par,
+                                                          double *__restrict__ part, const int *__restrict__ flag) {
+  update_xr_body(n, x, r, p, ap, part_pap, scal, par, part, flag);
+}
 
 // Convergence test, then beta = r1.r1 / r0.r0 and p = r + beta p.  Block 0
 // publishes the new r.r (into the other parity slot), the residual, the
 // iteration count and the converged flag.
 template <typename T>
-__global__ void __launch_bounds__(kCgThreads) k_update_p(int n, T *__restrict__ p, const T *__restrict__ r,
-                                                         const double *__restrict__ part, double *__restrict__ scal,
-                                                         int par, double tol, int *__restrict__ flag) {
+__device__ __forceinline__ void update_p_body(int n, T *__restrict__ p, const T *__restrict__ r,
+                                              const double *__restrict__ part, double *__restrict__ scal, int par,
+                                              double tol, int *__restrict__ flag) {
   if (flag[0]) return;
   __shared__ double sh[kCgThreads];
   const double rr1 = parts_sum(part, sh);
@@ -183,6 +209,12 @@ __global__ void __launch_bounds__(kCgThreads) k_update_p(int n, T *__restrict__ 
     flag[1] += 1;
     if (done) flag[0] = 1;
   }
+}
+template <typename T>
+__global__ void __launch_bounds__(kCgThreads) k_update_p(int n, T *__restrict__ p, const T *__restrict__ r,
+                                                         const double *__restrict__ part, double *__restrict__ scal,
+                                                         int par, double tol, int *__restrict__ flag) {
+  update_p_body(n, p, r, part, scal, par, tol, flag);
 }
 
 // FP64 residual r = b - ax and its partial max norms (outer refinement step).
@@ -430,10 +462,122 @@ int cg_reliable(dkt_plan *pl, const double *b, double *x, int max_iter, double t
   return it;
 }
 
+// One solve of a batch (dkt_cg_solve_batch): its vectors and workspace.
+struct CgDesc {
+  int n;
+  const double *b;
+  double *x, *r, *p, *ap, *part, *scal;
+  int *flag;
+};
+
+// Batched FP64 iteration kernels: solve blockIdx.y, block blockIdx.x of its
+// kCgBlocks, through the single-solve bodies.
+__global__ void __launch_bounds__(kCgThreads) kb_init(const CgDesc *__restrict__ d) {
+  const CgDesc e = d[blockIdx.y];
+  CG_LOOP(i, e.n) e.x[i] = 0.0;
+  init_body<double>(e.n, e.b, nullptr, e.r, e.p, e.part);
+}
+__global__ void __launch_bounds__(kCgThreads) kb_init_fin(const CgDesc *__restrict__ d, double tol) {
+  const CgDesc e = d[blockIdx.y];
+  init_fin_body(e.part, e.scal, tol, e.flag);
+}
+__global__ void __launch_bounds__(kCgThreads) kb_dot(const CgDesc *__restrict__ d) {
+  const CgDesc e = d[blockIdx.y];
+  dot_body<double>(e.n, e.p, e.ap, e.part + 2 * kCgBlocks, e.flag);
+}
+__global__ void __launch_bounds__(kCgThreads) kb_update_xr(const CgDesc *__restrict__ d, int par) {
+  const CgDesc e = d[blockIdx.y];
+  update_xr_body<double>(e.n, e.x, e.r, e.p, e.ap, e.part + 2 * kCgBlocks, e.scal, par, e.part, e.flag);
+}
+__global__ void __launch_bounds__(kCgThreads) kb_update_p(const CgDesc *__restrict__ d, int par, double tol) {
+  const CgDesc e = d[blockIdx.y];
+  update_p_body<double>(e.n, e.p, e.r, e.part, e.scal, par, tol, e.flag);
+}
+// out[3k .. 3k + 2] = converged flag, iterations, relative residual of solve k.
+__global__ void __launch_bounds__(kCgThreads) kb_status(const CgDesc *__restrict__ d, int nsolves,
+                                                        double *__restrict__ out) {
+  for (int k = threadIdx.x; k < nsolves; k += blockDim.x) {
+    out[3 * k] = d[k].flag[0];
+    out[3 * k + 1] = d[k].flag[1];
+    out[3 * k + 2] = d[k].scal[S_RESID];
+  }
+}
+
 }  // namespace
 }  // namespace dkt
 
 using namespace dkt;
+
+extern "C" int dkt_cg_solve_batch(dkt_plan *const *plans, int nplans, const double *const *b, double *const *x,
+                                  int max_iter, double tol, int masked, double *stats, void *stream) {
+  return guarded([&] {
+    if (!plans || !b || !x || nplans < 1) throw std::invalid_argument("null pointer or empty batch");
+    if (max_iter < 0 || !(tol >= 0.0)) throw std::invalid_argument("max_iter >= 0 and tol >= 0 required");
+    std::vector<dkt_plan *> pls(plans, plans + nplans);
+    for (int k = 0; k < nplans; k++) {
+      if (!pls[k] || !b[k] || !x[k]) throw std::invalid_argument("null plan or vector");
+      if (pls[k]->dry) throw std::invalid_argument("a dry-run plan cannot solve");
+      if ((const void *)b[k] == (const void *)x[k]) throw std::invalid_argument("b and x must not alias");
+      for (int j = 0; j < k; j++)
+        if (pls[j] == pls[k]) throw std::invalid_argument("a plan may appear once per batch (one CG workspace)");
+    }
+    hipStream_t st = (hipStream_t)stream;
+    const int K = nplans;
+    std::vector<CgDesc> hd(K);
+    std::vector<const double *> px(K);
+    std::vector<double *> py(K);
+    std::vector<const int *> cancel(K);
+    for (int k = 0; k < K; k++) {
+      ensure_workspace(pls[k], false);
+      CgWorkspace &w = pls[k]->cg;
+      hd[k] = CgDesc{(int)pls[k]->N, b[k], x[k], w.r.p, w.p.p, w.ap.p, w.part64.p, w.scal.p, w.flag.p};
+      px[k] = w.p.p;
+      py[k] = w.ap.p;
+      cancel[k] = w.flag.p;  // flag[0]: converged
+    }
+    // Matvecs of every plan in one launch per (dim, p), tiled plans only.
+    std::unique_ptr<MatvecLauncher> mv = batch_matvec_f64(pls, px, py, masked ? 1 : 0, cancel);
+    DevBuf<CgDesc> d;
+    d.alloc(K);
+    DevBuf<double> status;
+    status.alloc(3 * (size_t)K);
+    HIP_OK(hipMemcpyAsync(d.p, hd.data(), K * sizeof(CgDesc), hipMemcpyHostToDevice, st));
+    std::vector<double> hs(3 * (size_t)K);
+    auto poll = [&] {
+      hipLaunchKernelGGL(kb_status, dim3(1), dim3(kCgThreads), 0, st, d.p, K, status.p);
+      HIP_OK(hipGetLastError());
+      HIP_OK(hipMemcpyAsync(hs.data(), status.p, hs.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+      HIP_OK(hipStreamSynchronize(st));
+      for (int k = 0; k < K; k++)
+        if (hs[3 * k] == 0.0) return false;
+      return true;
+    };
+    const dim3 grid(kCgBlocks, K), blk(kCgThreads);
+    hipLaunchKernelGGL(kb_init, grid, blk, 0, st, d.p);
+    hipLaunchKernelGGL(kb_init_fin, dim3(1, K), blk, 0, st, d.p, tol);
+    HIP_OK(hipGetLastError());
+    int it = 0;
+    bool done = poll();
+    while (!done && it < max_iter) {
+      const int chunk = std::min(kCgChunk, max_iter - it);
+      for (int c = 0; c < chunk; c++) {
+        const int par = (it + c) & 1;
+        mv->launch(st);
+        hipLaunchKernelGGL(kb_dot, grid, blk, 0, st, d.p);
+        hipLaunchKernelGGL(kb_update_xr, grid, blk, 0, st, d.p, par);
+        hipLaunchKernelGGL(kb_update_p, grid, blk, 0, st, d.p, par, tol);
+        HIP_OK(hipGetLastError());
+      }
+      it += chunk;
+      done = poll();
+    }
+    if (stats)
+      for (int k = 0; k < K; k++) {
+        stats[2 * k] = hs[3 * k + 1];
+        stats[2 * k + 1] = hs[3 * k + 2];
+      }
+  });
+}
 
 extern "C" int dkt_cg_solve(dkt_plan *pl, int mode, const double *b, double *x, int max_iter, double tol, int masked,
                             double *stats, void *stream) {

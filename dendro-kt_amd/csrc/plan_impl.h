@@ -95,4 +95,15 @@ namespace dkt {
 // is nonzero at launch time, the kernels return without touching y.
 void plan_apply(dkt_plan *pl, int dtype, const void *x, void *y, int nrhs, int mask, hipStream_t st,
                 const int *cancel);
+
+// FP64 matvecs y_k = A_k x_k of many tiled plans, one launch per (dim, p)
+// (batch.hip); plan k's kernels return at once while *cancel[k] is nonzero.
+// Each result is bitwise the plan's own matvec.
+struct MatvecLauncher {
+  virtual ~MatvecLauncher() = default;
+  virtual void launch(hipStream_t st) = 0;
+};
+std::unique_ptr<MatvecLauncher> batch_matvec_f64(const std::vector<dkt_plan *> &plans,
+                                                 const std::vector<const double *> &x, const std::vector<double *> &y,
+                                                 int mask, const std::vector<const int *> &cancel);
 }  // namespace dkt

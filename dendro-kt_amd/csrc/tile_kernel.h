@@ -536,6 +536,7 @@ tile_kernel_batch(const BatchPlan<T> *__restrict__ plans, const int *__restrict_
   const int t = xcd_tile(blockIdx.x, gridDim.x);
   const int k = find_range(tile_start, nplans, t);
   const BatchPlan<T> &bp = plans[k];
+  if (bp.A.cancel && *bp.A.cancel) return;
   tile_body<T, DIM, P, ONE>(bp.A, t - tile_start[k], bp.kref, bp.carry_w, bp.x, bp.y, bp.partial);
 }
 
@@ -574,7 +575,7 @@ __global__ void __launch_bounds__(256) fixup_kernel_batch(const BatchPlan<T> *__
   const int k = find_range(block_start, nplans, blockIdx.x);
   const BatchPlan<T> &bp = plans[k];
   const int i = (blockIdx.x - block_start[k]) * 256 + threadIdx.x;
-  if (i >= bp.nshared) return;
+  if (i >= bp.nshared || (bp.A.cancel && *bp.A.cancel)) return;
   const int nq = bp.A.nq, ld = bp.A.ld;
   const int n = bp.snode[i];
   const int a = bp.soff[i], b = bp.soff[i + 1];

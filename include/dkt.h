@@ -196,6 +196,21 @@ int dkt_plan_matvec(dkt_plan *plan, int dtype, const void *x, void *y, int nrhs,
 int dkt_cg_solve(dkt_plan *plan, int mode, const double *b, double *x, int max_iter, double tol, int masked,
                  double *stats, void *stream);
 
+/* Batched FP64 conjugate gradients: solves A_k x_k = b_k for nplans tiled
+ * plans (any mix of (dim, p); no simple, per-level or conforming plans) in
+ * lockstep on one stream, each as dkt_cg_solve with DKT_CG_F64 from x_k = 0
+ * (x_k is overwritten).  Per iteration one matvec launch per (dim, p) group
+ * and one launch per vector step serve every solve; a converged solve's
+ * kernels return at once.  Each solve's arithmetic and reduction order are
+ * those of its own dkt_cg_solve: x_k is bitwise what dkt_cg_solve(plans[k],
+ * DKT_CG_F64, b_k, x_k = 0, ...) returns.  A plan may appear once per batch
+ * (it owns one CG workspace).  stats (may be NULL): 2 * nplans doubles,
+ * [2k] iterations and [2k + 1] final relative residual of solve k.
+ * No counterpart in the reference (HeatMat::cgSolve solves one system per
+ * call, FEM/examples/src/heatMat.cpp:165); for servers batching requests. */
+int dkt_cg_solve_batch(dkt_plan *const *plans, int nplans, const double *const *b, double *const *x, int max_iter,
+                       double tol, int masked, double *stats, void *stream);
+
 /* ---- Batches ----------------------------------------------------------- */
 /* One launch per (dim, p) over the tiles of many plans (independent meshes),
  * each with its own device vectors x[k], y[k] of [N_k][nrhs] (dtype 0 f64,

@@ -151,3 +151,42 @@ def test_cg_bitwise_reproducible_and_zero_rhs(dkt):
     x = torch.zeros_like(b)
     st = plan.cg_solve_torch(z, x, dkt.CG_F64, 100, 1e-8)
     assert st["iters"] == 0 and float(x.abs().max()) == 0.0
+
+
+@pytest.mark.gpu
+def test_cg_batch_bitwise_equals_single_solves(dkt, oracle):
+    """dkt_cg_solve_batch over meshes of different (dim, p), sizes and
+    iteration counts: every solve is bitwise the single FP64 solve from x = 0
+    (same iterations, residual and x), and its residual, recomputed with the
+    oracle's operator, is below tol.  A second batch with a small max_iter
+    stops the long solves early, again bitwise as the single solves do; a
+    plan listed twice is refused."""
+    torch = _torch()
+    names = ["sphere3d_small", "ex3_3d_p1", "gauss4d_small", "ex1_2d_p2"]
+    cases = []
+    for i, name in enumerate(names):
+        da, z, meta = fx.build_da(dkt, name)
+        b = _rhs(da, meta["dim"], 11 + i)
+        cases.append((name, da, meta, dkt.Plan(da, dkt.OP_STIFFNESS), torch.tensor(b, device="cuda")))
+    tol = 1e-8
+    for max_iter in (5000, 40):
+        singles = []
+        for name, da, meta, plan, bd in cases:
+            x = torch.zeros_like(bd)
+            st = plan.cg_solve_torch(bd, x, dkt.CG_F64, max_iter, tol)
+            singles.append((st, x.cpu().numpy()))
+        xs = [torch.full_like(bd, 7.0) for *_, bd in cases]  # overwritten: the batch starts from x = 0
+        stats = dkt.cg_solve_batch_torch([c[3] for c in cases], [c[4] for c in cases], xs, max_iter, tol)
+        torch.cuda.synchronize()
+        for (name, da, meta, plan, bd), (st1, x1), stb, xb in zip(cases, singles, stats, xs):
+            print(f"max_iter {max_iter} {name}: single {st1}, batch {stb}")
+            assert stb["iters"] == st1["iters"] and stb["resid"] == st1["resid"]
+            assert np.array_equal(xb.cpu().numpy(), x1)
+            if max_iter == 5000:
+                A = _op(oracle, meta, da.nodes())
+                b = bd.cpu().numpy()
+                assert np.abs(b - A(x1)).max() / np.abs(b).max() <= 1.01 * tol
+        if max_iter == 40:
+            assert any(s["iters"] == 40 for s in stats)
+    with pytest.raises(dkt.DktError):
+        dkt.cg_solve_batch_torch([cases[0][3]] * 2, [cases[0][4]] * 2, [torch.zeros_like(cases[0][4])] * 2)
