@@ -262,7 +262,8 @@ def serve_leg(dkt, nreq):
     requests alternating Poisson and heat-equation kinds (heat: b = M f with HeatVec's
     hooks, then CG), tol 1e-8, 4 HIP streams, plans built before the timed runs.
     Mixed-precision CG at 50/200/800 requests/s and FP64 CG at 200 requests/s;
-    latency runs from arrival to solution on the host."""
+    then FP64 CG with request batching (2 workers, up to 10 requests per batched
+    solve) at 200/800 requests/s; latency runs from arrival to solution on the host."""
     import importlib.util as ilu
     spec = ilu.spec_from_file_location("dkt_serve", os.path.join(HERE, "dendro-kt_amd", "serve.py"))
     sv = ilu.module_from_spec(spec)
@@ -285,12 +286,19 @@ def serve_leg(dkt, nreq):
             elems.append(da.n_elements)
         out["plan_build_s"] = round(time.perf_counter() - t0, 2)
         out["elements_min"], out["elements_max"] = int(min(elems)), int(max(elems))
-        runs = [(50, 1, "mixed"), (200, 1, "mixed"), (800, 1, "mixed"), (200, 0, "f64")]
-        for rate, mode, name in runs:
+        runs = [(50, 1, "mixed", srv), (200, 1, "mixed", srv), (800, 1, "mixed", srv), (200, 0, "f64", srv),
+                (200, 0, "f64_batched", None), (800, 0, "f64_batched", None)]
+        for rate, mode, name, server in runs:
+            if server is None:
+                if srv.batch == 0:
+                    srv.close()
+                    srv = sv.Server(dkt, n_streams=2, batch=10, cache=srv.cache)
+                    out["batched"] = {"workers": 2, "max_batch": 10}
+                server = srv
             reqs = [sv.Request(pool[int(rng.integers(len(pool)))], rhs_seed=int(rng.integers(1 << 30)), mode=mode,
                                kind="heat" if k % 2 else "poisson") for k in range(nreq)]
             log(f"serving {nreq} requests at {rate}/s ({name})")
-            done, wall = sv.run_open_loop(srv, reqs, rate, seed=rate)
+            done, wall = sv.run_open_loop(server, reqs, rate, seed=rate)
             errors = [r.error for r in done if r.error]
             if errors:
                 out[f"{name}_{rate}"] = {"errors": errors[:3]}

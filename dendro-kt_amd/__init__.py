@@ -115,6 +115,10 @@ def _bind_device(L):
     L.dkt_cg_solve.argtypes = [P, c.c_int, P, P, c.c_int, c.c_double, c.c_int, c.POINTER(c.c_double), P]
     L.dkt_cg_solve_batch.argtypes = [c.POINTER(P), c.c_int, c.POINTER(P), c.POINTER(P), c.c_int, c.c_double, c.c_int,
                                      c.POINTER(c.c_double), P]
+    L.dkt_cg_engine_create.argtypes = [c.c_int, c.c_int, c.POINTER(P)]
+    L.dkt_cg_engine_add.argtypes = [P, P, P, P, c.c_int, c.c_double, c.POINTER(c.c_int)]
+    L.dkt_cg_engine_step.argtypes = [P, c.c_int, P, c.POINTER(c.c_int), c.POINTER(c.c_double), c.POINTER(c.c_int)]
+    L.dkt_cg_engine_destroy.argtypes = [P]
     L.dkt_batch_create.argtypes = [c.POINTER(P), c.c_int, c.c_int, c.c_int, c.POINTER(P), c.POINTER(P), c.c_int,
                                    c.c_int, c.POINTER(P)]
     L.dkt_batch_run.argtypes = [P, P]
@@ -376,6 +380,54 @@ class Plan:
     def __del__(self):
         if getattr(self, "h", None) and _lib is not None:
             _lib.dkt_plan_destroy(self.h)
+            self.h = None
+
+
+class CgEngine:
+    """Continuous batching of FP64 CG solves (dkt_cg_engine): add() starts a
+    solve at the next step, step() runs chunks of 8 iterations of every running
+    solve and returns the ones that stopped as [(key, dict(iters, resid))].
+    Each solve is bitwise Plan.cg_solve from x = 0; x is overwritten."""
+
+    def __init__(self, max_slots, masked=True):
+        L = load()
+        self.max_slots = max_slots
+        h = ctypes.c_void_p()
+        _chk(L.dkt_cg_engine_create(max_slots, 1 if masked else 0, ctypes.byref(h)))
+        self.h = h
+        self.running = {}  # slot -> (key, plan, b, x): kept alive while the solve runs
+        self._fin = (ctypes.c_int * max_slots)()
+        self._st = (ctypes.c_double * (2 * max_slots))()
+
+    def add(self, plan, b, x, key=None, max_iter=10000, tol=1e-8):
+        import torch
+        if b.dtype != torch.float64 or x.dtype != torch.float64 or b.shape != (plan.n_nodes,) or x.shape != b.shape:
+            raise DktError("b and x must be float64 device vectors of the plan's node count")
+        if not (b.is_cuda and x.is_cuda and b.is_contiguous() and x.is_contiguous()):
+            raise DktError("b and x must be contiguous device tensors")
+        slot = ctypes.c_int(-1)
+        _chk(load().dkt_cg_engine_add(self.h, plan.h.value, ctypes.c_void_p(b.data_ptr()),
+                                      ctypes.c_void_p(x.data_ptr()), max_iter, tol, ctypes.byref(slot)))
+        self.running[slot.value] = (key, plan, b, x)
+        return slot.value
+
+    def step(self, chunks=1, stream=None):
+        import torch
+        s = stream if stream is not None else torch.cuda.current_stream().cuda_stream
+        nf = ctypes.c_int(0)
+        _chk(load().dkt_cg_engine_step(self.h, chunks, ctypes.c_void_p(s or 0), self._fin, self._st, ctypes.byref(nf)))
+        out = []
+        for i in range(nf.value):
+            key = self.running.pop(self._fin[i])[0]
+            out.append((key, dict(iters=int(self._st[2 * i]), resid=self._st[2 * i + 1])))
+        return out
+
+    def __len__(self):
+        return len(self.running)
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.dkt_cg_engine_destroy(self.h)
             self.h = None
 
 

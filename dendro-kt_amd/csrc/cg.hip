@@ -28,6 +28,7 @@
 // (or looser when tol / ||r|| is larger), then x += ||r|| d.
 #include <algorithm>
 #include <cmath>
+#include <deque>
 
 #include "../../include/dkt.h"
 #include "capi_common.h"
@@ -462,9 +463,11 @@ int cg_reliable(dkt_plan *pl, const double *b, double *x, int max_iter, double t
   return it;
 }
 
-// One solve of a batch (dkt_cg_solve_batch): its vectors and workspace.
+// One solve of a batch (dkt_cg_engine, dkt_cg_solve_batch): its vectors,
+// workspace and stop rule.
 struct CgDesc {
-  int n;
+  int n, max_iter;
+  double tol;
   const double *b;
   double *x, *r, *p, *ap, *part, *scal;
   int *flag;
@@ -477,9 +480,11 @@ __global__ void __launch_bounds__(kCgThreads) kb_init(const CgDesc *__restrict__
   CG_LOOP(i, e.n) e.x[i] = 0.0;
   init_body<double>(e.n, e.b, nullptr, e.r, e.p, e.part);
 }
-__global__ void __launch_bounds__(kCgThreads) kb_init_fin(const CgDesc *__restrict__ d, double tol) {
+// As k_init_fin; a solve with max_iter = 0 stops at once (flag[0] = 2).
+__global__ void __launch_bounds__(kCgThreads) kb_init_fin(const CgDesc *__restrict__ d) {
   const CgDesc e = d[blockIdx.y];
-  init_fin_body(e.part, e.scal, tol, e.flag);
+  init_fin_body(e.part, e.scal, e.tol, e.flag);
+  if (threadIdx.x == 0 && e.max_iter <= 0 && !e.flag[0]) e.flag[0] = 2;
 }
 __global__ void __launch_bounds__(kCgThreads) kb_dot(const CgDesc *__restrict__ d) {
   const CgDesc e = d[blockIdx.y];
@@ -489,11 +494,17 @@ __global__ void __launch_bounds__(kCgThreads) kb_update_xr(const CgDesc *__restr
   const CgDesc e = d[blockIdx.y];
   update_xr_body<double>(e.n, e.x, e.r, e.p, e.ap, e.part + 2 * kCgBlocks, e.scal, par, e.part, e.flag);
 }
-__global__ void __launch_bounds__(kCgThreads) kb_update_p(const CgDesc *__restrict__ d, int par, double tol) {
+// As k_update_p; after the solve's max_iter-th iteration it stops (flag[0] =
+// 2), as the single solve's host loop does.  Blocks that start after block 0
+// has raised the flag skip the update of p, which is no longer used.
+__global__ void __launch_bounds__(kCgThreads) kb_update_p(const CgDesc *__restrict__ d, int par) {
   const CgDesc e = d[blockIdx.y];
-  update_p_body<double>(e.n, e.p, e.r, e.part, e.scal, par, tol, e.flag);
+  if (e.flag[0]) return;
+  update_p_body<double>(e.n, e.p, e.r, e.part, e.scal, par, e.tol, e.flag);
+  if (blockIdx.x == 0 && threadIdx.x == 0 && !e.flag[0] && e.flag[1] >= e.max_iter) e.flag[0] = 2;
 }
-// out[3k .. 3k + 2] = converged flag, iterations, relative residual of solve k.
+// out[3k .. 3k + 2] = stop flag (0 running, 1 converged, 2 max_iter),
+// iterations, relative residual of solve k.
 __global__ void __launch_bounds__(kCgThreads) kb_status(const CgDesc *__restrict__ d, int nsolves,
                                                         double *__restrict__ out) {
   for (int k = threadIdx.x; k < nsolves; k += blockDim.x) {
@@ -507,77 +518,6 @@ __global__ void __launch_bounds__(kCgThreads) kb_status(const CgDesc *__restrict
 }  // namespace dkt
 
 using namespace dkt;
-
-extern "C" int dkt_cg_solve_batch(dkt_plan *const *plans, int nplans, const double *const *b, double *const *x,
-                                  int max_iter, double tol, int masked, double *stats, void *stream) {
-  return guarded([&] {
-    if (!plans || !b || !x || nplans < 1) throw std::invalid_argument("null pointer or empty batch");
-    if (max_iter < 0 || !(tol >= 0.0)) throw std::invalid_argument("max_iter >= 0 and tol >= 0 required");
-    std::vector<dkt_plan *> pls(plans, plans + nplans);
-    for (int k = 0; k < nplans; k++) {
-      if (!pls[k] || !b[k] || !x[k]) throw std::invalid_argument("null plan or vector");
-      if (pls[k]->dry) throw std::invalid_argument("a dry-run plan cannot solve");
-      if ((const void *)b[k] == (const void *)x[k]) throw std::invalid_argument("b and x must not alias");
-      for (int j = 0; j < k; j++)
-        if (pls[j] == pls[k]) throw std::invalid_argument("a plan may appear once per batch (one CG workspace)");
-    }
-    hipStream_t st = (hipStream_t)stream;
-    const int K = nplans;
-    std::vector<CgDesc> hd(K);
-    std::vector<const double *> px(K);
-    std::vector<double *> py(K);
-    std::vector<const int *> cancel(K);
-    for (int k = 0; k < K; k++) {
-      ensure_workspace(pls[k], false);
-      CgWorkspace &w = pls[k]->cg;
-      hd[k] = CgDesc{(int)pls[k]->N, b[k], x[k], w.r.p, w.p.p, w.ap.p, w.part64.p, w.scal.p, w.flag.p};
-      px[k] = w.p.p;
-      py[k] = w.ap.p;
-      cancel[k] = w.flag.p;  // flag[0]: converged
-    }
-    // Matvecs of every plan in one launch per (dim, p), tiled plans only.
-    std::unique_ptr<MatvecLauncher> mv = batch_matvec_f64(pls, px, py, masked ? 1 : 0, cancel);
-    DevBuf<CgDesc> d;
-    d.alloc(K);
-    DevBuf<double> status;
-    status.alloc(3 * (size_t)K);
-    HIP_OK(hipMemcpyAsync(d.p, hd.data(), K * sizeof(CgDesc), hipMemcpyHostToDevice, st));
-    std::vector<double> hs(3 * (size_t)K);
-    auto poll = [&] {
-      hipLaunchKernelGGL(kb_status, dim3(1), dim3(kCgThreads), 0, st, d.p, K, status.p);
-      HIP_OK(hipGetLastError());
-      HIP_OK(hipMemcpyAsync(hs.data(), status.p, hs.size() * sizeof(double), hipMemcpyDeviceToHost, st));
-      HIP_OK(hipStreamSynchronize(st));
-      for (int k = 0; k < K; k++)
-        if (hs[3 * k] == 0.0) return false;
-      return true;
-    };
-    const dim3 grid(kCgBlocks, K), blk(kCgThreads);
-    hipLaunchKernelGGL(kb_init, grid, blk, 0, st, d.p);
-    hipLaunchKernelGGL(kb_init_fin, dim3(1, K), blk, 0, st, d.p, tol);
-    HIP_OK(hipGetLastError());
-    int it = 0;
-    bool done = poll();
-    while (!done && it < max_iter) {
-      const int chunk = std::min(kCgChunk, max_iter - it);
-      for (int c = 0; c < chunk; c++) {
-        const int par = (it + c) & 1;
-        mv->launch(st);
-        hipLaunchKernelGGL(kb_dot, grid, blk, 0, st, d.p);
-        hipLaunchKernelGGL(kb_update_xr, grid, blk, 0, st, d.p, par);
-        hipLaunchKernelGGL(kb_update_p, grid, blk, 0, st, d.p, par, tol);
-        HIP_OK(hipGetLastError());
-      }
-      it += chunk;
-      done = poll();
-    }
-    if (stats)
-      for (int k = 0; k < K; k++) {
-        stats[2 * k] = hs[3 * k + 1];
-        stats[2 * k + 1] = hs[3 * k + 2];
-      }
-  });
-}
 
 extern "C" int dkt_cg_solve(dkt_plan *pl, int mode, const double *b, double *x, int max_iter, double tol, int masked,
                             double *stats, void *stream) {
@@ -631,6 +571,193 @@ extern "C" int dkt_cg_solve(dkt_plan *pl, int mode, const double *b, double *x, 
       stats[1] = resid;
       stats[2] = outer;
       stats[3] = matvecs;
+    }
+  });
+}
+
+// Continuous batching of FP64 CG solves.  Solves join (dkt_cg_engine_add)
+// and leave (finished in dkt_cg_engine_step) between chunks of kCgChunk
+// iterations; every chunk runs one matvec launch per (dim, p) of the active
+// plans and one launch per vector step for all active solves.  A solve starts
+// at the first step after it was added, so it always runs an even number of
+// iterations per chunk and its own iteration parity: its arithmetic is that of
+// its own dkt_cg_solve from x = 0.
+struct dkt_cg_engine {
+  // A slot owns its CG workspace (kept and grown across solves), so several
+  // solves may run on one plan at once.
+  struct Slot {
+    dkt_plan *pl = nullptr;
+    CgDesc d{};
+    bool active = false, fresh = false;
+    size_t cap = 0;
+    DevBuf<double> r, p, ap, part, scal;
+    DevBuf<int> flag;
+  };
+  int masked = 0;
+  std::deque<Slot> slots;  // fixed size after creation; Slot is not movable
+  bool dirty = true;                       // active set changed: rebuild the launches
+  std::vector<int> act;                    // active slots in descriptor order
+  std::unique_ptr<MatvecLauncher> mv;
+  DevBuf<CgDesc> dact, dnew;
+  DevBuf<double> status;
+  std::vector<CgDesc> hact, hnew;
+  std::vector<double> hs;
+};
+
+extern "C" int dkt_cg_engine_create(int max_slots, int masked, dkt_cg_engine **out) {
+  return guarded([&] {
+    if (!out || max_slots < 1 || max_slots > 65535) throw std::invalid_argument("max_slots must be 1..65535");
+    auto e = std::make_unique<dkt_cg_engine>();
+    e->masked = masked ? 1 : 0;
+    e->slots.resize(max_slots);
+    e->status.alloc(3 * (size_t)max_slots);
+    *out = e.release();
+  });
+}
+
+extern "C" int dkt_cg_engine_add(dkt_cg_engine *e, dkt_plan *pl, const double *b, double *x, int max_iter, double tol,
+                                 int *slot) {
+  return guarded([&] {
+    if (!e || !pl || !b || !x || !slot) throw std::invalid_argument("null pointer");
+    if (pl->dry) throw std::invalid_argument("a dry-run plan cannot solve");
+    if (pl->simple || pl->per_level || pl->conforming)
+      throw std::invalid_argument("batched solves need tiled device plans");
+    if (max_iter < 0 || !(tol >= 0.0)) throw std::invalid_argument("max_iter >= 0 and tol >= 0 required");
+    if ((const void *)b == (const void *)x) throw std::invalid_argument("b and x must not alias");
+    int free_slot = -1;
+    for (int k = 0; k < (int)e->slots.size() && free_slot < 0; k++)
+      if (!e->slots[k].active) free_slot = k;
+    if (free_slot < 0) throw std::invalid_argument("no free slot");
+    auto &s = e->slots[free_slot];
+    if (s.cap < pl->N) {
+      s.r.alloc(pl->N);
+      s.p.alloc(pl->N);
+      s.ap.alloc(pl->N);
+      s.cap = pl->N;
+    }
+    if (!s.part.p) {
+      s.part.alloc(4 * kCgBlocks);
+      s.scal.alloc(S_COUNT);
+      s.flag.alloc(4);
+    }
+    s.pl = pl;
+    s.d = CgDesc{(int)pl->N, max_iter, tol, b, x, s.r.p, s.p.p, s.ap.p, s.part.p, s.scal.p, s.flag.p};
+    s.active = true;
+    s.fresh = true;
+    e->dirty = true;
+    *slot = free_slot;
+  });
+}
+
+// Starts the solves added since the last step, runs `chunks` chunks of
+// iterations over the active solves and reports those that stopped:
+// finished[i] = slot, stats[2i] = iterations, stats[2i + 1] = relative
+// residual, *nfinished = their count (arrays of at least max_slots entries).
+// Their slots are free again.
+extern "C" int dkt_cg_engine_step(dkt_cg_engine *e, int chunks, void *stream, int *finished, double *stats,
+                                  int *nfinished) {
+  return guarded([&] {
+    if (!e || !finished || !nfinished || chunks < 0) throw std::invalid_argument("null pointer or chunks < 0");
+    hipStream_t st = (hipStream_t)stream;
+    *nfinished = 0;
+    if (e->dirty) {
+      e->act.clear();
+      e->hact.clear();
+      e->hnew.clear();
+      std::vector<dkt_plan *> pls;
+      std::vector<const double *> px;
+      std::vector<double *> py;
+      std::vector<const int *> cancel;
+      for (int k = 0; k < (int)e->slots.size(); k++) {
+        auto &s = e->slots[k];
+        if (!s.active) continue;
+        e->act.push_back(k);
+        e->hact.push_back(s.d);
+        if (s.fresh) e->hnew.push_back(s.d);
+        s.fresh = false;
+        pls.push_back(s.pl);
+        px.push_back(s.d.p);
+        py.push_back(s.d.ap);
+        cancel.push_back(s.d.flag);  // flag[0]: stopped
+      }
+      e->mv = pls.empty() ? nullptr : batch_matvec_f64(pls, px, py, e->masked, cancel);
+      if (e->dact.n < e->hact.size()) e->dact.alloc(e->hact.size());
+      if (e->dnew.n < e->hnew.size()) e->dnew.alloc(e->hnew.size());
+      if (!e->hact.empty())
+        HIP_OK(hipMemcpyAsync(e->dact.p, e->hact.data(), e->hact.size() * sizeof(CgDesc), hipMemcpyHostToDevice,
+                              st));
+      if (!e->hnew.empty()) {
+        HIP_OK(hipMemcpyAsync(e->dnew.p, e->hnew.data(), e->hnew.size() * sizeof(CgDesc), hipMemcpyHostToDevice,
+                              st));
+        const int nn = (int)e->hnew.size();
+        hipLaunchKernelGGL(kb_init, dim3(kCgBlocks, nn), dim3(kCgThreads), 0, st, e->dnew.p);
+        hipLaunchKernelGGL(kb_init_fin, dim3(1, nn), dim3(kCgThreads), 0, st, e->dnew.p);
+        HIP_OK(hipGetLastError());
+      }
+      e->dirty = false;
+    }
+    const int K = (int)e->act.size();
+    if (K == 0) return;
+    const dim3 grid(kCgBlocks, K), blk(kCgThreads);
+    for (int c = 0; c < chunks * kCgChunk; c++) {
+      const int par = c & 1;
+      e->mv->launch(st);
+      hipLaunchKernelGGL(kb_dot, grid, blk, 0, st, e->dact.p);
+      hipLaunchKernelGGL(kb_update_xr, grid, blk, 0, st, e->dact.p, par);
+      hipLaunchKernelGGL(kb_update_p, grid, blk, 0, st, e->dact.p, par);
+      HIP_OK(hipGetLastError());
+    }
+    hipLaunchKernelGGL(kb_status, dim3(1), dim3(kCgThreads), 0, st, e->dact.p, K, e->status.p);
+    HIP_OK(hipGetLastError());
+    e->hs.resize(3 * (size_t)K);
+    HIP_OK(hipMemcpyAsync(e->hs.data(), e->status.p, e->hs.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIP_OK(hipStreamSynchronize(st));
+    for (int i = 0; i < K; i++) {
+      if (e->hs[3 * i] == 0.0) continue;
+      const int k = e->act[i];
+      finished[*nfinished] = k;
+      if (stats) {
+        stats[2 * *nfinished] = e->hs[3 * i + 1];
+        stats[2 * *nfinished + 1] = e->hs[3 * i + 2];
+      }
+      (*nfinished)++;
+      e->slots[k].active = false;
+      e->dirty = true;
+    }
+  });
+}
+
+extern "C" int dkt_cg_engine_destroy(dkt_cg_engine *e) {
+  delete e;
+  return 0;
+}
+
+extern "C" int dkt_cg_solve_batch(dkt_plan *const *plans, int nplans, const double *const *b, double *const *x,
+                                  int max_iter, double tol, int masked, double *stats, void *stream) {
+  return guarded([&] {
+    if (!plans || !b || !x || nplans < 1) throw std::invalid_argument("null pointer or empty batch");
+    dkt_cg_engine *raw = nullptr;
+    if (dkt_cg_engine_create(nplans, masked, &raw) != 0) throw std::runtime_error(dkt_last_error());
+    std::unique_ptr<dkt_cg_engine> e(raw);
+    for (int k = 0; k < nplans; k++) {
+      int slot = -1;
+      if (dkt_cg_engine_add(e.get(), plans[k], b[k], x[k], max_iter, tol, &slot) != 0)
+        throw std::invalid_argument(dkt_last_error());
+    }
+    std::vector<int> fin(nplans);
+    std::vector<double> fst(2 * (size_t)nplans);
+    int left = nplans;
+    // The first step only starts the solves (they stop at once when b = 0 or max_iter = 0).
+    for (int chunks = 0; left > 0; chunks = 1) {
+      int nf = 0;
+      if (dkt_cg_engine_step(e.get(), chunks, stream, fin.data(), fst.data(), &nf) != 0)
+        throw std::runtime_error(dkt_last_error());
+      for (int i = 0; i < nf; i++)
+        if (stats) {
+          stats[2 * fin[i]] = fst[2 * i];
+          stats[2 * fin[i] + 1] = fst[2 * i + 1];
+        }
+      left -= nf;
     }
   });
 }

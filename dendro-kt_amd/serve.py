@@ -19,7 +19,13 @@ Design:
   the device (the C calls release the GIL), solves on the same plan are
   serialised because a plan owns one CG workspace;
 * every reduction on the device runs in a fixed order, so a request's answer
-  does not depend on what else is being served ("alone == in-mix").
+  does not depend on what else is being served ("alone == in-mix");
+* continuous batching (Server(batch=K)): each worker runs a CG engine
+  (dkt_cg_engine) of K slots; a queued request joins as soon as a slot is free
+  and leaves when its solve stops, between chunks of 8 iterations, so one
+  launch per step serves every running solve and no solve waits for a slower
+  one.  Solves are FP64; each answer is bitwise the request's own FP64 solve.
+  Every slot has its own CG workspace, so requests on one mesh run together.
 
 Latencies are measured from submission to completion of each request
 (end to end) and from plan availability to completion (solve).
@@ -31,6 +37,8 @@ import time
 from dataclasses import dataclass, field
 
 import numpy as np
+
+_EMPTY = object()  # no request available
 
 
 @dataclass(frozen=True)
@@ -131,17 +139,19 @@ class PlanCache:
 
 
 class Server:
-    def __init__(self, dkt, n_streams=4, hbm_budget_bytes=200e9):
+    def __init__(self, dkt, n_streams=4, hbm_budget_bytes=200e9, batch=0, cache=None):
         import torch
         self.torch = torch
         self.dkt = dkt
-        self.cache = PlanCache(dkt, hbm_budget_bytes)
+        self.cache = cache if cache is not None else PlanCache(dkt, hbm_budget_bytes)
         self.q = queue.Queue()
+        self.batch = batch
         self.done = []
         self.done_lock = threading.Lock()
         self.workers = []
+        work = self._work_batched if batch > 0 else self._work
         for i in range(n_streams):
-            t = threading.Thread(target=self._work, args=(torch.cuda.Stream(),), daemon=True)
+            t = threading.Thread(target=work, args=(torch.cuda.Stream(),), daemon=True)
             t.start()
             self.workers.append(t)
 
@@ -183,6 +193,90 @@ class Server:
             req.finished = time.perf_counter()
             with self.done_lock:
                 self.done.append(req)
+
+    def _rhs(self, req, da):
+        f = self.dkt.keyed_uniform(req.rhs_seed, da.nodes())
+        if req.kind == "poisson":
+            f[da.boundary()] = 0.0
+        elif req.kind != "heat":
+            raise ValueError(f"unknown request kind {req.kind}")
+        return f
+
+    def _finish(self, req):
+        req.finished = time.perf_counter()
+        with self.done_lock:
+            self.done.append(req)
+
+    def _next(self, block):
+        """A queued request (None: shut down; _EMPTY: none)."""
+        try:
+            return self.q.get(block=block, timeout=0.05 if block else None)
+        except queue.Empty:
+            return _EMPTY
+
+    def _work_batched(self, stream):
+        """Continuous batching: requests join the worker's CG engine as slots free
+        up and leave when their solve stops, between chunks of 8 iterations."""
+        torch = self.torch
+        engine = self.dkt.CgEngine(self.batch, masked=True)
+        running = {}  # id(req) -> (req, b, x)
+        stop = False
+        while True:
+            while not stop and len(running) < self.batch:
+                req = self._next(block=not running)
+                if req is _EMPTY:
+                    break
+                if req is None:
+                    stop = True
+                    break
+                if not req.started:
+                    req.started = time.perf_counter()
+                try:
+                    da, (plan, mass), lk, built = self.cache.get(req.mesh)
+                except Exception as e:  # reported per request
+                    req.error = f"{type(e).__name__}: {e}"
+                    self._finish(req)
+                    continue
+                req.plan_built = req.plan_built or built
+                try:
+                    with torch.cuda.stream(stream):
+                        req.solve_started = time.perf_counter()
+                        fd = torch.tensor(self._rhs(req, da), device="cuda")
+                        if req.kind == "heat":
+                            # the mass plan's matvec buffers are per plan: one user at a time
+                            bd = torch.empty_like(fd)
+                            with lk:
+                                mass.matvec_torch(fd, bd, masked=2, stream=stream.cuda_stream)
+                                stream.synchronize()
+                        else:
+                            bd = fd
+                        xd = torch.empty_like(fd)
+                        engine.add(plan, bd, xd, key=id(req), max_iter=req.max_iter, tol=req.tol)
+                    running[id(req)] = (req, bd, xd)
+                except Exception as e:
+                    req.error = f"{type(e).__name__}: {e}"
+                    self._finish(req)
+            if not running:
+                if stop:
+                    return
+                continue
+            try:
+                with torch.cuda.stream(stream):
+                    finished = engine.step(1, stream=stream.cuda_stream)
+                    out = []
+                    for key, st in finished:
+                        req, bd, xd = running.pop(key)
+                        req.stats = dict(iters=st["iters"], resid=st["resid"], refinements=0, matvecs=st["iters"])
+                        req.x = xd.cpu().numpy()
+                        out.append(req)
+                for req in out:
+                    self._finish(req)
+            except Exception as e:  # every running solve fails
+                for req, *_ in running.values():
+                    req.error = f"{type(e).__name__}: {e}"
+                    self._finish(req)
+                running.clear()
+                engine = self.dkt.CgEngine(self.batch, masked=True)
 
     def drain(self, n, timeout=None):
         t0 = time.perf_counter()

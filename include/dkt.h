@@ -197,19 +197,45 @@ int dkt_cg_solve(dkt_plan *plan, int mode, const double *b, double *x, int max_i
                  double *stats, void *stream);
 
 /* Batched FP64 conjugate gradients: solves A_k x_k = b_k for nplans tiled
- * plans (any mix of (dim, p); no simple, per-level or conforming plans) in
- * lockstep on one stream, each as dkt_cg_solve with DKT_CG_F64 from x_k = 0
+ * plans (any mix of (dim, p); no simple, per-level or conforming plans)
+ * together on one stream, each as dkt_cg_solve with DKT_CG_F64 from x_k = 0
  * (x_k is overwritten).  Per iteration one matvec launch per (dim, p) group
  * and one launch per vector step serve every solve; a converged solve's
  * kernels return at once.  Each solve's arithmetic and reduction order are
  * those of its own dkt_cg_solve: x_k is bitwise what dkt_cg_solve(plans[k],
- * DKT_CG_F64, b_k, x_k = 0, ...) returns.  A plan may appear once per batch
- * (it owns one CG workspace).  stats (may be NULL): 2 * nplans doubles,
+ * DKT_CG_F64, b_k, x_k = 0, ...) returns.  A plan may appear several times
+ * (each solve has its own workspace).  stats (may be NULL): 2 * nplans doubles,
  * [2k] iterations and [2k + 1] final relative residual of solve k.
  * No counterpart in the reference (HeatMat::cgSolve solves one system per
  * call, FEM/examples/src/heatMat.cpp:165); for servers batching requests. */
 int dkt_cg_solve_batch(dkt_plan *const *plans, int nplans, const double *const *b, double *const *x, int max_iter,
                        double tol, int masked, double *stats, void *stream);
+
+/* Continuous batching of the same FP64 solves (dkt_cg_solve_batch runs on
+ * it): solves join with dkt_cg_engine_add and leave when they stop, between
+ * chunks of 8 iterations, so a server keeps the device busy with whatever is
+ * running instead of waiting for the slowest solve of a batch.  Each solve is
+ * bitwise its own dkt_cg_solve(DKT_CG_F64) from x = 0 (x is overwritten), with
+ * its own tol and max_iter; masked is per engine.  No counterpart in the
+ * reference.
+ * dkt_cg_engine_create: at most max_slots (1..65535) solves at a time.
+ * dkt_cg_engine_add: plan (tiled; several solves may share one), device b
+ *   and x of its N nodes (they must stay valid until the solve stops); *slot
+ *   gets the solve's slot, whose workspace the engine keeps for later solves.
+ *   Fails when every slot is taken.
+ * dkt_cg_engine_step: starts the solves added since the last step, runs
+ *   chunks * 8 iterations of every running solve on `stream`, synchronizes,
+ *   and reports the solves that stopped (converged or max_iter reached):
+ *   finished[i] = slot, stats[2i] = iterations, stats[2i + 1] = relative
+ *   residual (stats may be NULL), *nfinished = their count; arrays of
+ *   max_slots entries.  Their slots are free again.  One engine, one thread. */
+typedef struct dkt_cg_engine dkt_cg_engine;
+int dkt_cg_engine_create(int max_slots, int masked, dkt_cg_engine **out);
+int dkt_cg_engine_add(dkt_cg_engine *engine, dkt_plan *plan, const double *b, double *x, int max_iter, double tol,
+                      int *slot);
+int dkt_cg_engine_step(dkt_cg_engine *engine, int chunks, void *stream, int *finished, double *stats,
+                       int *nfinished);
+int dkt_cg_engine_destroy(dkt_cg_engine *engine);
 
 /* ---- Batches ----------------------------------------------------------- */
 /* One launch per (dim, p) over the tiles of many plans (independent meshes),

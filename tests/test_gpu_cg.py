@@ -160,7 +160,7 @@ def test_cg_batch_bitwise_equals_single_solves(dkt, oracle):
     (same iterations, residual and x), and its residual, recomputed with the
     oracle's operator, is below tol.  A second batch with a small max_iter
     stops the long solves early, again bitwise as the single solves do; a
-    plan listed twice is refused."""
+    plan listed twice solves both systems."""
     torch = _torch()
     names = ["sphere3d_small", "ex3_3d_p1", "gauss4d_small", "ex1_2d_p2"]
     cases = []
@@ -188,5 +188,53 @@ def test_cg_batch_bitwise_equals_single_solves(dkt, oracle):
                 assert np.abs(b - A(x1)).max() / np.abs(b).max() <= 1.01 * tol
         if max_iter == 40:
             assert any(s["iters"] == 40 for s in stats)
+    # one plan twice in a batch (a workspace per solve), two right-hand sides
+    b2 = torch.tensor(_rhs(cases[2][1], cases[2][2]["dim"], 99), device="cuda")
+    x_a, x_b = torch.zeros_like(b2), torch.zeros_like(b2)
+    plan = cases[2][3]
+    st_a = plan.cg_solve_torch(cases[2][4], x_a, dkt.CG_F64, 5000, tol)
+    st_b = plan.cg_solve_torch(b2, x_b, dkt.CG_F64, 5000, tol)
+    xs2 = [torch.empty_like(b2), torch.empty_like(b2)]
+    stats = dkt.cg_solve_batch_torch([plan, plan], [cases[2][4], b2], xs2, 5000, tol)
+    assert stats[0]["iters"] == st_a["iters"] and stats[1]["iters"] == st_b["iters"]
+    assert torch.equal(xs2[0], x_a) and torch.equal(xs2[1], x_b)
+
+
+@pytest.mark.gpu
+def test_cg_engine_solves_joining_mid_flight(dkt):
+    """Continuous batching (dkt_cg_engine): solves that join while others are
+    running, with their own tol and max_iter, stop with their own iteration
+    count and are bitwise their single FP64 solves; a full engine refuses a
+    solve, and a freed slot takes the next one."""
+    torch = _torch()
+    cases = []
+    for i, name in enumerate(["gauss4d_small", "sphere3d_small", "ex3_3d_p1"]):
+        da, z, meta = fx.build_da(dkt, name)
+        cases.append((dkt.Plan(da, dkt.OP_STIFFNESS), torch.tensor(_rhs(da, meta["dim"], 21 + i), device="cuda")))
+    params = [(1e-8, 5000), (1e-6, 5000), (1e-8, 13)]
+    singles = []
+    for (plan, bd), (tol, mi) in zip(cases, params):
+        x = torch.zeros_like(bd)
+        singles.append((plan.cg_solve_torch(bd, x, dkt.CG_F64, mi, tol), x.cpu().numpy()))
+    eng = dkt.CgEngine(2)
+    xs = [torch.empty_like(bd) for _, bd in cases]
+    got = {}
+    eng.add(cases[0][0], cases[0][1], xs[0], key=0, max_iter=params[0][1], tol=params[0][0])
+    for key, st in eng.step(3):
+        got[key] = st
+    eng.add(cases[1][0], cases[1][1], xs[1], key=1, max_iter=params[1][1], tol=params[1][0])
     with pytest.raises(dkt.DktError):
-        dkt.cg_solve_batch_torch([cases[0][3]] * 2, [cases[0][4]] * 2, [torch.zeros_like(cases[0][4])] * 2)
+        eng.add(cases[2][0], cases[2][1], xs[2], key=2)
+    pending = [2]
+    while len(got) < 3:
+        for key, st in eng.step(1):
+            got[key] = st
+            if pending:
+                k = pending.pop()
+                eng.add(cases[k][0], cases[k][1], xs[k], key=k, max_iter=params[k][1], tol=params[k][0])
+    torch.cuda.synchronize()
+    for k, ((st1, x1), xb) in enumerate(zip(singles, xs)):
+        print(k, "single", st1, "engine", got[k])
+        assert got[k]["iters"] == st1["iters"] and got[k]["resid"] == st1["resid"]
+        assert np.array_equal(xb.cpu().numpy(), x1)
+    assert got[2]["iters"] == 13

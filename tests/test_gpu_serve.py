@@ -97,3 +97,36 @@ def test_heat_request_solves_mass_rhs(dkt):
     k_plan.matvec_torch(x, ax, masked=1)
     torch.cuda.synchronize()
     assert float((b - ax).abs().max() / b.abs().max()) <= 1.01 * r.tol
+
+
+@pytest.mark.gpu
+def test_batched_server_equals_single_fp64_solves(dkt):
+    """Server(batch=K): continuous batching on two workers' CG engines, with
+    more requests than slots and several requests on one mesh running at once.
+    Every answer (Poisson and heat kinds) is bitwise the request's FP64 solve
+    served alone."""
+    _torch()
+    sv = _serve_mod()
+    meshes = [sv.MeshSpec(4, 8, n, s) for n, s in ((300, 1), (800, 2), (1500, 3))]
+    meshes.append(sv.MeshSpec(3, 30, shape="sphere", center=(0.4, 0.5, 0.6), radius=0.2, max_level=5))
+    reqs = [sv.Request(m, rhs_seed=k, kind=kind, mode=0) for k, kind in ((1, "poisson"), (2, "heat"), (3, "poisson"))
+            for m in meshes]
+    srv = sv.Server(dkt, n_streams=2, batch=3)
+    try:
+        for r in reqs:
+            srv.submit(r)
+        done = srv.drain(len(reqs), timeout=300)
+    finally:
+        srv.close()
+    assert all(not r.error for r in done), [r.error for r in done]
+    assert all(r.stats["resid"] <= r.tol for r in done)
+    alone = sv.Server(dkt, n_streams=1)
+    try:
+        for i, r in enumerate(reqs):
+            alone.submit(sv.Request(r.mesh, r.rhs_seed, kind=r.kind, mode=0))
+            out = alone.drain(i + 1, timeout=300)[-1]
+            assert not out.error
+            assert out.stats["iters"] == r.stats["iters"]
+            assert np.array_equal(out.x, r.x)
+    finally:
+        alone.close()
