@@ -5,8 +5,8 @@
 // (normLInfty, include/mathUtils.tcc:120-127; dot, :204-221).
 //
 // Device design: the scalars never leave the device.  Dot products and max
-// norms are two-stage reductions over a fixed partition (kCgBlocks blocks of
-// kCgThreads work-items, fixed tree order): every block that needs alpha or
+// norms are two-stage reductions over a fixed partition (cg_blocks(n) blocks
+// of kCgThreads work-items, fixed tree order): every block that needs alpha or
 // beta re-reduces the same partials in the same order, so all blocks agree
 // bit for bit and the solve is reproducible run to run.  Convergence raises a
 // device flag; the kernels of iterations queued after it (matvec included)
@@ -83,7 +83,16 @@ __device__ __forceinline__ double parts_max(const double *part, double *sh) {
   return block_max(fmax(part[threadIdx.x], part[threadIdx.x + kCgThreads]), sh);
 }
 
-#define CG_LOOP(i, n) for (int i = blockIdx.x * kCgThreads + threadIdx.x; i < (n); i += kCgBlocks * kCgThreads)
+// Blocks of a solve over n entries: at least 4 entries per work-item, at most
+// kCgBlocks.  Blocks past it write no partials; the partial arrays keep zeros
+// there (zeroed with the workspace), which the fixed-order sums and max norms
+// over all kCgBlocks slots ignore.
+__host__ __device__ inline int cg_blocks(int n) {
+  const int b = (n + 4 * kCgThreads - 1) / (4 * kCgThreads);
+  return b < 1 ? 1 : (b > kCgBlocks ? kCgBlocks : b);
+}
+#define CG_LOOP(i, n) \
+  for (int i = blockIdx.x * kCgThreads + threadIdx.x; i < (n); i += cg_blocks(n) * kCgThreads)
 
 // The bodies of the iteration kernels run one solve's block blockIdx.x of
 // kCgBlocks; the single-solve kernels and the batched ones (one solve per
@@ -332,7 +341,7 @@ template <typename T>
 int cg_run(dkt_plan *pl, int dtype, const T *b, T *x, bool x_is_zero, int max_iter, double tol, int masked,
            const CgVectors<T> &v, int *flag, int *host, hipStream_t st, int *matvecs) {
   const int n = (int)pl->N;
-  const dim3 grid(kCgBlocks), blk(kCgThreads);
+  const dim3 grid(cg_blocks(n)), blk(kCgThreads);
   if (!x_is_zero) {
     plan_apply(pl, dtype, x, v.ap, 1, masked, st, nullptr);
     (*matvecs)++;
@@ -384,6 +393,7 @@ void ensure_workspace(dkt_plan *pl, bool mixed) {
     w.p.alloc(n);
     w.ap.alloc(n);
     w.part64.alloc(4 * kCgBlocks);
+    HIP_OK(hipMemset(w.part64.p, 0, 4 * kCgBlocks * sizeof(double)));
     w.scal.alloc(S_COUNT);
     w.flag.alloc(4);
     HIP_OK(hipHostMalloc((void **)&w.host, 4 * sizeof(int), hipHostMallocDefault));
@@ -396,6 +406,7 @@ void ensure_workspace(dkt_plan *pl, bool mixed) {
     w.x32.alloc(n);
     w.b32.alloc(n);
     w.part32.alloc(4 * kCgBlocks);
+    HIP_OK(hipMemset(w.part32.p, 0, 4 * kCgBlocks * sizeof(double)));
     w.scal32.alloc(S_COUNT);
   }
 }
@@ -406,7 +417,7 @@ int cg_reliable(dkt_plan *pl, const double *b, double *x, int max_iter, double t
                 int *matvecs, int *updates, double *resid) {
   CgWorkspace &w = pl->cg;
   const int n = (int)pl->N;
-  const dim3 grid(kCgBlocks), blk(kCgThreads);
+  const dim3 grid(cg_blocks(n)), blk(kCgThreads);
   CgVectors<float> v{w.r32.p, w.p32.p, w.ap32.p, w.part32.p, w.scal32.p};
   int *flag = w.flag.p, *host = w.host;
   // True FP64 residual of the current x into d64 and scal[S_RESID].
@@ -475,8 +486,16 @@ struct CgDesc {
 
 // Batched FP64 iteration kernels: solve blockIdx.y, block blockIdx.x of its
 // kCgBlocks, through the single-solve bodies.
+// Blocks at or past cg_blocks(n) of a solve return at once (the grid spans the
+// largest solve).  kb_init's block 0 zeroes the partial slots past them: an
+// engine slot's workspace is reused by solves of other sizes.
 __global__ void __launch_bounds__(kCgThreads) kb_init(const CgDesc *__restrict__ d) {
   const CgDesc e = d[blockIdx.y];
+  const int nb = cg_blocks(e.n);
+  if (blockIdx.x >= nb) return;
+  if (blockIdx.x == 0)
+    for (int j = threadIdx.x; j < 4 * kCgBlocks; j += kCgThreads)
+      if (j % kCgBlocks >= nb) e.part[j] = 0.0;
   CG_LOOP(i, e.n) e.x[i] = 0.0;
   init_body<double>(e.n, e.b, nullptr, e.r, e.p, e.part);
 }
@@ -488,10 +507,12 @@ __global__ void __launch_bounds__(kCgThreads) kb_init_fin(const CgDesc *__restri
 }
 __global__ void __launch_bounds__(kCgThreads) kb_dot(const CgDesc *__restrict__ d) {
   const CgDesc e = d[blockIdx.y];
+  if (blockIdx.x >= cg_blocks(e.n)) return;
   dot_body<double>(e.n, e.p, e.ap, e.part + 2 * kCgBlocks, e.flag);
 }
 __global__ void __launch_bounds__(kCgThreads) kb_update_xr(const CgDesc *__restrict__ d, int par) {
   const CgDesc e = d[blockIdx.y];
+  if (blockIdx.x >= cg_blocks(e.n)) return;
   update_xr_body<double>(e.n, e.x, e.r, e.p, e.ap, e.part + 2 * kCgBlocks, e.scal, par, e.part, e.flag);
 }
 // As k_update_p; after the solve's max_iter-th iteration it stops (flag[0] =
@@ -499,7 +520,7 @@ __global__ void __launch_bounds__(kCgThreads) kb_update_xr(const CgDesc *__restr
 // has raised the flag skip the update of p, which is no longer used.
 __global__ void __launch_bounds__(kCgThreads) kb_update_p(const CgDesc *__restrict__ d, int par) {
   const CgDesc e = d[blockIdx.y];
-  if (e.flag[0]) return;
+  if (blockIdx.x >= cg_blocks(e.n) || e.flag[0]) return;
   update_p_body<double>(e.n, e.p, e.r, e.part, e.scal, par, e.tol, e.flag);
   if (blockIdx.x == 0 && threadIdx.x == 0 && !e.flag[0] && e.flag[1] >= e.max_iter) e.flag[0] = 2;
 }
@@ -549,17 +570,17 @@ extern "C" int dkt_cg_solve(dkt_plan *pl, int mode, const double *b, double *x, 
       while (true) {
         plan_apply(pl, 0, x, w.ap.p, 1, masked ? 1 : 0, st, nullptr);
         matvecs++;
-        hipLaunchKernelGGL(k_resid64, dim3(kCgBlocks), dim3(kCgThreads), 0, st, n, b, w.ap.p, w.d64.p, w.part64.p);
+        hipLaunchKernelGGL(k_resid64, dim3(cg_blocks(n)), dim3(kCgThreads), 0, st, n, b, w.ap.p, w.d64.p, w.part64.p);
         hipLaunchKernelGGL(k_resid64_fin, dim3(1), dim3(kCgThreads), 0, st, w.part64.p, w.scal.p);
         HIP_OK(hipGetLastError());
         resid = read_scalar(w.scal.p, S_RESID, st);
         if (resid <= tol || iters >= max_iter || !std::isfinite(resid)) break;
-        hipLaunchKernelGGL(k_to32, dim3(kCgBlocks), dim3(kCgThreads), 0, st, n, w.d64.p, w.scal.p, w.b32.p, w.x32.p);
+        hipLaunchKernelGGL(k_to32, dim3(cg_blocks(n)), dim3(kCgThreads), 0, st, n, w.d64.p, w.scal.p, w.b32.p, w.x32.p);
         // Inner tolerance: kInnerTol, loosened when the outer target is already close.
         const double inner_tol = std::min(0.5, std::max(kInnerTol, 0.5 * tol / resid));
         const int inner = cg_run<float>(pl, 1, w.b32.p, w.x32.p, true, max_iter - iters, inner_tol, masked ? 1 : 0, v,
                                         w.flag.p, w.host, st, &matvecs);
-        hipLaunchKernelGGL(k_add32, dim3(kCgBlocks), dim3(kCgThreads), 0, st, n, x, w.x32.p, w.scal.p);
+        hipLaunchKernelGGL(k_add32, dim3(cg_blocks(n)), dim3(kCgThreads), 0, st, n, x, w.x32.p, w.scal.p);
         HIP_OK(hipGetLastError());
         iters += inner;
         outer++;
@@ -690,7 +711,9 @@ extern "C" int dkt_cg_engine_step(dkt_cg_engine *e, int chunks, void *stream, in
         HIP_OK(hipMemcpyAsync(e->dnew.p, e->hnew.data(), e->hnew.size() * sizeof(CgDesc), hipMemcpyHostToDevice,
                               st));
         const int nn = (int)e->hnew.size();
-        hipLaunchKernelGGL(kb_init, dim3(kCgBlocks, nn), dim3(kCgThreads), 0, st, e->dnew.p);
+        int nbn = 1;
+        for (const CgDesc &c : e->hnew) nbn = std::max(nbn, cg_blocks(c.n));
+        hipLaunchKernelGGL(kb_init, dim3(nbn, nn), dim3(kCgThreads), 0, st, e->dnew.p);
         hipLaunchKernelGGL(kb_init_fin, dim3(1, nn), dim3(kCgThreads), 0, st, e->dnew.p);
         HIP_OK(hipGetLastError());
       }
@@ -698,7 +721,9 @@ extern "C" int dkt_cg_engine_step(dkt_cg_engine *e, int chunks, void *stream, in
     }
     const int K = (int)e->act.size();
     if (K == 0) return;
-    const dim3 grid(kCgBlocks, K), blk(kCgThreads);
+    int nb = 1;
+    for (const CgDesc &c : e->hact) nb = std::max(nb, cg_blocks(c.n));
+    const dim3 grid(nb, K), blk(kCgThreads);
     for (int c = 0; c < chunks * kCgChunk; c++) {
       const int par = c & 1;
       e->mv->launch(st);
