@@ -630,8 +630,7 @@ extern "C" int dkt_cg_engine_create(int max_slots, int masked, dkt_cg_engine **o
     if (!out || max_slots < 1 || max_slots > 65535) throw std::invalid_argument("max_slots must be 1..65535");
     auto e = std::make_unique<dkt_cg_engine>();
     e->masked = masked ? 1 : 0;
-    e->slots.resize(max_slots);
-    e->status.alloc(3 * (size_t)max_slots);
+    e->slots.resize(max_slots);  // device memory is allocated by add and step
     *out = e.release();
   });
 }
@@ -721,6 +720,7 @@ extern "C" int dkt_cg_engine_step(dkt_cg_engine *e, int chunks, void *stream, in
     }
     const int K = (int)e->act.size();
     if (K == 0) return;
+    if (e->status.n < 3 * e->slots.size()) e->status.alloc(3 * e->slots.size());
     int nb = 1;
     for (const CgDesc &c : e->hact) nb = std::max(nb, cg_blocks(c.n));
     const dim3 grid(nb, K), blk(kCgThreads);

@@ -112,3 +112,41 @@ def test_plain_c_caller(tmp_path):
     out = subprocess.run([str(exe)], capture_output=True, text=True, timeout=300)
     assert out.returncode == 0, out.stderr
     assert "elements 82776 nodes 55639" in out.stdout, out.stdout
+
+
+def test_cg_engine_argument_checks(dkt):
+    """The batched-CG entry points validate without touching the device: bad
+    slot counts, a dry-run plan, null pointers, negative chunks and an empty
+    batch are refused with status codes; stepping an idle engine does
+    nothing."""
+    import gen_inputs as gi
+    L = dkt.load()
+    V = ctypes.c_void_p
+    pts = gi.gauss_points(3, 200, 3, 0.06, 9, 8)
+    da = dkt.DA(dkt.balance_tree(dkt.Octs.from_array(pts, 3, 8, dkt.CURVE_HILBERT), 1), 1)
+    h, _ = _dry_plan(dkt, da)
+    e = V()
+    try:
+        assert L.dkt_cg_engine_create(0, 1, ctypes.byref(e)) != 0 and b"max_slots" in L.dkt_last_error()
+        assert L.dkt_cg_engine_create(2, 1, ctypes.byref(e)) == 0
+        slot = ctypes.c_int(-1)
+        b = (ctypes.c_double * 4)()
+        x = (ctypes.c_double * 4)()
+        pb, px = ctypes.cast(b, V), ctypes.cast(x, V)
+        assert L.dkt_cg_engine_add(e, h, pb, px, 100, 1e-8, ctypes.byref(slot)) != 0
+        assert b"dry-run" in L.dkt_last_error()
+        assert L.dkt_cg_engine_add(e, h, None, px, 100, 1e-8, ctypes.byref(slot)) != 0
+        assert b"null" in L.dkt_last_error()
+        assert L.dkt_cg_engine_add(None, h, pb, px, 100, 1e-8, ctypes.byref(slot)) != 0
+        n = ctypes.c_int(0)
+        fin = (ctypes.c_int * 2)()
+        assert L.dkt_cg_engine_step(e, -1, None, fin, None, ctypes.byref(n)) != 0
+        assert L.dkt_cg_engine_step(e, 1, None, fin, None, ctypes.byref(n)) == 0 and n.value == 0  # nothing running
+        plans = (V * 1)(h.value)
+        assert L.dkt_cg_solve_batch(plans, 0, (V * 1)(pb.value), (V * 1)(px.value), 10, 1e-8, 1, None, None) != 0
+        assert L.dkt_cg_solve_batch(plans, 1, (V * 1)(pb.value), (V * 1)(px.value), 10, 1e-8, 1, None, None) != 0
+        assert b"dry-run" in L.dkt_last_error()
+    finally:
+        if e:
+            L.dkt_cg_engine_destroy(e)
+        L.dkt_plan_destroy(h)
