@@ -135,3 +135,64 @@ def test_output_only_mask_is_heatvec_compute_vec(dkt, name):
     torch.cuda.synchronize()
     y = yd.cpu().numpy()
     assert np.linalg.norm(y - ref) <= 1e-12 * np.linalg.norm(ref)
+
+
+def test_full_size_4d_workload_properties(dkt):
+    """The benchmark's 4D workload at full size (2e6-point Gaussian-mixture tree,
+    max level 9: 15,282,481 elements, 10,346,445 nodes, beyond what the oracle
+    runs in seconds), through properties that hold at any size:
+    - the host pipeline reproduces the mesh sizes exactly;
+    - three tilings (64, 128, 256 leaves: different tiles, replay blocks,
+      carry terms and partial slots) agree to FP64 rounding;
+    - a repeated matvec is bitwise identical;
+    - the stiffness annihilates constants (interpolated and replayed values
+      of a constant are that constant, K_ref 1 = 0), to rounding relative to
+      max |A x|, in FP64 and FP32;
+    - masked = 1 zeroes the boundary rows, and on an x that vanishes on the
+      boundary it equals the raw operator on interior rows bitwise."""
+    torch = _torch()
+    pts = dkt.gauss_points(4, 9, 2000000, 4, 0.05, 1)
+    da = dkt.DA(dkt.balance_tree(pts, 1), 1)
+    assert da.n_elements == 15282481 and da.n_nodes == 10346445
+    assert da.count(3) == 19419623 and da.count(4) == 14545541
+    x = torch.tensor(dkt.keyed_uniform(0, da.nodes()), device="cuda")
+    ys = {}
+    for tile in (64, 128, 256):
+        plan = dkt.Plan(da, dkt.OP_STIFFNESS, tile_leaves=tile)
+        y = torch.full_like(x, float("nan"))
+        plan.matvec_torch(x, y)
+        ys[tile] = y
+        if tile == 128:
+            y2 = torch.full_like(x, float("nan"))
+            plan.matvec_torch(x, y2)
+            torch.cuda.synchronize()
+            assert torch.equal(y, y2)
+            one = torch.ones_like(x)
+            z = torch.full_like(x, float("nan"))
+            plan.matvec_torch(one, z)
+            z32 = torch.full_like(x, float("nan"), dtype=torch.float32)
+            plan.matvec_torch(one.float(), z32)
+            torch.cuda.synchronize()
+            # to rounding: tile-start buffer values are sums of dyadic weights over long chains
+            ymax = float(y.abs().max())
+            assert float(z.abs().max()) <= 1e-13 * ymax and float(z32.abs().max()) <= 1e-5 * ymax
+            bnd = torch.tensor(da.boundary(), device="cuda", dtype=torch.long)
+            xi = x.clone()
+            xi[bnd] = 0.0
+            raw = torch.full_like(x, float("nan"))
+            msk = torch.full_like(x, float("nan"))
+            plan.matvec_torch(xi, raw)
+            plan.matvec_torch(xi, msk, masked=True)
+            torch.cuda.synchronize()
+            assert float(msk[bnd].abs().max()) == 0.0
+            interior = torch.ones(da.n_nodes, dtype=torch.bool, device="cuda")
+            interior[bnd] = False
+            assert torch.equal(raw[interior], msk[interior])
+        del plan
+    torch.cuda.synchronize()
+    ref = ys[128]
+    assert torch.isfinite(ref).all()
+    for tile in (64, 256):
+        rel = float(torch.linalg.norm(ys[tile] - ref) / torch.linalg.norm(ref))
+        print(f"tile {tile} vs 128: rel L2 {rel:.3e}")
+        assert rel <= 1e-13
