@@ -196,3 +196,42 @@ def test_full_size_4d_workload_properties(dkt):
         rel = float(torch.linalg.norm(ys[tile] - ref) / torch.linalg.norm(ref))
         print(f"tile {tile} vs 128: rel L2 {rel:.3e}")
         assert rel <= 1e-13
+
+
+def test_full_size_3d_p2_workload_properties(dkt):
+    """The p = 2 workload of the CG benchmark at full size (3D sphere tree refined
+    to level 9: 1,332,192 elements, 8,882,437 nodes) through size-independent
+    properties: two tilings of the reference-semantics operator agree to FP64
+    rounding and annihilate constants to rounding; the conforming operator is
+    symmetric (x.Ay = y.Ax to rounding) and annihilates constants."""
+    torch = _torch()
+    tree = dkt.balance_tree(dkt.sphere_tree(3, 30, 3, 9, (0.5, 0.5, 0.5), 0.3), 1)
+    da = dkt.DA(tree, 2)
+    assert da.n_elements == 1332192 and da.n_nodes == 8882437
+    x = torch.tensor(dkt.keyed_uniform(0, da.nodes()), device="cuda")
+    xb = torch.tensor(dkt.keyed_uniform(1, da.nodes()), device="cuda")
+    one = torch.ones_like(x)
+    ys = {}
+    for tile in (128, 256):
+        plan = dkt.Plan(da, dkt.OP_STIFFNESS, tile_leaves=tile)
+        y = torch.full_like(x, float("nan"))
+        z = torch.full_like(x, float("nan"))
+        plan.matvec_torch(x, y)
+        plan.matvec_torch(one, z)
+        torch.cuda.synchronize()
+        assert float(z.abs().max()) <= 1e-12 * float(y.abs().max())
+        ys[tile] = y
+        del plan
+    rel = float(torch.linalg.norm(ys[256] - ys[128]) / torch.linalg.norm(ys[128]))
+    print(f"p=2 tiles 256 vs 128: rel L2 {rel:.3e}")
+    assert rel <= 1e-13
+    conf = dkt.Plan(da, dkt.OP_STIFFNESS, conforming=True)
+    ax, ab, a1 = (torch.full_like(x, float("nan")) for _ in range(3))
+    conf.matvec_torch(x, ax)
+    conf.matvec_torch(xb, ab)
+    conf.matvec_torch(one, a1)
+    torch.cuda.synchronize()
+    s1, s2 = float(torch.dot(xb, ax)), float(torch.dot(x, ab))
+    print(f"conforming: x.Ay {s1:.17g} y.Ax {s2:.17g}")
+    assert abs(s1 - s2) <= 1e-12 * float(torch.linalg.norm(xb) * torch.linalg.norm(ax))
+    assert float(a1.abs().max()) <= 1e-12 * float(ax.abs().max())
