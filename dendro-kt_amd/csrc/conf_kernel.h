@@ -83,18 +83,54 @@ __global__ void __launch_bounds__(kTileThreads) conf_tile_kernel(ConfArgs A, con
     for (int r = 0; r < NPE; r++) loc[r] = 0;
   }
   const int npts = th.nnode + th.nv;
-  for (int j = tid; j < th.nnode; j += nthr) {
-    const int g = A.tile_nodes[th.node0 + j];
-    const bool z = A.dir_in && A.dir_in[g];
-    for (int q = 0; q < nq; q++) x_loc[j * nq + q] = z ? T(0) : x[(size_t)g * A.ld + q];
+  // Staging in batches of kCB per work-item: every load of a batch is issued
+  // before its stores, so a batch costs one memory round trip, not kCB.
+  constexpr int kCB = 4;
+  for (int j0 = tid; j0 < th.nnode; j0 += kCB * nthr) {
+    int g[kCB];
+#pragma unroll
+    for (int b = 0; b < kCB; b++) {
+      const int j = j0 + b * nthr;
+      g[b] = j < th.nnode ? A.tile_nodes[th.node0 + j] : -1;
+    }
+    for (int q = 0; q < nq; q++) {
+      T xv[kCB];
+      bool z[kCB];
+#pragma unroll
+      for (int b = 0; b < kCB; b++) {
+        z[b] = g[b] >= 0 && A.dir_in && A.dir_in[g[b]];
+        xv[b] = g[b] >= 0 ? x[(size_t)g[b] * A.ld + q] : T(0);
+      }
+#pragma unroll
+      for (int b = 0; b < kCB; b++)
+        if (g[b] >= 0) x_loc[(j0 + b * nthr) * nq + q] = z[b] ? T(0) : xv[b];
+    }
   }
   if (nq > 1)
     for (int j = tid; j < npts * nq; j += nthr) y_loc[j] = T(0);
-  for (int k = tid; k < th.nt; k += nthr) {
-    tj_s[k] = A.term_node[th.t0 + k];
-    tv_s[k] = A.term_virt[th.t0 + k];
-    tw_s[k] = term_w[th.t0 + k];
-    tp_s[k] = A.term_perm[th.t0 + k];
+  for (int k0 = tid; k0 < th.nt; k0 += kCB * nthr) {
+    uint16_t tj[kCB], tv[kCB], tp[kCB];
+    T tw[kCB];
+#pragma unroll
+    for (int b = 0; b < kCB; b++) {
+      const int k = k0 + b * nthr;
+      if (k < th.nt) {
+        tj[b] = A.term_node[th.t0 + k];
+        tv[b] = A.term_virt[th.t0 + k];
+        tw[b] = term_w[th.t0 + k];
+        tp[b] = A.term_perm[th.t0 + k];
+      }
+    }
+#pragma unroll
+    for (int b = 0; b < kCB; b++) {
+      const int k = k0 + b * nthr;
+      if (k < th.nt) {
+        tj_s[k] = tj[b];
+        tv_s[k] = tv[b];
+        tw_s[k] = tw[b];
+        tp_s[k] = tp[b];
+      }
+    }
   }
   for (int v = tid; v < th.nv; v += nthr) vo_s[v] = A.virt_off[th.v0 + v];
   if (tid == 0) vo_s[th.nv] = (uint16_t)th.nt;
