@@ -194,14 +194,26 @@ __global__ void __launch_bounds__(kTileThreads) conf_tile_kernel(ConfArgs A, con
   }
   __syncthreads();
 
-  // 4. Write out.
-  for (int j = tid; j < th.nnode; j += nthr) {
-    const int o = A.tile_out[th.node0 + j];
-    if (o >= 0) {
-      const bool z = A.dir_out && A.dir_out[o];
-      for (int q = 0; q < nq; q++) y[(size_t)o * A.ld + q] = z ? T(0) : y_loc[j * nq + q];
-    } else {
-      for (int q = 0; q < nq; q++) partial[(size_t)(-o - 1) * nq + q] = y_loc[j * nq + q];
+  // 4. Write out, the output slots of a batch of kCB loaded together.
+  for (int j0 = tid; j0 < th.nnode; j0 += kCB * nthr) {
+    int o[kCB];
+#pragma unroll
+    for (int b = 0; b < kCB; b++) {
+      const int j = j0 + b * nthr;
+      o[b] = j < th.nnode ? A.tile_out[th.node0 + j] : INT32_MIN;
+    }
+    bool z[kCB];
+#pragma unroll
+    for (int b = 0; b < kCB; b++) z[b] = o[b] >= 0 && A.dir_out && A.dir_out[o[b]];
+#pragma unroll
+    for (int b = 0; b < kCB; b++) {
+      const int j = j0 + b * nthr;
+      if (o[b] == INT32_MIN) continue;
+      if (o[b] >= 0) {
+        for (int q = 0; q < nq; q++) y[(size_t)o[b] * A.ld + q] = z[b] ? T(0) : y_loc[j * nq + q];
+      } else {
+        for (int q = 0; q < nq; q++) partial[(size_t)(-o[b] - 1) * nq + q] = y_loc[j * nq + q];
+      }
     }
   }
 }
