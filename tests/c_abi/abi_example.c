@@ -1,5 +1,5 @@
 /* A plain C caller of the drop-in boundary (include/dkt.h), with no Python or
- * torch: builds the config-0 geometry (3D sphere tree, Hilbert order), its DA
+ * torch: builds the level-7 3D sphere tree (Hilbert order), its DA
  * and a host-side (dry-run) tiled plan, prints the counts and exits 0 on
  * success.  Built and run by tests/test_abi.py. */
 #include <stdio.h>

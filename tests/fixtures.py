@@ -23,7 +23,7 @@ def load(name):
 def fixture_input(name, z):
     if "input" in z.files:
         return z["input"]
-    if name == "config0_sphere3d":
+    if name == "sphere3d_l7":
         return gi.sphere_tree(3, 30, 3, 7, (0.5, 0.5, 0.5), 0.3)
     raise KeyError(name)
 

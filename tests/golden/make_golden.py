@@ -77,7 +77,7 @@ def cases():
                        lambda: gi.cells_at_level(gi.gauss_points(2, 300, 2, 0.1, 5, 8), 8, 10),
                        ["poisson:hash0:raw", "ident:ones:raw"], True, [], False)
     # The 3D workload: sphere r=0.3 at the centre, base level 3, max level 7.
-    c["config0_sphere3d"] = (3, 30, 1, "hilbert", "balance",
+    c["sphere3d_l7"] = (3, 30, 1, "hilbert", "balance",
                              lambda: gi.sphere_tree(3, 30, 3, 7, (0.5, 0.5, 0.5), 0.3),
                              ["heat:hash0:masked", "poisson:hash0:raw"], False, [], True)
     return c

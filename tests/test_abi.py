@@ -97,7 +97,7 @@ def test_plan_matvec_argument_checks(dkt):
 
 def test_plain_c_caller(tmp_path):
     """The boundary is usable from C alone: a C program that includes dkt.h and
-    links libdkt.so builds the config-0 geometry (82,776 elements, 55,639 nodes)
+    links libdkt.so builds the level-7 3D sphere tree (82,776 elements, 55,639 nodes)
     and a host-side plan."""
     import shutil
     import subprocess

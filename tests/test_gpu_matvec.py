@@ -105,7 +105,7 @@ def test_multi_rhs_columns_match_single(dkt, nrhs):
 
 def test_repeated_matvec_is_deterministic(dkt):
     torch = _torch()
-    da, z, meta = fx.build_da(dkt, "config0_sphere3d")
+    da, z, meta = fx.build_da(dkt, "sphere3d_l7")
     plan = dkt.Plan(da, dkt.OP_STIFFNESS)
     xd = torch.tensor(fx.make_x("hash0", da.nodes(), 3), device="cuda")
     ys = []

@@ -32,7 +32,7 @@ def test_mesh_matches_reference(dkt, name):
             assert _sha(arr) == meta["sha_" + key], key
 
 
-@pytest.mark.parametrize("name", [n for n in fx.names() if n != "config0_sphere3d"])
+@pytest.mark.parametrize("name", [n for n in fx.names() if n != "sphere3d_l7"])
 def test_traversal_matches_reference_gathers(dkt, name):
     """Traversal leaves in the reference's visiting order; every present slot equals the
     node index the reference gathers there with vecIn[i] = i (FEM/include/matvec.h:380-400)."""
