@@ -22,7 +22,8 @@ Design:
   does not depend on what else is being served ("alone == in-mix");
 * continuous batching (Server(batch=K)): each worker runs a CG engine
   (dkt_cg_engine) of K slots; a queued request joins as soon as a slot is free
-  and leaves when its solve stops, between chunks of 8 iterations, so one
+  (smallest mesh first among the requests the worker holds, which bounds the
+  median latency under load) and leaves when its solve stops, between chunks of 8 iterations, so one
   launch per step serves every running solve and no solve waits for a slower
   one.  Solves are FP64; each answer is bitwise the request's own FP64 solve.
   Every slot has its own CG workspace, so requests on one mesh run together.
@@ -139,13 +140,14 @@ class PlanCache:
 
 
 class Server:
-    def __init__(self, dkt, n_streams=4, hbm_budget_bytes=200e9, batch=0, cache=None):
+    def __init__(self, dkt, n_streams=4, hbm_budget_bytes=200e9, batch=0, cache=None, max_wait=0.5):
         import torch
         self.torch = torch
         self.dkt = dkt
         self.cache = cache if cache is not None else PlanCache(dkt, hbm_budget_bytes)
         self.q = queue.Queue()
         self.batch = batch
+        self.max_wait = max_wait  # batched mode: seconds after which a waiting request goes first
         self.done = []
         self.done_lock = threading.Lock()
         self.workers = []
@@ -216,14 +218,18 @@ class Server:
 
     def _work_batched(self, stream):
         """Continuous batching: requests join the worker's CG engine as slots free
-        up and leave when their solve stops, between chunks of 8 iterations."""
+        up, smallest mesh first among the waiting ones, and leave when their
+        solve stops, between chunks of 8 iterations."""
         torch = self.torch
         engine = self.dkt.CgEngine(self.batch, masked=True)
         running = {}  # id(req) -> (req, b, x)
+        pending = []  # (nodes, arrival, req, da, plan, mass, lock): taken from the queue, not started
         stop = False
         while True:
-            while not stop and len(running) < self.batch:
-                req = self._next(block=not running)
+            # Take queued requests (at most 2 * batch waiting here, so that the
+            # other workers get theirs), then start the smallest meshes first.
+            while not stop and len(pending) < 2 * self.batch:
+                req = self._next(block=not running and not pending)
                 if req is _EMPTY:
                     break
                 if req is None:
@@ -238,6 +244,12 @@ class Server:
                     self._finish(req)
                     continue
                 req.plan_built = req.plan_built or built
+                pending.append((da.n_nodes, req.submitted, req, da, plan, mass, lk))
+            # requests waiting longer than max_wait go first, in arrival order (no starvation)
+            now = time.perf_counter()
+            pending.sort(key=lambda t: (0, t[1], 0) if now - t[1] > self.max_wait else (1, t[0], t[1]))
+            while pending and len(running) < self.batch:
+                _, _, req, da, plan, mass, lk = pending.pop(0)
                 try:
                     with torch.cuda.stream(stream):
                         req.solve_started = time.perf_counter()
@@ -257,7 +269,7 @@ class Server:
                     req.error = f"{type(e).__name__}: {e}"
                     self._finish(req)
             if not running:
-                if stop:
+                if stop and not pending:
                     return
                 continue
             try:
