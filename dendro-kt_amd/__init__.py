@@ -29,6 +29,7 @@ OP_MASS = 1
 OP_IDENTITY = 2
 OP_CAPTURED = 3
 PLAN_SIMPLE = 1
+PLAN_DRYRUN = 2
 PLAN_CONFORMING = 4
 PLAN_TILE_SHIFT = 8
 CG_F64 = 0
@@ -36,12 +37,23 @@ CG_MIXED = 1
 CG_MIXED_RESTART = 2
 CG_MIXED_RELIABLE = 3
 BATCH_GRAPH = 1
+ERR_BREAKDOWN = 2
+STOP_NAMES = {1: "converged", 2: "max_iter", 3: "breakdown", 4: "stalled"}
 
 _lib = None
 
 
 class DktError(RuntimeError):
     pass
+
+
+class CgBreakdown(DktError):
+    """CG stopped because p.Ap <= 0 or a step was not finite (DKT_ERR_BREAKDOWN);
+    .stats holds the solve statistics, x the last iterate."""
+
+    def __init__(self, msg, stats):
+        super().__init__(msg)
+        self.stats = stats
 
 
 def lib_path():
@@ -117,6 +129,7 @@ def _bind_device(L):
                                      c.POINTER(c.c_double), P]
     L.dkt_cg_engine_create.argtypes = [c.c_int, c.c_int, c.POINTER(P)]
     L.dkt_cg_engine_add.argtypes = [P, P, P, P, c.c_int, c.c_double, c.POINTER(c.c_int)]
+    L.dkt_cg_engine_add_mode.argtypes = [P, P, c.c_int, P, P, c.c_int, c.c_double, c.POINTER(c.c_int)]
     L.dkt_cg_engine_step.argtypes = [P, c.c_int, P, c.POINTER(c.c_int), c.POINTER(c.c_double), c.POINTER(c.c_int)]
     L.dkt_cg_engine_destroy.argtypes = [P]
     L.dkt_batch_create.argtypes = [c.POINTER(P), c.c_int, c.c_int, c.c_int, c.POINTER(P), c.POINTER(P), c.c_int,
@@ -308,7 +321,8 @@ class Plan:
     masked: 0 raw, 1 HeatMat's Dirichlet hooks (x and y), 2 HeatVec's (y only).
     """
 
-    def __init__(self, da, op=OP_STIFFNESS, op_data=None, simple=False, tile_leaves=0, conforming=False):
+    def __init__(self, da, op=OP_STIFFNESS, op_data=None, simple=False, tile_leaves=0, conforming=False,
+                 dry_run=False):
         L = load()
         if not hasattr(L, "dkt_plan_create"):
             raise DktError("libdkt.so was built without the device plan")
@@ -319,7 +333,8 @@ class Plan:
         if op_data is not None:
             self._op_data = np.ascontiguousarray(op_data, dtype=np.float64)
             ptr = self._op_data.ctypes.data_as(ctypes.c_void_p)
-        flags = (PLAN_SIMPLE if simple else 0) | (PLAN_CONFORMING if conforming else 0)
+        flags = (PLAN_SIMPLE if simple else 0) | (PLAN_CONFORMING if conforming else 0) | \
+            (PLAN_DRYRUN if dry_run else 0)
         if tile_leaves:
             flags |= (int(tile_leaves).bit_length() - 1) << PLAN_TILE_SHIFT
         _chk(L.dkt_plan_create(da.h, op, ptr, flags, ctypes.byref(h)))
@@ -360,22 +375,32 @@ class Plan:
         s = stream if stream is not None else torch.cuda.current_stream().cuda_stream
         self.matvec(x.data_ptr(), y.data_ptr(), nrhs, 0 if x.dtype == torch.float64 else 1, masked, s)
 
-    def cg_solve(self, b_ptr, x_ptr, mode=CG_F64, max_iter=10000, tol=1e-8, masked=True, stream=None):
+    def cg_solve(self, b_ptr, x_ptr, mode=CG_F64, max_iter=10000, tol=1e-8, masked=True, stream=None,
+                 raise_on_breakdown=True):
         """HeatMat::cgSolve on the device (x: initial guess in, solution out).  Returns
-        dict(iters, resid, refinements, matvecs)."""
-        st = (ctypes.c_double * 4)()
-        _chk(load().dkt_cg_solve(self.h, mode, ctypes.c_void_p(b_ptr), ctypes.c_void_p(x_ptr), max_iter, tol,
-                                 1 if masked else 0, st, ctypes.c_void_p(stream or 0)))
-        return dict(iters=int(st[0]), resid=st[1], refinements=int(st[2]), matvecs=int(st[3]))
+        dict(iters, resid, refinements, matvecs, stop) with stop one of
+        "converged", "max_iter", "breakdown", "stalled".  A breakdown raises
+        CgBreakdown (carrying the statistics) unless raise_on_breakdown is False."""
+        st = (ctypes.c_double * 5)()
+        rc = load().dkt_cg_solve(self.h, mode, ctypes.c_void_p(b_ptr), ctypes.c_void_p(x_ptr), max_iter, tol,
+                                 1 if masked else 0, st, ctypes.c_void_p(stream or 0))
+        if rc not in (0, ERR_BREAKDOWN):
+            _chk(rc)
+        out = dict(iters=int(st[0]), resid=st[1], refinements=int(st[2]), matvecs=int(st[3]),
+                   stop=STOP_NAMES.get(int(st[4]), "unknown"))
+        if rc == ERR_BREAKDOWN and raise_on_breakdown:
+            raise CgBreakdown(load().dkt_last_error().decode(), out)
+        return out
 
-    def cg_solve_torch(self, b, x, mode=CG_F64, max_iter=10000, tol=1e-8, masked=True, stream=None):
+    def cg_solve_torch(self, b, x, mode=CG_F64, max_iter=10000, tol=1e-8, masked=True, stream=None,
+                       raise_on_breakdown=True):
         import torch
         if b.dtype != torch.float64 or x.dtype != torch.float64 or b.shape != (self.n_nodes,) or x.shape != b.shape:
             raise DktError("b and x must be float64 device vectors of the plan's node count")
         if not (b.is_cuda and x.is_cuda and b.is_contiguous() and x.is_contiguous()):
             raise DktError("b and x must be contiguous device tensors")
         s = stream if stream is not None else torch.cuda.current_stream().cuda_stream
-        return self.cg_solve(b.data_ptr(), x.data_ptr(), mode, max_iter, tol, masked, s)
+        return self.cg_solve(b.data_ptr(), x.data_ptr(), mode, max_iter, tol, masked, s, raise_on_breakdown)
 
     def __del__(self):
         if getattr(self, "h", None) and _lib is not None:
@@ -384,10 +409,11 @@ class Plan:
 
 
 class CgEngine:
-    """Continuous batching of FP64 CG solves (dkt_cg_engine): add() starts a
-    solve at the next step, step() runs chunks of 8 iterations of every running
-    solve and returns the ones that stopped as [(key, dict(iters, resid))].
-    Each solve is bitwise Plan.cg_solve from x = 0; x is overwritten."""
+    """Continuous batching of CG solves (dkt_cg_engine): add() starts a solve
+    (mode CG_F64 or CG_MIXED / CG_MIXED_RELIABLE) at the next step, step()
+    runs chunks of 8 iterations of every running solve and returns the ones
+    that stopped as [(key, dict(iters, resid, stop))].  Each solve is bitwise
+    Plan.cg_solve in its mode from x = 0; x is overwritten."""
 
     def __init__(self, max_slots, masked=True):
         L = load()
@@ -397,17 +423,17 @@ class CgEngine:
         self.h = h
         self.running = {}  # slot -> (key, plan, b, x): kept alive while the solve runs
         self._fin = (ctypes.c_int * max_slots)()
-        self._st = (ctypes.c_double * (2 * max_slots))()
+        self._st = (ctypes.c_double * (3 * max_slots))()
 
-    def add(self, plan, b, x, key=None, max_iter=10000, tol=1e-8):
+    def add(self, plan, b, x, key=None, max_iter=10000, tol=1e-8, mode=CG_F64):
         import torch
         if b.dtype != torch.float64 or x.dtype != torch.float64 or b.shape != (plan.n_nodes,) or x.shape != b.shape:
             raise DktError("b and x must be float64 device vectors of the plan's node count")
         if not (b.is_cuda and x.is_cuda and b.is_contiguous() and x.is_contiguous()):
             raise DktError("b and x must be contiguous device tensors")
         slot = ctypes.c_int(-1)
-        _chk(load().dkt_cg_engine_add(self.h, plan.h.value, ctypes.c_void_p(b.data_ptr()),
-                                      ctypes.c_void_p(x.data_ptr()), max_iter, tol, ctypes.byref(slot)))
+        _chk(load().dkt_cg_engine_add_mode(self.h, plan.h.value, mode, ctypes.c_void_p(b.data_ptr()),
+                                           ctypes.c_void_p(x.data_ptr()), max_iter, tol, ctypes.byref(slot)))
         self.running[slot.value] = (key, plan, b, x)
         return slot.value
 
@@ -419,7 +445,8 @@ class CgEngine:
         out = []
         for i in range(nf.value):
             key = self.running.pop(self._fin[i])[0]
-            out.append((key, dict(iters=int(self._st[2 * i]), resid=self._st[2 * i + 1])))
+            out.append((key, dict(iters=int(self._st[3 * i]), resid=self._st[3 * i + 1],
+                                  stop=STOP_NAMES.get(int(self._st[3 * i + 2]), "unknown"))))
         return out
 
     def __len__(self):
@@ -448,10 +475,11 @@ def cg_solve_batch_torch(plans, bs, xs, max_iter=10000, tol=1e-8, masked=True, s
     hp = (V * n)(*[pl.h.value for pl in plans])
     hb = (V * n)(*[b.data_ptr() for b in bs])
     hx = (V * n)(*[x.data_ptr() for x in xs])
-    st = (ctypes.c_double * (2 * n))()
+    st = (ctypes.c_double * (3 * n))()
     s = stream if stream is not None else torch.cuda.current_stream().cuda_stream
     _chk(load().dkt_cg_solve_batch(hp, n, hb, hx, max_iter, tol, 1 if masked else 0, st, V(s or 0)))
-    return [dict(iters=int(st[2 * k]), resid=st[2 * k + 1]) for k in range(n)]
+    return [dict(iters=int(st[3 * k]), resid=st[3 * k + 1], stop=STOP_NAMES.get(int(st[3 * k + 2]), "unknown"))
+            for k in range(n)]
 
 
 class Batch:

@@ -45,7 +45,9 @@ constexpr double kInnerTol = 1e-4;
 constexpr int kMaxOuter = 64;
 constexpr double kReliableDrop = 0.1;
 // Scalar slots.
-enum { S_RR0 = 0, S_RR1 = 1, S_NORMB = 2, S_RESID = 3, S_SCALE = 4, S_BETA = 5, S_COUNT = 8 };
+enum { S_RR0 = 0, S_RR1 = 1, S_NORMB = 2, S_RESID = 3, S_SCALE = 4, S_BETA = 5, S_PAR = 6, S_TOLIN = 7, S_COUNT = 8 };
+// Stop reasons in flag[0] (0: running), as reported in the solve statistics.
+constexpr int kStopConverged = 1, kStopMaxIter = 2, kStopBreakdown = 3, kStopStalled = 4;
 
 template <typename T>
 __device__ __forceinline__ T block_sum(T v, T *sh) {
@@ -169,10 +171,19 @@ template <typename T>
 __device__ __forceinline__ void update_xr_body(int n, T *__restrict__ x, T *__restrict__ r, const T *__restrict__ p,
                                                const T *__restrict__ ap, const double *__restrict__ part_pap,
                                                const double *__restrict__ scal, int par, double *__restrict__ part,
-                                               const int *__restrict__ flag) {
+                                               int *__restrict__ flag) {
   if (flag[0]) return;
   __shared__ double sh[kCgThreads];
-  const T alpha = (T)(scal[S_RR0 + par] / parts_sum(part_pap, sh));
+  // Breakdown guard: p.Ap <= 0 (the operator is not positive definite on p)
+  // or a non-finite step stops the solve before x or r change.  Every block
+  // reduces the same partials in the same order, so all of them agree.
+  const double pap = parts_sum(part_pap, sh);
+  const double a64 = scal[S_RR0 + par] / pap;
+  if (!(pap > 0.0) || !isfinite(a64)) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) flag[0] = kStopBreakdown;
+    return;
+  }
+  const T alpha = (T)a64;
   double rr = 0, mr = 0;
   CG_LOOP(i, n) {
     x[i] += alpha * p[i];
@@ -193,7 +204,7 @@ __global__ void __launch_bounds__(kCgThreads) k_update_xr(int n, T *__restrict__
                                                           const T *__restrict__ p, const T *__restrict__ ap,
                                                           const double *__restrict__ part_pap,
                                                           const double *__restrict__ scal, int par,
-                                                          double *__restrict__ part, const int *__restrict__ flag) {
+                                                          double *__restrict__ part, int *__restrict__ flag) {
   update_xr_body(n, x, r, p, ap, part_pap, scal, par, part, flag);
 }
 
@@ -336,10 +347,11 @@ struct CgVectors {
 };
 
 // Runs up to max_iter CG iterations on A x = b in precision T from the current x.
-// Returns the iterations performed; the final relative residual is scal[S_RESID].
+// Returns the iterations performed; the final relative residual is scal[S_RESID]
+// and *stop the stop reason (kStop*).
 template <typename T>
 int cg_run(dkt_plan *pl, int dtype, const T *b, T *x, bool x_is_zero, int max_iter, double tol, int masked,
-           const CgVectors<T> &v, int *flag, int *host, hipStream_t st, int *matvecs) {
+           const CgVectors<T> &v, int *flag, int *host, hipStream_t st, int *matvecs, int *stop) {
   const int n = (int)pl->N;
   const dim3 grid(cg_blocks(n)), blk(kCgThreads);
   if (!x_is_zero) {
@@ -352,7 +364,10 @@ int cg_run(dkt_plan *pl, int dtype, const T *b, T *x, bool x_is_zero, int max_it
   int it = 0;
   HIP_OK(hipMemcpyAsync(host, flag, 2 * sizeof(int), hipMemcpyDeviceToHost, st));
   HIP_OK(hipStreamSynchronize(st));
-  if (host[0]) return 0;
+  if (host[0]) {
+    *stop = kStopConverged;
+    return 0;
+  }
   while (it < max_iter) {
     const int chunk = std::min(kCgChunk, max_iter - it);
     for (int c = 0; c < chunk; c++) {
@@ -369,6 +384,7 @@ int cg_run(dkt_plan *pl, int dtype, const T *b, T *x, bool x_is_zero, int max_it
     HIP_OK(hipStreamSynchronize(st));
     if (host[0]) break;
   }
+  *stop = host[0] ? host[0] : kStopMaxIter;
   *matvecs += host[1];
   return host[1];
 }
@@ -414,7 +430,7 @@ void ensure_workspace(dkt_plan *pl, bool mixed) {
 // Mixed-precision CG with reliable updates (see the file comment).  Returns
 // the FP32 iterations; *resid is the final true relative residual.
 int cg_reliable(dkt_plan *pl, const double *b, double *x, int max_iter, double tol, int masked, hipStream_t st,
-                int *matvecs, int *updates, double *resid) {
+                int *matvecs, int *updates, double *resid, int *stop) {
   CgWorkspace &w = pl->cg;
   const int n = (int)pl->N;
   const dim3 grid(cg_blocks(n)), blk(kCgThreads);
@@ -430,13 +446,14 @@ int cg_reliable(dkt_plan *pl, const double *b, double *x, int max_iter, double t
     return read_scalar(w.scal.p, S_RESID, st);
   };
   *resid = true_residual();
+  *stop = *resid <= tol ? kStopConverged : (std::isfinite(*resid) ? kStopMaxIter : kStopBreakdown);
   if (*resid <= tol || max_iter == 0 || !std::isfinite(*resid)) return 0;
   // Start: r32 = p32 = r, x32 = 0 (a restart; the r.r goes to slot 0, flags cleared).
   hipLaunchKernelGGL(k_rel32, grid, blk, 0, st, n, w.d64.p, v.r, v.p, w.x32.p, 2, v.part);
   hipLaunchKernelGGL(k_rel32_fin, dim3(1), blk, 0, st, v.part, v.scal, w.scal.p, 1, 2, flag);
   HIP_OK(hipGetLastError());
   double last = *resid;
-  int it = 0, stalls = 0;
+  int it = 0, stalls = 0, it_restart = 0;
   double tol_in = std::max(tol, kReliableDrop * last);
   while (it < max_iter) {
     const int chunk = std::min(kCgChunk, max_iter - it);
@@ -452,16 +469,33 @@ int cg_reliable(dkt_plan *pl, const double *b, double *x, int max_iter, double t
     HIP_OK(hipMemcpyAsync(host, flag, 2 * sizeof(int), hipMemcpyDeviceToHost, st));
     HIP_OK(hipStreamSynchronize(st));
     it = host[1];  // iterations completed (the chunk's launches after a stop return at once)
+    const bool broke = host[0] == kStopBreakdown;
     if (!host[0] && it < max_iter) continue;
     // Reliable update: fold the FP32 update into x, replace the residual.
     hipLaunchKernelGGL(k_add32u, grid, blk, 0, st, n, x, w.x32.p);
     *resid = true_residual();
     (*updates)++;
-    if (*resid <= tol || it >= max_iter || !std::isfinite(*resid)) break;
-    // No progress since the last replacement: restart the search direction.
+    if (*resid <= tol) {
+      *stop = kStopConverged;
+      break;
+    }
+    if (!std::isfinite(*resid) || (broke && it == it_restart)) {
+      *stop = kStopBreakdown;  // non-finite, or p.Ap <= 0 right after a restart
+      break;
+    }
+    if (it >= max_iter) {
+      *stop = kStopMaxIter;
+      break;
+    }
+    // No progress since the last replacement, or a breakdown of the FP32
+    // iteration: restart the search direction.
     stalls = *resid < 0.9 * last ? 0 : stalls + 1;
-    const int restart = stalls > 0 ? 1 : 0;
-    if (stalls > 8) break;
+    const int restart = (stalls > 0 || broke) ? 1 : 0;
+    if (stalls > 8) {
+      *stop = kStopStalled;
+      break;
+    }
+    if (restart) it_restart = it;
     const int par = (it - 1) & 1;  // parity of the iteration that stopped
     hipLaunchKernelGGL(k_rel32, grid, blk, 0, st, n, w.d64.p, v.r, v.p, w.x32.p, restart, v.part);
     hipLaunchKernelGGL(k_rel32_fin, dim3(1), blk, 0, st, v.part, v.scal, w.scal.p, par, restart, flag);
@@ -474,23 +508,28 @@ int cg_reliable(dkt_plan *pl, const double *b, double *x, int max_iter, double t
   return it;
 }
 
-// One solve of a batch (dkt_cg_engine, dkt_cg_solve_batch): its vectors,
-// workspace and stop rule.
+// One solve of a batch (dkt_cg_engine, dkt_cg_solve_batch): its iteration
+// vectors (FP64, or FP32 for the iteration of a mixed-precision solve),
+// workspace and stop rule.  tolp non-null: the iteration stops at the
+// relative residual *tolp (a mixed solve's inner target) instead of tol.
+template <typename T>
 struct CgDesc {
   int n, max_iter;
   double tol;
-  const double *b;
-  double *x, *r, *p, *ap, *part, *scal;
+  const double *tolp;
+  const T *b;
+  T *x, *r, *p, *ap;
+  double *part, *scal;
   int *flag;
 };
 
-// Batched FP64 iteration kernels: solve blockIdx.y, block blockIdx.x of its
+// Batched iteration kernels: solve blockIdx.y, block blockIdx.x of its
 // kCgBlocks, through the single-solve bodies.
 // Blocks at or past cg_blocks(n) of a solve return at once (the grid spans the
 // largest solve).  kb_init's block 0 zeroes the partial slots past them: an
 // engine slot's workspace is reused by solves of other sizes.
-__global__ void __launch_bounds__(kCgThreads) kb_init(const CgDesc *__restrict__ d) {
-  const CgDesc e = d[blockIdx.y];
+__global__ void __launch_bounds__(kCgThreads) kb_init(const CgDesc<double> *__restrict__ d) {
+  const CgDesc<double> e = d[blockIdx.y];
   const int nb = cg_blocks(e.n);
   if (blockIdx.x >= nb) return;
   if (blockIdx.x == 0)
@@ -500,38 +539,196 @@ __global__ void __launch_bounds__(kCgThreads) kb_init(const CgDesc *__restrict__
   init_body<double>(e.n, e.b, nullptr, e.r, e.p, e.part);
 }
 // As k_init_fin; a solve with max_iter = 0 stops at once (flag[0] = 2).
-__global__ void __launch_bounds__(kCgThreads) kb_init_fin(const CgDesc *__restrict__ d) {
-  const CgDesc e = d[blockIdx.y];
+__global__ void __launch_bounds__(kCgThreads) kb_init_fin(const CgDesc<double> *__restrict__ d) {
+  const CgDesc<double> e = d[blockIdx.y];
   init_fin_body(e.part, e.scal, e.tol, e.flag);
-  if (threadIdx.x == 0 && e.max_iter <= 0 && !e.flag[0]) e.flag[0] = 2;
+  if (threadIdx.x == 0 && e.max_iter <= 0 && !e.flag[0]) e.flag[0] = kStopMaxIter;
 }
-__global__ void __launch_bounds__(kCgThreads) kb_dot(const CgDesc *__restrict__ d) {
-  const CgDesc e = d[blockIdx.y];
+template <typename T>
+__global__ void __launch_bounds__(kCgThreads) kb_dot(const CgDesc<T> *__restrict__ d) {
+  const CgDesc<T> e = d[blockIdx.y];
   if (blockIdx.x >= cg_blocks(e.n)) return;
-  dot_body<double>(e.n, e.p, e.ap, e.part + 2 * kCgBlocks, e.flag);
+  dot_body<T>(e.n, e.p, e.ap, e.part + 2 * kCgBlocks, e.flag);
 }
-__global__ void __launch_bounds__(kCgThreads) kb_update_xr(const CgDesc *__restrict__ d, int par) {
-  const CgDesc e = d[blockIdx.y];
+template <typename T>
+__global__ void __launch_bounds__(kCgThreads) kb_update_xr(const CgDesc<T> *__restrict__ d, int par) {
+  const CgDesc<T> e = d[blockIdx.y];
   if (blockIdx.x >= cg_blocks(e.n)) return;
-  update_xr_body<double>(e.n, e.x, e.r, e.p, e.ap, e.part + 2 * kCgBlocks, e.scal, par, e.part, e.flag);
+  update_xr_body<T>(e.n, e.x, e.r, e.p, e.ap, e.part + 2 * kCgBlocks, e.scal, par, e.part, e.flag);
 }
 // As k_update_p; after the solve's max_iter-th iteration it stops (flag[0] =
 // 2), as the single solve's host loop does.  Blocks that start after block 0
-// has raised the flag skip the update of p, which is no longer used.
-__global__ void __launch_bounds__(kCgThreads) kb_update_p(const CgDesc *__restrict__ d, int par) {
-  const CgDesc e = d[blockIdx.y];
+// has raised the flag skip the update of p, which is no longer used.  The
+// parity of the iteration that stopped goes to scal[S_PAR] (a mixed solve's
+// residual replacement continues from it).
+template <typename T>
+__global__ void __launch_bounds__(kCgThreads) kb_update_p(const CgDesc<T> *__restrict__ d, int par) {
+  const CgDesc<T> e = d[blockIdx.y];
   if (blockIdx.x >= cg_blocks(e.n) || e.flag[0]) return;
-  update_p_body<double>(e.n, e.p, e.r, e.part, e.scal, par, e.tol, e.flag);
-  if (blockIdx.x == 0 && threadIdx.x == 0 && !e.flag[0] && e.flag[1] >= e.max_iter) e.flag[0] = 2;
+  update_p_body<T>(e.n, e.p, e.r, e.part, e.scal, par, e.tolp ? *e.tolp : e.tol, e.flag);
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    if (!e.flag[0] && e.flag[1] >= e.max_iter) e.flag[0] = kStopMaxIter;
+    if (e.flag[0]) e.scal[S_PAR] = par;
+  }
 }
-// out[3k .. 3k + 2] = stop flag (0 running, 1 converged, 2 max_iter),
-// iterations, relative residual of solve k.
-__global__ void __launch_bounds__(kCgThreads) kb_status(const CgDesc *__restrict__ d, int nsolves,
+
+// Residual replacement of mixed-precision solves in an engine (the device
+// side of cg_reliable's host loop, for every slot at once).  upd: 0 nothing
+// to do this step, 1 the FP32 iteration stopped (inner target, max_iter or
+// breakdown): fold x32 into x and replace the residual, 2 the solve starts
+// (x = 0, r = b).  st: [0] lowest true residual so far, [1] stalls,
+// [2] FP32 iterations at the last restart, [3] restart (1) or keep the
+// search direction (0).  A stopped solve's flag[0] becomes kFinal + reason.
+constexpr int kFinal = 16;
+struct RelDesc {
+  int n, max_iter;
+  double tol;
+  const double *b;
+  double *x, *r64, *ap64, *part64, *scal64;
+  float *x32, *r32, *p32;
+  double *part32, *scal32;
+  double *st;
+  int *flag;
+  const int *upd;
+};
+
+// x += x32, x32 = 0 (start: x = 0, x32 = 0).
+__global__ void __launch_bounds__(kCgThreads) kr_fold(const RelDesc *__restrict__ d) {
+  const RelDesc e = d[blockIdx.y];
+  const int u = *e.upd;
+  if (!u || blockIdx.x >= cg_blocks(e.n)) return;
+  CG_LOOP(i, e.n) {
+    e.x[i] = u == 2 ? 0.0 : e.x[i] + (double)e.x32[i];
+    e.x32[i] = 0.0f;
+  }
+}
+// r64 = b - A x (start: r64 = b); partials of max|r|, max|b|.  At the start,
+// block 0 zeroes the partial slots past cg_blocks(n) of both workspaces.
+__global__ void __launch_bounds__(kCgThreads) kr_resid(const RelDesc *__restrict__ d) {
+  const RelDesc e = d[blockIdx.y];
+  const int u = *e.upd;
+  const int nb = cg_blocks(e.n);
+  if (!u || blockIdx.x >= nb) return;
+  if (u == 2 && blockIdx.x == 0)
+    for (int j = threadIdx.x; j < 4 * kCgBlocks; j += kCgThreads)
+      if (j % kCgBlocks >= nb) {
+        e.part64[j] = 0.0;
+        e.part32[j] = 0.0;
+      }
+  __shared__ double sh[kCgThreads];
+  double mr = 0, mb = 0;
+  CG_LOOP(i, e.n) {
+    const double ri = u == 2 ? e.b[i] : e.b[i] - e.ap64[i];
+    e.r64[i] = ri;
+    mr = fmax(mr, fabs(ri));
+    mb = fmax(mb, fabs(e.b[i]));
+  }
+  mr = block_max(mr, sh);
+  mb = block_max(mb, sh);
+  if (threadIdx.x == 0) {
+    e.part64[blockIdx.x] = mr;
+    e.part64[kCgBlocks + blockIdx.x] = mb;
+  }
+}
+// The stop decision of cg_reliable, per solve.
+__global__ void __launch_bounds__(kCgThreads) kr_decide(const RelDesc *__restrict__ d) {
+  const RelDesc e = d[blockIdx.y];
+  const int u = *e.upd;
+  if (!u) return;
+  __shared__ double sh[kCgThreads];
+  const double mr = parts_max(e.part64, sh);
+  double mb = parts_max(e.part64 + kCgBlocks, sh);
+  if (threadIdx.x != 0) return;
+  if (mb == 0.0) mb = 1.0;
+  const double resid = mr / mb;
+  e.scal64[S_NORMB] = mb;
+  e.scal64[S_RESID] = resid;
+  double *st = e.st;
+  if (u == 2) {  // start
+    e.flag[1] = 0;
+    st[0] = resid;
+    st[1] = 0;
+    st[2] = 0;
+    st[3] = 1;
+    if (resid <= e.tol) e.flag[0] = kFinal + kStopConverged;
+    else if (!isfinite(resid)) e.flag[0] = kFinal + kStopBreakdown;
+    else if (e.max_iter <= 0) e.flag[0] = kFinal + kStopMaxIter;
+    else e.flag[0] = 0;
+    e.scal32[S_TOLIN] = fmax(e.tol, kReliableDrop * resid);
+    return;
+  }
+  const int it = e.flag[1];
+  const bool broke = e.flag[0] == kStopBreakdown;
+  if (resid <= e.tol) {
+    e.flag[0] = kFinal + kStopConverged;
+  } else if (!isfinite(resid) || (broke && it == (int)st[2])) {
+    e.flag[0] = kFinal + kStopBreakdown;
+  } else if (it >= e.max_iter) {
+    e.flag[0] = kFinal + kStopMaxIter;
+  } else {
+    const double last = st[0];
+    st[1] = resid < 0.9 * last ? 0.0 : st[1] + 1.0;
+    const int restart = (st[1] > 0 || broke) ? 1 : 0;
+    if (st[1] > 8) {
+      e.flag[0] = kFinal + kStopStalled;
+    } else {
+      if (restart) st[2] = it;
+      st[3] = restart;
+      st[0] = fmin(last, resid);
+      e.scal32[S_TOLIN] = fmax(e.tol, kReliableDrop * resid);
+      e.flag[0] = 0;
+    }
+  }
+}
+// r32 = r64 (restart: p32 = r32 too); partials of r32.r32.
+__global__ void __launch_bounds__(kCgThreads) kr_rel32(const RelDesc *__restrict__ d) {
+  const RelDesc e = d[blockIdx.y];
+  if (!*e.upd || e.flag[0] || blockIdx.x >= cg_blocks(e.n)) return;
+  const bool restart = e.st[3] != 0.0;
+  __shared__ double sh[kCgThreads];
+  double rr = 0;
+  CG_LOOP(i, e.n) {
+    const float ri = (float)e.r64[i];
+    e.r32[i] = ri;
+    if (restart) e.p32[i] = ri;
+    rr += (double)ri * (double)ri;
+  }
+  rr = block_sum(rr, sh);
+  if (threadIdx.x == 0) e.part32[blockIdx.x] = rr;
+}
+// The new r.r goes to parity slot 0 (every engine step starts at parity 0);
+// beta = new r.r / r.r before the iteration that stopped (0 on a restart).
+__global__ void __launch_bounds__(kCgThreads) kr_rel32_fin(const RelDesc *__restrict__ d) {
+  const RelDesc e = d[blockIdx.y];
+  if (!*e.upd || e.flag[0]) return;
+  __shared__ double sh[kCgThreads];
+  const double rr = parts_sum(e.part32, sh);
+  if (threadIdx.x == 0) {
+    const int par = *e.upd == 2 ? 0 : (int)e.scal32[S_PAR];
+    e.scal32[S_BETA] = e.st[3] != 0.0 ? 0.0 : rr / e.scal32[S_RR0 + par];
+    e.scal32[S_RR0] = rr;
+    e.scal32[S_NORMB] = e.scal64[S_NORMB];
+  }
+}
+// p32 = r32 + beta p32 (residual replaced without a restart).
+__global__ void __launch_bounds__(kCgThreads) kr_pupd32(const RelDesc *__restrict__ d) {
+  const RelDesc e = d[blockIdx.y];
+  if (!*e.upd || e.flag[0] || e.st[3] != 0.0 || blockIdx.x >= cg_blocks(e.n)) return;
+  const float beta = (float)e.scal32[S_BETA];
+  CG_LOOP(i, e.n) e.p32[i] = e.r32[i] + beta * e.p32[i];
+}
+
+// out[3k .. 3k + 2] = flag[0], iterations, relative residual of solve k.
+struct StatDesc {
+  const int *flag;
+  const double *resid;
+};
+__global__ void __launch_bounds__(kCgThreads) kb_status(const StatDesc *__restrict__ d, int nsolves,
                                                         double *__restrict__ out) {
   for (int k = threadIdx.x; k < nsolves; k += blockDim.x) {
     out[3 * k] = d[k].flag[0];
     out[3 * k + 1] = d[k].flag[1];
-    out[3 * k + 2] = d[k].scal[S_RESID];
+    out[3 * k + 2] = *d[k].resid;
   }
 }
 
@@ -542,7 +739,8 @@ using namespace dkt;
 
 extern "C" int dkt_cg_solve(dkt_plan *pl, int mode, const double *b, double *x, int max_iter, double tol, int masked,
                             double *stats, void *stream) {
-  return guarded([&] {
+  int stop = 0;
+  const int rc = guarded([&] {
     if (!pl || !b || !x) throw std::invalid_argument("null pointer");
     if (pl->dry) throw std::invalid_argument("a dry-run plan cannot solve");
     if (max_iter < 0 || !(tol >= 0.0)) throw std::invalid_argument("max_iter >= 0 and tol >= 0 required");
@@ -560,10 +758,11 @@ extern "C" int dkt_cg_solve(dkt_plan *pl, int mode, const double *b, double *x, 
     double resid = 0.0;
     if (!mixed) {
       CgVectors<double> v{w.r.p, w.p.p, w.ap.p, w.part64.p, w.scal.p};
-      iters = cg_run<double>(pl, 0, b, x, false, max_iter, tol, masked ? 1 : 0, v, w.flag.p, w.host, st, &matvecs);
+      iters = cg_run<double>(pl, 0, b, x, false, max_iter, tol, masked ? 1 : 0, v, w.flag.p, w.host, st, &matvecs,
+                             &stop);
       resid = read_scalar(w.scal.p, S_RESID, st);
     } else if (mode == DKT_CG_MIXED_RELIABLE) {
-      iters = cg_reliable(pl, b, x, max_iter, tol, masked ? 1 : 0, st, &matvecs, &outer, &resid);
+      iters = cg_reliable(pl, b, x, max_iter, tol, masked ? 1 : 0, st, &matvecs, &outer, &resid, &stop);
     } else {
       const int n = (int)pl->N;
       CgVectors<float> v{w.r32.p, w.p32.p, w.ap32.p, w.part32.p, w.scal32.p};  // FP64 reductions
@@ -574,17 +773,32 @@ extern "C" int dkt_cg_solve(dkt_plan *pl, int mode, const double *b, double *x, 
         hipLaunchKernelGGL(k_resid64_fin, dim3(1), dim3(kCgThreads), 0, st, w.part64.p, w.scal.p);
         HIP_OK(hipGetLastError());
         resid = read_scalar(w.scal.p, S_RESID, st);
-        if (resid <= tol || iters >= max_iter || !std::isfinite(resid)) break;
+        if (resid <= tol) {
+          stop = kStopConverged;
+          break;
+        }
+        if (!std::isfinite(resid)) {
+          stop = kStopBreakdown;
+          break;
+        }
+        if (iters >= max_iter) {
+          stop = kStopMaxIter;
+          break;
+        }
         hipLaunchKernelGGL(k_to32, dim3(cg_blocks(n)), dim3(kCgThreads), 0, st, n, w.d64.p, w.scal.p, w.b32.p, w.x32.p);
         // Inner tolerance: kInnerTol, loosened when the outer target is already close.
         const double inner_tol = std::min(0.5, std::max(kInnerTol, 0.5 * tol / resid));
+        int inner_stop = 0;
         const int inner = cg_run<float>(pl, 1, w.b32.p, w.x32.p, true, max_iter - iters, inner_tol, masked ? 1 : 0, v,
-                                        w.flag.p, w.host, st, &matvecs);
+                                        w.flag.p, w.host, st, &matvecs, &inner_stop);
         hipLaunchKernelGGL(k_add32, dim3(cg_blocks(n)), dim3(kCgThreads), 0, st, n, x, w.x32.p, w.scal.p);
         HIP_OK(hipGetLastError());
         iters += inner;
         outer++;
-        if (inner == 0 || outer >= kMaxOuter) break;  // no progress possible: report the last residual
+        if (inner == 0 || outer >= kMaxOuter) {  // no progress possible: report the last residual
+          stop = inner == 0 && inner_stop == kStopBreakdown ? kStopBreakdown : kStopStalled;
+          break;
+        }
       }
     }
     if (stats) {
@@ -592,37 +806,62 @@ extern "C" int dkt_cg_solve(dkt_plan *pl, int mode, const double *b, double *x, 
       stats[1] = resid;
       stats[2] = outer;
       stats[3] = matvecs;
+      stats[4] = stop;
     }
   });
+  if (rc == 0 && stop == kStopBreakdown) {
+    g_last_error = "CG breakdown: p.Ap <= 0 or a non-finite step (the operator is not positive definite on the "
+                   "search direction); x holds the last iterate";
+    return 2;
+  }
+  return rc;
 }
 
-// Continuous batching of FP64 CG solves.  Solves join (dkt_cg_engine_add)
-// and leave (finished in dkt_cg_engine_step) between chunks of kCgChunk
-// iterations; every chunk runs one matvec launch per (dim, p) of the active
-// plans and one launch per vector step for all active solves.  A solve starts
-// at the first step after it was added, so it always runs an even number of
-// iterations per chunk and its own iteration parity: its arithmetic is that of
-// its own dkt_cg_solve from x = 0.
+// Continuous batching of CG solves.  Solves join (dkt_cg_engine_add) and
+// leave (finished in dkt_cg_engine_step) between chunks of kCgChunk
+// iterations; every chunk runs one matvec launch per (kind, dim, p) and
+// precision of the active plans and one launch per vector step for all
+// active solves of a precision.  A solve starts at the first step after it
+// was added, so every step starts at iteration parity 0 and each solve's
+// arithmetic is that of its own dkt_cg_solve from x = 0:
+//   FP64 solves (DKT_CG_F64) run the FP64 iteration;
+//   mixed solves (DKT_CG_MIXED_RELIABLE) run the FP32 iteration and, at the
+//   start of the step after their FP32 residual reached its target (or
+//   max_iter, or broke down), the FP64 residual replacement of cg_reliable
+//   (kr_* kernels) for all such solves at once.
 struct dkt_cg_engine {
   // A slot owns its CG workspace (kept and grown across solves), so several
   // solves may run on one plan at once.
   struct Slot {
     dkt_plan *pl = nullptr;
-    CgDesc d{};
+    int mode = DKT_CG_F64;
+    CgDesc<double> d{};
+    CgDesc<float> d32{};
+    RelDesc rel{};
     bool active = false, fresh = false;
-    size_t cap = 0;
-    DevBuf<double> r, p, ap, part, scal;
+    int upd = 0;  // mixed: residual replacement due at the next step (1), start (2)
+    size_t cap = 0, cap32 = 0;
+    DevBuf<double> r, p, ap, part, scal, st;
+    DevBuf<float> r32, p32, ap32, x32;
+    DevBuf<double> part32, scal32;
     DevBuf<int> flag;
   };
   int masked = 0;
   std::deque<Slot> slots;  // fixed size after creation; Slot is not movable
-  bool dirty = true;                       // active set changed: rebuild the launches
-  std::vector<int> act;                    // active slots in descriptor order
-  std::unique_ptr<MatvecLauncher> mv;
-  DevBuf<CgDesc> dact, dnew;
+  bool dirty = true;       // active set changed: rebuild the launches
+  std::vector<int> act64, act32;  // active slots of each precision, in descriptor order
+  std::unique_ptr<MatvecLauncher> mv64, mv32, mvu;
+  DevBuf<CgDesc<double>> d64, dnew;
+  DevBuf<CgDesc<float>> d32;
+  DevBuf<RelDesc> drel;
+  DevBuf<StatDesc> dstat;
+  DevBuf<int> dupd;  // [slot] upd, [max_slots + slot] 0 when the slot's FP64 matvec runs this step
   DevBuf<double> status;
-  std::vector<CgDesc> hact, hnew;
+  int *hupd = nullptr;  // pinned host staging of dupd
   std::vector<double> hs;
+  ~dkt_cg_engine() {
+    if (hupd) (void)hipHostFree(hupd);
+  }
 };
 
 extern "C" int dkt_cg_engine_create(int max_slots, int masked, dkt_cg_engine **out) {
@@ -635,33 +874,67 @@ extern "C" int dkt_cg_engine_create(int max_slots, int masked, dkt_cg_engine **o
   });
 }
 
-extern "C" int dkt_cg_engine_add(dkt_cg_engine *e, dkt_plan *pl, const double *b, double *x, int max_iter, double tol,
-                                 int *slot) {
+extern "C" int dkt_cg_engine_add_mode(dkt_cg_engine *e, dkt_plan *pl, int mode, const double *b, double *x,
+                                      int max_iter, double tol, int *slot) {
   return guarded([&] {
     if (!e || !pl || !b || !x || !slot) throw std::invalid_argument("null pointer");
     if (pl->dry) throw std::invalid_argument("a dry-run plan cannot solve");
-    if (pl->simple || pl->per_level || pl->conforming)
-      throw std::invalid_argument("batched solves need tiled device plans");
+    if (pl->simple || pl->per_level) throw std::invalid_argument("batched solves need tiled device plans");
     if (max_iter < 0 || !(tol >= 0.0)) throw std::invalid_argument("max_iter >= 0 and tol >= 0 required");
     if ((const void *)b == (const void *)x) throw std::invalid_argument("b and x must not alias");
+    if (mode == DKT_CG_MIXED) mode = DKT_CG_MIXED_RELIABLE;
+    if (mode != DKT_CG_F64 && mode != DKT_CG_MIXED_RELIABLE)
+      throw std::invalid_argument("batched solves run DKT_CG_F64 or DKT_CG_MIXED(_RELIABLE)");
     int free_slot = -1;
     for (int k = 0; k < (int)e->slots.size() && free_slot < 0; k++)
       if (!e->slots[k].active) free_slot = k;
     if (free_slot < 0) throw std::invalid_argument("no free slot");
     auto &s = e->slots[free_slot];
-    if (s.cap < pl->N) {
-      s.r.alloc(pl->N);
-      s.p.alloc(pl->N);
-      s.ap.alloc(pl->N);
-      s.cap = pl->N;
+    const size_t n = pl->N;
+    if (!e->hupd) {
+      const size_t S = e->slots.size();
+      HIP_OK(hipHostMalloc((void **)&e->hupd, 2 * S * sizeof(int), hipHostMallocDefault));
+      e->dupd.alloc(2 * S);
+    }
+    if (s.cap < n) {
+      s.r.alloc(n);
+      s.p.alloc(n);
+      s.ap.alloc(n);
+      s.cap = n;
     }
     if (!s.part.p) {
       s.part.alloc(4 * kCgBlocks);
       s.scal.alloc(S_COUNT);
       s.flag.alloc(4);
+      s.st.alloc(4);
+    }
+    if (mode == DKT_CG_MIXED_RELIABLE) {
+      if (s.cap32 < n) {
+        s.r32.alloc(n);
+        s.p32.alloc(n);
+        s.ap32.alloc(n);
+        s.x32.alloc(n);
+        s.cap32 = n;
+      }
+      if (!s.part32.p) {
+        s.part32.alloc(4 * kCgBlocks);
+        s.scal32.alloc(S_COUNT);
+      }
     }
     s.pl = pl;
-    s.d = CgDesc{(int)pl->N, max_iter, tol, b, x, s.r.p, s.p.p, s.ap.p, s.part.p, s.scal.p, s.flag.p};
+    s.mode = mode;
+    const int N = (int)n;
+    s.d = CgDesc<double>{N, max_iter, tol, nullptr, b, x, s.r.p, s.p.p, s.ap.p, s.part.p, s.scal.p, s.flag.p};
+    if (mode == DKT_CG_MIXED_RELIABLE) {
+      // FP64 residual path: r -> r64, ap -> ap64, part/scal -> the FP64 reductions.
+      s.d32 = CgDesc<float>{N, max_iter, tol, s.scal32.p + S_TOLIN, nullptr, s.x32.p, s.r32.p, s.p32.p, s.ap32.p,
+                            s.part32.p, s.scal32.p, s.flag.p};
+      s.rel = RelDesc{N, max_iter, tol, b, x, s.r.p, s.ap.p, s.part.p, s.scal.p, s.x32.p, s.r32.p, s.p32.p,
+                      s.part32.p, s.scal32.p, s.st.p, s.flag.p, nullptr};
+      s.upd = 2;
+    } else {
+      s.upd = 0;
+    }
     s.active = true;
     s.fresh = true;
     e->dirty = true;
@@ -669,81 +942,157 @@ extern "C" int dkt_cg_engine_add(dkt_cg_engine *e, dkt_plan *pl, const double *b
   });
 }
 
-// Starts the solves added since the last step, runs `chunks` chunks of
-// iterations over the active solves and reports those that stopped:
-// finished[i] = slot, stats[2i] = iterations, stats[2i + 1] = relative
-// residual, *nfinished = their count (arrays of at least max_slots entries).
-// Their slots are free again.
+extern "C" int dkt_cg_engine_add(dkt_cg_engine *e, dkt_plan *pl, const double *b, double *x, int max_iter, double tol,
+                                 int *slot) {
+  return dkt_cg_engine_add_mode(e, pl, DKT_CG_F64, b, x, max_iter, tol, slot);
+}
+
+// Starts the solves added since the last step, runs the due residual
+// replacements and `chunks` chunks of iterations over the active solves, and
+// reports those that stopped: finished[i] = slot, stats[3i] = iterations,
+// stats[3i + 1] = relative residual, stats[3i + 2] = stop reason,
+// *nfinished = their count.  Their slots are free again.
 extern "C" int dkt_cg_engine_step(dkt_cg_engine *e, int chunks, void *stream, int *finished, double *stats,
                                   int *nfinished) {
   return guarded([&] {
     if (!e || !finished || !nfinished || chunks < 0) throw std::invalid_argument("null pointer or chunks < 0");
     hipStream_t st = (hipStream_t)stream;
     *nfinished = 0;
+    const int S = (int)e->slots.size();
     if (e->dirty) {
-      e->act.clear();
-      e->hact.clear();
-      e->hnew.clear();
-      std::vector<dkt_plan *> pls;
-      std::vector<const double *> px;
-      std::vector<double *> py;
-      std::vector<const int *> cancel;
-      for (int k = 0; k < (int)e->slots.size(); k++) {
+      e->act64.clear();
+      e->act32.clear();
+      std::vector<CgDesc<double>> h64, hnew;
+      std::vector<CgDesc<float>> h32;
+      std::vector<RelDesc> hrel;
+      std::vector<StatDesc> hstat;
+      std::vector<dkt_plan *> p64, p32;
+      std::vector<const void *> x64, x32, xu;
+      std::vector<void *> y64, y32, yu;
+      std::vector<const int *> c64, c32, cu;
+      for (int k = 0; k < S; k++) {
         auto &s = e->slots[k];
         if (!s.active) continue;
-        e->act.push_back(k);
-        e->hact.push_back(s.d);
-        if (s.fresh) e->hnew.push_back(s.d);
+        if (s.mode == DKT_CG_F64) {
+          e->act64.push_back(k);
+          h64.push_back(s.d);
+          if (s.fresh) hnew.push_back(s.d);
+          p64.push_back(s.pl);
+          x64.push_back(s.d.p);
+          y64.push_back(s.d.ap);
+          c64.push_back(s.d.flag);  // flag[0]: stopped
+        } else {
+          e->act32.push_back(k);
+          h32.push_back(s.d32);
+          RelDesc r = s.rel;
+          r.upd = e->dupd.p + k;
+          hrel.push_back(r);
+          p32.push_back(s.pl);
+          x32.push_back(s.d32.p);
+          y32.push_back(s.d32.ap);
+          c32.push_back(s.d32.flag);
+          xu.push_back(s.rel.x);
+          yu.push_back(s.rel.ap64);
+          cu.push_back(e->dupd.p + S + k);
+        }
         s.fresh = false;
-        pls.push_back(s.pl);
-        px.push_back(s.d.p);
-        py.push_back(s.d.ap);
-        cancel.push_back(s.d.flag);  // flag[0]: stopped
       }
-      e->mv = pls.empty() ? nullptr : batch_matvec_f64(pls, px, py, e->masked, cancel);
-      if (e->dact.n < e->hact.size()) e->dact.alloc(e->hact.size());
-      if (e->dnew.n < e->hnew.size()) e->dnew.alloc(e->hnew.size());
-      if (!e->hact.empty())
-        HIP_OK(hipMemcpyAsync(e->dact.p, e->hact.data(), e->hact.size() * sizeof(CgDesc), hipMemcpyHostToDevice,
-                              st));
-      if (!e->hnew.empty()) {
-        HIP_OK(hipMemcpyAsync(e->dnew.p, e->hnew.data(), e->hnew.size() * sizeof(CgDesc), hipMemcpyHostToDevice,
-                              st));
-        const int nn = (int)e->hnew.size();
+      for (int k : e->act64) hstat.push_back(StatDesc{e->slots[k].d.flag, e->slots[k].d.scal + S_RESID});
+      for (int k : e->act32) hstat.push_back(StatDesc{e->slots[k].d.flag, e->slots[k].rel.scal64 + S_RESID});
+      e->mv64 = p64.empty() ? nullptr : batch_matvec(0, p64, x64, y64, e->masked, c64);
+      e->mv32 = p32.empty() ? nullptr : batch_matvec(1, p32, x32, y32, e->masked, c32);
+      e->mvu = p32.empty() ? nullptr : batch_matvec(0, p32, xu, yu, e->masked, cu);
+      auto put = [&](auto &buf, const auto &v) {
+        if (v.empty()) return;
+        if (buf.n < v.size()) buf.alloc(v.size());
+        HIP_OK(hipMemcpyAsync(buf.p, v.data(), v.size() * sizeof(v[0]), hipMemcpyHostToDevice, st));
+      };
+      put(e->d64, h64);
+      put(e->dnew, hnew);
+      put(e->d32, h32);
+      put(e->drel, hrel);
+      put(e->dstat, hstat);
+      if (!hnew.empty()) {
+        const int nn = (int)hnew.size();
         int nbn = 1;
-        for (const CgDesc &c : e->hnew) nbn = std::max(nbn, cg_blocks(c.n));
+        for (const auto &c : hnew) nbn = std::max(nbn, cg_blocks(c.n));
         hipLaunchKernelGGL(kb_init, dim3(nbn, nn), dim3(kCgThreads), 0, st, e->dnew.p);
         hipLaunchKernelGGL(kb_init_fin, dim3(1, nn), dim3(kCgThreads), 0, st, e->dnew.p);
         HIP_OK(hipGetLastError());
       }
       e->dirty = false;
     }
-    const int K = (int)e->act.size();
-    if (K == 0) return;
-    if (e->status.n < 3 * e->slots.size()) e->status.alloc(3 * e->slots.size());
-    int nb = 1;
-    for (const CgDesc &c : e->hact) nb = std::max(nb, cg_blocks(c.n));
-    const dim3 grid(nb, K), blk(kCgThreads);
+    const int K64 = (int)e->act64.size(), K32 = (int)e->act32.size();
+    if (K64 + K32 == 0) return;
+    int nb64 = 1, nb32 = 1;
+    for (int k : e->act64) nb64 = std::max(nb64, cg_blocks(e->slots[k].d.n));
+    for (int k : e->act32) nb32 = std::max(nb32, cg_blocks(e->slots[k].d.n));
+    // Residual replacements (and starts) of the mixed solves that are due.
+    bool any_upd = false;
+    for (int k = 0; k < S; k++) {
+      const int u = e->slots[k].active && e->slots[k].mode != DKT_CG_F64 ? e->slots[k].upd : 0;
+      e->hupd[k] = u;
+      e->hupd[S + k] = u == 1 ? 0 : 1;  // the FP64 matvec of x runs for replacements only
+      any_upd = any_upd || u != 0;
+    }
+    if (any_upd) {
+      HIP_OK(hipMemcpyAsync(e->dupd.p, e->hupd, 2 * S * sizeof(int), hipMemcpyHostToDevice, st));
+      const dim3 g(nb32, K32), b1(1, K32), blk(kCgThreads);
+      hipLaunchKernelGGL(kr_fold, g, blk, 0, st, e->drel.p);
+      e->mvu->launch(st);
+      hipLaunchKernelGGL(kr_resid, g, blk, 0, st, e->drel.p);
+      hipLaunchKernelGGL(kr_decide, b1, blk, 0, st, e->drel.p);
+      hipLaunchKernelGGL(kr_rel32, g, blk, 0, st, e->drel.p);
+      hipLaunchKernelGGL(kr_rel32_fin, b1, blk, 0, st, e->drel.p);
+      hipLaunchKernelGGL(kr_pupd32, g, blk, 0, st, e->drel.p);
+      HIP_OK(hipGetLastError());
+      for (int k : e->act32) e->slots[k].upd = 0;
+    }
+    const dim3 blk(kCgThreads);
     for (int c = 0; c < chunks * kCgChunk; c++) {
       const int par = c & 1;
-      e->mv->launch(st);
-      hipLaunchKernelGGL(kb_dot, grid, blk, 0, st, e->dact.p);
-      hipLaunchKernelGGL(kb_update_xr, grid, blk, 0, st, e->dact.p, par);
-      hipLaunchKernelGGL(kb_update_p, grid, blk, 0, st, e->dact.p, par);
+      if (K64) {
+        const dim3 g(nb64, K64);
+        e->mv64->launch(st);
+        hipLaunchKernelGGL((kb_dot<double>), g, blk, 0, st, e->d64.p);
+        hipLaunchKernelGGL((kb_update_xr<double>), g, blk, 0, st, e->d64.p, par);
+        hipLaunchKernelGGL((kb_update_p<double>), g, blk, 0, st, e->d64.p, par);
+      }
+      if (K32) {
+        const dim3 g(nb32, K32);
+        e->mv32->launch(st);
+        hipLaunchKernelGGL((kb_dot<float>), g, blk, 0, st, e->d32.p);
+        hipLaunchKernelGGL((kb_update_xr<float>), g, blk, 0, st, e->d32.p, par);
+        hipLaunchKernelGGL((kb_update_p<float>), g, blk, 0, st, e->d32.p, par);
+      }
       HIP_OK(hipGetLastError());
     }
-    hipLaunchKernelGGL(kb_status, dim3(1), dim3(kCgThreads), 0, st, e->dact.p, K, e->status.p);
+    const int K = K64 + K32;
+    if (e->status.n < 3 * (size_t)S) e->status.alloc(3 * (size_t)S);
+    hipLaunchKernelGGL(kb_status, dim3(1), blk, 0, st, e->dstat.p, K, e->status.p);
     HIP_OK(hipGetLastError());
     e->hs.resize(3 * (size_t)K);
     HIP_OK(hipMemcpyAsync(e->hs.data(), e->status.p, e->hs.size() * sizeof(double), hipMemcpyDeviceToHost, st));
     HIP_OK(hipStreamSynchronize(st));
     for (int i = 0; i < K; i++) {
-      if (e->hs[3 * i] == 0.0) continue;
-      const int k = e->act[i];
+      const bool mixed = i >= K64;
+      const int k = mixed ? e->act32[i - K64] : e->act64[i];
+      const int f = (int)e->hs[3 * i];
+      int reason = 0;
+      if (!mixed) {
+        reason = f;
+      } else if (f >= kFinal) {
+        reason = f - kFinal;
+      } else if (f != 0) {
+        e->slots[k].upd = 1;  // FP32 iteration stopped: residual replacement at the next step
+        continue;
+      }
+      if (!reason) continue;
       finished[*nfinished] = k;
       if (stats) {
-        stats[2 * *nfinished] = e->hs[3 * i + 1];
-        stats[2 * *nfinished + 1] = e->hs[3 * i + 2];
+        stats[3 * *nfinished] = e->hs[3 * i + 1];
+        stats[3 * *nfinished + 1] = e->hs[3 * i + 2];
+        stats[3 * *nfinished + 2] = reason;
       }
       (*nfinished)++;
       e->slots[k].active = false;
@@ -770,7 +1119,7 @@ extern "C" int dkt_cg_solve_batch(dkt_plan *const *plans, int nplans, const doub
         throw std::invalid_argument(dkt_last_error());
     }
     std::vector<int> fin(nplans);
-    std::vector<double> fst(2 * (size_t)nplans);
+    std::vector<double> fst(3 * (size_t)nplans);
     int left = nplans;
     // The first step only starts the solves (they stop at once when b = 0 or max_iter = 0).
     for (int chunks = 0; left > 0; chunks = 1) {
@@ -779,8 +1128,9 @@ extern "C" int dkt_cg_solve_batch(dkt_plan *const *plans, int nplans, const doub
         throw std::runtime_error(dkt_last_error());
       for (int i = 0; i < nf; i++)
         if (stats) {
-          stats[2 * fin[i]] = fst[2 * i];
-          stats[2 * fin[i] + 1] = fst[2 * i + 1];
+          stats[3 * fin[i]] = fst[3 * i];
+          stats[3 * fin[i] + 1] = fst[3 * i + 1];
+          stats[3 * fin[i] + 2] = fst[3 * i + 2];
         }
       left -= nf;
     }

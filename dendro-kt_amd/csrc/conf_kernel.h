@@ -43,15 +43,15 @@ struct ConfArgs {
   int max_points, max_virt, max_terms, tile_leaves;
 };
 
+// One tile of one plan (the workgroup may have more work-items than the
+// plan's tile_leaves; they idle in the element phases and help in the loops).
 template <typename T, int DIM, int P>
-__global__ void __launch_bounds__(kTileThreads) conf_tile_kernel(ConfArgs A, const T *__restrict__ kref,
-                                                                 const T *__restrict__ term_w,
-                                                                 const T *__restrict__ x, T *__restrict__ y,
-                                                                 T *__restrict__ partial) {
+__device__ __forceinline__ void conf_tile_body(const ConfArgs &A, int tile, const T *__restrict__ kref,
+                                              const T *__restrict__ term_w, const T *__restrict__ x,
+                                              T *__restrict__ y, T *__restrict__ partial) {
   constexpr int NPE = Pow<P + 1, DIM>::v;
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  if (A.cancel && *A.cancel) return;
-  const ConfTileHdr th = A.tiles[xcd_tile(blockIdx.x, gridDim.x)];
+  const ConfTileHdr th = A.tiles[tile];
   const int nq = A.nq;
   size_t off = 0;
   auto carve = [&](size_t bytes) {
@@ -216,6 +216,15 @@ __global__ void __launch_bounds__(kTileThreads) conf_tile_kernel(ConfArgs A, con
       }
     }
   }
+}
+
+template <typename T, int DIM, int P>
+__global__ void __launch_bounds__(kTileThreads) conf_tile_kernel(ConfArgs A, const T *__restrict__ kref,
+                                                                 const T *__restrict__ term_w,
+                                                                 const T *__restrict__ x, T *__restrict__ y,
+                                                                 T *__restrict__ partial) {
+  if (A.cancel && *A.cancel) return;
+  conf_tile_body<T, DIM, P>(A, xcd_tile(blockIdx.x, gridDim.x), kref, term_w, x, y, partial);
 }
 
 // LDS bytes of conf_tile_kernel for nq right-hand sides.

@@ -1055,21 +1055,33 @@ void build_conf_tiled(dkt_plan *pl, const MeshParams &mp, const Traversal &tr, i
 // ---------------------------------------------------------------------------
 // Launch.
 // ---------------------------------------------------------------------------
+// Scratch of `need` elements for one matvec: the caller's buffer when `ext`
+// is given (too small: an error, nothing is allocated), else the plan's own,
+// allocated or grown on first use.
+template <typename T>
+T *matvec_scratch(dkt_plan *pl, size_t need, const ExtScratch *ext) {
+  if (ext) {
+    if (need * sizeof(T) > ext->bytes)
+      throw std::invalid_argument("matvec scratch of " + std::to_string(need * sizeof(T)) +
+                                  " bytes exceeds the fixed scratch of " + std::to_string(ext->bytes) + " bytes");
+    return (T *)ext->p;
+  }
+  if (sizeof(T) == 8) {
+    if (pl->scratch64.n < need) pl->scratch64.alloc(need);
+    return (T *)pl->scratch64.p;
+  }
+  if (pl->scratch32.n < need) pl->scratch32.alloc(need);
+  return (T *)pl->scratch32.p;
+}
+
 template <typename T, int DIM, int P>
-void run_simple(dkt_plan *pl, int nrhs, const T *x, T *y, int mask, hipStream_t st, const int *cancel) {
+void run_simple(dkt_plan *pl, int nrhs, const T *x, T *y, int mask, hipStream_t st, const int *cancel,
+                const ExtScratch *ext) {
   constexpr int NPE = Pow<P + 1, DIM>::v;
   const bool f64 = sizeof(T) == 8;
   const T *kmat = f64 ? (const T *)pl->kmat64.p : (const T *)pl->kmat32.p;
   const T *hcw = f64 ? (const T *)pl->hc_w64.p : (const T *)pl->hc_w32.p;
-  const size_t need = pl->nbuf * (size_t)nrhs;
-  T *buf;
-  if (f64) {
-    if (pl->scratch64.n < need) pl->scratch64.alloc(need);
-    buf = (T *)pl->scratch64.p;
-  } else {
-    if (pl->scratch32.n < need) pl->scratch32.alloc(need);
-    buf = (T *)pl->scratch32.p;
-  }
+  T *buf = matvec_scratch<T>(pl, pl->nbuf * (size_t)nrhs, ext);
   const uint8_t *dir_in = mask == 1 ? pl->dirichlet.p : nullptr;
   const uint8_t *dir_out = mask ? pl->dirichlet.p : nullptr;
   const int blk = 256;
@@ -1087,7 +1099,8 @@ void run_simple(dkt_plan *pl, int nrhs, const T *x, T *y, int mask, hipStream_t 
 }
 
 template <typename T, int DIM, int P>
-void run_tiled(dkt_plan *pl, int nrhs, const T *x, T *y, int mask, hipStream_t st, const int *cancel) {
+void run_tiled(dkt_plan *pl, int nrhs, const T *x, T *y, int mask, hipStream_t st, const int *cancel,
+               const ExtScratch *ext) {
   constexpr int NPE = Pow<P + 1, DIM>::v;
   const bool f64 = sizeof(T) == 8;
   const T *kmat = P == 2 ? (f64 ? (const T *)pl->kfac64.p : (const T *)pl->kfac32.p)
@@ -1101,15 +1114,7 @@ void run_tiled(dkt_plan *pl, int nrhs, const T *x, T *y, int mask, hipStream_t s
   if (lds_for(1) > kLdsLimit) throw std::invalid_argument("tile does not fit in LDS");
   int chunk = 1;
   while (chunk < nrhs && lds_for(chunk + 1) <= kLdsLimit) chunk++;
-  const size_t need = std::max<size_t>(1, pl->npartial * (size_t)chunk);
-  T *part;
-  if (f64) {
-    if (pl->scratch64.n < need) pl->scratch64.alloc(need);
-    part = (T *)pl->scratch64.p;
-  } else {
-    if (pl->scratch32.n < need) pl->scratch32.alloc(need);
-    part = (T *)pl->scratch32.p;
-  }
+  T *part = matvec_scratch<T>(pl, std::max<size_t>(1, pl->npartial * (size_t)chunk), ext);
   TileArgs A;
   A.tiles = pl->tiles.p;
   A.tile_nodes = pl->tile_nodes.p;
@@ -1131,7 +1136,7 @@ void run_tiled(dkt_plan *pl, int nrhs, const T *x, T *y, int mask, hipStream_t s
   for (int q0 = 0; q0 < nrhs; q0 += chunk) {
     const int nq = std::min(chunk, nrhs - q0);
     A.nq = nq;
-    if (pl->profiling) HIP_OK(hipEventRecord(pl->next_event(), st));
+    if (pl->profiling && !ext) HIP_OK(hipEventRecord(pl->next_event(), st));
     if (pl->ntiles) {
       if (nq == 1)
         hipLaunchKernelGGL((tile_kernel<T, DIM, P, true>), dim3(pl->ntiles), dim3(pl->tile_leaves), lds_for(nq), st, A,
@@ -1141,19 +1146,20 @@ void run_tiled(dkt_plan *pl, int nrhs, const T *x, T *y, int mask, hipStream_t s
                            A, kmat, cw, x + q0, y + q0, part);
       HIP_OK(hipGetLastError());
     }
-    if (pl->profiling) HIP_OK(hipEventRecord(pl->next_event(), st));
+    if (pl->profiling && !ext) HIP_OK(hipEventRecord(pl->next_event(), st));
     if (pl->nshared) {
       const int blk = 256;
       hipLaunchKernelGGL((fixup_kernel<T>), dim3((pl->nshared + blk - 1) / blk), dim3(blk), 0, st,
                          (int)pl->nshared, pl->snode.p, pl->soff.p, part, A.dir_out, nq, nrhs, y + q0, cancel);
       HIP_OK(hipGetLastError());
     }
-    if (pl->profiling) HIP_OK(hipEventRecord(pl->next_event(), st));
+    if (pl->profiling && !ext) HIP_OK(hipEventRecord(pl->next_event(), st));
   }
 }
 
 template <typename T, int DIM, int P>
-void run_conf(dkt_plan *pl, int nrhs, const T *x, T *y, int mask, hipStream_t st, const int *cancel) {
+void run_conf(dkt_plan *pl, int nrhs, const T *x, T *y, int mask, hipStream_t st, const int *cancel,
+              const ExtScratch *ext) {
   constexpr int NPE = Pow<P + 1, DIM>::v;
   const bool f64 = sizeof(T) == 8;
   const T *kmat = P == 2 ? (f64 ? (const T *)pl->kfac64.p : (const T *)pl->kfac32.p)
@@ -1163,15 +1169,7 @@ void run_conf(dkt_plan *pl, int nrhs, const T *x, T *y, int mask, hipStream_t st
   if (lds_for(1) > kLdsLimit) throw std::invalid_argument("tile does not fit in LDS");
   int chunk = 1;
   while (chunk < nrhs && lds_for(chunk + 1) <= kLdsLimit) chunk++;
-  const size_t need = std::max<size_t>(1, pl->npartial * (size_t)chunk);
-  T *part;
-  if (f64) {
-    if (pl->scratch64.n < need) pl->scratch64.alloc(need);
-    part = (T *)pl->scratch64.p;
-  } else {
-    if (pl->scratch32.n < need) pl->scratch32.alloc(need);
-    part = (T *)pl->scratch32.p;
-  }
+  T *part = matvec_scratch<T>(pl, std::max<size_t>(1, pl->npartial * (size_t)chunk), ext);
   ConfArgs A;
   A.tiles = pl->ctiles.p;
   A.tile_nodes = pl->tile_nodes.p;
@@ -1195,34 +1193,35 @@ void run_conf(dkt_plan *pl, int nrhs, const T *x, T *y, int mask, hipStream_t st
   for (int q0 = 0; q0 < nrhs; q0 += chunk) {
     const int nq = std::min(chunk, nrhs - q0);
     A.nq = nq;
-    if (pl->profiling) HIP_OK(hipEventRecord(pl->next_event(), st));
+    if (pl->profiling && !ext) HIP_OK(hipEventRecord(pl->next_event(), st));
     if (pl->ntiles) {
       hipLaunchKernelGGL((conf_tile_kernel<T, DIM, P>), dim3(pl->ntiles), dim3(pl->tile_leaves), lds_for(nq), st, A,
                          kmat, tw, x + q0, y + q0, part);
       HIP_OK(hipGetLastError());
     }
-    if (pl->profiling) HIP_OK(hipEventRecord(pl->next_event(), st));
+    if (pl->profiling && !ext) HIP_OK(hipEventRecord(pl->next_event(), st));
     if (pl->nshared) {
       hipLaunchKernelGGL((fixup_kernel<T>), dim3((pl->nshared + 255) / 256), dim3(256), 0, st, (int)pl->nshared,
                          pl->snode.p, pl->soff.p, part, A.dir_out, nq, nrhs, y + q0, cancel);
       HIP_OK(hipGetLastError());
     }
-    if (pl->profiling) HIP_OK(hipEventRecord(pl->next_event(), st));
+    if (pl->profiling && !ext) HIP_OK(hipEventRecord(pl->next_event(), st));
   }
 }
 
 template <typename T>
-void run_matvec(dkt_plan *pl, int nrhs, const T *x, T *y, int mask, hipStream_t st, const int *cancel) {
+void run_matvec(dkt_plan *pl, int nrhs, const T *x, T *y, int mask, hipStream_t st, const int *cancel,
+                const ExtScratch *ext) {
   const bool simple = pl->simple || pl->per_level;
-#define DKT_DISPATCH(D, P)                                    \
-  if (pl->dim == D && pl->order == P) {                       \
-    if (simple)                                               \
-      run_simple<T, D, P>(pl, nrhs, x, y, mask, st, cancel);  \
-    else if (pl->conforming)                                  \
-      run_conf<T, D, P>(pl, nrhs, x, y, mask, st, cancel);    \
-    else                                                      \
-      run_tiled<T, D, P>(pl, nrhs, x, y, mask, st, cancel);   \
-    return;                                                   \
+#define DKT_DISPATCH(D, P)                                         \
+  if (pl->dim == D && pl->order == P) {                            \
+    if (simple)                                                    \
+      run_simple<T, D, P>(pl, nrhs, x, y, mask, st, cancel, ext);  \
+    else if (pl->conforming)                                       \
+      run_conf<T, D, P>(pl, nrhs, x, y, mask, st, cancel, ext);    \
+    else                                                           \
+      run_tiled<T, D, P>(pl, nrhs, x, y, mask, st, cancel, ext);   \
+    return;                                                        \
   }
   DKT_DISPATCH(2, 1)
   DKT_DISPATCH(3, 1)
@@ -1237,12 +1236,12 @@ void run_matvec(dkt_plan *pl, int nrhs, const T *x, T *y, int mask, hipStream_t 
 
 namespace dkt {
 void plan_apply(dkt_plan *pl, int dtype, const void *x, void *y, int nrhs, int mask, hipStream_t st,
-                const int *cancel) {
+                const int *cancel, const ExtScratch *ext) {
   if (mask < 0 || mask > 2) throw std::invalid_argument("masked must be 0, 1 or 2");
   if (dtype == 0)
-    run_matvec<double>(pl, nrhs, (const double *)x, (double *)y, mask, st, cancel);
+    run_matvec<double>(pl, nrhs, (const double *)x, (double *)y, mask, st, cancel, ext);
   else if (dtype == 1)
-    run_matvec<float>(pl, nrhs, (const float *)x, (float *)y, mask, st, cancel);
+    run_matvec<float>(pl, nrhs, (const float *)x, (float *)y, mask, st, cancel, ext);
   else
     throw std::invalid_argument("dtype must be 0 (f64) or 1 (f32)");
 }

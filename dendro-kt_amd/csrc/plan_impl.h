@@ -91,19 +91,27 @@ struct dkt_plan {
 };
 
 namespace dkt {
+// Device scratch supplied by the caller instead of the plan's own buffer.
+struct ExtScratch {
+  void *p;
+  size_t bytes;
+};
 // y = A x on `st` (dtype 0 f64, 1 f32; mask as dkt_plan_matvec's `masked`).  When `cancel` is non-null and *cancel
-// is nonzero at launch time, the kernels return without touching y.
+// is nonzero at launch time, the kernels return without touching y.  With `ext`, the matvec's partial sums (or
+// the simple path's contributions) go to ext->p, and a matvec that needs more than ext->bytes is refused; no
+// device memory is allocated then.
 void plan_apply(dkt_plan *pl, int dtype, const void *x, void *y, int nrhs, int mask, hipStream_t st,
-                const int *cancel);
+                const int *cancel, const ExtScratch *ext = nullptr);
 
-// FP64 matvecs y_k = A_k x_k of many tiled plans, one launch per (dim, p)
-// (batch.hip); plan k's kernels return at once while *cancel[k] is nonzero.
-// Each result is bitwise the plan's own matvec.
+// Matvecs y_k = A_k x_k (dtype 0 f64, 1 f32, one column) of many tiled plans,
+// reference or conforming, one launch per (kind, dim, p) (batch.hip); plan
+// k's kernels return at once while *cancel[k] is nonzero.  Each result is
+// bitwise the plan's own matvec.
 struct MatvecLauncher {
   virtual ~MatvecLauncher() = default;
   virtual void launch(hipStream_t st) = 0;
 };
-std::unique_ptr<MatvecLauncher> batch_matvec_f64(const std::vector<dkt_plan *> &plans,
-                                                 const std::vector<const double *> &x, const std::vector<double *> &y,
-                                                 int mask, const std::vector<const int *> &cancel);
+std::unique_ptr<MatvecLauncher> batch_matvec(int dtype, const std::vector<dkt_plan *> &plans,
+                                             const std::vector<const void *> &x, const std::vector<void *> &y,
+                                             int mask, const std::vector<const int *> &cancel);
 }  // namespace dkt

@@ -3,33 +3,42 @@ solves are single-shot calls of HeatMat::cgSolve, FEM/examples/src/heatMat.cpp:1
 
 A request names a mesh (a seeded Gaussian point cloud, or a tree refined around
 a sphere, that the host refines, balances and enumerates, as ot::DA(tree, ...)
-does), a kind and a right-hand side; the server answers with x = A^-1 b to a
-relative L_inf residual.  Kinds:
+does), a kind, a precision mode and a right-hand side; the server answers with
+x = A^-1 b to a relative L_inf residual.  Kinds:
 * "poisson": b is given per node (zero on the boundary), A the
   Dirichlet-masked stiffness (HeatMat::cgSolve);
 * "heat": the reference's heat-equation driver (FEM/examples/src/heatEq.cpp):
   b = M f by the mass operator with HeatVec's hooks (feVector::computeVec,
   FEM/include/feVector.h:122-168), then the same CG on the stiffness.
-Design:
 
-* plan cache: mesh -> (DA, device plan), keyed by the mesh spec, with LRU
-  eviction when the estimated device footprint exceeds the HBM budget
-  (admission control: a plan larger than the whole budget is refused);
-* workers: one thread per HIP stream; solves on different streams overlap on
-  the device (the C calls release the GIL), solves on the same plan are
-  serialised because a plan owns one CG workspace;
+Operator: by default the conforming operator sum_e C_e^T K_e C_e (symmetric,
+so CG converges at every size); Server(operator="reference") serves the
+reference traversal's operator (fem::matvec semantics, not symmetric at
+hanging nodes: FP64 CG can stall on it at scale).
+
+Pipeline:
+* intake: builder threads take submitted requests, lease their mesh's plans
+  from the plan cache (building them on a miss), form the right-hand side on
+  the host and put the request on one ready list shared by all workers;
+* admission (PlanCache): a mesh's device footprint is known before anything is
+  allocated on the device (DKT_PLAN_DRYRUN plans give the map bytes), and
+  every lease counts against the HBM budget until its request finishes: the
+  maps of the leased plans plus the request's vectors.  Unleased plans are
+  evicted least recently used first; a request that does not fit waits for
+  leases to end (and is refused when it could never fit);
+* workers: one thread per HIP stream.  Batched (Server(batch=K)): each worker
+  runs a CG engine (dkt_cg_engine) of K slots, FP64 and mixed-precision
+  solves side by side; a ready request joins as soon as a slot is free and
+  leaves when its solve stops, between chunks of 8 iterations (continuous
+  batching).  Unbatched: one solve at a time per stream;
+* order: select_next picks the request a free slot takes: requests that have
+  waited longer than max_wait first, oldest first; otherwise the smallest
+  mesh, then the oldest;
 * every reduction on the device runs in a fixed order, so a request's answer
-  does not depend on what else is being served ("alone == in-mix");
-* continuous batching (Server(batch=K)): each worker runs a CG engine
-  (dkt_cg_engine) of K slots; a queued request joins as soon as a slot is free
-  (smallest mesh first among the requests the worker holds, which bounds the
-  median latency under load) and leaves when its solve stops, between chunks of 8 iterations, so one
-  launch per step serves every running solve and no solve waits for a slower
-  one.  Solves are FP64; each answer is bitwise the request's own FP64 solve.
-  Every slot has its own CG workspace, so requests on one mesh run together.
+  does not depend on what else is being served ("alone == in-mix").
 
 Latencies are measured from submission to completion of each request
-(end to end) and from plan availability to completion (solve).
+(end to end) and from the start of its solve to completion.
 """
 import collections
 import queue
@@ -39,7 +48,11 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-_EMPTY = object()  # no request available
+CG_F64, CG_MIXED, CG_MIXED_RESTART, CG_MIXED_RELIABLE = 0, 1, 2, 3
+# Device bytes per node of one request: f, b, x (FP64), the CG workspace
+# (r, p, Ap in FP64; r, p, Ap, x in FP32 for the mixed modes) and the mixed
+# modes' FP64 residual.
+VEC_BYTES_PER_NODE = 8 * 7 + 4 * 5
 
 
 @dataclass(frozen=True)
@@ -72,7 +85,7 @@ class Request:
     mesh: MeshSpec
     rhs_seed: int
     tol: float = 1e-8
-    mode: int = 0
+    mode: int = CG_F64
     max_iter: int = 100000
     kind: str = "poisson"
     # filled in by the server
@@ -81,121 +94,195 @@ class Request:
     solve_started: float = 0.0
     finished: float = 0.0
     plan_built: bool = False
+    nodes: int = 0
     stats: dict = field(default_factory=dict)
     x: object = None
     error: str = ""
 
 
+def select_next(ready, now, max_wait):
+    """Index in `ready` (items with .req.submitted and .req.nodes) of the
+    request a free slot takes next: among requests that have waited longer
+    than max_wait, the oldest; otherwise the smallest mesh, then the oldest.
+    None when `ready` is empty."""
+    if not ready:
+        return None
+    aged = [i for i, it in enumerate(ready) if now - it.req.submitted > max_wait]
+    if aged:
+        return min(aged, key=lambda i: (ready[i].req.submitted, i))
+    return min(range(len(ready)), key=lambda i: (ready[i].req.nodes, ready[i].req.submitted, i))
+
+
+class AdmissionError(MemoryError):
+    pass
+
+
+class Lease:
+    """A request's hold on a cached mesh: its DA and plans, and the bytes it
+    counts against the budget until released."""
+
+    def __init__(self, cache, key, entry, vec_bytes):
+        self.cache, self.key, self.entry, self.vec_bytes = cache, key, entry, vec_bytes
+        self.da, self.plans, self.lock = entry["da"], entry["plans"], entry["lock"]
+        self.released = False
+
+    def release(self):
+        if not self.released:
+            self.released = True
+            self.cache._release(self)
+
+
 class PlanCache:
-    def __init__(self, dkt, budget_bytes):
+    """Mesh -> (DA, stiffness plan, mass plan) under an HBM budget.
+
+    acquire(spec) returns a Lease.  A miss builds the DA on the host and the
+    plans first as dry runs (host tiling only, no device memory) to learn the
+    map bytes; only when those plus the request's vectors fit the budget,
+    after evicting unleased entries least recently used first, are the device
+    plans built.  A request that fits the budget but not the memory left by
+    current leases waits for them to end (at most wait_timeout seconds); one
+    larger than the whole budget is refused with AdmissionError.
+
+    `build(spec)` may be given to replace the construction: it returns
+    (da, map_bytes, make_plans) where make_plans() builds the device plans."""
+
+    def __init__(self, dkt, budget_bytes, operator="conforming", build=None, wait_timeout=120.0):
+        if operator not in ("conforming", "reference"):
+            raise ValueError(f"unknown operator {operator}")
         self.dkt = dkt
-        self.budget = budget_bytes
-        self.lock = threading.Lock()
-        self.entries = collections.OrderedDict()  # spec -> (da, plans, bytes, lock)
-        self.building = {}
+        self.budget = float(budget_bytes)
+        self.operator = operator
+        self.build = build or self._build
+        self.wait_timeout = wait_timeout
+        self.cv = threading.Condition()
+        self.entries = collections.OrderedDict()  # spec -> dict(da, plans, map_bytes, refs, lock)
+        self.building = set()
+        self.leased_vec_bytes = 0.0
 
-    @staticmethod
-    def footprint(plans, n_nodes):
-        # device maps (stiffness and mass plans) + CG workspace (FP64 and FP32
-        # vectors) + the request's f, b and x
-        return sum(p.info()["map_bytes"] for p in plans) + n_nodes * (8 * 7 + 4 * 5)
+    def _build(self, spec):
+        dkt = self.dkt
+        conf = self.operator == "conforming"
+        da = dkt.DA(spec.tree(dkt), spec.order)
+        map_bytes = sum(dkt.Plan(da, op, conforming=conf, dry_run=True).info()["map_bytes"]
+                        for op in (dkt.OP_STIFFNESS, dkt.OP_MASS))
+        return da, map_bytes, lambda: (dkt.Plan(da, dkt.OP_STIFFNESS, conforming=conf),
+                                       dkt.Plan(da, dkt.OP_MASS, conforming=conf))
 
-    def get(self, spec):
-        """(da, (stiffness plan, mass plan), plan_lock, built_now).  Builds on a miss;
-        concurrent misses on one spec wait for a single build."""
+    def used(self):
+        """Bytes counted against the budget: every cached entry's maps plus the
+        vectors of every live lease."""
+        with self.cv:
+            return sum(e["map_bytes"] for e in self.entries.values()) + self.leased_vec_bytes
+
+    def _make_room(self, need, deadline):
+        """With self.cv held: evicts unleased entries (LRU first) and waits for
+        leases to end until `need` more bytes fit."""
         while True:
-            with self.lock:
-                if spec in self.entries:
-                    self.entries.move_to_end(spec)
-                    da, plans, _, lk = self.entries[spec]
-                    return da, plans, lk, False
-                ev = self.building.get(spec)
-                if ev is None:
-                    ev = threading.Event()
-                    self.building[spec] = ev
-                    owner = True
-                else:
-                    owner = False
-            if not owner:
-                ev.wait()
+            used = sum(e["map_bytes"] for e in self.entries.values()) + self.leased_vec_bytes
+            if used + need <= self.budget:
+                return
+            victim = next((k for k, e in self.entries.items() if e["refs"] == 0), None)
+            if victim is not None:
+                del self.entries[victim]
                 continue
+            left = deadline - time.monotonic()
+            if left <= 0:
+                raise AdmissionError(f"admission timed out: {need / 1e9:.2f} GB needed, "
+                                     f"{used / 1e9:.2f} of {self.budget / 1e9:.2f} GB leased")
+            self.cv.wait(min(left, 0.5))
+
+    def acquire(self, spec):
+        """(Lease, built_now).  Builds on a miss; concurrent misses on one spec
+        wait for a single build."""
+        deadline = time.monotonic() + self.wait_timeout
+        with self.cv:
+            while spec in self.building:
+                self.cv.wait(0.5)
+            if spec in self.entries:
+                e = self.entries[spec]
+                vec = VEC_BYTES_PER_NODE * e["da"].n_nodes
+                if e["map_bytes"] + vec > self.budget:
+                    raise AdmissionError(f"mesh needs {(e['map_bytes'] + vec) / 1e9:.2f} GB, "
+                                         f"budget {self.budget / 1e9:.2f} GB")
+                e["refs"] += 1  # held while waiting: not evicted under us
+                try:
+                    self._make_room(vec, deadline)
+                except Exception:
+                    e["refs"] -= 1
+                    raise
+                self.entries.move_to_end(spec)
+                self.leased_vec_bytes += vec
+                return Lease(self, spec, e, vec), False
+            self.building.add(spec)
+        try:
+            da, map_bytes, make_plans = self.build(spec)  # host only: no device memory yet
+            vec = VEC_BYTES_PER_NODE * da.n_nodes
+            if map_bytes + vec > self.budget:
+                raise AdmissionError(f"mesh needs {(map_bytes + vec) / 1e9:.2f} GB, "
+                                     f"budget {self.budget / 1e9:.2f} GB")
+            with self.cv:
+                self._make_room(map_bytes + vec, deadline)
+                # reserve the bytes before the device allocation
+                e = dict(da=da, plans=None, map_bytes=map_bytes, refs=1, lock=threading.Lock())
+                self.entries[spec] = e
+                self.leased_vec_bytes += vec
             try:
-                dkt = self.dkt
-                da = dkt.DA(spec.tree(dkt), spec.order)
-                plans = (dkt.Plan(da, dkt.OP_STIFFNESS), dkt.Plan(da, dkt.OP_MASS))
-                nbytes = self.footprint(plans, da.n_nodes)
-                if nbytes > self.budget:
-                    raise MemoryError(f"plan needs {nbytes / 1e9:.1f} GB, budget {self.budget / 1e9:.1f} GB")
-                with self.lock:
-                    used = sum(e[2] for e in self.entries.values())
-                    while self.entries and used + nbytes > self.budget:
-                        _, old = self.entries.popitem(last=False)
-                        used -= old[2]
-                    self.entries[spec] = (da, plans, nbytes, threading.Lock())
-                    return da, plans, self.entries[spec][3], True
-            finally:
-                with self.lock:
-                    self.building.pop(spec, None)
-                ev.set()
+                e["plans"] = make_plans()
+            except Exception:
+                with self.cv:
+                    self.entries.pop(spec, None)
+                    self.leased_vec_bytes -= vec
+                    self.cv.notify_all()
+                raise
+            return Lease(self, spec, e, vec), True
+        finally:
+            with self.cv:
+                self.building.discard(spec)
+                self.cv.notify_all()
+
+    def _release(self, lease):
+        with self.cv:
+            lease.entry["refs"] -= 1
+            self.leased_vec_bytes -= lease.vec_bytes
+            self.cv.notify_all()
+
+
+@dataclass
+class Ready:
+    req: Request
+    lease: Lease
+    f: np.ndarray
 
 
 class Server:
-    def __init__(self, dkt, n_streams=4, hbm_budget_bytes=200e9, batch=0, cache=None, max_wait=0.5):
+    def __init__(self, dkt, n_streams=4, hbm_budget_bytes=200e9, batch=0, cache=None, max_wait=0.5,
+                 operator="conforming", n_builders=2):
         import torch
         self.torch = torch
         self.dkt = dkt
-        self.cache = cache if cache is not None else PlanCache(dkt, hbm_budget_bytes)
-        self.q = queue.Queue()
+        self.cache = cache if cache is not None else PlanCache(dkt, hbm_budget_bytes, operator)
+        self.intake = queue.Queue()
+        self.ready = []
+        self.ready_cv = threading.Condition()
         self.batch = batch
-        self.max_wait = max_wait  # batched mode: seconds after which a waiting request goes first
+        self.max_wait = max_wait
+        self.closing = False
         self.done = []
         self.done_lock = threading.Lock()
-        self.workers = []
-        work = self._work_batched if batch > 0 else self._work
-        for i in range(n_streams):
-            t = threading.Thread(target=work, args=(torch.cuda.Stream(),), daemon=True)
+        self.builders = [threading.Thread(target=self._build_loop, daemon=True) for _ in range(n_builders)]
+        for t in self.builders:
             t.start()
-            self.workers.append(t)
+        work = self._work_batched if batch > 0 else self._work
+        self.workers = [threading.Thread(target=work, args=(torch.cuda.Stream(),), daemon=True)
+                        for _ in range(n_streams)]
+        for t in self.workers:
+            t.start()
 
     def submit(self, req):
         req.submitted = time.perf_counter()
-        self.q.put(req)
+        self.intake.put(req)
 
-    def _work(self, stream):
-        torch = self.torch
-        while True:
-            req = self.q.get()
-            if req is None:
-                return
-            try:
-                req.started = time.perf_counter()
-                da, (plan, mass), lk, built = self.cache.get(req.mesh)
-                req.plan_built = built
-                req.solve_started = time.perf_counter()
-                f = self.dkt.keyed_uniform(req.rhs_seed, da.nodes())
-                if req.kind == "poisson":
-                    f[da.boundary()] = 0.0
-                elif req.kind != "heat":
-                    raise ValueError(f"unknown request kind {req.kind}")
-                with torch.cuda.stream(stream):
-                    fd = torch.tensor(f, device="cuda")
-                    xd = torch.zeros_like(fd)
-                    with lk:
-                        if req.kind == "heat":
-                            bd = torch.empty_like(fd)
-                            mass.matvec_torch(fd, bd, masked=2, stream=stream.cuda_stream)
-                        else:
-                            bd = fd
-                        req.stats = plan.cg_solve_torch(bd, xd, req.mode, req.max_iter, req.tol,
-                                                        stream=stream.cuda_stream)
-                    req.x = xd.cpu().numpy()
-                stream.synchronize()
-            except Exception as e:  # reported per request
-                req.error = f"{type(e).__name__}: {e}"
-            req.finished = time.perf_counter()
-            with self.done_lock:
-                self.done.append(req)
-
+    # -- intake ---------------------------------------------------------------
     def _rhs(self, req, da):
         f = self.dkt.keyed_uniform(req.rhs_seed, da.nodes())
         if req.kind == "poisson":
@@ -204,72 +291,112 @@ class Server:
             raise ValueError(f"unknown request kind {req.kind}")
         return f
 
-    def _finish(self, req):
+    def _build_loop(self):
+        while True:
+            req = self.intake.get()
+            if req is None:
+                return
+            req.started = time.perf_counter()
+            lease = None
+            try:
+                if self.batch > 0 and req.mode not in (CG_F64, CG_MIXED, CG_MIXED_RELIABLE):
+                    raise ValueError(f"batched serving runs modes CG_F64 and CG_MIXED(_RELIABLE), not {req.mode}")
+                lease, built = self.cache.acquire(req.mesh)
+                req.plan_built = built
+                req.nodes = lease.da.n_nodes
+                item = Ready(req, lease, self._rhs(req, lease.da))
+            except Exception as e:  # reported per request
+                if lease is not None:
+                    lease.release()
+                req.error = f"{type(e).__name__}: {e}"
+                self._finish(req)
+                continue
+            with self.ready_cv:
+                self.ready.append(item)
+                self.ready_cv.notify()
+
+    def _take(self, block):
+        """The next ready request by select_next (None: none, or shutting down)."""
+        with self.ready_cv:
+            while True:
+                i = select_next(self.ready, time.perf_counter(), self.max_wait)
+                if i is not None:
+                    return self.ready.pop(i)
+                if not block or self.closing:
+                    return None
+                self.ready_cv.wait(0.05)
+
+    def _finish(self, req, lease=None):
+        if lease is not None:
+            lease.release()
         req.finished = time.perf_counter()
         with self.done_lock:
             self.done.append(req)
 
-    def _next(self, block):
-        """A queued request (None: shut down; _EMPTY: none)."""
-        try:
-            return self.q.get(block=block, timeout=0.05 if block else None)
-        except queue.Empty:
-            return _EMPTY
+    def _mass_rhs(self, item, fd, stream):
+        """b = M f with HeatVec's hooks for heat requests; b = f otherwise."""
+        if item.req.kind != "heat":
+            return fd
+        bd = self.torch.empty_like(fd)
+        _, mass = item.lease.plans
+        with item.lease.lock:  # a plan's matvec scratch serves one caller at a time
+            mass.matvec_torch(fd, bd, masked=2, stream=stream.cuda_stream)
+            stream.synchronize()
+        return bd
+
+    # -- workers --------------------------------------------------------------
+    def _work(self, stream):
+        torch = self.torch
+        while True:
+            item = self._take(block=True)
+            if item is None:
+                if self.closing:
+                    return
+                continue
+            req = item.req
+            try:
+                req.solve_started = time.perf_counter()
+                with torch.cuda.stream(stream):
+                    fd = torch.tensor(item.f, device="cuda")
+                    bd = self._mass_rhs(item, fd, stream)
+                    xd = torch.zeros_like(fd)
+                    plan, _ = item.lease.plans
+                    with item.lease.lock:  # a plan owns one CG workspace
+                        st = plan.cg_solve_torch(bd, xd, req.mode, req.max_iter, req.tol,
+                                                 stream=stream.cuda_stream)
+                    req.stats = dict(st, converged=st["stop"] == "converged")
+                    req.x = xd.cpu().numpy()
+                stream.synchronize()
+            except Exception as e:  # reported per request
+                req.error = f"{type(e).__name__}: {e}"
+            self._finish(req, item.lease)
 
     def _work_batched(self, stream):
-        """Continuous batching: requests join the worker's CG engine as slots free
-        up, smallest mesh first among the waiting ones, and leave when their
-        solve stops, between chunks of 8 iterations."""
+        """Continuous batching: ready requests join the worker's CG engine as
+        slots free up and leave when their solve stops."""
         torch = self.torch
         engine = self.dkt.CgEngine(self.batch, masked=True)
-        running = {}  # id(req) -> (req, b, x)
-        pending = []  # (nodes, arrival, req, da, plan, mass, lock): taken from the queue, not started
-        stop = False
+        running = {}  # id(req) -> (item, b, x)
         while True:
-            # Take queued requests (at most 2 * batch waiting here, so that the
-            # other workers get theirs), then start the smallest meshes first.
-            while not stop and len(pending) < 2 * self.batch:
-                req = self._next(block=not running and not pending)
-                if req is _EMPTY:
+            while len(running) < self.batch:
+                item = self._take(block=not running)
+                if item is None:
                     break
-                if req is None:
-                    stop = True
-                    break
-                if not req.started:
-                    req.started = time.perf_counter()
-                try:
-                    da, (plan, mass), lk, built = self.cache.get(req.mesh)
-                except Exception as e:  # reported per request
-                    req.error = f"{type(e).__name__}: {e}"
-                    self._finish(req)
-                    continue
-                req.plan_built = req.plan_built or built
-                pending.append((da.n_nodes, req.submitted, req, da, plan, mass, lk))
-            # requests waiting longer than max_wait go first, in arrival order (no starvation)
-            now = time.perf_counter()
-            pending.sort(key=lambda t: (0, t[1], 0) if now - t[1] > self.max_wait else (1, t[0], t[1]))
-            while pending and len(running) < self.batch:
-                _, _, req, da, plan, mass, lk = pending.pop(0)
+                req = item.req
                 try:
                     with torch.cuda.stream(stream):
                         req.solve_started = time.perf_counter()
-                        fd = torch.tensor(self._rhs(req, da), device="cuda")
-                        if req.kind == "heat":
-                            # the mass plan's matvec buffers are per plan: one user at a time
-                            bd = torch.empty_like(fd)
-                            with lk:
-                                mass.matvec_torch(fd, bd, masked=2, stream=stream.cuda_stream)
-                                stream.synchronize()
-                        else:
-                            bd = fd
+                        fd = torch.tensor(item.f, device="cuda")
+                        bd = self._mass_rhs(item, fd, stream)
                         xd = torch.empty_like(fd)
-                        engine.add(plan, bd, xd, key=id(req), max_iter=req.max_iter, tol=req.tol)
-                    running[id(req)] = (req, bd, xd)
+                        engine.add(item.lease.plans[0], bd, xd, key=id(req), max_iter=req.max_iter, tol=req.tol,
+                                   mode=req.mode)
+                    running[id(req)] = (item, bd, xd)
                 except Exception as e:
                     req.error = f"{type(e).__name__}: {e}"
-                    self._finish(req)
+                    self._finish(req, item.lease)
             if not running:
-                if stop and not pending:
+                if self.closing:
                     return
                 continue
             try:
@@ -277,16 +404,20 @@ class Server:
                     finished = engine.step(1, stream=stream.cuda_stream)
                     out = []
                     for key, st in finished:
-                        req, bd, xd = running.pop(key)
-                        req.stats = dict(iters=st["iters"], resid=st["resid"], refinements=0, matvecs=st["iters"])
+                        item, bd, xd = running.pop(key)
+                        req = item.req
+                        req.stats = dict(iters=st["iters"], resid=st["resid"], stop=st["stop"],
+                                         converged=st["stop"] == "converged", refinements=0, matvecs=st["iters"])
+                        if st["stop"] == "breakdown":
+                            req.error = "CgBreakdown: p.Ap <= 0 or a non-finite step"
                         req.x = xd.cpu().numpy()
-                        out.append(req)
-                for req in out:
-                    self._finish(req)
+                        out.append(item)
+                for item in out:
+                    self._finish(item.req, item.lease)
             except Exception as e:  # every running solve fails
-                for req, *_ in running.values():
-                    req.error = f"{type(e).__name__}: {e}"
-                    self._finish(req)
+                for item, *_ in running.values():
+                    item.req.error = f"{type(e).__name__}: {e}"
+                    self._finish(item.req, item.lease)
                 running.clear()
                 engine = self.dkt.CgEngine(self.batch, masked=True)
 
@@ -301,8 +432,13 @@ class Server:
             time.sleep(0.002)
 
     def close(self):
-        for _ in self.workers:
-            self.q.put(None)
+        for _ in self.builders:
+            self.intake.put(None)
+        for t in self.builders:
+            t.join()
+        with self.ready_cv:
+            self.closing = True
+            self.ready_cv.notify_all()
         for t in self.workers:
             t.join()
 

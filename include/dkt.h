@@ -187,8 +187,19 @@ int dkt_plan_matvec(dkt_plan *plan, int dtype, const void *x, void *y, int nrhs,
  * b, x: device double [N]; x holds the initial guess on entry.  Reductions
  * run in a fixed order (bitwise reproducible).  stats (may be NULL): [0]
  * CG iterations (FP32 iterations for the mixed modes), [1] final relative
- * residual, [2] residual replacements / refinement steps, [3] matvecs.
- * Returns 0 also when max_iter is reached; compare stats[1] with tol. */
+ * residual, [2] residual replacements / refinement steps, [3] matvecs,
+ * [4] stop reason: DKT_CG_CONVERGED, DKT_CG_MAX_ITER, DKT_CG_BREAKDOWN
+ * (p.Ap <= 0 or a non-finite step: the operator is not positive definite on
+ * the search direction; the iteration stops before x changes) or
+ * DKT_CG_STALLED (the mixed modes' refinement made no progress).
+ * Returns 0 when the solve converged, reached max_iter or stalled (compare
+ * stats[1] with tol), DKT_ERR_BREAKDOWN on a breakdown (stats are filled and
+ * x holds the last iterate), 1 on invalid arguments or a device error. */
+#define DKT_CG_CONVERGED 1
+#define DKT_CG_MAX_ITER 2
+#define DKT_CG_BREAKDOWN 3
+#define DKT_CG_STALLED 4
+#define DKT_ERR_BREAKDOWN 2
 #define DKT_CG_F64 0
 #define DKT_CG_MIXED 1
 #define DKT_CG_MIXED_RESTART 2
@@ -197,42 +208,53 @@ int dkt_cg_solve(dkt_plan *plan, int mode, const double *b, double *x, int max_i
                  double *stats, void *stream);
 
 /* Batched FP64 conjugate gradients: solves A_k x_k = b_k for nplans tiled
- * plans (any mix of (dim, p); no simple, per-level or conforming plans)
+ * plans (any mix of (dim, p), reference or conforming; no simple or
+ * per-level plans)
  * together on one stream, each as dkt_cg_solve with DKT_CG_F64 from x_k = 0
  * (x_k is overwritten).  Per iteration one matvec launch per (dim, p) group
  * and one launch per vector step serve every solve; a converged solve's
  * kernels return at once.  Each solve's arithmetic and reduction order are
  * those of its own dkt_cg_solve: x_k is bitwise what dkt_cg_solve(plans[k],
  * DKT_CG_F64, b_k, x_k = 0, ...) returns.  A plan may appear several times
- * (each solve has its own workspace).  stats (may be NULL): 2 * nplans doubles,
- * [2k] iterations and [2k + 1] final relative residual of solve k.
+ * (each solve has its own workspace).  stats (may be NULL): 3 * nplans doubles,
+ * [3k] iterations, [3k + 1] final relative residual and [3k + 2] stop reason
+ * (DKT_CG_CONVERGED, DKT_CG_MAX_ITER or DKT_CG_BREAKDOWN) of solve k.
  * No counterpart in the reference (HeatMat::cgSolve solves one system per
  * call, FEM/examples/src/heatMat.cpp:165); for servers batching requests. */
 int dkt_cg_solve_batch(dkt_plan *const *plans, int nplans, const double *const *b, double *const *x, int max_iter,
                        double tol, int masked, double *stats, void *stream);
 
-/* Continuous batching of the same FP64 solves (dkt_cg_solve_batch runs on
- * it): solves join with dkt_cg_engine_add and leave when they stop, between
- * chunks of 8 iterations, so a server keeps the device busy with whatever is
- * running instead of waiting for the slowest solve of a batch.  Each solve is
- * bitwise its own dkt_cg_solve(DKT_CG_F64) from x = 0 (x is overwritten), with
- * its own tol and max_iter; masked is per engine.  No counterpart in the
- * reference.
+/* Continuous batching of CG solves (dkt_cg_solve_batch runs on it): solves
+ * join with dkt_cg_engine_add(_mode) and leave when they stop, between chunks
+ * of 8 iterations, so a server keeps the device busy with whatever is running
+ * instead of waiting for the slowest solve of a batch.  Each solve is bitwise
+ * its own dkt_cg_solve from x = 0 in its mode (x is overwritten), with its own
+ * tol and max_iter; masked is per engine.  No counterpart in the reference.
  * dkt_cg_engine_create: at most max_slots (1..65535) solves at a time.
- * dkt_cg_engine_add: plan (tiled; several solves may share one), device b
- *   and x of its N nodes (they must stay valid until the solve stops); *slot
- *   gets the solve's slot, whose workspace the engine keeps for later solves.
- *   Fails when every slot is taken.
+ * dkt_cg_engine_add: an FP64 solve (DKT_CG_F64) on a tiled plan (reference or
+ *   conforming; several solves may share one), device b and x of its N nodes
+ *   (they must stay valid until the solve stops); *slot gets the solve's
+ *   slot, whose workspace the engine keeps for later solves.  Fails when
+ *   every slot is taken.
+ * dkt_cg_engine_add_mode: the same with mode DKT_CG_F64, or
+ *   DKT_CG_MIXED / DKT_CG_MIXED_RELIABLE: mixed precision with reliable
+ *   updates (FP32 iteration and matvecs, FP64 residual replacements, stop on
+ *   the FP64 residual), bitwise dkt_cg_solve(DKT_CG_MIXED_RELIABLE) from
+ *   x = 0.  The replacements of every mixed solve that is due run together
+ *   at the start of a step.
  * dkt_cg_engine_step: starts the solves added since the last step, runs
  *   chunks * 8 iterations of every running solve on `stream`, synchronizes,
- *   and reports the solves that stopped (converged or max_iter reached):
- *   finished[i] = slot, stats[2i] = iterations, stats[2i + 1] = relative
- *   residual (stats may be NULL), *nfinished = their count; arrays of
- *   max_slots entries.  Their slots are free again.  One engine, one thread. */
+ *   and reports the solves that stopped (converged, max_iter reached or
+ *   broken down): finished[i] = slot, stats[3i] = iterations, stats[3i + 1]
+ *   = relative residual, stats[3i + 2] = stop reason (DKT_CG_*; stats may be
+ *   NULL), *nfinished = their count; finished has max_slots entries, stats
+ *   3 * max_slots.  Their slots are free again.  One engine, one thread. */
 typedef struct dkt_cg_engine dkt_cg_engine;
 int dkt_cg_engine_create(int max_slots, int masked, dkt_cg_engine **out);
 int dkt_cg_engine_add(dkt_cg_engine *engine, dkt_plan *plan, const double *b, double *x, int max_iter, double tol,
                       int *slot);
+int dkt_cg_engine_add_mode(dkt_cg_engine *engine, dkt_plan *plan, int mode, const double *b, double *x, int max_iter,
+                           double tol, int *slot);
 int dkt_cg_engine_step(dkt_cg_engine *engine, int chunks, void *stream, int *finished, double *stats,
                        int *nfinished);
 int dkt_cg_engine_destroy(dkt_cg_engine *engine);

@@ -238,3 +238,61 @@ def test_cg_engine_solves_joining_mid_flight(dkt):
         assert got[k]["iters"] == st1["iters"] and got[k]["resid"] == st1["resid"]
         assert np.array_equal(xb.cpu().numpy(), x1)
     assert got[2]["iters"] == 13
+
+
+@pytest.mark.gpu
+def test_cg_engine_mixed_and_conforming(dkt):
+    """The engine runs FP64 and mixed-precision (reliable-update) solves side
+    by side, on reference and conforming plans: each solve is bitwise its own
+    dkt_cg_solve from x = 0 in its mode (iterations, residual, stop reason and
+    x), including solves that join while others are mid-flight, a solve
+    stopped by max_iter, and two solves sharing one plan; every converged
+    mixed solve reaches the FP64 tolerance on its true residual."""
+    torch = _torch()
+    specs = [("gauss4d_small", False, dkt.CG_MIXED_RELIABLE, 1e-8, 5000),
+             ("sphere3d_small", True, dkt.CG_MIXED_RELIABLE, 1e-8, 5000),
+             ("ex3_3d_p2", True, dkt.CG_F64, 1e-8, 5000),
+             ("sphere3d_small", True, dkt.CG_MIXED_RELIABLE, 1e-8, 30),
+             ("ex3_3d_p1", False, dkt.CG_F64, 1e-8, 5000),
+             ("sphere3d_small", True, dkt.CG_F64, 1e-10, 5000)]
+    plans = {}
+    cases = []
+    for i, (name, conf, mode, tol, mi) in enumerate(specs):
+        if (name, conf) not in plans:
+            da, z, meta = fx.build_da(dkt, name)
+            plans[(name, conf)] = (da, meta, dkt.Plan(da, dkt.OP_STIFFNESS, conforming=conf))
+        da, meta, plan = plans[(name, conf)]
+        bd = torch.tensor(_rhs(da, meta["dim"], 41 + i), device="cuda")
+        cases.append((plan, bd, mode, tol, mi))
+    singles = []
+    for plan, bd, mode, tol, mi in cases:
+        x = torch.zeros_like(bd)
+        singles.append((plan.cg_solve_torch(bd, x, mode, mi, tol), x.cpu().numpy()))
+    eng = dkt.CgEngine(3)
+    xs = [torch.full_like(bd, 3.0) for _, bd, *_ in cases]
+    got = {}
+    todo = list(range(len(cases)))
+
+    def add(k):
+        plan, bd, mode, tol, mi = cases[k]
+        eng.add(plan, bd, xs[k], key=k, max_iter=mi, tol=tol, mode=mode)
+
+    for _ in range(3):
+        add(todo.pop(0))
+    steps = 0
+    while len(got) < len(cases):
+        for key, st in eng.step(1 if steps % 3 else 2):
+            got[key] = st
+            if todo:
+                add(todo.pop(0))
+        steps += 1
+        assert steps < 20000
+    torch.cuda.synchronize()
+    for k, ((st1, x1), xb) in enumerate(zip(singles, xs)):
+        print(k, specs[k], "single", st1, "engine", got[k], flush=True)
+        assert got[k]["iters"] == st1["iters"] and got[k]["resid"] == st1["resid"]
+        assert got[k]["stop"] == st1["stop"]
+        assert np.array_equal(xb.cpu().numpy(), x1)
+        if specs[k][4] == 5000:
+            assert st1["stop"] == "converged" and st1["resid"] <= specs[k][3]
+    assert got[3]["stop"] == "max_iter" and got[3]["iters"] >= 30

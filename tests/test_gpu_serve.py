@@ -1,9 +1,10 @@
-"""Serving harness: concurrent CG requests (Poisson and heat-equation kinds) over
-cached plans on several HIP streams.  A request's answer must be bitwise the
-same whether it is served alone or in a mix (fixed-order device reductions),
-every request must reach its tolerance, a heat request's right-hand side must
-be the output-masked mass matvec, and a plan larger than the HBM budget is
-refused."""
+"""Serving harness: concurrent CG requests (Poisson and heat-equation kinds,
+FP64 and mixed precision) over cached plans on several HIP streams, with and
+without continuous batching.  A request's answer must be bitwise the same
+whether it is served alone or in a mix (fixed-order device reductions), every
+request must reach its tolerance, a heat request's right-hand side must be the
+output-masked mass matvec (conforming and reference operators), and a mesh
+larger than the HBM budget is refused before any device allocation."""
 import importlib.util
 import os
 import sys
@@ -69,17 +70,19 @@ def test_admission_refuses_plans_over_budget(dkt):
         (r,) = srv.drain(1, timeout=120)
     finally:
         srv.close()
-    assert r.error.startswith("MemoryError")
+    assert r.error.startswith("AdmissionError")
 
 
 @pytest.mark.gpu
-def test_heat_request_solves_mass_rhs(dkt):
+@pytest.mark.parametrize("operator", ["conforming", "reference"])
+def test_heat_request_solves_mass_rhs(dkt, operator):
     """A heat request solves K x = b with b = M f (boundary rows zeroed): check the
-    residual with the device operators, both themselves checked against the reference."""
+    residual with the device operators, both themselves checked against the
+    reference (reference operator) or the assembled numpy operator (conforming)."""
     torch = _torch()
     sv = _serve_mod()
     spec = sv.MeshSpec(3, 30, shape="sphere", center=(0.5, 0.5, 0.5), radius=0.25, max_level=5)
-    srv = sv.Server(dkt, n_streams=2)
+    srv = sv.Server(dkt, n_streams=2, operator=operator)
     try:
         r = sv.Request(spec, rhs_seed=7, kind="heat")
         srv.submit(r)
@@ -88,7 +91,8 @@ def test_heat_request_solves_mass_rhs(dkt):
         srv.close()
     assert not r.error, r.error
     da = dkt.DA(spec.tree(dkt), 1)
-    k_plan, m_plan = dkt.Plan(da, dkt.OP_STIFFNESS), dkt.Plan(da, dkt.OP_MASS)
+    conf = operator == "conforming"
+    k_plan, m_plan = dkt.Plan(da, dkt.OP_STIFFNESS, conforming=conf), dkt.Plan(da, dkt.OP_MASS, conforming=conf)
     f = torch.tensor(dkt.keyed_uniform(7, da.nodes()), device="cuda")
     b = torch.empty_like(f)
     m_plan.matvec_torch(f, b, masked=2)
@@ -127,6 +131,44 @@ def test_batched_server_equals_single_fp64_solves(dkt):
             out = alone.drain(i + 1, timeout=300)[-1]
             assert not out.error
             assert out.stats["iters"] == r.stats["iters"]
+            assert np.array_equal(out.x, r.x)
+    finally:
+        alone.close()
+
+
+@pytest.mark.gpu
+def test_batched_server_mixed_and_fp64_equal_alone(dkt):
+    """Server(batch=K) with FP64 and mixed-precision requests in one engine
+    (conforming operator): every answer is bitwise the request served alone
+    without batching (mixed: dkt_cg_solve with reliable updates), every
+    request converges to its tolerance, and a mode the engine does not run is
+    refused per request."""
+    _torch()
+    sv = _serve_mod()
+    meshes = [sv.MeshSpec(4, 8, n, s) for n, s in ((400, 5), (1200, 6))]
+    meshes.append(sv.MeshSpec(3, 30, shape="sphere", center=(0.45, 0.5, 0.55), radius=0.25, max_level=6))
+    reqs = [sv.Request(m, rhs_seed=k, kind=kind, mode=mode)
+            for k, kind, mode in ((1, "poisson", sv.CG_MIXED), (2, "heat", sv.CG_F64), (3, "poisson", sv.CG_F64),
+                                  (4, "heat", sv.CG_MIXED))
+            for m in meshes]
+    bad = sv.Request(meshes[0], rhs_seed=9, mode=sv.CG_MIXED_RESTART)
+    srv = sv.Server(dkt, n_streams=2, batch=4)
+    try:
+        for r in reqs + [bad]:
+            srv.submit(r)
+        done = srv.drain(len(reqs) + 1, timeout=300)
+    finally:
+        srv.close()
+    assert bad.error.startswith("ValueError")
+    assert all(not r.error for r in reqs), [r.error for r in reqs]
+    assert all(r.stats["converged"] and r.stats["resid"] <= r.tol for r in reqs)
+    alone = sv.Server(dkt, n_streams=1)
+    try:
+        for i, r in enumerate(reqs):
+            alone.submit(sv.Request(r.mesh, r.rhs_seed, kind=r.kind, mode=r.mode))
+            out = alone.drain(i + 1, timeout=300)[-1]
+            assert not out.error
+            assert out.stats["iters"] == r.stats["iters"] and out.stats["resid"] == r.stats["resid"]
             assert np.array_equal(out.x, r.x)
     finally:
         alone.close()
