@@ -86,6 +86,27 @@ def test_flattened_traversal_matches_reference(dkt, name, key):
     assert np.linalg.norm(y - ref) <= 1e-13 * max(np.linalg.norm(ref), 1e-300)
 
 
+@pytest.mark.parametrize("name,key", _cases())
+def test_plan_cpu_restatement_matches_reference(dkt, oracle, name, key):
+    """The CPU baseline's restatement over the flattened traversal
+    (oracle/plan_cpu.c), on 1 and 3 threads, against the reference's outputs."""
+    import importlib.util
+    spec = importlib.util.spec_from_file_location("plan_cpu", os.path.join(REPO, "oracle", "plan_cpu.py"))
+    pc = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(pc)
+    O = pc.load()
+    da, z, meta = fx.build_da(dkt, name)
+    nodes = da.nodes()
+    _, op, xs, masked = key.split("_")
+    x = fx.make_x(xs, nodes, meta["dim"])
+    tr = da.traversal()
+    ref = z[key]
+    for nt in (1, 3):
+        y = pc.matvec(O, tr, meta["dim"], meta["order"], da.n_nodes, x, op, nt,
+                      da.boundary() if masked == "masked" else None)
+        assert np.linalg.norm(y - ref) <= 1e-13 * max(np.linalg.norm(ref), 1e-300), (nt,)
+
+
 @pytest.mark.parametrize("dim,p", [(2, 1), (3, 1), (4, 1), (2, 2), (3, 2)])
 def test_stiffness_annihilates_constants(dkt, oracle, dim, p):
     maxd = 8 if p == 1 else 10
