@@ -32,6 +32,7 @@ PLAN_SIMPLE = 1
 PLAN_DRYRUN = 2
 PLAN_CONFORMING = 4
 PLAN_TILE_SHIFT = 8
+NOALLOC_SCRATCH_BYTES = 32 << 20  # DKT_NOALLOC_SCRATCH_BYTES
 CG_F64 = 0
 CG_MIXED = 1
 CG_MIXED_RESTART = 2
@@ -121,6 +122,7 @@ def _bind_device(L):
     L.dkt_plan_create.argtypes = [P, c.c_int, P, c.c_int, c.POINTER(P)]
     L.dkt_plan_destroy.argtypes = [P]
     L.dkt_plan_matvec.argtypes = [P, c.c_int, P, P, c.c_int, c.c_int, P]
+    L.dkt_plan_matvec_noalloc.argtypes = [P, c.c_int, P, P, c.c_int, c.c_int, P]
     L.dkt_plan_info.argtypes = [P, c.POINTER(c.c_double), c.c_int]
     L.dkt_plan_profile.argtypes = [P, c.c_int]
     L.dkt_plan_profile_read.argtypes = [P, c.POINTER(c.c_double), c.POINTER(c.c_double), c.POINTER(c.c_int)]
@@ -363,7 +365,13 @@ class Plan:
         _chk(load().dkt_plan_matvec(self.h, dtype, ctypes.c_void_p(x_ptr), ctypes.c_void_p(y_ptr), nrhs,
                                     int(masked), ctypes.c_void_p(stream or 0)))
 
-    def matvec_torch(self, x, y, masked=False, stream=None):
+    def matvec_noalloc(self, x_ptr, y_ptr, nrhs=1, dtype=0, masked=False, stream=None):
+        """As matvec, with the scratch in the library's fixed device array
+        (dkt_plan_matvec_noalloc): no device memory is allocated."""
+        _chk(load().dkt_plan_matvec_noalloc(self.h, dtype, ctypes.c_void_p(x_ptr), ctypes.c_void_p(y_ptr), nrhs,
+                                            int(masked), ctypes.c_void_p(stream or 0)))
+
+    def matvec_torch(self, x, y, masked=False, stream=None, noalloc=False):
         import torch
         if x.dtype != y.dtype or x.dtype not in (torch.float64, torch.float32):
             raise DktError("x and y must both be float64 or float32")
@@ -373,7 +381,8 @@ class Plan:
         if x.shape[0] != self.n_nodes or y.shape != x.shape:
             raise DktError("x/y shape does not match the plan")
         s = stream if stream is not None else torch.cuda.current_stream().cuda_stream
-        self.matvec(x.data_ptr(), y.data_ptr(), nrhs, 0 if x.dtype == torch.float64 else 1, masked, s)
+        fn = self.matvec_noalloc if noalloc else self.matvec
+        fn(x.data_ptr(), y.data_ptr(), nrhs, 0 if x.dtype == torch.float64 else 1, masked, s)
 
     def cg_solve(self, b_ptr, x_ptr, mode=CG_F64, max_iter=10000, tol=1e-8, masked=True, stream=None,
                  raise_on_breakdown=True):

@@ -1351,6 +1351,49 @@ extern "C" int dkt_plan_matvec(dkt_plan *pl, int dtype, const void *x, void *y, 
   });
 }
 
+// ---------------------------------------------------------------------------
+// Matvec without allocation: the partial sums go to a fixed device array.
+// ---------------------------------------------------------------------------
+namespace {
+// One copy per device, held until the process ends.  Host code never passes
+// `scratch` itself to a kernel (in host code the name is a host-side stand-in,
+// not a device address): it uses the address hipGetSymbolAddress returns for
+// the current device.
+__device__ __align__(256) unsigned char scratch[DKT_NOALLOC_SCRATCH_BYTES];
+
+std::mutex g_scratch_mu;
+std::vector<void *> g_scratch_dev;  // device address of `scratch`, per device (null: not looked up yet)
+
+// Address of the current device's `scratch`, looked up once per device with
+// that device current.
+void *scratch_address() {
+  int dev = -1;
+  HIP_OK(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if ((size_t)dev >= g_scratch_dev.size()) g_scratch_dev.resize(dev + 1, nullptr);
+  if (!g_scratch_dev[dev]) {
+    void *ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr)
+      throw std::runtime_error(std::string("hipGetSymbolAddress(scratch) failed: ") + hipGetErrorString(e));
+    g_scratch_dev[dev] = ptr;
+  }
+  return g_scratch_dev[dev];
+}
+}  // namespace
+
+extern "C" int dkt_plan_matvec_noalloc(dkt_plan *pl, int dtype, const void *x, void *y, int nrhs, int masked,
+                                       void *stream) {
+  return guarded([&] {
+    if (!pl || !x || !y) throw std::invalid_argument("null pointer");
+    if (pl->dry) throw std::invalid_argument("a dry-run plan cannot run a matvec");
+    if (nrhs < 1 || nrhs > 64) throw std::invalid_argument("nrhs must be 1..64");
+    if (x == y) throw std::invalid_argument("x and y must not alias");
+    const ExtScratch ext{scratch_address(), (size_t)DKT_NOALLOC_SCRATCH_BYTES};
+    plan_apply(pl, dtype, x, y, nrhs, masked, (hipStream_t)stream, nullptr, &ext);
+  });
+}
+
 // info: nodes, leaves, hanging leaves, nPe, device map bytes, tiles, shared nodes,
 // partial slots, max nodes per tile, the tile statistics, leaves per tile,
 // max constraint terms per conforming tile, max hanging leaves per tile, LDS

@@ -168,6 +168,19 @@ int dkt_plan_destroy(dkt_plan *plan);
  * boundary entries of y are zero (x is read as given). */
 int dkt_plan_matvec(dkt_plan *plan, int dtype, const void *x, void *y, int nrhs, int masked, void *stream);
 
+/* dkt_plan_matvec that allocates no device memory, for callers that cannot
+ * allocate (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  No counterpart in the reference.  Same arguments and
+ * result, bit for bit, as dkt_plan_matvec.  The matvec's scratch (the partial
+ * sums of nodes shared by several tiles; the contribution buffer of simple
+ * plans) lives in one fixed device array of DKT_NOALLOC_SCRATCH_BYTES per GPU,
+ * held until the process ends; a matvec that needs more (partial slots x
+ * columns per launch x element size) is refused with an error and nothing
+ * runs.  Every call on one GPU shares that array: calls that overlap on one
+ * GPU (other streams or threads) race and must be ordered by the caller. */
+#define DKT_NOALLOC_SCRATCH_BYTES (32u << 20)
+int dkt_plan_matvec_noalloc(dkt_plan *plan, int dtype, const void *x, void *y, int nrhs, int masked, void *stream);
+
 /* ---- Solver ------------------------------------------------------------ */
 /* Conjugate gradients on A x = b with the plan's operator, as
  * HeatMat::cgSolve (FEM/examples/src/heatMat.cpp:165-325): unpreconditioned,
