@@ -38,6 +38,7 @@ CG_MIXED = 1
 CG_MIXED_RESTART = 2
 CG_MIXED_RELIABLE = 3
 BATCH_GRAPH = 1
+BATCH_PERSISTENT = 2
 ERR_BREAKDOWN = 2
 STOP_NAMES = {1: "converged", 2: "max_iter", 3: "breakdown", 4: "stalled"}
 
@@ -127,6 +128,8 @@ def _bind_device(L):
     L.dkt_plan_profile.argtypes = [P, c.c_int]
     L.dkt_plan_profile_read.argtypes = [P, c.POINTER(c.c_double), c.POINTER(c.c_double), c.POINTER(c.c_int)]
     L.dkt_cg_solve.argtypes = [P, c.c_int, P, P, c.c_int, c.c_double, c.c_int, c.POINTER(c.c_double), P]
+    L.dkt_bicgstab_solve.argtypes = [P, P, P, c.c_int, c.c_double, c.c_int, c.POINTER(c.c_double), P]
+    L.dkt_plan_set_dirichlet.argtypes = [P, c.POINTER(c.c_uint32), c.c_size_t]
     L.dkt_cg_solve_batch.argtypes = [c.POINTER(P), c.c_int, c.POINTER(P), c.POINTER(P), c.c_int, c.c_double, c.c_int,
                                      c.POINTER(c.c_double), P]
     L.dkt_cg_engine_create.argtypes = [c.c_int, c.c_int, c.POINTER(P)]
@@ -138,6 +141,7 @@ def _bind_device(L):
                                    c.c_int, c.POINTER(P)]
     L.dkt_batch_run.argtypes = [P, P]
     L.dkt_batch_destroy.argtypes = [P]
+    L.dkt_batch_status.argtypes = [P, c.POINTER(c.c_int)]
 
 
 def _chk(rc):
@@ -411,6 +415,33 @@ class Plan:
         s = stream if stream is not None else torch.cuda.current_stream().cuda_stream
         return self.cg_solve(b.data_ptr(), x.data_ptr(), mode, max_iter, tol, masked, s, raise_on_breakdown)
 
+    def bicgstab_solve_torch(self, b, x, max_iter=10000, tol=1e-8, masked=True, stream=None,
+                             raise_on_breakdown=True):
+        """BiCGStab on the plan's operator (nonsymmetric operators, e.g. the
+        space-time heat form); x: initial guess in, solution out.  Returns the
+        same statistics as cg_solve."""
+        import torch
+        if b.dtype != torch.float64 or x.dtype != torch.float64 or b.shape != (self.n_nodes,) or x.shape != b.shape:
+            raise DktError("b and x must be float64 device vectors of the plan's node count")
+        if not (b.is_cuda and x.is_cuda and b.is_contiguous() and x.is_contiguous()):
+            raise DktError("b and x must be contiguous device tensors")
+        s = stream if stream is not None else torch.cuda.current_stream().cuda_stream
+        st = (ctypes.c_double * 5)()
+        rc = load().dkt_bicgstab_solve(self.h, ctypes.c_void_p(b.data_ptr()), ctypes.c_void_p(x.data_ptr()),
+                                       max_iter, tol, 1 if masked else 0, st, ctypes.c_void_p(s or 0))
+        if rc not in (0, ERR_BREAKDOWN):
+            _chk(rc)
+        out = dict(iters=int(st[0]), resid=st[1], refinements=0, matvecs=int(st[3]),
+                   stop=STOP_NAMES.get(int(st[4]), "unknown"))
+        if rc == ERR_BREAKDOWN and raise_on_breakdown:
+            raise CgBreakdown(load().dkt_last_error().decode(), out)
+        return out
+
+    def set_dirichlet(self, ids):
+        """Replaces the Dirichlet node set of the masked operators."""
+        ids = np.ascontiguousarray(ids, dtype=np.uint32)
+        _chk(load().dkt_plan_set_dirichlet(self.h, ids.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)), len(ids)))
+
     def __del__(self):
         if getattr(self, "h", None) and _lib is not None:
             _lib.dkt_plan_destroy(self.h)
@@ -499,7 +530,7 @@ class Batch:
     run (needs a non-default stream) and replayed afterwards.
     """
 
-    def __init__(self, plans, xs, ys, masked=False, graph=False):
+    def __init__(self, plans, xs, ys, masked=False, graph=False, persistent=False):
         import torch
         L = load()
         n = len(plans)
@@ -520,14 +551,21 @@ class Batch:
         xp = (P * n)(*[x.data_ptr() for x in xs])
         yp = (P * n)(*[y.data_ptr() for y in ys])
         h = P()
-        _chk(L.dkt_batch_create(ph, n, 0 if dt == torch.float64 else 1, nrhs, xp, yp, int(masked),
-                                BATCH_GRAPH if graph else 0, ctypes.byref(h)))
+        flags = (BATCH_GRAPH if graph else 0) | (BATCH_PERSISTENT if persistent else 0)
+        _chk(L.dkt_batch_create(ph, n, 0 if dt == torch.float64 else 1, nrhs, xp, yp, int(masked), flags,
+                                ctypes.byref(h)))
         self.h = h
 
     def run(self, stream=None):
         import torch
         s = stream if stream is not None else torch.cuda.current_stream().cuda_stream
         _chk(load().dkt_batch_run(self.h, ctypes.c_void_p(s)))
+
+    def status(self):
+        """1 when a persistent launch's fix-up gave up waiting (results invalid), else 0."""
+        err = ctypes.c_int(0)
+        _chk(load().dkt_batch_status(self.h, ctypes.byref(err)))
+        return err.value
 
     def __del__(self):
         if getattr(self, "h", None) and _lib is not None:

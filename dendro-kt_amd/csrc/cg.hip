@@ -411,6 +411,8 @@ void ensure_workspace(dkt_plan *pl, bool mixed) {
     w.part64.alloc(4 * kCgBlocks);
     HIP_OK(hipMemset(w.part64.p, 0, 4 * kCgBlocks * sizeof(double)));
     w.scal.alloc(S_COUNT);
+  }
+  if (!w.flag.p) {
     w.flag.alloc(4);
     HIP_OK(hipHostMalloc((void **)&w.host, 4 * sizeof(int), hipHostMallocDefault));
   }
@@ -729,6 +731,145 @@ __global__ void __launch_bounds__(kCgThreads) kb_status(const StatDesc *__restri
     out[3 * k] = d[k].flag[0];
     out[3 * k + 1] = d[k].flag[1];
     out[3 * k + 2] = *d[k].resid;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// BiCGStab (van der Vorst 1992) for nonsymmetric operators (the stabilised
+// space-time heat form), FP64, with the same fixed-order reductions as CG:
+// every block re-reduces the same partials, so the solve is bitwise
+// reproducible.  No counterpart in the reference's library (its heat solves
+// go through PETSc's KSP, FEM/examples/src/heatEq.cpp:153-165).
+// Scalars: B_RHO r^.r of the current residual, B_RHO0 of the previous one,
+// B_ALPHA, B_OMEGA, B_NORMB, B_RESID.
+// ---------------------------------------------------------------------------
+enum { B_RHO = 0, B_RHO0 = 1, B_ALPHA = 2, B_OMEGA = 3, B_NORMB = 4, B_RESID = 5, B_COUNT = 8 };
+
+struct BiVecs {
+  double *r, *rh, *p, *v, *s, *t, *part, *scal;
+};
+
+// r = b - ax, r^ = r, p = v = 0; partials of r.r, max|r|, max|b|.
+__global__ void __launch_bounds__(kCgThreads) kbi_init(int n, const double *__restrict__ b,
+                                                       const double *__restrict__ ax, BiVecs w) {
+  __shared__ double sh[kCgThreads];
+  double rr = 0, mr = 0, mb = 0;
+  CG_LOOP(i, n) {
+    const double ri = b[i] - ax[i];
+    w.r[i] = ri;
+    w.rh[i] = ri;
+    w.p[i] = 0.0;
+    w.v[i] = 0.0;
+    rr += ri * ri;
+    mr = fmax(mr, fabs(ri));
+    mb = fmax(mb, fabs(b[i]));
+  }
+  rr = block_sum(rr, sh);
+  mr = block_max(mr, sh);
+  mb = block_max(mb, sh);
+  if (threadIdx.x == 0) {
+    w.part[blockIdx.x] = rr;
+    w.part[kCgBlocks + blockIdx.x] = mr;
+    w.part[2 * kCgBlocks + blockIdx.x] = mb;
+  }
+}
+__global__ void __launch_bounds__(kCgThreads) kbi_init_fin(BiVecs w, double tol, int *__restrict__ flag) {
+  __shared__ double sh[kCgThreads];
+  const double rr = parts_sum(w.part, sh);
+  const double mr = parts_max(w.part + kCgBlocks, sh);
+  double mb = parts_max(w.part + 2 * kCgBlocks, sh);
+  if (threadIdx.x == 0) {
+    if (mb == 0.0) mb = 1.0;
+    w.scal[B_RHO] = rr;
+    w.scal[B_RHO0] = 1.0;
+    w.scal[B_ALPHA] = 1.0;
+    w.scal[B_OMEGA] = 1.0;
+    w.scal[B_NORMB] = mb;
+    w.scal[B_RESID] = mr / mb;
+    flag[0] = mr / mb <= tol ? kStopConverged : 0;
+    flag[1] = 0;
+  }
+}
+// p = r + beta (p - omega v), beta = (rho / rho0) (alpha / omega).
+__global__ void __launch_bounds__(kCgThreads) kbi_p(int n, BiVecs w, const int *__restrict__ flag) {
+  if (flag[0]) return;
+  const double beta = (w.scal[B_RHO] / w.scal[B_RHO0]) * (w.scal[B_ALPHA] / w.scal[B_OMEGA]);
+  const double om = w.scal[B_OMEGA];
+  CG_LOOP(i, n) w.p[i] = w.r[i] + beta * (w.p[i] - om * w.v[i]);
+}
+// partials of a.b into part + slot * kCgBlocks (and c.d into the next slot when c is given).
+__global__ void __launch_bounds__(kCgThreads) kbi_dot(int n, const double *__restrict__ a, const double *__restrict__ b,
+                                                      const double *__restrict__ c, const double *__restrict__ d,
+                                                      double *__restrict__ part, const int *__restrict__ flag) {
+  if (flag[0]) return;
+  __shared__ double sh[kCgThreads];
+  double ab = 0, cd = 0;
+  CG_LOOP(i, n) {
+    ab += a[i] * b[i];
+    if (c) cd += c[i] * d[i];
+  }
+  ab = block_sum(ab, sh);
+  if (c) cd = block_sum(cd, sh);
+  if (threadIdx.x == 0) {
+    part[blockIdx.x] = ab;
+    if (c) part[kCgBlocks + blockIdx.x] = cd;
+  }
+}
+// alpha = rho / (r^.v); s = r - alpha v.  r^.v = 0 or a non-finite alpha: breakdown.
+__global__ void __launch_bounds__(kCgThreads) kbi_s(int n, BiVecs w, int *__restrict__ flag) {
+  if (flag[0]) return;
+  __shared__ double sh[kCgThreads];
+  const double rv = parts_sum(w.part + 3 * kCgBlocks, sh);
+  const double alpha = w.scal[B_RHO] / rv;
+  if (!(rv != 0.0) || !isfinite(alpha)) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) flag[0] = kStopBreakdown;
+    return;
+  }
+  CG_LOOP(i, n) w.s[i] = w.r[i] - alpha * w.v[i];
+  if (blockIdx.x == 0 && threadIdx.x == 0) w.scal[B_ALPHA] = alpha;
+}
+// omega = (t.s) / (t.t) (0 when t = 0); x += alpha p + omega s; r = s - omega t;
+// partials of r^.r and max|r|.
+__global__ void __launch_bounds__(kCgThreads) kbi_xr(int n, double *__restrict__ x, BiVecs w,
+                                                     const int *__restrict__ flag) {
+  if (flag[0]) return;
+  __shared__ double sh[kCgThreads];
+  const double ts = parts_sum(w.part + 4 * kCgBlocks, sh);
+  const double tt = parts_sum(w.part + 5 * kCgBlocks, sh);
+  const double om = tt > 0.0 ? ts / tt : 0.0;
+  const double alpha = w.scal[B_ALPHA];
+  double rr = 0, mr = 0;
+  CG_LOOP(i, n) {
+    x[i] += alpha * w.p[i] + om * w.s[i];
+    const double ri = w.s[i] - om * w.t[i];
+    w.r[i] = ri;
+    rr += w.rh[i] * ri;
+    mr = fmax(mr, fabs(ri));
+  }
+  rr = block_sum(rr, sh);
+  mr = block_max(mr, sh);
+  if (threadIdx.x == 0) {
+    w.part[blockIdx.x] = rr;
+    w.part[kCgBlocks + blockIdx.x] = mr;
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) w.scal[B_OMEGA] = om;
+}
+// rho0 = rho, rho = r^.r, the residual and the stop test; omega = 0 or rho =
+// 0 with the residual above tol: breakdown (the next beta is undefined).
+__global__ void __launch_bounds__(kCgThreads) kbi_fin(BiVecs w, double tol, int max_iter, int *__restrict__ flag) {
+  if (flag[0]) return;
+  __shared__ double sh[kCgThreads];
+  const double rho = parts_sum(w.part, sh);
+  const double mr = parts_max(w.part + kCgBlocks, sh);
+  if (threadIdx.x == 0) {
+    const double resid = mr / w.scal[B_NORMB];
+    w.scal[B_RHO0] = w.scal[B_RHO];
+    w.scal[B_RHO] = rho;
+    w.scal[B_RESID] = resid;
+    flag[1] += 1;
+    if (resid <= tol) flag[0] = kStopConverged;
+    else if (!isfinite(resid) || rho == 0.0 || w.scal[B_OMEGA] == 0.0) flag[0] = kStopBreakdown;
+    else if (flag[1] >= max_iter) flag[0] = kStopMaxIter;
   }
 }
 
@@ -1135,4 +1276,77 @@ extern "C" int dkt_cg_solve_batch(dkt_plan *const *plans, int nplans, const doub
       left -= nf;
     }
   });
+}
+
+extern "C" int dkt_bicgstab_solve(dkt_plan *pl, const double *b, double *x, int max_iter, double tol, int masked,
+                                  double *stats, void *stream) {
+  int stop = 0;
+  const int rc = guarded([&] {
+    if (!pl || !b || !x) throw std::invalid_argument("null pointer");
+    if (pl->dry) throw std::invalid_argument("a dry-run plan cannot solve");
+    if (max_iter < 0 || !(tol >= 0.0)) throw std::invalid_argument("max_iter >= 0 and tol >= 0 required");
+    if ((const void *)b == (const void *)x) throw std::invalid_argument("b and x must not alias");
+    hipStream_t st = (hipStream_t)stream;
+    const int n = (int)pl->N;
+    CgWorkspace &w = pl->cg;
+    if (w.n != pl->N) {
+      w.release();
+      w.n = pl->N;
+    }
+    if (!w.bi.p) {
+      w.bi.alloc(6 * pl->N);
+      w.bipart.alloc(6 * kCgBlocks);
+      HIP_OK(hipMemset(w.bipart.p, 0, 6 * kCgBlocks * sizeof(double)));
+      w.biscal.alloc(B_COUNT);
+    }
+    if (!w.flag.p) {
+      w.flag.alloc(4);
+      HIP_OK(hipHostMalloc((void **)&w.host, 4 * sizeof(int), hipHostMallocDefault));
+    }
+    double *base = w.bi.p;
+    BiVecs v{base, base + pl->N, base + 2 * pl->N, base + 3 * pl->N, base + 4 * pl->N, base + 5 * pl->N,
+             w.bipart.p, w.biscal.p};
+    int *flag = w.flag.p, *host = w.host;
+    const int mask = masked ? 1 : 0;
+    const dim3 grid(cg_blocks(n)), blk(kCgThreads);
+    plan_apply(pl, 0, x, v.v, 1, mask, st, nullptr);
+    hipLaunchKernelGGL(kbi_init, grid, blk, 0, st, n, b, v.v, v);
+    hipLaunchKernelGGL(kbi_init_fin, dim3(1), blk, 0, st, v, tol, flag);
+    HIP_OK(hipGetLastError());
+    HIP_OK(hipMemcpyAsync(host, flag, 2 * sizeof(int), hipMemcpyDeviceToHost, st));
+    HIP_OK(hipStreamSynchronize(st));
+    int it = 0;
+    stop = host[0] ? kStopConverged : (max_iter == 0 ? kStopMaxIter : 0);
+    while (!stop) {
+      for (int c = 0; c < kCgChunk; c++) {
+        hipLaunchKernelGGL(kbi_p, grid, blk, 0, st, n, v, flag);
+        plan_apply(pl, 0, v.p, v.v, 1, mask, st, flag);
+        hipLaunchKernelGGL(kbi_dot, grid, blk, 0, st, n, v.rh, v.v, (const double *)nullptr,
+                           (const double *)nullptr, v.part + 3 * kCgBlocks, flag);
+        hipLaunchKernelGGL(kbi_s, grid, blk, 0, st, n, v, flag);
+        plan_apply(pl, 0, v.s, v.t, 1, mask, st, flag);
+        hipLaunchKernelGGL(kbi_dot, grid, blk, 0, st, n, v.t, v.s, v.t, v.t, v.part + 4 * kCgBlocks, flag);
+        hipLaunchKernelGGL(kbi_xr, grid, blk, 0, st, n, x, v, flag);
+        hipLaunchKernelGGL(kbi_fin, dim3(1), blk, 0, st, v, tol, max_iter, flag);
+        HIP_OK(hipGetLastError());
+      }
+      HIP_OK(hipMemcpyAsync(host, flag, 2 * sizeof(int), hipMemcpyDeviceToHost, st));
+      HIP_OK(hipStreamSynchronize(st));
+      it = host[1];
+      stop = host[0];
+    }
+    const double resid = read_scalar(w.biscal.p, B_RESID, st);
+    if (stats) {
+      stats[0] = it;
+      stats[1] = resid;
+      stats[2] = 0;
+      stats[3] = 1 + 2 * it;
+      stats[4] = stop;
+    }
+  });
+  if (rc == 0 && stop == kStopBreakdown) {
+    g_last_error = "BiCGStab breakdown: r^.v = 0, rho = 0 or omega = 0; x holds the last iterate";
+    return DKT_ERR_BREAKDOWN;
+  }
+  return rc;
 }

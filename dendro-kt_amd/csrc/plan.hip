@@ -1394,6 +1394,19 @@ extern "C" int dkt_plan_matvec_noalloc(dkt_plan *pl, int dtype, const void *x, v
   });
 }
 
+extern "C" int dkt_plan_set_dirichlet(dkt_plan *pl, const uint32_t *ids, size_t n) {
+  return guarded([&] {
+    if (!pl || (n && !ids)) throw std::invalid_argument("null pointer");
+    std::vector<uint8_t> dir(pl->N, 0);
+    for (size_t i = 0; i < n; i++) {
+      if (ids[i] >= pl->N) throw std::invalid_argument("Dirichlet node id out of range");
+      dir[ids[i]] = 1;
+    }
+    if (!pl->dry) HIP_OK(hipDeviceSynchronize());  // no matvec in flight reads the old set
+    pl->dirichlet.upload(dir);
+  });
+}
+
 // info: nodes, leaves, hanging leaves, nPe, device map bytes, tiles, shared nodes,
 // partial slots, max nodes per tile, the tile statistics, leaves per tile,
 // max constraint terms per conforming tile, max hanging leaves per tile, LDS

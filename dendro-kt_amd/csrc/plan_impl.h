@@ -13,9 +13,10 @@ struct CgWorkspace {
   dkt::DevBuf<double> r, p, ap, d64, part64, scal, part32, scal32;  // part32/scal32: reductions of the FP32 solve
   dkt::DevBuf<float> r32, p32, ap32, x32, b32;
   dkt::DevBuf<int> flag;  // [0] cancel (converged), [1] iterations done
+  dkt::DevBuf<double> bi, bipart, biscal;  // BiCGStab: r, r^, p, v, s, t; partials; scalars
   int *host = nullptr;    // pinned copy of flag
   void release() {
-    for (auto *b : {&r, &p, &ap, &d64, &part64, &scal, &part32, &scal32}) b->release();
+    for (auto *b : {&r, &p, &ap, &d64, &part64, &scal, &part32, &scal32, &bi, &bipart, &biscal}) b->release();
     for (auto *b : {&r32, &p32, &ap32, &x32, &b32}) b->release();
     flag.release();
     if (host) (void)hipHostFree(host);

@@ -181,6 +181,16 @@ int dkt_plan_matvec(dkt_plan *plan, int dtype, const void *x, void *y, int nrhs,
 #define DKT_NOALLOC_SCRATCH_BYTES (32u << 20)
 int dkt_plan_matvec_noalloc(dkt_plan *plan, int dtype, const void *x, void *y, int nrhs, int masked, void *stream);
 
+/* Replaces the plan's Dirichlet node set (the nodes the masked matvecs and
+ * the solvers' masked operators zero; dkt_plan_create takes the DA's boundary
+ * nodes, DA::getBoundaryNodeIndices, as HeatMat's hooks do) with ids[0..n).
+ * For operators whose essential boundary is part of the domain boundary, such
+ * as the space-time heat form (spatial boundary and initial time only).  Waits
+ * for the device to finish work that may read the old set.  No counterpart in
+ * the reference (its hooks are the leaf class's preMatVec/postMatVec,
+ * FEM/include/feMatrix.h:91-121). */
+int dkt_plan_set_dirichlet(dkt_plan *plan, const uint32_t *ids, size_t n);
+
 /* ---- Solver ------------------------------------------------------------ */
 /* Conjugate gradients on A x = b with the plan's operator, as
  * HeatMat::cgSolve (FEM/examples/src/heatMat.cpp:165-325): unpreconditioned,
@@ -219,6 +229,17 @@ int dkt_plan_matvec_noalloc(dkt_plan *plan, int dtype, const void *x, void *y, i
 #define DKT_CG_MIXED_RELIABLE 3
 int dkt_cg_solve(dkt_plan *plan, int mode, const double *b, double *x, int max_iter, double tol, int masked,
                  double *stats, void *stream);
+
+/* BiCGStab (van der Vorst) on A x = b for nonsymmetric operators, FP64, e.g.
+ * the stabilised space-time heat form of a DKT_OP_CAPTURED plan.  Same
+ * arguments, stop rule (relative L_inf residual), masks, stats layout and
+ * status codes as dkt_cg_solve (stats[3]: 1 + 2 matvecs per iteration);
+ * breakdown (r^.v = 0, rho = 0, omega = 0) returns DKT_ERR_BREAKDOWN.
+ * Reductions run in a fixed order (bitwise reproducible).  No counterpart in
+ * the reference library (its heat driver hands the operator to PETSc's KSP,
+ * FEM/examples/src/heatEq.cpp:153-165). */
+int dkt_bicgstab_solve(dkt_plan *plan, const double *b, double *x, int max_iter, double tol, int masked,
+                       double *stats, void *stream);
 
 /* Batched FP64 conjugate gradients: solves A_k x_k = b_k for nplans tiled
  * plans (any mix of (dim, p), reference or conforming; no simple or
@@ -283,9 +304,22 @@ int dkt_cg_engine_destroy(dkt_cg_engine *engine);
  * call, FEM/include/matvec.h:231-240). */
 typedef struct dkt_batch dkt_batch;
 #define DKT_BATCH_GRAPH 1
+/* DKT_BATCH_PERSISTENT: one persistent launch per run for the whole batch,
+ * any mix of (dim, p): the tiles of every plan (work claimed from eight
+ * SFC-contiguous queues, one per XCD) and then the fix-up of their shared
+ * nodes, each fix-up block waiting on its plan's tile-completion counter; the
+ * kernel resets its counters on exit, so with DKT_BATCH_GRAPH a replay is one
+ * graph node.  Tiled reference-semantics plans only.  Switched off:
+ * dkt_batch_create refuses the flag (its launch did not finish on the device
+ * and the cause is not found). */
+#define DKT_BATCH_PERSISTENT 2
 int dkt_batch_create(dkt_plan *const *plans, int nplans, int dtype, int nrhs, const void *const *x, void *const *y,
                      int masked, int flags, dkt_batch **out);
 int dkt_batch_run(dkt_batch *batch, void *stream);
+/* Persistent batches: *err = 1 when a fix-up block gave up waiting for its
+ * plan's tiles (a bounded spin; the results of that run are not valid), else 0.
+ * Synchronizes with the device. */
+int dkt_batch_status(dkt_batch *batch, int *err);
 int dkt_batch_destroy(dkt_batch *batch);
 
 /* info[0..8] = nodes, leaves, hanging leaves, nodes per element, device map

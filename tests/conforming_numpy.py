@@ -64,14 +64,17 @@ def constraint_rows(cells, nodes, dim, p, maxd):
     return rows
 
 
-def assemble(cells, nodes, dim, p, maxd, op="poisson"):
-    """Dense conforming operator over the mesh elements (small meshes only)."""
+def assemble(cells, nodes, dim, p, maxd, op="poisson", level_mats=None):
+    """Dense conforming operator over the mesh elements (small meshes only);
+    level_mats[level] replaces the element matrix (e.g. a nonsymmetric form)."""
     K, hp = pn.ref_matrix(dim, p, op)
     n = nodes.shape[0]
     A = np.zeros((n, n))
     rows = constraint_rows(cells, nodes, dim, p, maxd)
     for e, row in enumerate(rows):
         lev = int(cells[e, dim])
+        if level_mats is not None:
+            K, hp = level_mats[lev], None
         scale = 1.0 if hp is None else (2.0 ** (-float(lev))) ** hp
         C = np.zeros((len(row), n))
         for r, comb in enumerate(row):
