@@ -10,6 +10,7 @@
 // result is bitwise the same as its own dkt_plan_matvec: the tile work and
 // the fix-up order are the plan's.
 #include <algorithm>
+#include <functional>
 #include <map>
 #include <memory>
 #include <tuple>
@@ -273,6 +274,279 @@ struct ConfGroup : GroupBase {
   }
 };
 
+// ---------------------------------------------------------------------------
+// One persistent launch for a whole batch (DKT_BATCH_PERSISTENT): the tiles of
+// every plan and column chunk, any mix of (dim, p), then the fix-up of their
+// shared nodes, in a single kernel.
+//
+// Work is claimed with global counters: tile work comes from eight queues, one
+// contiguous run of tiles per queue, and workgroup b takes from queue b % 8
+// first (workgroups are dispatched to the eight XCDs round-robin, so each XCD's
+// L2 sees one SFC-contiguous run, as xcd_tile arranges for the plain
+// launches), then from the other queues in turn.  A workgroup moves on to the
+// fix-up blocks only when every queue is empty, so every tile is then claimed
+// by a running workgroup; a fix-up block of a (plan, chunk) record waits until
+// every tile of that record has finished (a per-record completion counter,
+// released by each tile after its write-out), a wait that ends once the tiles
+// in flight finish.  The wait polls the counter with relaxed atomic loads and
+// acquires once when it is complete: an acquiring load per poll would
+// invalidate the L2 on every poll and starve the very tiles it waits for.  A
+// bounded wait records an error flag instead of hanging.  The last workgroup
+// to leave resets every counter, so the next launch (a graph replay) starts
+// from zero with no memset node.
+// Each plan's result is bitwise its own dkt_plan_matvec (same tile work, same
+// fix-up order).
+// ---------------------------------------------------------------------------
+constexpr int kQueues = 8;
+constexpr long long kSpinLimit = 1ll << 20;  // polls (about a microsecond each) before a wait gives up
+
+struct PersistCtl {
+  unsigned int next[kQueues];
+  unsigned int fix_next;
+  unsigned int exited;
+  unsigned int err;
+};
+
+template <typename T>
+struct PersistRec {
+  BatchPlan<T> bp;
+  int kind;  // 0: (2,1) 1: (3,1) 2: (4,1) 3: (2,2) 4: (3,2)
+  int ntiles;
+};
+
+// The tile bodies a persistent kernel carries: bit k of KINDS for kind k.  A
+// kernel's registers are those of its largest body, so a batch runs the
+// smallest instantiation that covers its kinds.
+constexpr int kKindsAll = 31, kKinds3D = (1 << 1) | (1 << 4);
+
+template <typename T, bool ONE, int KINDS>
+__device__ __forceinline__ void persist_tile(const PersistRec<T> &rec, int t) {
+  const BatchPlan<T> &bp = rec.bp;
+  if ((KINDS & 1) && rec.kind == 0) tile_body<T, 2, 1, ONE>(bp.A, t, bp.kref, bp.carry_w, bp.x, bp.y, bp.partial);
+  if ((KINDS & 2) && rec.kind == 1) tile_body<T, 3, 1, ONE>(bp.A, t, bp.kref, bp.carry_w, bp.x, bp.y, bp.partial);
+  if ((KINDS & 4) && rec.kind == 2) tile_body<T, 4, 1, ONE>(bp.A, t, bp.kref, bp.carry_w, bp.x, bp.y, bp.partial);
+  if ((KINDS & 8) && rec.kind == 3) tile_body<T, 2, 2, ONE>(bp.A, t, bp.kref, bp.carry_w, bp.x, bp.y, bp.partial);
+  if ((KINDS & 16) && rec.kind == 4) tile_body<T, 3, 2, ONE>(bp.A, t, bp.kref, bp.carry_w, bp.x, bp.y, bp.partial);
+}
+
+template <typename T, bool ONE, int KINDS>
+__global__ void __launch_bounds__(kTileThreads) tile_kernel_persistent(
+    const PersistRec<T> *__restrict__ recs, const int *__restrict__ tile_start, const int *__restrict__ fix_start,
+    int nrec, int ntiles, int nfix, unsigned int *__restrict__ done, PersistCtl *__restrict__ ctl) {
+  __shared__ int s_item;
+  __shared__ int s_ok;
+  for (int qi = 0; qi < kQueues; qi++) {
+    const int q = (blockIdx.x + qi) % kQueues;  // own queue first, then the others
+    const int lo = (int)((long long)ntiles * q / kQueues), hi = (int)((long long)ntiles * (q + 1) / kQueues);
+  for (;;) {
+    if (threadIdx.x == 0) s_item = lo + (int)atomicAdd(&ctl->next[q], 1u);
+    __syncthreads();
+    const int t = s_item;
+    __syncthreads();
+    if (t >= hi) break;
+    const int r = find_range(tile_start, nrec, t);
+    const PersistRec<T> &rec = recs[r];
+    if (!(rec.bp.A.cancel && *rec.bp.A.cancel)) persist_tile<T, ONE, KINDS>(rec, t - tile_start[r]);
+    __syncthreads();  // every work-item's write-out is issued
+    if (threadIdx.x == 0) {
+      __threadfence();
+      atomicAdd(&done[r], 1u);
+    }
+  }
+  }
+  for (;;) {
+    if (threadIdx.x == 0) s_item = (int)atomicAdd(&ctl->fix_next, 1u);
+    __syncthreads();
+    const int f = s_item;
+    __syncthreads();
+    if (f >= nfix) break;
+    const int r = find_range(fix_start, nrec, f);
+    const PersistRec<T> &rec = recs[r];
+    if (threadIdx.x == 0) {
+      long long spins = 0;
+      int ok = 1;
+      const unsigned int need = (unsigned int)rec.ntiles;
+      while (__hip_atomic_load(&done[r], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < need) {
+        if (++spins > kSpinLimit) {
+          atomicOr(&ctl->err, 1u);
+          ok = 0;
+          break;
+        }
+        __builtin_amdgcn_s_sleep(8);
+      }
+      __atomic_thread_fence(__ATOMIC_ACQUIRE);  // the record's partials are visible from here on
+      s_ok = ok;
+    }
+    __syncthreads();
+    const BatchPlan<T> &bp = rec.bp;
+    if (!s_ok || (bp.A.cancel && *bp.A.cancel)) continue;
+    const int nq = bp.A.nq, ld = bp.A.ld;
+    for (int k = threadIdx.x; k < 256; k += blockDim.x) {
+      const int i = (f - fix_start[r]) * 256 + k;
+      if (i >= bp.nshared) break;
+      const int n = bp.snode[i];
+      const int a = bp.soff[i], b = bp.soff[i + 1];
+      const bool z = bp.A.dir_out && bp.A.dir_out[n];
+      for (int qq = 0; qq < nq; qq++) bp.y[(size_t)n * ld + qq] = z ? T(0) : sum_partials(bp.partial, a, b, nq, qq);
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __threadfence();
+    if (atomicAdd(&ctl->exited, 1u) == gridDim.x - 1) {  // the last workgroup: reset for the next launch
+      for (int i = 0; i < kQueues; i++) ctl->next[i] = 0;
+      ctl->fix_next = 0;
+      for (int i = 0; i < nrec; i++) done[i] = 0;
+      __threadfence();
+      ctl->exited = 0;
+    }
+  }
+}
+
+int plan_kind(const dkt_plan *pl) {
+  if (pl->order == 1) return pl->dim - 2;
+  if (pl->dim == 2) return 3;
+  if (pl->dim == 3) return 4;
+  throw std::invalid_argument("unsupported (dim, order)");
+}
+
+template <typename T>
+struct PersistGroup : GroupBase {
+  DevBuf<PersistRec<T>> recs;
+  DevBuf<int> tile_start, fix_start;
+  DevBuf<unsigned int> done;
+  DevBuf<PersistCtl> ctl;
+  DevBuf<T> partial;
+  int nrec = 0, ntiles = 0, nfix = 0, threads = 0, grid = 0, kinds = 0;
+  bool one = false;
+  size_t lds = 0;
+  using KernelFn = void (*)(const PersistRec<T> *, const int *, const int *, int, int, int, unsigned int *,
+                            PersistCtl *);
+  KernelFn fn = nullptr;
+
+  template <bool ONE>
+  KernelFn pick() {
+    switch (kinds) {
+      case 1: return tile_kernel_persistent<T, ONE, 1>;
+      case 2: return tile_kernel_persistent<T, ONE, 2>;
+      case 4: return tile_kernel_persistent<T, ONE, 4>;
+      case 8: return tile_kernel_persistent<T, ONE, 8>;
+      case 16: return tile_kernel_persistent<T, ONE, 16>;
+      default:
+        if ((kinds & ~kKinds3D) == 0) return tile_kernel_persistent<T, ONE, kKinds3D>;
+        return tile_kernel_persistent<T, ONE, kKindsAll>;
+    }
+  }
+
+  PersistGroup(const std::vector<dkt_plan *> &pls, const std::vector<const void *> &xs, const std::vector<void *> &ys,
+               int nrhs, int masked) {
+    const bool f64 = sizeof(T) == 8;
+    auto dims_of = [](const dkt_plan *pl) {
+      return TileDims{pl->max_nodes, pl->max_grp, pl->max_he, pl->max_rd, pl->max_cr, pl->tile_leaves, pl->max_hang};
+    };
+    auto lds_of = [&](const dkt_plan *pl, int nq) {
+      const TileDims D = dims_of(pl);
+      switch (plan_kind(pl)) {
+        case 0: return tile_lds_bytes<T, 4>(D, nq);
+        case 1: return tile_lds_bytes<T, 8>(D, nq);
+        case 2: return tile_lds_bytes<T, 16>(D, nq);
+        case 3: return tile_lds_bytes<T, 9>(D, nq);
+        default: return tile_lds_bytes<T, 27>(D, nq);
+      }
+    };
+    // Columns per launch: the most that every plan's tiles fit in LDS; one
+    // column runs the single-column tile bodies.
+    int chunk = nrhs;
+    for (dkt_plan *pl : pls) {
+      if (lds_of(pl, 1) > kLdsLimit) throw std::invalid_argument("tile does not fit in LDS");
+      while (chunk > 1 && lds_of(pl, chunk) > kLdsLimit) chunk--;
+    }
+    one = chunk == 1;
+    std::vector<PersistRec<T>> hr;
+    std::vector<int> ts{0}, fs{0};
+    std::vector<size_t> poff;
+    size_t ptot = 0;
+    for (int q0 = 0; q0 < nrhs; q0 += chunk) {
+      const int nq = std::min(chunk, nrhs - q0);
+      if (nq != chunk && !one) {
+        // a short last chunk runs with the multi-column body too (nq is a runtime argument there)
+      }
+      for (size_t k = 0; k < pls.size(); k++) {
+        dkt_plan *pl = pls[k];
+        threads = std::max(threads, pl->tile_leaves);
+        lds = std::max(lds, lds_of(pl, nq));
+        PersistRec<T> r{};
+        TileArgs &A = r.bp.A;
+        A.tiles = pl->tiles.p;
+        A.tile_nodes = pl->tile_nodes.p;
+        A.tile_out = pl->tile_out.p;
+        A.leaf_loc = pl->leaf_loc.p;
+        A.leaf_aux = pl->leaf_aux.p;
+        A.group_ploc = pl->group_ploc.p;
+        A.hentries = pl->hentries.p;
+        A.readers = pl->readers.p;
+        A.blocks = pl->blocks.p;
+        A.carry_key = pl->carry_key.p;
+        A.dir_in = masked == 1 ? pl->dirichlet.p : nullptr;
+        A.dir_out = masked ? pl->dirichlet.p : nullptr;
+        A.cancel = nullptr;
+        A.op = pl->op;
+        A.hp = pl->hp;
+        A.nq = nq;
+        A.ld = nrhs;
+        A.dims = dims_of(pl);
+        r.bp.kref = pl->order == 2 ? (f64 ? (const T *)pl->kfac64.p : (const T *)pl->kfac32.p)
+                                   : (f64 ? (const T *)pl->kmat64.p : (const T *)pl->kmat32.p);
+        r.bp.carry_w = f64 ? (const T *)pl->carry_w64.p : (const T *)pl->carry_w32.p;
+        r.bp.x = (const T *)xs[k] + q0;
+        r.bp.y = (T *)ys[k] + q0;
+        poff.push_back(ptot);
+        ptot += pl->npartial * (size_t)nq;
+        r.bp.snode = pl->snode.p;
+        r.bp.soff = pl->soff.p;
+        r.bp.nshared = (int)pl->nshared;
+        r.kind = plan_kind(pl);
+        kinds |= 1 << r.kind;
+        r.ntiles = (int)pl->ntiles;
+        hr.push_back(r);
+        ts.push_back(ts.back() + (int)pl->ntiles);
+        fs.push_back(fs.back() + (int)((pl->nshared + 255) / 256));
+      }
+    }
+    partial.alloc(std::max<size_t>(1, ptot));
+    for (size_t i = 0; i < hr.size(); i++) hr[i].bp.partial = partial.p + poff[i];
+    nrec = (int)hr.size();
+    ntiles = ts.back();
+    nfix = fs.back();
+    recs.upload(hr);
+    tile_start.upload(ts);
+    fix_start.upload(fs);
+    done.upload(std::vector<unsigned int>(nrec, 0u));
+    ctl.upload(std::vector<PersistCtl>(1, PersistCtl{}));
+    // Persistent grid: as many workgroups as fit on the device at once (at
+    // least one per queue), never more than there is work.
+    int dev = 0, cus = 0, per_cu = 0;
+    HIP_OK(hipGetDevice(&dev));
+    HIP_OK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    if (lds > kLdsLimit) throw std::invalid_argument("tile does not fit in LDS");
+    fn = one ? pick<true>() : pick<false>();
+    HIP_OK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, threads, lds));
+    grid = std::max(kQueues, std::min(std::max(1, cus * std::max(1, per_cu)), std::max(ntiles + nfix, kQueues)));
+  }
+
+  void launch(hipStream_t st) override {
+    hipLaunchKernelGGL(fn, dim3(grid), dim3(threads), lds, st, (const PersistRec<T> *)recs.p,
+                       (const int *)tile_start.p, (const int *)fix_start.p, nrec, ntiles, nfix, done.p, ctl.p);
+    HIP_OK(hipGetLastError());
+  }
+
+  unsigned int error_flag() {
+    PersistCtl h{};
+    HIP_OK(hipMemcpy(&h, ctl.p, sizeof(h), hipMemcpyDeviceToHost));
+    return h.err;
+  }
+};
+
 template <typename T>
 GroupBase *make_group(bool conforming, int dim, int p, const std::vector<dkt_plan *> &pls,
                       const std::vector<const void *> &xs, const std::vector<void *> &ys, int nrhs, int masked,
@@ -336,6 +610,7 @@ std::unique_ptr<MatvecLauncher> dkt::batch_matvec(int dtype, const std::vector<d
 
 struct dkt_batch {
   std::vector<std::unique_ptr<GroupBase>> groups;
+  std::function<unsigned int()> error_flag;  // persistent launches: the spin-limit flag
   bool use_graph = false;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -357,6 +632,30 @@ extern "C" int dkt_batch_create(dkt_plan *const *plans, int nplans, int dtype, i
       if (x[k] == (const void *)y[k]) throw std::invalid_argument("x and y must not alias");
     }
     auto b = std::make_unique<dkt_batch>();
+    if (flags & DKT_BATCH_PERSISTENT)
+      throw std::invalid_argument(
+          "DKT_BATCH_PERSISTENT is switched off: its launch did not finish on the device and the cause is not "
+          "found; use the per-(dim, p) launches (with DKT_BATCH_GRAPH for a replayed graph)");
+    if (flags & DKT_BATCH_PERSISTENT) {
+      std::vector<dkt_plan *> pls(plans, plans + nplans);
+      for (dkt_plan *pl : pls)
+        if (pl->dry || pl->simple || pl->per_level || pl->conforming)
+          throw std::invalid_argument("a persistent batch runs tiled reference-semantics plans");
+      std::vector<const void *> xs(x, x + nplans);
+      std::vector<void *> ys(y, y + nplans);
+      if (dtype == 0) {
+        auto *g = new PersistGroup<double>(pls, xs, ys, nrhs, masked);
+        b->error_flag = [g] { return g->error_flag(); };
+        b->groups.emplace_back(g);
+      } else {
+        auto *g = new PersistGroup<float>(pls, xs, ys, nrhs, masked);
+        b->error_flag = [g] { return g->error_flag(); };
+        b->groups.emplace_back(g);
+      }
+      b->use_graph = (flags & DKT_BATCH_GRAPH) != 0;
+      *out = b.release();
+      return;
+    }
     for (const auto &kv : by_kind_of(plans, nplans)) {
       std::vector<dkt_plan *> pls;
       std::vector<const void *> xs;
@@ -405,4 +704,11 @@ extern "C" int dkt_batch_run(dkt_batch *b, void *stream) {
 extern "C" int dkt_batch_destroy(dkt_batch *b) {
   delete b;
   return 0;
+}
+
+extern "C" int dkt_batch_status(dkt_batch *b, int *err) {
+  return guarded([&] {
+    if (!b || !err) throw std::invalid_argument("null pointer");
+    *err = b->error_flag ? (int)b->error_flag() : 0;
+  });
 }

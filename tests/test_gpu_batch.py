@@ -83,6 +83,47 @@ def test_batch_graph_replays(dkt):
 
 
 @pytest.mark.gpu
+@pytest.mark.skip(reason="DKT_BATCH_PERSISTENT is switched off: its launch did not finish on the device (the run "
+                         "was stopped for silence, twice) and the cause is not found")
+@pytest.mark.parametrize("dtype,nrhs,graph", [("f64", 1, True), ("f32", 8, True), ("f64", 3, False)])
+def test_persistent_batch_matches_per_plan(dkt, dtype, nrhs, graph):
+    """DKT_BATCH_PERSISTENT: one persistent launch for every (dim, p) of the
+    batch with the fix-up fused (2D/3D/4D, p = 1 and 2, several tile sizes):
+    each run, eager or replayed from a one-node graph, is bitwise every plan's
+    own matvec; the launch resets its own counters between runs (replays with
+    eager work and other inputs in between) and reports no spin timeout."""
+    torch = _torch()
+    dt = torch.float32 if dtype == "f32" else torch.float64
+    das = _meshes(dkt, 10, 17)
+    plans = [dkt.Plan(da, dkt.OP_STIFFNESS, tile_leaves=(64, 128, 256)[k % 3]) for k, da in enumerate(das)]
+    xs, ys = [], []
+    for k, da in enumerate(das):
+        x = torch.tensor(np.stack([dkt.keyed_uniform(k * 16 + q, da.nodes()) for q in range(nrhs)], axis=1),
+                         dtype=dt, device="cuda").contiguous()
+        xs.append(x if nrhs > 1 else x[:, 0].contiguous())
+        ys.append(torch.zeros_like(xs[-1]))
+    s = torch.cuda.Stream()
+    b = dkt.Batch(plans, xs, ys, masked=True, graph=graph, persistent=True)
+    for rep in range(4):
+        for x in xs:  # other inputs every run
+            x.mul_(-1.0 if rep % 2 else 1.0)
+        refs = []
+        for p, x in zip(plans, xs):
+            r = torch.empty_like(x)
+            p.matvec_torch(x, r, masked=True)
+            refs.append(r)
+        torch.cuda.synchronize()
+        with torch.cuda.stream(s):
+            for y in ys:
+                y.fill_(float("nan"))
+            b.run(s.cuda_stream)
+        s.synchronize()
+        assert b.status() == 0
+        for k, (y, ref) in enumerate(zip(ys, refs)):
+            assert torch.equal(y, ref), (rep, k)
+
+
+@pytest.mark.gpu
 def test_autotuner_picks_a_candidate_and_caches(dkt):
     """The autotuner times every candidate tile size that fits, returns the fastest
     per precision (for one and for eight columns), and answers again from its cache."""
