@@ -209,7 +209,8 @@ def batch_leg(dkt, torch, ntrees):
     counts log-uniform in [1e3, 2e5], each refined around a random sphere with
     centre U[0.2,0.8]^3 and radius U[0.1,0.3]), p = 1 and 2 alternating, 8
     right-hand sides each, FP32, tile sizes per (dim, p) from the autotuner, one
-    hipGraph replay per step against the same plans launched one by one."""
+    hipGraph replay per step (one launch per (dim, p) plus its fix-up), against the
+    same plans launched one by one."""
     wl = _pkg_mod("dkt_workloads", "workloads.py")
     at = _pkg_mod("dkt_autotune", "autotune.py")
     t0 = time.time()
@@ -257,14 +258,15 @@ def batch_leg(dkt, torch, ntrees):
 
 
 def serve_leg(dkt, nreq, rates=(50, 200, 800)):
-    """Served CG under Poisson arrivals: a pool of 10 meshes (workloads.serve_pool:
+    """Served solves under Poisson arrivals: a pool of 10 meshes (workloads.serve_pool:
     70% 3D trees refined around random spheres, 30% 4D Gaussian-mixture trees, p = 1,
-    element counts log-uniform in [1e3, 1e6]), requests alternating Poisson and
-    heat-equation kinds (heat: b = M f with HeatVec's hooks, then CG), tol 1e-8
-    (relative L_inf), the conforming operator, an HBM admission budget of 200 GB,
-    plans built before the timed runs.  Continuous batching (2 workers of 8 slots)
-    with mixed-precision CG (FP32 iteration, FP64 reliable updates) and with FP64
-    CG, each at every rate; latency runs from arrival to solution on the host."""
+    element counts log-uniform in [1e3, 1e6]), requests alternating Poisson and heat
+    kinds (3D heat: b = M f with HeatVec's hooks, then CG; 4D heat: the stabilised
+    space-time heat form, BiCGStab in FP64 on a direct worker), tol 1e-8 (relative
+    L_inf), the conforming operator, an HBM admission budget of 200 GB, plans built
+    before the timed runs.  Continuous batching (2 workers of 8 slots) with
+    mixed-precision CG (FP32 iteration, FP64 reliable updates) and with FP64 CG,
+    each at every rate on the same request list; latency runs from arrival to solution on the host."""
     sv = _pkg_mod("dkt_serve", "serve.py")
     wl = _pkg_mod("dkt_workloads", "workloads.py")
     pool = [spec for spec, _ in wl.serve_pool(sv, 10, 1e3, 1e6, seed=0)]
@@ -274,20 +276,27 @@ def serve_leg(dkt, nreq, rates=(50, 200, 800)):
     t0 = time.perf_counter()
     elems, nodes = [], []
     for m in pool:
-        lease, _ = cache.acquire(m)
-        elems.append(lease.da.n_elements)
-        nodes.append(lease.da.n_nodes)
-        lease.release()
+        for kind in (("poisson",) if m.dim == 3 else ("poisson", "spacetime")):
+            lease, _ = cache.acquire(m, kind)
+            if kind == "poisson":
+                elems.append(lease.da.n_elements)
+                nodes.append(lease.da.n_nodes)
+            lease.release()
     out["plan_build_s"] = round(time.perf_counter() - t0, 2)
     out["elements"] = sorted(int(e) for e in elems)
-    rng = np.random.default_rng(0)
-    for name, mode in (("mixed", sv.CG_MIXED), ("f64", sv.CG_F64)):
-        for rate in rates:
+    for rate in rates:
+        # one request list per rate, served once with each CG mode
+        rng = np.random.default_rng(rate)
+        drawn = []
+        for k in range(nreq):
+            m = pool[int(rng.integers(len(pool)))]
+            kind = ("heat" if m.dim == 3 else "spacetime") if k % 2 else "poisson"
+            drawn.append((m, int(rng.integers(1 << 30)), kind))
+        for name, mode in (("mixed", sv.CG_MIXED), ("f64", sv.CG_F64)):
             srv = sv.Server(dkt, n_streams=2, batch=8, cache=cache)
             try:
-                reqs = [sv.Request(pool[int(rng.integers(len(pool)))], rhs_seed=int(rng.integers(1 << 30)),
-                                   mode=mode, kind="heat" if k % 2 else "poisson", max_iter=50000)
-                        for k in range(nreq)]
+                reqs = [sv.Request(m, rhs_seed=seed, kind=kind, max_iter=50000,
+                                   mode=sv.CG_F64 if kind == "spacetime" else mode) for m, seed, kind in drawn]
                 log(f"serving {nreq} requests at {rate}/s ({name}, batched)")
                 done, wall = sv.run_open_loop(srv, reqs, rate, seed=rate)
             finally:
@@ -302,7 +311,8 @@ def serve_leg(dkt, nreq, rates=(50, 200, 800)):
                 "e2e_ms": {k: round(v, 1) for k, v in sv.percentiles(e2e).items()},
                 "max_resid": max(r.stats["resid"] for r in done),
                 "not_converged": sum(1 for r in done if not r.stats.get("converged")),
-                "iters_mean": round(float(np.mean([r.stats["iters"] for r in done])), 1)}
+                "iters_mean": round(float(np.mean([r.stats["iters"] for r in done])), 1),
+                "kinds": {k: sum(1 for r in done if r.kind == k) for k in ("poisson", "heat", "spacetime")}}
             log(f"  {out[f'{name}_{rate}_per_s']}")
     return out
 

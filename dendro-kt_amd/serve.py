@@ -9,7 +9,12 @@ x = A^-1 b to a relative L_inf residual.  Kinds:
   Dirichlet-masked stiffness (HeatMat::cgSolve);
 * "heat": the reference's heat-equation driver (FEM/examples/src/heatEq.cpp):
   b = M f by the mass operator with HeatVec's hooks (feVector::computeVec,
-  FEM/include/feVector.h:122-168), then the same CG on the stiffness.
+  FEM/include/feVector.h:122-168), then the same CG on the stiffness;
+* "spacetime" (4D meshes: x, y, z and time): the stabilised space-time heat
+  form of the SC19 paper (spacetime.py): b = l(f) for f keyed uniform, the
+  essential boundary on the spatial boundary and at t = 0, solved by BiCGStab
+  in FP64 (the form is not symmetric); these requests run on direct workers
+  beside the CG engines.
 
 Operator: by default the conforming operator sum_e C_e^T K_e C_e (symmetric,
 so CG converges at every size); Server(operator="reference") serves the
@@ -49,6 +54,23 @@ from dataclasses import dataclass, field
 import numpy as np
 
 CG_F64, CG_MIXED, CG_MIXED_RESTART, CG_MIXED_RELIABLE = 0, 1, 2, 3
+KINDS = ("poisson", "heat", "spacetime")
+
+
+def _spacetime():
+    """The package's spacetime module (this file is loaded by path, as the package is)."""
+    import importlib.util
+    import os
+    import sys
+    if "dkt_spacetime" not in sys.modules:
+        spec = importlib.util.spec_from_file_location(
+            "dkt_spacetime", os.path.join(os.path.dirname(os.path.abspath(__file__)), "spacetime.py"))
+        m = importlib.util.module_from_spec(spec)
+        sys.modules["dkt_spacetime"] = m
+        spec.loader.exec_module(m)
+    return sys.modules["dkt_spacetime"]
+
+
 # Device bytes per node of one request: f, b, x (FP64), the CG workspace
 # (r, p, Ap in FP64; r, p, Ap, x in FP32 for the mixed modes) and the mixed
 # modes' FP64 residual.
@@ -143,8 +165,9 @@ class PlanCache:
     current leases waits for them to end (at most wait_timeout seconds); one
     larger than the whole budget is refused with AdmissionError.
 
-    `build(spec)` may be given to replace the construction: it returns
-    (da, map_bytes, make_plans) where make_plans() builds the device plans."""
+    `build(key)` may be given to replace the construction (the key is then the
+    spec itself): it returns (da, map_bytes, make_plans) where make_plans()
+    builds the device plans."""
 
     def __init__(self, dkt, budget_bytes, operator="conforming", build=None, wait_timeout=120.0):
         if operator not in ("conforming", "reference"):
@@ -159,10 +182,26 @@ class PlanCache:
         self.building = set()
         self.leased_vec_bytes = 0.0
 
-    def _build(self, spec):
+    def _build(self, key):
+        spec, kind = key
         dkt = self.dkt
         conf = self.operator == "conforming"
         da = dkt.DA(spec.tree(dkt), spec.order)
+        if kind == "spacetime":
+            st = _spacetime()
+            if spec.dim != 4 or spec.order != 1:
+                raise ValueError("space-time requests need a 4D p = 1 mesh")
+            mats = (st.level_matrices(spec.maxdepth), st.rhs_matrices(spec.maxdepth))
+            map_bytes = sum(dkt.Plan(da, dkt.OP_CAPTURED, op_data=m, conforming=conf, dry_run=True).info()["map_bytes"]
+                            for m in mats)
+
+            def make():
+                plans = tuple(dkt.Plan(da, dkt.OP_CAPTURED, op_data=m, conforming=conf) for m in mats)
+                ids = st.dirichlet_nodes(da.nodes(), spec.maxdepth)
+                for p in plans:
+                    p.set_dirichlet(ids)
+                return plans
+            return da, map_bytes, make
         map_bytes = sum(dkt.Plan(da, op, conforming=conf, dry_run=True).info()["map_bytes"]
                         for op in (dkt.OP_STIFFNESS, dkt.OP_MASS))
         return da, map_bytes, lambda: (dkt.Plan(da, dkt.OP_STIFFNESS, conforming=conf),
@@ -191,10 +230,12 @@ class PlanCache:
                                      f"{used / 1e9:.2f} of {self.budget / 1e9:.2f} GB leased")
             self.cv.wait(min(left, 0.5))
 
-    def acquire(self, spec):
-        """(Lease, built_now).  Builds on a miss; concurrent misses on one spec
-        wait for a single build."""
+    def acquire(self, spec, kind="poisson"):
+        """(Lease, built_now) for a mesh and a request kind (Poisson and heat
+        share the stiffness and mass plans; space-time has its own).  Builds on a
+        miss; concurrent misses on one key wait for a single build."""
         deadline = time.monotonic() + self.wait_timeout
+        spec = (spec, "spacetime" if kind == "spacetime" else "cg") if self.build == self._build else spec
         with self.cv:
             while spec in self.building:
                 self.cv.wait(0.5)
@@ -262,7 +303,8 @@ class Server:
         self.dkt = dkt
         self.cache = cache if cache is not None else PlanCache(dkt, hbm_budget_bytes, operator)
         self.intake = queue.Queue()
-        self.ready = []
+        self.ready = []  # CG requests (the engines' in batched mode)
+        self.ready_direct = []  # batched mode: space-time requests, for the direct workers
         self.ready_cv = threading.Condition()
         self.batch = batch
         self.max_wait = max_wait
@@ -275,6 +317,8 @@ class Server:
         work = self._work_batched if batch > 0 else self._work
         self.workers = [threading.Thread(target=work, args=(torch.cuda.Stream(),), daemon=True)
                         for _ in range(n_streams)]
+        if batch > 0:  # a direct worker for the requests the engines do not run
+            self.workers.append(threading.Thread(target=self._work, args=(torch.cuda.Stream(), True), daemon=True))
         for t in self.workers:
             t.start()
 
@@ -287,7 +331,7 @@ class Server:
         f = self.dkt.keyed_uniform(req.rhs_seed, da.nodes())
         if req.kind == "poisson":
             f[da.boundary()] = 0.0
-        elif req.kind != "heat":
+        elif req.kind not in KINDS:
             raise ValueError(f"unknown request kind {req.kind}")
         return f
 
@@ -299,9 +343,13 @@ class Server:
             req.started = time.perf_counter()
             lease = None
             try:
+                if req.kind not in KINDS:
+                    raise ValueError(f"unknown request kind {req.kind}")
+                if req.kind == "spacetime" and req.mode != CG_F64:
+                    raise ValueError("space-time requests are solved in FP64 (BiCGStab): mode must be CG_F64")
                 if self.batch > 0 and req.mode not in (CG_F64, CG_MIXED, CG_MIXED_RELIABLE):
                     raise ValueError(f"batched serving runs modes CG_F64 and CG_MIXED(_RELIABLE), not {req.mode}")
-                lease, built = self.cache.acquire(req.mesh)
+                lease, built = self.cache.acquire(req.mesh, req.kind)
                 req.plan_built = built
                 req.nodes = lease.da.n_nodes
                 item = Ready(req, lease, self._rhs(req, lease.da))
@@ -312,16 +360,18 @@ class Server:
                 self._finish(req)
                 continue
             with self.ready_cv:
-                self.ready.append(item)
-                self.ready_cv.notify()
+                direct = self.batch > 0 and req.kind == "spacetime"
+                (self.ready_direct if direct else self.ready).append(item)
+                self.ready_cv.notify_all()
 
-    def _take(self, block):
+    def _take(self, block, direct=False):
         """The next ready request by select_next (None: none, or shutting down)."""
+        lst = self.ready_direct if direct else self.ready
         with self.ready_cv:
             while True:
-                i = select_next(self.ready, time.perf_counter(), self.max_wait)
+                i = select_next(lst, time.perf_counter(), self.max_wait)
                 if i is not None:
-                    return self.ready.pop(i)
+                    return lst.pop(i)
                 if not block or self.closing:
                     return None
                 self.ready_cv.wait(0.05)
@@ -334,8 +384,10 @@ class Server:
             self.done.append(req)
 
     def _mass_rhs(self, item, fd, stream):
-        """b = M f with HeatVec's hooks for heat requests; b = f otherwise."""
-        if item.req.kind != "heat":
+        """b = M f with HeatVec's hooks for heat requests (b = l(f) with the
+        space-time form's essential boundary for space-time ones); b = f
+        otherwise."""
+        if item.req.kind not in ("heat", "spacetime"):
             return fd
         bd = self.torch.empty_like(fd)
         _, mass = item.lease.plans
@@ -345,10 +397,10 @@ class Server:
         return bd
 
     # -- workers --------------------------------------------------------------
-    def _work(self, stream):
+    def _work(self, stream, direct=False):
         torch = self.torch
         while True:
-            item = self._take(block=True)
+            item = self._take(block=True, direct=direct)
             if item is None:
                 if self.closing:
                     return
@@ -361,9 +413,12 @@ class Server:
                     bd = self._mass_rhs(item, fd, stream)
                     xd = torch.zeros_like(fd)
                     plan, _ = item.lease.plans
-                    with item.lease.lock:  # a plan owns one CG workspace
-                        st = plan.cg_solve_torch(bd, xd, req.mode, req.max_iter, req.tol,
-                                                 stream=stream.cuda_stream)
+                    with item.lease.lock:  # a plan owns one solver workspace
+                        if req.kind == "spacetime":
+                            st = plan.bicgstab_solve_torch(bd, xd, req.max_iter, req.tol, stream=stream.cuda_stream)
+                        else:
+                            st = plan.cg_solve_torch(bd, xd, req.mode, req.max_iter, req.tol,
+                                                     stream=stream.cuda_stream)
                     req.stats = dict(st, converged=st["stop"] == "converged")
                     req.x = xd.cpu().numpy()
                 stream.synchronize()

@@ -6,8 +6,9 @@
   drawn log-uniformly from [lo, hi]; order p alternating 1 and 2;
 * serve_pool: request meshes, dim 3 (a random-sphere tree) or 4 (a
   Gaussian-mixture tree, one point per leaf) in a 70/30 split, p = 1, with
-  element counts log-uniform in [lo, hi] (the 4D point count is scaled to the
-  target through the measured elements-per-point ratio of the generator).
+  element counts log-uniform (stratified) in [lo, hi] (the sphere radius and
+  the 4D point count are fitted to the target through the measured
+  elements-per-size ratios of the generators).
 
 There is no counterpart in the reference (its benchmark builds one tree per
 run, bench/src/matvec_bench_adaptive.cpp:199-216); these generators only fix
@@ -54,30 +55,34 @@ def gauss4d_points(target):
 
 
 def serve_pool(serve_mod, n, lo=1e3, hi=1e6, seed=0, frac3d=0.7):
-    """n MeshSpecs of the serving configuration."""
+    """n MeshSpecs of the serving configuration.  The element targets are
+    log-uniform and stratified: target i is drawn from the i-th of n equal
+    slices of [log lo, log hi], so every pool spans the whole range."""
     rng = np.random.default_rng(seed)
     pool = []
+    span = np.log(hi) - np.log(lo)
     for i in range(n):
-        target = float(np.exp(rng.uniform(np.log(lo), np.log(hi))))
+        target = float(np.exp(np.log(lo) + span * (i + rng.uniform()) / n))
         if rng.uniform() < frac3d:
             c = tuple(float(v) for v in rng.uniform(0.2, 0.8, 3))
-            r = float(rng.uniform(0.1, 0.3))
-            pool.append((serve_mod.MeshSpec(3, 30, shape="sphere", center=c, radius=r,
-                                            max_level=_sphere_level(target, r)), target))
+            lev, r = _sphere_fit(target, float(rng.uniform(0.1, 0.3)))
+            pool.append((serve_mod.MeshSpec(3, 30, shape="sphere", center=c, radius=r, max_level=lev), target))
         else:
             pool.append((serve_mod.MeshSpec(4, 9, gauss4d_points(target), 1000 + i), target))
     return pool
 
 
-def _sphere_level(target, radius):
-    """Refinement level of a sphere tree (base level 2) whose element count is
-    near `target`: measured, the 2:1 balanced tree holds about
-    58 r^2 4^level elements (the surface band of 4 pi r^2 / h^2 cells of size
-    h = 2^-level with its transition layers), on top of the 64 base cells."""
-    best, best_err = 3, None
+def _sphere_fit(target, radius):
+    """(level, radius) of a sphere tree (base level 2) with about `target`
+    elements: measured, the 2:1 balanced tree holds about 58 r^2 4^level
+    elements (the surface band of 4 pi r^2 / h^2 cells of size h = 2^-level
+    with its transition layers) on top of the 64 base cells.  The level is
+    the one whose fitting radius is closest to `radius` in log scale, and the
+    radius is then set to fit the target (kept within [0.1, 0.3])."""
+    best = None
     for lev in range(3, 14):
-        est = 64 + 58.0 * radius ** 2 * 4.0 ** lev
-        err = abs(math.log(est / target))
-        if best_err is None or err < best_err:
-            best, best_err = lev, err
-    return best
+        r = math.sqrt(max(target - 64.0, 1.0) / (58.0 * 4.0 ** lev))
+        err = abs(math.log(r / radius))
+        if best is None or err < best[0]:
+            best = (err, lev, min(max(r, 0.1), 0.3))
+    return best[1], best[2]

@@ -172,3 +172,57 @@ def test_batched_server_mixed_and_fp64_equal_alone(dkt):
             assert np.array_equal(out.x, r.x)
     finally:
         alone.close()
+
+
+@pytest.mark.gpu
+def test_spacetime_requests_served_beside_cg(dkt):
+    """Space-time heat requests (4D meshes, BiCGStab on the stabilised space-time
+    form) served by the batched server's direct worker while Poisson requests
+    run on the CG engines: every space-time answer meets its tolerance, checked
+    with a fresh space-time plan (b = l(f), essential boundary = spatial
+    boundary and t = 0), and equals the request served alone, bit for bit; a
+    space-time request in mixed precision or on a 3D mesh is refused."""
+    torch = _torch()
+    sv = _serve_mod()
+    st = sv._spacetime()
+    meshes = [sv.MeshSpec(4, 8, n, s) for n, s in ((300, 21), (900, 22))]
+    reqs = [sv.Request(m, rhs_seed=k, kind=kind) for k, kind in ((1, "spacetime"), (2, "poisson"), (3, "spacetime"))
+            for m in meshes]
+    bad = [sv.Request(meshes[0], rhs_seed=5, kind="spacetime", mode=sv.CG_MIXED),
+           sv.Request(sv.MeshSpec(3, 30, shape="sphere", center=(0.5, 0.5, 0.5), radius=0.2, max_level=4),
+                      rhs_seed=6, kind="spacetime")]
+    srv = sv.Server(dkt, n_streams=1, batch=3)
+    try:
+        for r in reqs + bad:
+            srv.submit(r)
+        srv.drain(len(reqs) + len(bad), timeout=300)
+    finally:
+        srv.close()
+    assert all(r.error for r in bad), [r.error for r in bad]
+    assert all(not r.error for r in reqs), [r.error for r in reqs]
+    for r in reqs:
+        assert r.stats["stop"] == "converged" and r.stats["resid"] <= r.tol
+        if r.kind != "spacetime":
+            continue
+        da = dkt.DA(r.mesh.tree(dkt), 1)
+        A = dkt.Plan(da, dkt.OP_CAPTURED, op_data=st.level_matrices(8), conforming=True)
+        F = dkt.Plan(da, dkt.OP_CAPTURED, op_data=st.rhs_matrices(8), conforming=True)
+        ids = st.dirichlet_nodes(da.nodes(), 8)
+        A.set_dirichlet(ids)
+        F.set_dirichlet(ids)
+        f = torch.tensor(dkt.keyed_uniform(r.rhs_seed, da.nodes()), device="cuda")
+        b = torch.empty_like(f)
+        F.matvec_torch(f, b, masked=2)
+        ax = torch.empty_like(f)
+        A.matvec_torch(torch.tensor(r.x, device="cuda"), ax, masked=True)
+        torch.cuda.synchronize()
+        assert float((b - ax).abs().max() / b.abs().max()) <= 1.01 * r.tol
+    alone = sv.Server(dkt, n_streams=1)
+    try:
+        for i, r in enumerate(reqs):
+            alone.submit(sv.Request(r.mesh, r.rhs_seed, kind=r.kind))
+            out = alone.drain(i + 1, timeout=300)[-1]
+            assert not out.error, out.error
+            assert np.array_equal(out.x, r.x)
+    finally:
+        alone.close()
