@@ -32,18 +32,42 @@ struct GroupBase {
   virtual void launch(hipStream_t st) = 0;
 };
 
+// Device copies of a group's tables: in the caller's arena when there is one,
+// else in buffers the group owns.
+struct GroupMem {
+  DevArena *arena = nullptr;
+  std::vector<std::unique_ptr<DevBuf<unsigned char>>> own;
+  template <typename T>
+  T *upload(const std::vector<T> &v) {
+    if (arena) return arena->upload(v);
+    own.emplace_back(new DevBuf<unsigned char>());
+    own.back()->alloc(std::max<size_t>(1, v.size() * sizeof(T)));
+    if (!v.empty()) HIP_OK(hipMemcpy(own.back()->p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+    return (T *)own.back()->p;
+  }
+  template <typename T>
+  T *alloc(size_t count) {
+    if (arena) return (T *)arena->take(count * sizeof(T));
+    own.emplace_back(new DevBuf<unsigned char>());
+    own.back()->alloc(std::max<size_t>(1, count * sizeof(T)));
+    return (T *)own.back()->p;
+  }
+};
+
 template <typename T, int DIM, int P>
 struct Group : GroupBase {
+  GroupMem mem;
   // One table of plan records per column chunk (columns [q0, q0 + nq)).
-  std::vector<std::unique_ptr<DevBuf<BatchPlan<T>>>> tables;
-  DevBuf<int> tile_start, block_start;
-  DevBuf<T> partial;
+  std::vector<BatchPlan<T> *> tables;
+  int *tile_start = nullptr, *block_start = nullptr;
+  T *partial = nullptr;
   int nplans = 0, ntiles = 0, nblocks = 0, threads = 0;
   std::vector<size_t> lds;
   std::vector<int> nqs;  // columns per chunk
 
   Group(const std::vector<dkt_plan *> &pls, const std::vector<const void *> &xs, const std::vector<void *> &ys,
-        int nrhs, int masked, const std::vector<const int *> &cancel) {
+        int nrhs, int masked, const std::vector<const int *> &cancel, DevArena *arena = nullptr) {
+    mem.arena = arena;
     constexpr int NPE = Pow<P + 1, DIM>::v;
     const bool f64 = sizeof(T) == 8;
     auto dims_of = [](const dkt_plan *pl) {
@@ -65,7 +89,7 @@ struct Group : GroupBase {
       ts.push_back(ts.back() + (int)pl->ntiles);
       bs.push_back(bs.back() + (int)((pl->nshared + 255) / 256));
     }
-    partial.alloc(std::max<size_t>(1, ptot));
+    partial = mem.alloc<T>(std::max<size_t>(1, ptot));
     for (int q0 = 0; q0 < nrhs; q0 += chunk) {
       const int nq = std::min(chunk, nrhs - q0);
       std::vector<BatchPlan<T>> hp(pls.size());
@@ -97,33 +121,32 @@ struct Group : GroupBase {
         hp[k].carry_w = f64 ? (const T *)pl->carry_w64.p : (const T *)pl->carry_w32.p;
         hp[k].x = (const T *)xs[k] + q0;
         hp[k].y = (T *)ys[k] + q0;
-        hp[k].partial = partial.p + poff[k];
+        hp[k].partial = partial + poff[k];
         hp[k].snode = pl->snode.p;
         hp[k].soff = pl->soff.p;
         hp[k].nshared = (int)pl->nshared;
       }
-      tables.emplace_back(new DevBuf<BatchPlan<T>>());
-      tables.back()->upload(hp);
+      tables.push_back(mem.upload(hp));
       lds.push_back(need);
       nqs.push_back(nq);
     }
     nplans = (int)pls.size();
     ntiles = ts.back();
     nblocks = bs.back();
-    tile_start.upload(ts);
-    block_start.upload(bs);
+    tile_start = mem.upload(ts);
+    block_start = mem.upload(bs);
   }
 
   void launch(hipStream_t st) override {
     for (size_t c = 0; c < tables.size(); c++) {
       if (ntiles && nqs[c] == 1)
         hipLaunchKernelGGL((tile_kernel_batch<T, DIM, P, true>), dim3(ntiles), dim3(threads), lds[c], st,
-                           tables[c]->p, tile_start.p, nplans);
+                           tables[c], tile_start, nplans);
       else if (ntiles)
         hipLaunchKernelGGL((tile_kernel_batch<T, DIM, P, false>), dim3(ntiles), dim3(threads), lds[c], st,
-                           tables[c]->p, tile_start.p, nplans);
+                           tables[c], tile_start, nplans);
       if (nblocks)
-        hipLaunchKernelGGL((fixup_kernel_batch<T>), dim3(nblocks), dim3(256), 0, st, tables[c]->p, block_start.p,
+        hipLaunchKernelGGL((fixup_kernel_batch<T>), dim3(nblocks), dim3(256), 0, st, tables[c], block_start,
                            nplans);
       HIP_OK(hipGetLastError());
     }
@@ -180,15 +203,17 @@ __global__ void __launch_bounds__(kTileThreads) conf_tile_kernel_batch(const Con
 // own matvec (same tile work, same fix-up order).
 template <typename T, int DIM, int P>
 struct ConfGroup : GroupBase {
-  std::vector<std::unique_ptr<DevBuf<ConfBatchPlan<T>>>> tables;
-  std::vector<std::unique_ptr<DevBuf<FixRec<T>>>> fixes;
-  DevBuf<int> tile_start, block_start;
-  DevBuf<T> partial;
+  GroupMem mem;
+  std::vector<ConfBatchPlan<T> *> tables;
+  std::vector<FixRec<T> *> fixes;
+  int *tile_start = nullptr, *block_start = nullptr;
+  T *partial = nullptr;
   int nplans = 0, ntiles = 0, nblocks = 0, threads = 0;
   std::vector<size_t> lds;
 
   ConfGroup(const std::vector<dkt_plan *> &pls, const std::vector<const void *> &xs, const std::vector<void *> &ys,
-            int nrhs, int masked, const std::vector<const int *> &cancel) {
+            int nrhs, int masked, const std::vector<const int *> &cancel, DevArena *arena = nullptr) {
+    mem.arena = arena;
     constexpr int NPE = Pow<P + 1, DIM>::v;
     const bool f64 = sizeof(T) == 8;
     auto lds_of = [](const dkt_plan *pl, int nq) {
@@ -209,7 +234,7 @@ struct ConfGroup : GroupBase {
       ts.push_back(ts.back() + (int)pl->ntiles);
       bs.push_back(bs.back() + (int)((pl->nshared + 255) / 256));
     }
-    partial.alloc(std::max<size_t>(1, ptot));
+    partial = mem.alloc<T>(std::max<size_t>(1, ptot));
     for (int q0 = 0; q0 < nrhs; q0 += chunk) {
       const int nq = std::min(chunk, nrhs - q0);
       std::vector<ConfBatchPlan<T>> hp(pls.size());
@@ -244,30 +269,28 @@ struct ConfGroup : GroupBase {
         hp[k].term_w = f64 ? (const T *)pl->term_w64.p : (const T *)pl->term_w32.p;
         hp[k].x = (const T *)xs[k] + q0;
         hp[k].y = (T *)ys[k] + q0;
-        hp[k].partial = partial.p + poff[k];
-        hf[k] = FixRec<T>{pl->snode.p, pl->soff.p, partial.p + poff[k], (T *)ys[k] + q0, A.dir_out, A.cancel,
+        hp[k].partial = partial + poff[k];
+        hf[k] = FixRec<T>{pl->snode.p, pl->soff.p, partial + poff[k], (T *)ys[k] + q0, A.dir_out, A.cancel,
                           nq, nrhs, (int)pl->nshared};
       }
-      tables.emplace_back(new DevBuf<ConfBatchPlan<T>>());
-      tables.back()->upload(hp);
-      fixes.emplace_back(new DevBuf<FixRec<T>>());
-      fixes.back()->upload(hf);
+      tables.push_back(mem.upload(hp));
+      fixes.push_back(mem.upload(hf));
       lds.push_back(need);
     }
     nplans = (int)pls.size();
     ntiles = ts.back();
     nblocks = bs.back();
-    tile_start.upload(ts);
-    block_start.upload(bs);
+    tile_start = mem.upload(ts);
+    block_start = mem.upload(bs);
   }
 
   void launch(hipStream_t st) override {
     for (size_t c = 0; c < tables.size(); c++) {
       if (ntiles)
         hipLaunchKernelGGL((conf_tile_kernel_batch<T, DIM, P>), dim3(ntiles), dim3(threads), lds[c], st,
-                           tables[c]->p, tile_start.p, nplans);
+                           tables[c], tile_start, nplans);
       if (nblocks)
-        hipLaunchKernelGGL((fixup_kernel_rec<T>), dim3(nblocks), dim3(256), 0, st, fixes[c]->p, block_start.p,
+        hipLaunchKernelGGL((fixup_kernel_rec<T>), dim3(nblocks), dim3(256), 0, st, fixes[c], block_start,
                            nplans);
       HIP_OK(hipGetLastError());
     }
@@ -550,11 +573,11 @@ struct PersistGroup : GroupBase {
 template <typename T>
 GroupBase *make_group(bool conforming, int dim, int p, const std::vector<dkt_plan *> &pls,
                       const std::vector<const void *> &xs, const std::vector<void *> &ys, int nrhs, int masked,
-                      const std::vector<const int *> &cancel = {}) {
+                      const std::vector<const int *> &cancel = {}, DevArena *arena = nullptr) {
 #define DKT_GROUP(D, PP)                                                                       \
   if (dim == D && p == PP) {                                                                   \
-    if (conforming) return new ConfGroup<T, D, PP>(pls, xs, ys, nrhs, masked, cancel);         \
-    return new Group<T, D, PP>(pls, xs, ys, nrhs, masked, cancel);                             \
+    if (conforming) return new ConfGroup<T, D, PP>(pls, xs, ys, nrhs, masked, cancel, arena);  \
+    return new Group<T, D, PP>(pls, xs, ys, nrhs, masked, cancel, arena);                      \
   }
   DKT_GROUP(2, 1)
   DKT_GROUP(3, 1)
@@ -588,7 +611,7 @@ struct GroupsLauncher : MatvecLauncher {
 
 std::unique_ptr<MatvecLauncher> dkt::batch_matvec(int dtype, const std::vector<dkt_plan *> &plans,
                                                   const std::vector<const void *> &x, const std::vector<void *> &y,
-                                                  int mask, const std::vector<const int *> &cancel) {
+                                                  int mask, const std::vector<const int *> &cancel, DevArena *arena) {
   auto out = std::make_unique<GroupsLauncher>();
   for (const auto &kv : by_kind_of(plans.data(), (int)plans.size())) {
     std::vector<dkt_plan *> pls;
@@ -602,8 +625,8 @@ std::unique_ptr<MatvecLauncher> dkt::batch_matvec(int dtype, const std::vector<d
       cs.push_back(cancel[k]);
     }
     const auto [conf, dim, p] = kv.first;
-    out->groups.emplace_back(dtype == 0 ? make_group<double>(conf != 0, dim, p, pls, xs, ys, 1, mask, cs)
-                                        : make_group<float>(conf != 0, dim, p, pls, xs, ys, 1, mask, cs));
+    out->groups.emplace_back(dtype == 0 ? make_group<double>(conf != 0, dim, p, pls, xs, ys, 1, mask, cs, arena)
+                                        : make_group<float>(conf != 0, dim, p, pls, xs, ys, 1, mask, cs, arena));
   }
   return out;
 }

@@ -992,6 +992,9 @@ struct dkt_cg_engine {
   bool dirty = true;       // active set changed: rebuild the launches
   std::vector<int> act64, act32;  // active slots of each precision, in descriptor order
   std::unique_ptr<MatvecLauncher> mv64, mv32, mvu;
+  // Tables and partial sums of the batched matvecs, rebuilt at every change of
+  // the active set without allocating (each step ends synchronized).
+  DevArena arena;
   DevBuf<CgDesc<double>> d64, dnew;
   DevBuf<CgDesc<float>> d32;
   DevBuf<RelDesc> drel;
@@ -1140,9 +1143,13 @@ extern "C" int dkt_cg_engine_step(dkt_cg_engine *e, int chunks, void *stream, in
       }
       for (int k : e->act64) hstat.push_back(StatDesc{e->slots[k].d.flag, e->slots[k].d.scal + S_RESID});
       for (int k : e->act32) hstat.push_back(StatDesc{e->slots[k].d.flag, e->slots[k].rel.scal64 + S_RESID});
-      e->mv64 = p64.empty() ? nullptr : batch_matvec(0, p64, x64, y64, e->masked, c64);
-      e->mv32 = p32.empty() ? nullptr : batch_matvec(1, p32, x32, y32, e->masked, c32);
-      e->mvu = p32.empty() ? nullptr : batch_matvec(0, p32, xu, yu, e->masked, cu);
+      e->mv64.reset();
+      e->mv32.reset();
+      e->mvu.reset();
+      e->arena.reset(st);
+      e->mv64 = p64.empty() ? nullptr : batch_matvec(0, p64, x64, y64, e->masked, c64, &e->arena);
+      e->mv32 = p32.empty() ? nullptr : batch_matvec(1, p32, x32, y32, e->masked, c32, &e->arena);
+      e->mvu = p32.empty() ? nullptr : batch_matvec(0, p32, xu, yu, e->masked, cu, &e->arena);
       auto put = [&](auto &buf, const auto &v) {
         if (v.empty()) return;
         if (buf.n < v.size()) buf.alloc(v.size());

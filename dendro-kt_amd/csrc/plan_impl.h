@@ -1,5 +1,6 @@
 // The device plan object shared by the matvec (plan.hip) and the solvers (cg.hip).
 #pragma once
+#include <algorithm>
 #include <memory>
 #include <vector>
 
@@ -104,15 +105,72 @@ struct ExtScratch {
 void plan_apply(dkt_plan *pl, int dtype, const void *x, void *y, int nrhs, int mask, hipStream_t st,
                 const int *cancel, const ExtScratch *ext = nullptr);
 
+// Grow-only device memory for launch tables that are rebuilt often (the CG
+// engine rebuilds its batched matvecs whenever a solve joins or leaves):
+// reset() starts a rebuild, take() hands out 256-byte aligned pieces, upload()
+// copies a host vector into one with hipMemcpyAsync on the rebuild's stream.
+// A rebuild that outgrows the arena adds a block; the next reset() replaces
+// the blocks by one of their total size, so after the first few rebuilds no
+// device memory is allocated or freed (hipFree waits for the whole device,
+// other streams included).  The host copies stay alive until the next reset();
+// the caller must have synchronized the stream before it resets.
+struct DevArena {
+  struct Block {
+    DevBuf<unsigned char> buf;
+    size_t used = 0;
+  };
+  std::vector<std::unique_ptr<Block>> blocks;
+  std::vector<std::vector<unsigned char>> staging;
+  hipStream_t stream = nullptr;
+
+  void reset(hipStream_t st) {
+    stream = st;
+    staging.clear();
+    if (blocks.size() > 1) {
+      size_t total = 0;
+      for (auto &b : blocks) total += b->buf.n;
+      blocks.clear();
+      add_block(total);
+    }
+    for (auto &b : blocks) b->used = 0;
+  }
+  void *take(size_t bytes) {
+    bytes = (std::max<size_t>(bytes, 1) + 255) & ~(size_t)255;
+    Block *b = blocks.empty() ? nullptr : blocks.back().get();
+    if (!b || b->used + bytes > b->buf.n) b = add_block(std::max(bytes, blocks.empty() ? (size_t)1 << 20 : 2 * b->buf.n));
+    void *p = b->buf.p + b->used;
+    b->used += bytes;
+    return p;
+  }
+  template <typename T>
+  T *upload(const std::vector<T> &v) {
+    T *p = (T *)take(v.size() * sizeof(T));
+    if (v.empty()) return p;
+    staging.emplace_back((const unsigned char *)v.data(), (const unsigned char *)(v.data() + v.size()));
+    HIP_OK(hipMemcpyAsync(p, staging.back().data(), v.size() * sizeof(T), hipMemcpyHostToDevice, stream));
+    return p;
+  }
+
+ private:
+  Block *add_block(size_t bytes) {
+    blocks.emplace_back(new Block());
+    blocks.back()->buf.alloc(bytes);
+    return blocks.back().get();
+  }
+};
+
 // Matvecs y_k = A_k x_k (dtype 0 f64, 1 f32, one column) of many tiled plans,
 // reference or conforming, one launch per (kind, dim, p) (batch.hip); plan
 // k's kernels return at once while *cancel[k] is nonzero.  Each result is
-// bitwise the plan's own matvec.
+// bitwise the plan's own matvec.  With an arena, the launch tables and
+// partial sums live in it (nothing is allocated once it is large enough) and
+// the launcher is valid until the arena's next reset().
 struct MatvecLauncher {
   virtual ~MatvecLauncher() = default;
   virtual void launch(hipStream_t st) = 0;
 };
 std::unique_ptr<MatvecLauncher> batch_matvec(int dtype, const std::vector<dkt_plan *> &plans,
                                              const std::vector<const void *> &x, const std::vector<void *> &y,
-                                             int mask, const std::vector<const int *> &cancel);
+                                             int mask, const std::vector<const int *> &cancel,
+                                             DevArena *arena = nullptr);
 }  // namespace dkt
