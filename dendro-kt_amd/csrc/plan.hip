@@ -166,6 +166,22 @@ void one_d(int p, std::vector<double> &K, std::vector<double> &M) {
 // (B^T)^(x dim) (B)^(x dim) and the stiffness (B^T)^(x dim) diag(sum_d lambda)
 // B^(x dim).  M1 = L L^T (Cholesky), C = L^-1 K1 L^-T = Q diag(lambda) Q^T
 // (Jacobi rotations), B = Q^T L^T.  Returns B row-major, then lambda.
+// The device table of q2_apply in precision T: B and lambda from q2_factors,
+// then D[r] = sum_d lambda[i_d(r)] for the 3^dim ranks, summed in T from zero,
+// axis 0 first.
+template <typename T>
+std::vector<T> q2_table(const std::vector<double> &kf, int dim) {
+  std::vector<T> out(kf.begin(), kf.begin() + 12);
+  int n = 1;
+  for (int d = 0; d < dim; d++) n *= 3;
+  for (int r = 0; r < n; r++) {
+    T s = T(0);
+    for (int d = 0, rr = r; d < dim; d++, rr /= 3) s += (T)kf[9 + rr % 3];
+    out.push_back(s);
+  }
+  return out;
+}
+
 std::vector<double> q2_factors() {
   std::vector<double> K1, M1;
   one_d(2, K1, M1);
@@ -1289,8 +1305,8 @@ extern "C" int dkt_plan_create(const dkt_da *da, int op, const double *op_data, 
     pl->kmat32.upload(cast<float>(km));
     if (mp.order == 2) {
       const std::vector<double> kf = q2_factors();
-      pl->kfac64.upload(kf);
-      pl->kfac32.upload(cast<float>(kf));
+      pl->kfac64.upload(q2_table<double>(kf, mp.dim));
+      pl->kfac32.upload(q2_table<float>(kf, mp.dim));
     }
     std::vector<uint8_t> dir(pl->N, 0);
     for (u32 b : ns.bdy) dir[b] = 1;
