@@ -193,6 +193,37 @@ def cg_p2_leg(dkt, torch, level):
     return out
 
 
+def conf_p2_matvec(dkt, torch, args):
+    """The conforming 3D p=2 stiffness matvec alone (for kernel counters): the
+    level-7 sphere geometry (radius 0.3 at the centre, base level 3) refined to
+    --p2-level, x keyed uniform, FP64 then FP32."""
+    t0 = time.time()
+    da = dkt.DA(dkt.balance_tree(dkt.sphere_tree(3, 30, 3, args.p2_level, (0.5, 0.5, 0.5), 0.3), 1), 2)
+    plan = dkt.Plan(da, dkt.OP_STIFFNESS, conforming=True)
+    log(f"conforming p=2 level {args.p2_level}: {da.n_elements} elements, {da.n_nodes} nodes in {time.time() - t0:.1f}s")
+    if args.calib:
+        cal = torch.zeros(1 << 25, dtype=torch.float64, device="cuda")
+        cal2 = torch.empty_like(cal)
+        torch.neg(cal, out=cal2)
+        torch.cuda.synchronize()
+        del cal, cal2
+    x = torch.tensor(dkt.keyed_uniform(0, da.nodes()), device="cuda")
+    out = {"elements": da.n_elements, "nodes": da.n_nodes}
+    for name, xv in (("f64", x), ("f32", x.float())):
+        yv = torch.empty_like(xv)
+        for _ in range(args.warmup):
+            plan.matvec_torch(xv, yv)
+        torch.cuda.synchronize()
+        plan.profile(True)
+        for _ in range(args.steps):
+            plan.matvec_torch(xv, yv)
+        tile_ms, tot_ms, n = plan.profile_read()
+        plan.profile(False)
+        out[name] = {"kernel_ms": round(tile_ms / max(n, 1), 4), "matvec_ms": round(tot_ms / max(n, 1), 4)}
+        log(f"  {name}: {out[name]}")
+    print(json.dumps(out), flush=True)
+
+
 def _pkg_mod(name, file):
     import importlib.util as ilu
     if name in sys.modules:
@@ -335,6 +366,9 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the batched multi-tree leg")
     ap.add_argument("--batch-trees", type=int, default=64)
     ap.add_argument("--serve-requests", type=int, default=60)
+    ap.add_argument("--conf-p2-matvec", action="store_true",
+                    help="profiling mode: only the conforming 3D p=2 stiffness matvec on the level --p2-level "
+                         "sphere (FP64 then FP32, --warmup and --steps each); prints kernel times, not the bench line")
     ap.add_argument("--calib", action="store_true",
                     help="run one 256 MiB elementwise copy (torch.neg) first: the reference dispatch that scales "
                          "FETCH_SIZE / WRITE_SIZE counters to bytes")
@@ -349,6 +383,9 @@ def main():
     if not torch.cuda.is_available():
         log("no HIP device")
         sys.exit(1)
+    if args.conf_p2_matvec:
+        conf_p2_matvec(dkt, torch, args)
+        return
     da, mesh_s = build_mesh(dkt, args.points, args.maxdepth, args.seed)
     N, L, H, npe = da.n_nodes, da.count(3), da.count(4), da.npe
     t0 = time.time()
