@@ -216,9 +216,7 @@ struct ConfGroup : GroupBase {
     mem.arena = arena;
     constexpr int NPE = Pow<P + 1, DIM>::v;
     const bool f64 = sizeof(T) == 8;
-    auto lds_of = [](const dkt_plan *pl, int nq) {
-      return conf_lds_bytes<T, NPE>(pl->max_nodes, pl->max_virt, pl->max_terms, nq);
-    };
+    auto lds_of = [](const dkt_plan *pl, int nq) { return conf_lds_bytes<T, NPE>(pl->max_nodes, nq); };
     int chunk = nrhs;
     for (dkt_plan *pl : pls) {
       if (lds_of(pl, 1) > kLdsLimit) throw std::invalid_argument("tile does not fit in LDS");
@@ -251,8 +249,10 @@ struct ConfGroup : GroupBase {
         A.leaf_lev = pl->leaf_lev.p;
         A.virt_off = pl->virt_off.p;
         A.term_node = pl->term_node.p;
-        A.term_virt = pl->term_virt.p;
-        A.term_perm = pl->term_perm.p;
+        A.term_svirt = pl->term_svirt.p;
+        A.seg_node = pl->seg_node.p;
+        A.seg_off = pl->seg_off.p;
+        A.nterm = (int)pl->nterm;
         A.dir_in = masked == 1 ? pl->dirichlet.p : nullptr;
         A.dir_out = masked ? pl->dirichlet.p : nullptr;
         A.cancel = cancel.empty() ? nullptr : cancel[k];
@@ -261,8 +261,6 @@ struct ConfGroup : GroupBase {
         A.nq = nq;
         A.ld = nrhs;
         A.max_points = pl->max_nodes;
-        A.max_virt = pl->max_virt;
-        A.max_terms = pl->max_terms;
         A.tile_leaves = pl->tile_leaves;
         hp[k].kref = P == 2 ? (f64 ? (const T *)pl->kfac64.p : (const T *)pl->kfac32.p)
                             : (f64 ? (const T *)pl->kmat64.p : (const T *)pl->kmat32.p);

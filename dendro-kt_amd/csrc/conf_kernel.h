@@ -16,6 +16,11 @@
 //   4. write-out as tile_kernel (owned nodes to y, shared ones to partial slots
 //      summed by fixup_kernel).
 // Every sum runs in an order fixed by the plan: bitwise reproducible.
+// LDS holds the local point values only.  The constraint terms are read from
+// global memory where they are used: in row order for step 1, and a second
+// copy sorted by node, cut into one segment per node, for step 3.  Staging the
+// terms cost 14 bytes of LDS per term (28.6 KB of the 49.6 KB of a 256-element
+// 3D p = 2 tile in FP64), which held the kernel to three workgroups per CU.
 #pragma once
 #include "plan_common.h"
 
@@ -26,6 +31,7 @@ struct ConfTileHdr {
   int32_t node0, nnode;  // mesh nodes (tile_nodes / tile_out); local points [0, nnode)
   int32_t v0, nv;        // hanging points (virt_off), local points [nnode, nnode + nv)
   int32_t t0, nt;        // constraint terms of the hanging points' rows
+  int32_t s0, ns;        // segments of the node-sorted terms (seg_node / seg_off), one per node with terms
 };
 
 struct ConfArgs {
@@ -33,14 +39,16 @@ struct ConfArgs {
   const int32_t *tile_nodes, *tile_out;
   const uint16_t *leaf_loc;   // [E][nPe] local point of every rank
   const uint8_t *leaf_lev;    // [E] element level
-  const uint16_t *virt_off;   // [hanging point] tile-relative first term of its row
-  const uint16_t *term_node;  // tile-local node per term
-  const uint16_t *term_virt;  // hanging point (0-based within the tile) per term
-  const uint16_t *term_perm;  // tile-relative term indices sorted by node
+  const uint16_t *virt_off;    // [hanging point] tile-relative first term of its row
+  const uint16_t *term_node;   // tile-local node per term, row order
+  const uint16_t *term_svirt;  // hanging point (0-based within the tile) per term, sorted by node
+  const uint16_t *seg_node;    // [segment] tile-local node
+  const uint16_t *seg_off;     // [segment] tile-relative first sorted term
   const uint8_t *dir_in, *dir_out;
   const int *cancel;
   int op, hp, nq, ld;
-  int max_points, max_virt, max_terms, tile_leaves;
+  int max_points, tile_leaves;
+  int nterm;  // term weights: [0, nterm) in row order, then [nterm, 2 nterm) sorted by node
 };
 
 // One tile of one plan (the workgroup may have more work-items than the
@@ -62,14 +70,9 @@ __device__ __forceinline__ void conf_tile_body(const ConfArgs &A, int tile, cons
   T *x_loc = reinterpret_cast<T *>(carve((size_t)A.max_points * nq * sizeof(T)));
   // One column: the accumulator reuses x_loc once every element has read it.
   T *y_loc = nq == 1 ? x_loc : reinterpret_cast<T *>(carve((size_t)A.max_points * nq * sizeof(T)));
-  T *tw_s = reinterpret_cast<T *>(carve((size_t)A.max_terms * sizeof(T)));
-  uint16_t *tj_s = reinterpret_cast<uint16_t *>(carve((size_t)A.max_terms * 2));
-  uint16_t *tv_s = reinterpret_cast<uint16_t *>(carve((size_t)A.max_terms * 2));
-  uint16_t *tp_s = reinterpret_cast<uint16_t *>(carve((size_t)A.max_terms * 2));
-  uint16_t *vo_s = reinterpret_cast<uint16_t *>(carve((size_t)(A.max_virt + 1) * 2));
   const int tid = threadIdx.x, nthr = blockDim.x;
 
-  // 1. Stage nodes and rows.
+  // 1. Stage the nodes, then evaluate the hanging points' rows.
   const bool active = tid < th.nleaf;
   uint16_t loc[NPE];
   int lev = 0;
@@ -108,38 +111,15 @@ __device__ __forceinline__ void conf_tile_body(const ConfArgs &A, int tile, cons
   }
   if (nq > 1)
     for (int j = tid; j < npts * nq; j += nthr) y_loc[j] = T(0);
-  for (int k0 = tid; k0 < th.nt; k0 += kCB * nthr) {
-    uint16_t tj[kCB], tv[kCB], tp[kCB];
-    T tw[kCB];
-#pragma unroll
-    for (int b = 0; b < kCB; b++) {
-      const int k = k0 + b * nthr;
-      if (k < th.nt) {
-        tj[b] = A.term_node[th.t0 + k];
-        tv[b] = A.term_virt[th.t0 + k];
-        tw[b] = term_w[th.t0 + k];
-        tp[b] = A.term_perm[th.t0 + k];
-      }
-    }
-#pragma unroll
-    for (int b = 0; b < kCB; b++) {
-      const int k = k0 + b * nthr;
-      if (k < th.nt) {
-        tj_s[k] = tj[b];
-        tv_s[k] = tv[b];
-        tw_s[k] = tw[b];
-        tp_s[k] = tp[b];
-      }
-    }
-  }
-  for (int v = tid; v < th.nv; v += nthr) vo_s[v] = A.virt_off[th.v0 + v];
-  if (tid == 0) vo_s[th.nv] = (uint16_t)th.nt;
   __syncthreads();
+  // Hanging point values: each row over the staged nodes, terms in row order.
+  const uint16_t *tnode = A.term_node + th.t0;
+  const T *tw = term_w + th.t0;
   for (int v = tid; v < th.nv; v += nthr) {
-    const int k0 = vo_s[v], k1 = vo_s[v + 1];
+    const int k0 = A.virt_off[th.v0 + v], k1 = v + 1 < th.nv ? A.virt_off[th.v0 + v + 1] : th.nt;
     for (int q = 0; q < nq; q++) {
       T s = T(0);
-      for (int k = k0; k < k1; k++) s += tw_s[k] * x_loc[tj_s[k] * nq + q];
+      for (int k = k0; k < k1; k++) s += tw[k] * x_loc[tnode[k] * nq + q];
       x_loc[(th.nnode + v) * nq + q] = s;
     }
   }
@@ -179,16 +159,15 @@ __device__ __forceinline__ void conf_tile_body(const ConfArgs &A, int tile, cons
     }
   }
 
-  // 3. Transposed rows: per node, its terms in sorted order (segment heads).
-  for (int i = tid; i < th.nt; i += nthr) {
-    const int j = tj_s[tp_s[i]];
-    if (i > 0 && tj_s[tp_s[i - 1]] == j) continue;
+  // 3. Transposed rows: per node, its terms in sorted order (one segment per node).
+  const uint16_t *svirt = A.term_svirt + th.t0;
+  const T *sw = term_w + A.nterm + th.t0;
+  for (int g = tid; g < th.ns; g += nthr) {
+    const int j = A.seg_node[th.s0 + g];
+    const int m0 = A.seg_off[th.s0 + g], m1 = g + 1 < th.ns ? A.seg_off[th.s0 + g + 1] : th.nt;
     for (int q = 0; q < nq; q++) {
       T acc = T(0);
-      for (int m = i; m < th.nt && tj_s[tp_s[m]] == j; m++) {
-        const int k = tp_s[m];
-        acc += tw_s[k] * y_loc[(th.nnode + tv_s[k]) * nq + q];
-      }
+      for (int m = m0; m < m1; m++) acc += sw[m] * y_loc[(th.nnode + svirt[m]) * nq + q];
       y_loc[j * nq + q] += acc;
     }
   }
@@ -229,10 +208,9 @@ __global__ void __launch_bounds__(kTileThreads) conf_tile_kernel(ConfArgs A, con
 
 // LDS bytes of conf_tile_kernel for nq right-hand sides.
 template <typename T, int NPE>
-inline size_t conf_lds_bytes(int max_points, int max_virt, int max_terms, int nq) {
+inline size_t conf_lds_bytes(int max_points, int nq) {
   auto r16 = [](size_t b) { return (b + 15) & ~(size_t)15; };
-  return (nq == 1 ? 1 : 2) * r16((size_t)max_points * nq * sizeof(T)) + r16((size_t)max_terms * sizeof(T)) +
-         3 * r16((size_t)max_terms * 2) + r16((size_t)(max_virt + 1) * 2);
+  return (nq == 1 ? 1 : 2) * r16((size_t)max_points * nq * sizeof(T));
 }
 
 }  // namespace dkt

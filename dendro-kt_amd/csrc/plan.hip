@@ -964,9 +964,9 @@ void build_conf_tiled(dkt_plan *pl, const MeshParams &mp, const Traversal &tr, i
   std::vector<int32_t> last_tile(N, -1), local(N, -1);
   std::vector<ConfTileHdr> hdr(ntiles);
   std::vector<int32_t> tnodes;
-  std::vector<uint16_t> lloc(L * (size_t)npe, 0), voff, tnode, tvirt, tperm;
+  std::vector<uint16_t> lloc(L * (size_t)npe, 0), voff, tnode, tvirt, tsvirt, sgnode, sgoff;
   std::vector<uint8_t> llev(L, 0);
-  std::vector<double> tw;
+  std::vector<double> tw, tsw;
   std::vector<std::pair<size_t, int32_t>> vref;
   int max_points = 0, max_virt = 0, max_terms = 0;
   for (size_t t = 0; t < ntiles; t++) {
@@ -1013,17 +1013,28 @@ void build_conf_tiled(dkt_plan *pl, const MeshParams &mp, const Traversal &tr, i
     H.nt = (int32_t)(tnode.size() - H.t0);
     if (nn + H.nv >= 0xFFFF) throw std::runtime_error("tile has too many points");
     for (const auto &vr : vref) lloc[vr.first] = (uint16_t)(nn + vr.second);
+    // The terms again, sorted by node (row order within a node), one segment per node.
     std::vector<uint16_t> perm(H.nt);
     for (int k = 0; k < H.nt; k++) perm[k] = (uint16_t)k;
     std::stable_sort(perm.begin(), perm.end(),
                      [&](uint16_t a, uint16_t b) { return tnode[H.t0 + a] < tnode[H.t0 + b]; });
-    tperm.insert(tperm.end(), perm.begin(), perm.end());
+    H.s0 = (int32_t)sgnode.size();
+    for (int i = 0; i < H.nt; i++) {
+      const size_t k = H.t0 + perm[i];
+      tsvirt.push_back(tvirt[k]);
+      tsw.push_back(tw[k]);
+      if (i == 0 || tnode[H.t0 + perm[i - 1]] != tnode[k]) {
+        sgnode.push_back(tnode[k]);
+        sgoff.push_back((uint16_t)i);
+      }
+    }
+    H.ns = (int32_t)(sgnode.size() - H.s0);
     max_points = std::max(max_points, nn + H.nv);
     max_virt = std::max(max_virt, H.nv);
     max_terms = std::max(max_terms, H.nt);
   }
   // Halve the tiles while one right-hand side in FP64 does not fit in LDS.
-  pl->lds_bytes = conf_lds_bytes<double, 4>(max_points, max_virt, max_terms, 1);
+  pl->lds_bytes = conf_lds_bytes<double, 4>(max_points, 1);
   if (pl->lds_bytes > (size_t)kLdsLimit && T > 16) return build_conf_tiled(pl, mp, tr, T / 2);
   // Owned nodes go to y; a node in several tiles gets a partial slot per tile
   // (node-major, tile order), summed by fixup_kernel.
@@ -1058,14 +1069,18 @@ void build_conf_tiled(dkt_plan *pl, const MeshParams &mp, const Traversal &tr, i
   pl->leaf_lev.upload(nonempty(llev));
   pl->virt_off.upload(nonempty(voff));
   pl->term_node.upload(nonempty(tnode));
-  pl->term_virt.upload(nonempty(tvirt));
-  pl->term_perm.upload(nonempty(tperm));
-  pl->term_w64.upload(nonempty(tw));
-  pl->term_w32.upload(cast<float>(nonempty(tw)));
+  pl->term_svirt.upload(nonempty(tsvirt));
+  pl->seg_node.upload(nonempty(sgnode));
+  pl->seg_off.upload(nonempty(sgoff));
+  std::vector<double> wboth(tw);
+  wboth.insert(wboth.end(), tsw.begin(), tsw.end());
+  pl->nterm = tw.size();
+  pl->term_w64.upload(nonempty(wboth));
+  pl->term_w32.upload(cast<float>(nonempty(wboth)));
   pl->snode.upload(nonempty(snode));
   pl->soff.upload(soff);
   pl->map_bytes = (double)(hdr.size() * sizeof(ConfTileHdr) + tnodes.size() * 8 + lloc.size() * 2 + llev.size() +
-                           voff.size() * 2 + tnode.size() * 14 + snode.size() * 8);
+                           voff.size() * 2 + tnode.size() * 20 + sgnode.size() * 4 + snode.size() * 8);
 }
 
 // ---------------------------------------------------------------------------
@@ -1181,7 +1196,7 @@ void run_conf(dkt_plan *pl, int nrhs, const T *x, T *y, int mask, hipStream_t st
   const T *kmat = P == 2 ? (f64 ? (const T *)pl->kfac64.p : (const T *)pl->kfac32.p)
                          : (f64 ? (const T *)pl->kmat64.p : (const T *)pl->kmat32.p);
   const T *tw = f64 ? (const T *)pl->term_w64.p : (const T *)pl->term_w32.p;
-  auto lds_for = [&](int nq) { return conf_lds_bytes<T, NPE>(pl->max_nodes, pl->max_virt, pl->max_terms, nq); };
+  auto lds_for = [&](int nq) { return conf_lds_bytes<T, NPE>(pl->max_nodes, nq); };
   if (lds_for(1) > kLdsLimit) throw std::invalid_argument("tile does not fit in LDS");
   int chunk = 1;
   while (chunk < nrhs && lds_for(chunk + 1) <= kLdsLimit) chunk++;
@@ -1194,8 +1209,9 @@ void run_conf(dkt_plan *pl, int nrhs, const T *x, T *y, int mask, hipStream_t st
   A.leaf_lev = pl->leaf_lev.p;
   A.virt_off = pl->virt_off.p;
   A.term_node = pl->term_node.p;
-  A.term_virt = pl->term_virt.p;
-  A.term_perm = pl->term_perm.p;
+  A.term_svirt = pl->term_svirt.p;
+  A.seg_node = pl->seg_node.p;
+  A.seg_off = pl->seg_off.p;
   A.dir_in = mask == 1 ? pl->dirichlet.p : nullptr;
   A.dir_out = mask ? pl->dirichlet.p : nullptr;
   A.cancel = cancel;
@@ -1203,9 +1219,8 @@ void run_conf(dkt_plan *pl, int nrhs, const T *x, T *y, int mask, hipStream_t st
   A.hp = pl->hp;
   A.ld = nrhs;
   A.max_points = pl->max_nodes;
-  A.max_virt = pl->max_virt;
-  A.max_terms = pl->max_terms;
   A.tile_leaves = pl->tile_leaves;
+  A.nterm = (int)pl->nterm;
   for (int q0 = 0; q0 < nrhs; q0 += chunk) {
     const int nq = std::min(chunk, nrhs - q0);
     A.nq = nq;
