@@ -187,7 +187,7 @@ struct ConfBatchPlan {
 };
 
 // All tiles of a batch of same-(dim, p) conforming plans in one launch.
-template <typename T, int DIM, int P>
+template <typename T, int DIM, int P, bool CAP>
 __global__ void __launch_bounds__(kTileThreads) conf_tile_kernel_batch(const ConfBatchPlan<T> *__restrict__ plans,
                                                                        const int *__restrict__ tile_start,
                                                                        int nplans) {
@@ -195,13 +195,13 @@ __global__ void __launch_bounds__(kTileThreads) conf_tile_kernel_batch(const Con
   const int k = find_range(tile_start, nplans, t);
   const ConfBatchPlan<T> &bp = plans[k];
   if (bp.A.cancel && *bp.A.cancel) return;
-  conf_tile_body<T, DIM, P>(bp.A, t - tile_start[k], bp.kref, bp.term_w, bp.x, bp.y, bp.partial);
+  conf_tile_body<T, DIM, P, CAP>(bp.A, t - tile_start[k], bp.kref, bp.term_w, bp.x, bp.y, bp.partial);
 }
 
-// Conforming plans of one (dim, p): one tile launch and one fix-up launch per
+// Conforming plans of one (dim, p) (CAP: with per-level captured matrices): one tile launch and one fix-up launch per
 // column chunk that fits the tiles' LDS.  Each result is bitwise the plan's
 // own matvec (same tile work, same fix-up order).
-template <typename T, int DIM, int P>
+template <typename T, int DIM, int P, bool CAP>
 struct ConfGroup : GroupBase {
   GroupMem mem;
   std::vector<ConfBatchPlan<T> *> tables;
@@ -262,8 +262,8 @@ struct ConfGroup : GroupBase {
         A.ld = nrhs;
         A.max_points = pl->max_nodes;
         A.tile_leaves = pl->tile_leaves;
-        hp[k].kref = P == 2 ? (f64 ? (const T *)pl->kfac64.p : (const T *)pl->kfac32.p)
-                            : (f64 ? (const T *)pl->kmat64.p : (const T *)pl->kmat32.p);
+        hp[k].kref = P == 2 && !pl->per_level ? (f64 ? (const T *)pl->kfac64.p : (const T *)pl->kfac32.p)
+                                              : (f64 ? (const T *)pl->kmat64.p : (const T *)pl->kmat32.p);
         hp[k].term_w = f64 ? (const T *)pl->term_w64.p : (const T *)pl->term_w32.p;
         hp[k].x = (const T *)xs[k] + q0;
         hp[k].y = (T *)ys[k] + q0;
@@ -285,7 +285,7 @@ struct ConfGroup : GroupBase {
   void launch(hipStream_t st) override {
     for (size_t c = 0; c < tables.size(); c++) {
       if (ntiles)
-        hipLaunchKernelGGL((conf_tile_kernel_batch<T, DIM, P>), dim3(ntiles), dim3(threads), lds[c], st,
+        hipLaunchKernelGGL((conf_tile_kernel_batch<T, DIM, P, CAP>), dim3(ntiles), dim3(threads), lds[c], st,
                            tables[c], tile_start, nplans);
       if (nblocks)
         hipLaunchKernelGGL((fixup_kernel_rec<T>), dim3(nblocks), dim3(256), 0, st, fixes[c], block_start,
@@ -569,13 +569,16 @@ struct PersistGroup : GroupBase {
 };
 
 template <typename T>
-GroupBase *make_group(bool conforming, int dim, int p, const std::vector<dkt_plan *> &pls,
+// kind: 0 reference-semantics tiles, 1 conforming tiles, 2 conforming tiles
+// with per-level captured element matrices.
+GroupBase *make_group(int kind, int dim, int p, const std::vector<dkt_plan *> &pls,
                       const std::vector<const void *> &xs, const std::vector<void *> &ys, int nrhs, int masked,
                       const std::vector<const int *> &cancel = {}, DevArena *arena = nullptr) {
-#define DKT_GROUP(D, PP)                                                                       \
-  if (dim == D && p == PP) {                                                                   \
-    if (conforming) return new ConfGroup<T, D, PP>(pls, xs, ys, nrhs, masked, cancel, arena);  \
-    return new Group<T, D, PP>(pls, xs, ys, nrhs, masked, cancel, arena);                      \
+#define DKT_GROUP(D, PP)                                                                             \
+  if (dim == D && p == PP) {                                                                         \
+    if (kind == 2) return new ConfGroup<T, D, PP, true>(pls, xs, ys, nrhs, masked, cancel, arena);   \
+    if (kind == 1) return new ConfGroup<T, D, PP, false>(pls, xs, ys, nrhs, masked, cancel, arena);  \
+    return new Group<T, D, PP>(pls, xs, ys, nrhs, masked, cancel, arena);                            \
   }
   DKT_GROUP(2, 1)
   DKT_GROUP(3, 1)
@@ -586,14 +589,14 @@ GroupBase *make_group(bool conforming, int dim, int p, const std::vector<dkt_pla
   throw std::invalid_argument("unsupported (dim, order)");
 }
 
-// Plans grouped by (conforming, dim, p): indices into the caller's list.
+// Plans grouped by (kind as make_group's, dim, p): indices into the caller's list.
 std::map<std::tuple<int, int, int>, std::vector<int>> by_kind_of(dkt_plan *const *plans, int nplans) {
   std::map<std::tuple<int, int, int>, std::vector<int>> by_kind;
   for (int k = 0; k < nplans; k++) {
     dkt_plan *pl = plans[k];
-    if (pl->dry || pl->simple || pl->per_level)
+    if (pl->dry || pl->simple || (pl->per_level && !pl->conforming))
       throw std::invalid_argument("batch plans must be tiled device plans (reference or conforming)");
-    by_kind[{pl->conforming ? 1 : 0, pl->dim, pl->order}].push_back(k);
+    by_kind[{pl->conforming ? (pl->per_level ? 2 : 1) : 0, pl->dim, pl->order}].push_back(k);
   }
   return by_kind;
 }
@@ -623,8 +626,8 @@ std::unique_ptr<MatvecLauncher> dkt::batch_matvec(int dtype, const std::vector<d
       cs.push_back(cancel[k]);
     }
     const auto [conf, dim, p] = kv.first;
-    out->groups.emplace_back(dtype == 0 ? make_group<double>(conf != 0, dim, p, pls, xs, ys, 1, mask, cs, arena)
-                                        : make_group<float>(conf != 0, dim, p, pls, xs, ys, 1, mask, cs, arena));
+    out->groups.emplace_back(dtype == 0 ? make_group<double>(conf, dim, p, pls, xs, ys, 1, mask, cs, arena)
+                                        : make_group<float>(conf, dim, p, pls, xs, ys, 1, mask, cs, arena));
   }
   return out;
 }
@@ -687,8 +690,8 @@ extern "C" int dkt_batch_create(dkt_plan *const *plans, int nplans, int dtype, i
         ys.push_back(y[k]);
       }
       const auto [conf, dim, p] = kv.first;
-      GroupBase *g = dtype == 0 ? make_group<double>(conf != 0, dim, p, pls, xs, ys, nrhs, masked)
-                                : make_group<float>(conf != 0, dim, p, pls, xs, ys, nrhs, masked);
+      GroupBase *g = dtype == 0 ? make_group<double>(conf, dim, p, pls, xs, ys, nrhs, masked)
+                                : make_group<float>(conf, dim, p, pls, xs, ys, nrhs, masked);
       b->groups.emplace_back(g);
     }
     b->use_graph = (flags & DKT_BATCH_GRAPH) != 0;

@@ -1023,7 +1023,8 @@ extern "C" int dkt_cg_engine_add_mode(dkt_cg_engine *e, dkt_plan *pl, int mode, 
   return guarded([&] {
     if (!e || !pl || !b || !x || !slot) throw std::invalid_argument("null pointer");
     if (pl->dry) throw std::invalid_argument("a dry-run plan cannot solve");
-    if (pl->simple || pl->per_level) throw std::invalid_argument("batched solves need tiled device plans");
+    if (pl->simple || (pl->per_level && !pl->conforming))
+      throw std::invalid_argument("batched solves need tiled device plans");
     if (max_iter < 0 || !(tol >= 0.0)) throw std::invalid_argument("max_iter >= 0 and tol >= 0 required");
     if ((const void *)b == (const void *)x) throw std::invalid_argument("b and x must not alias");
     if (mode == DKT_CG_MIXED) mode = DKT_CG_MIXED_RELIABLE;

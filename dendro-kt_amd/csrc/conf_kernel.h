@@ -53,7 +53,10 @@ struct ConfArgs {
 
 // One tile of one plan (the workgroup may have more work-items than the
 // plan's tile_leaves; they idle in the element phases and help in the loops).
-template <typename T, int DIM, int P>
+// CAP: per-level captured element matrices (DKT_OP_CAPTURED), kref pointing
+// to [level][nPe][nPe]; a separate instantiation, since the dense product's
+// registers would otherwise count against the closed-form operators too.
+template <typename T, int DIM, int P, bool CAP = false>
 __device__ __forceinline__ void conf_tile_body(const ConfArgs &A, int tile, const T *__restrict__ kref,
                                               const T *__restrict__ term_w, const T *__restrict__ x,
                                               T *__restrict__ y, T *__restrict__ partial) {
@@ -133,7 +136,19 @@ __device__ __forceinline__ void conf_tile_body(const ConfArgs &A, int tile, cons
       T u[NPE];
 #pragma unroll
       for (int r = 0; r < NPE; r++) u[r] = x_loc[loc[r] * nq + q];
-      if constexpr (P == 1) {
+      if constexpr (CAP) {
+        // Per-level element matrices (row-major, v = K u; the level carries
+        // the element size, so no scale): the lanes of a wavefront mostly
+        // share a level, so each entry is one broadcast load.
+        const T *K = kref + (size_t)lev * NPE * NPE;
+#pragma unroll
+        for (int i = 0; i < NPE; i++) {
+          T acc = T(0);
+#pragma unroll
+          for (int j = 0; j < NPE; j++) acc += K[i * NPE + j] * u[j];
+          v[i] = acc;
+        }
+      } else if constexpr (P == 1) {
         if (A.op == 0)
           q1_apply<T, DIM, 0>(u, v, scale);
         else if (A.op == 1)
@@ -197,13 +212,13 @@ __device__ __forceinline__ void conf_tile_body(const ConfArgs &A, int tile, cons
   }
 }
 
-template <typename T, int DIM, int P>
+template <typename T, int DIM, int P, bool CAP = false>
 __global__ void __launch_bounds__(kTileThreads) conf_tile_kernel(ConfArgs A, const T *__restrict__ kref,
                                                                  const T *__restrict__ term_w,
                                                                  const T *__restrict__ x, T *__restrict__ y,
                                                                  T *__restrict__ partial) {
   if (A.cancel && *A.cancel) return;
-  conf_tile_body<T, DIM, P>(A, xcd_tile(blockIdx.x, gridDim.x), kref, term_w, x, y, partial);
+  conf_tile_body<T, DIM, P, CAP>(A, xcd_tile(blockIdx.x, gridDim.x), kref, term_w, x, y, partial);
 }
 
 // LDS bytes of conf_tile_kernel for nq right-hand sides.

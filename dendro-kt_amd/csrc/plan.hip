@@ -1193,8 +1193,8 @@ void run_conf(dkt_plan *pl, int nrhs, const T *x, T *y, int mask, hipStream_t st
               const ExtScratch *ext) {
   constexpr int NPE = Pow<P + 1, DIM>::v;
   const bool f64 = sizeof(T) == 8;
-  const T *kmat = P == 2 ? (f64 ? (const T *)pl->kfac64.p : (const T *)pl->kfac32.p)
-                         : (f64 ? (const T *)pl->kmat64.p : (const T *)pl->kmat32.p);
+  const T *kmat = P == 2 && !pl->per_level ? (f64 ? (const T *)pl->kfac64.p : (const T *)pl->kfac32.p)
+                                           : (f64 ? (const T *)pl->kmat64.p : (const T *)pl->kmat32.p);
   const T *tw = f64 ? (const T *)pl->term_w64.p : (const T *)pl->term_w32.p;
   auto lds_for = [&](int nq) { return conf_lds_bytes<T, NPE>(pl->max_nodes, nq); };
   if (lds_for(1) > kLdsLimit) throw std::invalid_argument("tile does not fit in LDS");
@@ -1226,8 +1226,12 @@ void run_conf(dkt_plan *pl, int nrhs, const T *x, T *y, int mask, hipStream_t st
     A.nq = nq;
     if (pl->profiling && !ext) HIP_OK(hipEventRecord(pl->next_event(), st));
     if (pl->ntiles) {
-      hipLaunchKernelGGL((conf_tile_kernel<T, DIM, P>), dim3(pl->ntiles), dim3(pl->tile_leaves), lds_for(nq), st, A,
-                         kmat, tw, x + q0, y + q0, part);
+      if (pl->per_level)
+        hipLaunchKernelGGL((conf_tile_kernel<T, DIM, P, true>), dim3(pl->ntiles), dim3(pl->tile_leaves), lds_for(nq),
+                           st, A, kmat, tw, x + q0, y + q0, part);
+      else
+        hipLaunchKernelGGL((conf_tile_kernel<T, DIM, P, false>), dim3(pl->ntiles), dim3(pl->tile_leaves),
+                           lds_for(nq), st, A, kmat, tw, x + q0, y + q0, part);
       HIP_OK(hipGetLastError());
     }
     if (pl->profiling && !ext) HIP_OK(hipEventRecord(pl->next_event(), st));
@@ -1243,7 +1247,7 @@ void run_conf(dkt_plan *pl, int nrhs, const T *x, T *y, int mask, hipStream_t st
 template <typename T>
 void run_matvec(dkt_plan *pl, int nrhs, const T *x, T *y, int mask, hipStream_t st, const int *cancel,
                 const ExtScratch *ext) {
-  const bool simple = pl->simple || pl->per_level;
+  const bool simple = pl->simple || (pl->per_level && !pl->conforming);
 #define DKT_DISPATCH(D, P)                                         \
   if (pl->dim == D && pl->order == P) {                            \
     if (simple)                                                    \
@@ -1328,7 +1332,7 @@ extern "C" int dkt_plan_create(const dkt_da *da, int op, const double *op_data, 
     pl->dirichlet.upload(dir);
     const int tshift = (flags >> DKT_PLAN_TILE_SHIFT) & 15;
     if (tshift && (tshift < 6 || tshift > 8)) throw std::invalid_argument("tile size must be 64, 128 or 256 leaves");
-    if (pl->simple || pl->per_level)
+    if (pl->simple || (pl->per_level && !pl->conforming))
       build_simple(pl.get(), mp, tr);
     else if (pl->conforming)
       build_conf_tiled(pl.get(), mp, tr, tshift ? 1 << tshift : 256);

@@ -297,7 +297,7 @@ class Ready:
 
 class Server:
     def __init__(self, dkt, n_streams=4, hbm_budget_bytes=200e9, batch=0, cache=None, max_wait=0.5,
-                 operator="conforming", n_builders=2):
+                 operator="conforming", n_builders=2, n_direct=1):
         import torch
         self.torch = torch
         self.dkt = dkt
@@ -317,8 +317,9 @@ class Server:
         work = self._work_batched if batch > 0 else self._work
         self.workers = [threading.Thread(target=work, args=(torch.cuda.Stream(),), daemon=True)
                         for _ in range(n_streams)]
-        if batch > 0:  # a direct worker for the requests the engines do not run
-            self.workers.append(threading.Thread(target=self._work, args=(torch.cuda.Stream(), True), daemon=True))
+        if batch > 0:  # direct workers for the requests the engines do not run
+            self.workers += [threading.Thread(target=self._work, args=(torch.cuda.Stream(), True), daemon=True)
+                             for _ in range(n_direct)]
         for t in self.workers:
             t.start()
 

@@ -32,7 +32,10 @@ extern "C" {
  *   FEM/examples/src/heatVec.cpp:31-78).
  * DKT_OP_IDENTITY: identity elemental op (test/testMatvec.cpp:187-198).
  * DKT_OP_CAPTURED: dense per-level elemental matrices supplied by the caller
- *   (captured from a user elementalMatVec, FEM/include/feMatrix.h:56). */
+ *   (captured from a user elementalMatVec, FEM/include/feMatrix.h:56),
+ *   op_data = [maxdepth + 1][nPe][nPe] row-major (out = K in).  With the
+ *   reference's traversal semantics they run on the per-leaf/per-node kernels;
+ *   with DKT_PLAN_CONFORMING on the tiled conforming kernel. */
 #define DKT_OP_STIFFNESS 0
 #define DKT_OP_MASS 1
 #define DKT_OP_IDENTITY 2
@@ -46,9 +49,8 @@ extern "C" {
 /* The conforming operator sum_e C_e^T K_e C_e (symmetric; see
  * dkt_da_conforming_terms) instead of the reference's traversal semantics;
  * runs on tiles of mesh elements (default 256, halved while the LDS
- * footprint does not fit; at most 2048
- * constraint terms per tile), or on the per-leaf/per-node kernels with
- * DKT_PLAN_SIMPLE. */
+ * footprint does not fit; at most 2048 constraint terms per tile), or on the
+ * per-leaf/per-node kernels with DKT_PLAN_SIMPLE. */
 #define DKT_PLAN_CONFORMING 4
 /* Leaves per tile of the tiled path: flags |= log2(leaves) << DKT_PLAN_TILE_SHIFT
  * with leaves in {64, 128, 256}; 0 = 128, halved while the LDS footprint does
