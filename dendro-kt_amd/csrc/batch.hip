@@ -193,7 +193,7 @@ __global__ void __launch_bounds__(kTileThreads) conf_tile_kernel_batch(const Con
                                                                        int nplans) {
   const int t = xcd_tile(blockIdx.x, gridDim.x);
   const int k = find_range(tile_start, nplans, t);
-  const ConfBatchPlan<T> &bp = plans[k];
+  const ConfBatchPlan<T> bp = plans[k];  // a copy: its fields live in scalar registers
   if (bp.A.cancel && *bp.A.cancel) return;
   conf_tile_body<T, DIM, P, CAP>(bp.A, t - tile_start[k], bp.kref, bp.term_w, bp.x, bp.y, bp.partial);
 }

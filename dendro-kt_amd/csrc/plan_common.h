@@ -257,6 +257,23 @@ __device__ __forceinline__ void q1_apply(const T *u, T *v, T scale) {
 // table, not summed per work-item: per-lane sums kept the 27 products live in
 // vector registers (166 against 113 VGPRs for FP64 in 3D).  scale is a power of
 // two, so applying it last gives the bits of applying it with D.
+// *p for an address that is the same on every lane, held in scalar registers
+// from then on.  The compiler does this itself for a kernel's own pointer
+// arguments; through a pointer read from memory (a batch's plan record) it
+// cannot prove that the kernel's stores leave the value unchanged, and would
+// keep one copy per lane in vector registers.
+template <typename T>
+__device__ __forceinline__ T uniform_load(const T *p) {
+  if constexpr (sizeof(T) == 8) {
+    const uint64_t b = __builtin_bit_cast(uint64_t, *p);
+    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)b);
+    const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(b >> 32));
+    return __builtin_bit_cast(T, ((uint64_t)hi << 32) | lo);
+  } else {
+    return __builtin_bit_cast(T, (uint32_t)__builtin_amdgcn_readfirstlane(__builtin_bit_cast(uint32_t, *p)));
+  }
+}
+
 template <typename T, int DIM>
 __device__ __forceinline__ void q2_apply(T *u, const T *__restrict__ f, int op, T scale) {
   constexpr int NPE = Pow<3, DIM>::v;
@@ -267,7 +284,7 @@ __device__ __forceinline__ void q2_apply(T *u, const T *__restrict__ f, int op, 
   }
   T B[9];
 #pragma unroll
-  for (int k = 0; k < 9; k++) B[k] = f[k];
+  for (int k = 0; k < 9; k++) B[k] = uniform_load(f + k);
   constexpr int strides[4] = {1, 3, 9, 27};
 #pragma unroll
   for (int d = 0; d < DIM; d++) {
@@ -283,7 +300,7 @@ __device__ __forceinline__ void q2_apply(T *u, const T *__restrict__ f, int op, 
   }
   if (op == 0)
 #pragma unroll
-    for (int r = 0; r < NPE; r++) u[r] *= f[12 + r];
+    for (int r = 0; r < NPE; r++) u[r] *= uniform_load(f + 12 + r);
 #pragma unroll
   for (int d = 0; d < DIM; d++) {
     const int st = strides[d];

@@ -518,14 +518,19 @@ struct BatchPlan {
   int nshared;
 };
 
-// Index of the last start <= v (starts ascending, starts[0] = 0).
+// Index of the last start <= v (starts ascending, starts[0] = 0).  v is the
+// same for the whole workgroup at every call site (a tile or block index), so
+// the result is made wave-uniform explicitly: the record it selects is then
+// read with scalar loads into scalar registers instead of occupying vector
+// registers for its pointers (tile_kernel_batch<float, 3, 2, false> otherwise
+// needs more VGPRs than the single-plan kernel).
 __device__ __forceinline__ int find_range(const int *__restrict__ starts, int n, int v) {
   int lo = 0, hi = n;
   while (hi - lo > 1) {
     const int mid = (lo + hi) >> 1;
     if (starts[mid] <= v) lo = mid; else hi = mid;
   }
-  return lo;
+  return __builtin_amdgcn_readfirstlane(lo);
 }
 
 // All tiles of a batch of same-(dim, p) plans in one launch: batch tile
@@ -535,7 +540,7 @@ __global__ void __launch_bounds__(kTileThreads) __attribute__((amdgpu_waves_per_
 tile_kernel_batch(const BatchPlan<T> *__restrict__ plans, const int *__restrict__ tile_start, int nplans) {
   const int t = xcd_tile(blockIdx.x, gridDim.x);
   const int k = find_range(tile_start, nplans, t);
-  const BatchPlan<T> &bp = plans[k];
+  const BatchPlan<T> bp = plans[k];  // a copy: its fields live in scalar registers
   if (bp.A.cancel && *bp.A.cancel) return;
   tile_body<T, DIM, P, ONE>(bp.A, t - tile_start[k], bp.kref, bp.carry_w, bp.x, bp.y, bp.partial);
 }
