@@ -239,9 +239,11 @@ def batch_leg(dkt, torch, ntrees):
     """Batched matvec: ntrees independent 3D trees (workloads.batch_trees: element
     counts log-uniform in [1e3, 2e5], each refined around a random sphere with
     centre U[0.2,0.8]^3 and radius U[0.1,0.3]), p = 1 and 2 alternating, 8
-    right-hand sides each, FP32, tile sizes per (dim, p) from the autotuner, one
-    hipGraph replay per step (one launch per (dim, p) plus its fix-up), against the
-    same plans launched one by one."""
+    right-hand sides each, FP32, tile sizes and columns per launch from the
+    autotuner, one hipGraph replay per step (per (dim, p): its tile launches and
+    fix-ups), with the vectors node-major ([N][8]) and column-major ([8][N]; one
+    column per launch), the faster layout reported, against the same plans
+    launched one by one."""
     wl = _pkg_mod("dkt_workloads", "workloads.py")
     at = _pkg_mod("dkt_autotune", "autotune.py")
     t0 = time.time()
@@ -251,18 +253,18 @@ def batch_leg(dkt, torch, ntrees):
     log(f"batch: {ntrees} trees, {sizes.sum()} elements (min {sizes.min()}, max {sizes.max()}), "
         f"built in {time.time() - t0:.1f}s")
     nrhs = 8
-    tiles = {}
+    tiles, cols = {}, {}
     for p in (1, 2):
         group = [da for da in das if da.order == p]
         if group:
             sample = max(group, key=lambda d: d.n_elements)
-            tiles[p] = at.autotune(dkt, sample, nrhs=nrhs)["best"]["f32"]
+            r = at.autotune(dkt, sample, nrhs=nrhs)
+            tiles[p], cols[p] = r["best"]["f32"], r["columns"]["f32"]
     plans = [dkt.Plan(da, dkt.OP_STIFFNESS, tile_leaves=tiles[da.order]) for da in das]
     xs = [torch.rand(da.n_nodes, nrhs, dtype=torch.float32, device="cuda") * 2 - 1 for da in das]
     ys = [torch.empty_like(x) for x in xs]
     dofs = sum(da.n_nodes for da in das) * nrhs
     s = torch.cuda.Stream()
-    b = dkt.Batch(plans, xs, ys, graph=True)
 
     def timed(fn, reps=20):
         with torch.cuda.stream(s):
@@ -277,12 +279,29 @@ def batch_leg(dkt, torch, ntrees):
         e1.synchronize()
         return e0.elapsed_time(e1) / reps
 
-    ms_graph = timed(lambda: b.run(s.cuda_stream))
+    # One columns-per-launch value serves the whole batch: the autotuner's
+    # choices per (dim, p), each timed on the batch, the faster kept.
+    per_cols = {}
+    for c in sorted(set(cols.values())):
+        b = dkt.Batch(plans, xs, ys, graph=True, columns=c)
+        per_cols[c] = timed(lambda: b.run(s.cuda_stream))
+    # The same columns stored column-major ([nrhs][N] per tree): one
+    # contiguous column per launch.
+    xc = [x.t().contiguous() for x in xs]
+    yc = [torch.empty_like(x) for x in xc]
+    bc = dkt.Batch(plans, xc, yc, graph=True, column_major=True)
+    ms_colmajor = timed(lambda: bc.run(s.cuda_stream))
+    best_cols = min(per_cols, key=per_cols.get)
+    layout = "node-major" if per_cols[best_cols] <= ms_colmajor else "column-major"
+    ms_graph = min(per_cols[best_cols], ms_colmajor)
     ms_each = timed(lambda: [p.matvec_torch(x, y, stream=s.cuda_stream) for p, x, y in zip(plans, xs, ys)])
     return {"trees": ntrees, "elements": int(sizes.sum()), "elements_min": int(sizes.min()),
             "elements_median": int(np.median(sizes)), "elements_max": int(sizes.max()),
             "nodes": sum(da.n_nodes for da in das), "nrhs": nrhs, "dtype": "f32",
             "tile_leaves": {f"3d_p{p}": t for p, t in tiles.items()},
+            "columns_per_launch": {f"3d_p{p}": c for p, c in cols.items()}, "columns_used": best_cols,
+            "ms_graph_by_columns": {str(c): round(v, 4) for c, v in per_cols.items()},
+            "ms_graph_column_major": round(ms_colmajor, 4), "layout": layout,
             "ms_graph": round(ms_graph, 4), "ms_one_launch_per_tree": round(ms_each, 4),
             "gdof_s_graph": round(dofs / (ms_graph * 1e-3) / 1e9, 3),
             "gdof_s_one_launch_per_tree": round(dofs / (ms_each * 1e-3) / 1e9, 3)}

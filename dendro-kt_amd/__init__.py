@@ -39,6 +39,8 @@ CG_MIXED_RESTART = 2
 CG_MIXED_RELIABLE = 3
 BATCH_GRAPH = 1
 BATCH_PERSISTENT = 2
+BATCH_COLUMNS_SHIFT = 8  # DKT_BATCH_COLUMNS_SHIFT
+BATCH_COLUMN_MAJOR = 4
 ERR_BREAKDOWN = 2
 STOP_NAMES = {1: "converged", 2: "max_iter", 3: "breakdown", 4: "stalled"}
 
@@ -527,10 +529,14 @@ class Batch:
 
     ``xs``/``ys`` are torch device tensors [N_k] or [N_k, nrhs] (one dtype);
     with ``graph=True`` the launches are captured into a hipGraph on the first
-    run (needs a non-default stream) and replayed afterwards.
+    run (needs a non-default stream) and replayed afterwards.  ``columns``
+    caps the right-hand sides per launch (a power of two; 1 runs the
+    single-column kernels once per column; 0 = as many as fit in LDS).  With
+    ``column_major=True`` the vectors are [nrhs, N_k] and every launch carries
+    one contiguous column.
     """
 
-    def __init__(self, plans, xs, ys, masked=False, graph=False, persistent=False):
+    def __init__(self, plans, xs, ys, masked=False, graph=False, persistent=False, columns=0, column_major=False):
         import torch
         L = load()
         n = len(plans)
@@ -539,11 +545,18 @@ class Batch:
         dt = xs[0].dtype
         if dt not in (torch.float64, torch.float32):
             raise DktError("x must be float64 or float32")
-        nrhs = 1 if xs[0].dim() == 1 else xs[0].shape[1]
+        if column_major:  # [nrhs, N_k]
+            if xs[0].dim() != 2:
+                raise DktError("column-major vectors are [nrhs, N] tensors")
+            nrhs = xs[0].shape[0]
+        else:
+            nrhs = 1 if xs[0].dim() == 1 else xs[0].shape[1]
         for p, x, y in zip(plans, xs, ys):
-            if x.dtype != dt or y.dtype != dt or x.shape != y.shape or x.shape[0] != p.n_nodes:
+            n_k = x.shape[1] if column_major else x.shape[0]
+            if x.dtype != dt or y.dtype != dt or x.shape != y.shape or n_k != p.n_nodes:
                 raise DktError("vector shape or dtype does not match")
-            if (1 if x.dim() == 1 else x.shape[1]) != nrhs or not (x.is_contiguous() and y.is_contiguous()):
+            cols = x.shape[0] if column_major else (1 if x.dim() == 1 else x.shape[1])
+            if cols != nrhs or not (x.is_contiguous() and y.is_contiguous()):
                 raise DktError("all vectors need the same nrhs and contiguous storage")
         P = ctypes.c_void_p
         self._keep = (list(plans), list(xs), list(ys))
@@ -552,6 +565,12 @@ class Batch:
         yp = (P * n)(*[y.data_ptr() for y in ys])
         h = P()
         flags = (BATCH_GRAPH if graph else 0) | (BATCH_PERSISTENT if persistent else 0)
+        if column_major:
+            flags |= BATCH_COLUMN_MAJOR
+        if columns:
+            if columns & (columns - 1) or not 1 <= columns <= 64:
+                raise DktError("columns per launch must be a power of two in [1, 64]")
+            flags |= (columns.bit_length()) << BATCH_COLUMNS_SHIFT
         _chk(L.dkt_batch_create(ph, n, 0 if dt == torch.float64 else 1, nrhs, xp, yp, int(masked), flags,
                                 ctypes.byref(h)))
         self.h = h

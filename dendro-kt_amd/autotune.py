@@ -1,11 +1,17 @@
-"""Tile-size autotuner for the tiled matvec (no counterpart in the reference).
+"""Tile-size and columns-per-launch autotuner for the tiled matvec (no
+counterpart in the reference).
 
 The tiled kernel's tile size trades LDS footprint (occupancy) against shared
-nodes between tiles.  ``autotune`` builds the plan of a sample mesh with each
-candidate size, times the FP64 and FP32 matvecs with HIP events on the launch
-stream, and returns the fastest size per precision.  Results are cached per
-(dim, order) for the process and can be stored to a JSON file, so a server
-tunes once per (dim, p, columns) and passes ``tile_leaves`` to later plans.
+nodes between tiles.  With several right-hand sides a second choice appears:
+how many columns one launch carries.  More columns read the plan's maps once
+for all of them; fewer columns run kernels with fewer registers and more
+waves per SIMD (one column runs the single-column kernels).  ``autotune``
+builds the plan of a sample mesh with each candidate tile size, times the
+FP64 and FP32 matvecs with HIP events on the launch stream (with nrhs > 1 as
+a one-plan batch for each candidate column count), and returns the fastest
+choice per precision.  Results are cached per (dim, order, nrhs) for the
+process and can be stored to a JSON file, so a server tunes once and passes
+``tile_leaves`` (and ``columns`` to Batch) to later plans.
 """
 import json
 import os
@@ -14,9 +20,20 @@ CANDIDATES = (64, 128, 256)
 _cache = {}
 
 
+def _column_candidates(nrhs):
+    if nrhs <= 1:
+        return (0,)
+    out, c = [], 1
+    while c < nrhs:
+        out.append(c)
+        c *= 2
+    return tuple(out) + (0,)  # 0: as many as fit in LDS
+
+
 def autotune(dkt, da, candidates=CANDIDATES, reps=10, warmup=3, nrhs=1):
-    """Fastest tile size per precision for matvecs with `nrhs` columns on `da`:
-    dict(best={"f64": T, "f32": T}, ms={T: {"f64": ms, "f32": ms}})."""
+    """Fastest tile size (and, for nrhs > 1, columns per launch) per precision
+    for matvecs with `nrhs` columns on `da`: dict(best={"f64": T, "f32": T},
+    columns={"f64": c, "f32": c}, ms={(T, c): {"f64": ms, "f32": ms}})."""
     import torch
     key = (da.dim, da.order, nrhs)
     if key in _cache:
@@ -30,31 +47,52 @@ def autotune(dkt, da, candidates=CANDIDATES, reps=10, warmup=3, nrhs=1):
             plan = dkt.Plan(da, dkt.OP_STIFFNESS, tile_leaves=T)
         except dkt.DktError:
             continue  # does not fit in LDS
-        row = {}
-        for name, dt in (("f64", torch.float64), ("f32", torch.float32)):
-            x = x64.to(dt)
-            y = torch.empty_like(x)
-            for _ in range(warmup):
-                plan.matvec_torch(x, y)
-            torch.cuda.synchronize()
-            plan.profile(True)
-            for _ in range(reps):
-                plan.matvec_torch(x, y)
-            _, tot, n = plan.profile_read()
-            plan.profile(False)
-            row[name] = tot / max(n, 1)
-        times[T] = row
+        for c in _column_candidates(nrhs):
+            row = {}
+            for name, dt in (("f64", torch.float64), ("f32", torch.float32)):
+                x = x64.to(dt)
+                y = torch.empty_like(x)
+                if nrhs == 1:
+                    for _ in range(warmup):
+                        plan.matvec_torch(x, y)
+                    torch.cuda.synchronize()
+                    plan.profile(True)
+                    for _ in range(reps):
+                        plan.matvec_torch(x, y)
+                    _, tot, n = plan.profile_read()
+                    plan.profile(False)
+                    row[name] = tot / max(n, 1)
+                else:
+                    b = dkt.Batch([plan], [x], [y], columns=c)
+                    st = torch.cuda.current_stream()
+                    for _ in range(warmup):
+                        b.run(st.cuda_stream)
+                    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                    e0.record(st)
+                    for _ in range(reps):
+                        b.run(st.cuda_stream)
+                    e1.record(st)
+                    e1.synchronize()
+                    row[name] = e0.elapsed_time(e1) / reps
+                    del b
+            times[(T, c)] = row
         del plan
     if not times:
         raise RuntimeError("no tile size fits")
-    best = {name: min(times, key=lambda T: times[T][name]) for name in ("f64", "f32")}
-    _cache[key] = {"best": best, "ms": times}
+    best, cols = {}, {}
+    for name in ("f64", "f32"):
+        T, c = min(times, key=lambda k: times[k][name])
+        best[name], cols[name] = T, c
+    _cache[key] = {"best": best, "columns": cols, "ms": times}
     return _cache[key]
 
 
 def save(path):
+    out = {}
+    for k, v in _cache.items():
+        out[f"{k[0]}d_p{k[1]}_q{k[2]}"] = dict(v, ms={f"{T}_{c}": r for (T, c), r in v["ms"].items()})
     with open(path, "w") as f:
-        json.dump({f"{k[0]}d_p{k[1]}_q{k[2]}": v for k, v in _cache.items()}, f, indent=1)
+        json.dump(out, f, indent=1)
 
 
 def load(path):
@@ -65,5 +103,6 @@ def load(path):
             dim, rest = k.split("d_p")
             p, q = rest.split("_q")
             v["best"] = {n: int(t) for n, t in v["best"].items()}
-            v["ms"] = {int(t): r for t, r in v["ms"].items()}
+            v["columns"] = {n: int(c) for n, c in v.get("columns", {"f64": 0, "f32": 0}).items()}
+            v["ms"] = {tuple(int(a) for a in t.split("_")): r for t, r in v["ms"].items()}
             _cache[(int(dim), int(p), int(q))] = v

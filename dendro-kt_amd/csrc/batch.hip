@@ -66,7 +66,8 @@ struct Group : GroupBase {
   std::vector<int> nqs;  // columns per chunk
 
   Group(const std::vector<dkt_plan *> &pls, const std::vector<const void *> &xs, const std::vector<void *> &ys,
-        int nrhs, int masked, const std::vector<const int *> &cancel, DevArena *arena = nullptr) {
+        int nrhs, int masked, const std::vector<const int *> &cancel, DevArena *arena = nullptr, int max_cols = 0,
+        bool colmajor = false) {
     mem.arena = arena;
     constexpr int NPE = Pow<P + 1, DIM>::v;
     const bool f64 = sizeof(T) == 8;
@@ -74,7 +75,8 @@ struct Group : GroupBase {
       return TileDims{pl->max_nodes, pl->max_grp, pl->max_he, pl->max_rd, pl->max_cr, pl->tile_leaves, pl->max_hang};
     };
     // Columns per launch: the most every plan's tiles fit in LDS.
-    int chunk = nrhs;
+    // Column-major vectors ([nrhs][N]): one contiguous column per launch.
+    int chunk = colmajor ? 1 : (max_cols > 0 ? std::min(nrhs, max_cols) : nrhs);
     for (dkt_plan *pl : pls) {
       if (tile_lds_bytes<T, NPE>(dims_of(pl), 1) > kLdsLimit) throw std::invalid_argument("tile does not fit in LDS");
       while (chunk > 1 && tile_lds_bytes<T, NPE>(dims_of(pl), chunk) > kLdsLimit) chunk--;
@@ -114,13 +116,14 @@ struct Group : GroupBase {
         A.op = pl->op;
         A.hp = pl->hp;
         A.nq = nq;
-        A.ld = nrhs;
+        A.ld = colmajor ? 1 : nrhs;
         A.dims = dims_of(pl);
         hp[k].kref = P == 2 ? (f64 ? (const T *)pl->kfac64.p : (const T *)pl->kfac32.p)
                             : (f64 ? (const T *)pl->kmat64.p : (const T *)pl->kmat32.p);
         hp[k].carry_w = f64 ? (const T *)pl->carry_w64.p : (const T *)pl->carry_w32.p;
-        hp[k].x = (const T *)xs[k] + q0;
-        hp[k].y = (T *)ys[k] + q0;
+        const size_t coff = colmajor ? (size_t)q0 * pl->N : (size_t)q0;
+        hp[k].x = (const T *)xs[k] + coff;
+        hp[k].y = (T *)ys[k] + coff;
         hp[k].partial = partial + poff[k];
         hp[k].snode = pl->snode.p;
         hp[k].soff = pl->soff.p;
@@ -212,12 +215,13 @@ struct ConfGroup : GroupBase {
   std::vector<size_t> lds;
 
   ConfGroup(const std::vector<dkt_plan *> &pls, const std::vector<const void *> &xs, const std::vector<void *> &ys,
-            int nrhs, int masked, const std::vector<const int *> &cancel, DevArena *arena = nullptr) {
+            int nrhs, int masked, const std::vector<const int *> &cancel, DevArena *arena = nullptr,
+            int max_cols = 0, bool colmajor = false) {
     mem.arena = arena;
     constexpr int NPE = Pow<P + 1, DIM>::v;
     const bool f64 = sizeof(T) == 8;
     auto lds_of = [](const dkt_plan *pl, int nq) { return conf_lds_bytes<T, NPE>(pl->max_nodes, nq); };
-    int chunk = nrhs;
+    int chunk = colmajor ? 1 : (max_cols > 0 ? std::min(nrhs, max_cols) : nrhs);
     for (dkt_plan *pl : pls) {
       if (lds_of(pl, 1) > kLdsLimit) throw std::invalid_argument("tile does not fit in LDS");
       while (chunk > 1 && lds_of(pl, chunk) > kLdsLimit) chunk--;
@@ -259,17 +263,18 @@ struct ConfGroup : GroupBase {
         A.op = pl->op;
         A.hp = pl->hp;
         A.nq = nq;
-        A.ld = nrhs;
+        A.ld = colmajor ? 1 : nrhs;
         A.max_points = pl->max_nodes;
         A.tile_leaves = pl->tile_leaves;
         hp[k].kref = P == 2 && !pl->per_level ? (f64 ? (const T *)pl->kfac64.p : (const T *)pl->kfac32.p)
                                               : (f64 ? (const T *)pl->kmat64.p : (const T *)pl->kmat32.p);
         hp[k].term_w = f64 ? (const T *)pl->term_w64.p : (const T *)pl->term_w32.p;
-        hp[k].x = (const T *)xs[k] + q0;
-        hp[k].y = (T *)ys[k] + q0;
+        const size_t coff = colmajor ? (size_t)q0 * pl->N : (size_t)q0;
+        hp[k].x = (const T *)xs[k] + coff;
+        hp[k].y = (T *)ys[k] + coff;
         hp[k].partial = partial + poff[k];
-        hf[k] = FixRec<T>{pl->snode.p, pl->soff.p, partial + poff[k], (T *)ys[k] + q0, A.dir_out, A.cancel,
-                          nq, nrhs, (int)pl->nshared};
+        hf[k] = FixRec<T>{pl->snode.p, pl->soff.p, partial + poff[k], (T *)ys[k] + coff, A.dir_out, A.cancel,
+                          nq, A.ld, (int)pl->nshared};
       }
       tables.push_back(mem.upload(hp));
       fixes.push_back(mem.upload(hf));
@@ -573,12 +578,15 @@ template <typename T>
 // with per-level captured element matrices.
 GroupBase *make_group(int kind, int dim, int p, const std::vector<dkt_plan *> &pls,
                       const std::vector<const void *> &xs, const std::vector<void *> &ys, int nrhs, int masked,
-                      const std::vector<const int *> &cancel = {}, DevArena *arena = nullptr) {
-#define DKT_GROUP(D, PP)                                                                             \
-  if (dim == D && p == PP) {                                                                         \
-    if (kind == 2) return new ConfGroup<T, D, PP, true>(pls, xs, ys, nrhs, masked, cancel, arena);   \
-    if (kind == 1) return new ConfGroup<T, D, PP, false>(pls, xs, ys, nrhs, masked, cancel, arena);  \
-    return new Group<T, D, PP>(pls, xs, ys, nrhs, masked, cancel, arena);                            \
+                      const std::vector<const int *> &cancel = {}, DevArena *arena = nullptr, int max_cols = 0,
+                      bool colmajor = false) {
+#define DKT_GROUP(D, PP)                                                                                     \
+  if (dim == D && p == PP) {                                                                                 \
+    if (kind == 2)                                                                                           \
+      return new ConfGroup<T, D, PP, true>(pls, xs, ys, nrhs, masked, cancel, arena, max_cols, colmajor);    \
+    if (kind == 1)                                                                                           \
+      return new ConfGroup<T, D, PP, false>(pls, xs, ys, nrhs, masked, cancel, arena, max_cols, colmajor);   \
+    return new Group<T, D, PP>(pls, xs, ys, nrhs, masked, cancel, arena, max_cols, colmajor);                \
   }
   DKT_GROUP(2, 1)
   DKT_GROUP(3, 1)
@@ -655,6 +663,9 @@ extern "C" int dkt_batch_create(dkt_plan *const *plans, int nplans, int dtype, i
       if (!plans[k] || !x[k] || !y[k]) throw std::invalid_argument("null plan or vector");
       if (x[k] == (const void *)y[k]) throw std::invalid_argument("x and y must not alias");
     }
+    const int cbits = (flags >> DKT_BATCH_COLUMNS_SHIFT) & 15;
+    if (cbits > 7) throw std::invalid_argument("columns per launch must be 1, 2, 4, ..., 64");
+    const int max_cols = cbits ? 1 << (cbits - 1) : 0;
     auto b = std::make_unique<dkt_batch>();
     if (flags & DKT_BATCH_PERSISTENT)
       throw std::invalid_argument(
@@ -690,8 +701,10 @@ extern "C" int dkt_batch_create(dkt_plan *const *plans, int nplans, int dtype, i
         ys.push_back(y[k]);
       }
       const auto [conf, dim, p] = kv.first;
-      GroupBase *g = dtype == 0 ? make_group<double>(conf, dim, p, pls, xs, ys, nrhs, masked)
-                                : make_group<float>(conf, dim, p, pls, xs, ys, nrhs, masked);
+      const bool cm = (flags & DKT_BATCH_COLUMN_MAJOR) != 0;
+      GroupBase *g = dtype == 0
+                         ? make_group<double>(conf, dim, p, pls, xs, ys, nrhs, masked, {}, nullptr, max_cols, cm)
+                         : make_group<float>(conf, dim, p, pls, xs, ys, nrhs, masked, {}, nullptr, max_cols, cm);
       b->groups.emplace_back(g);
     }
     b->use_graph = (flags & DKT_BATCH_GRAPH) != 0;

@@ -315,6 +315,18 @@ typedef struct dkt_batch dkt_batch;
  * dkt_batch_create refuses the flag (its launch did not finish on the device
  * and the cause is not found). */
 #define DKT_BATCH_PERSISTENT 2
+/* Columns per launch for nrhs > 1: flags |= (log2(columns) + 1) <<
+ * DKT_BATCH_COLUMNS_SHIFT with columns in {1, 2, 4, ..., 64}; 0 = as many
+ * as the tiles' LDS allows.  One column per launch runs the single-column
+ * tile kernels (fewer registers, more waves per SIMD) once per column, over
+ * the same [N][nrhs] vectors; the autotuner chooses it per (dim, p).  Each
+ * column's result is bitwise the same either way. */
+#define DKT_BATCH_COLUMNS_SHIFT 8
+/* x[k], y[k] column-major, [nrhs][N_k] (column q at q * N_k), instead of
+ * [N_k][nrhs]: every launch carries one contiguous column (the single-column
+ * kernels, whose reads of x are then unit-stride).  Bitwise the results of the
+ * node-major layout, column by column. */
+#define DKT_BATCH_COLUMN_MAJOR 4
 int dkt_batch_create(dkt_plan *const *plans, int nplans, int dtype, int nrhs, const void *const *x, void *const *y,
                      int masked, int flags, dkt_batch **out);
 int dkt_batch_run(dkt_batch *batch, void *stream);

@@ -125,8 +125,9 @@ def test_persistent_batch_matches_per_plan(dkt, dtype, nrhs, graph):
 
 @pytest.mark.gpu
 def test_autotuner_picks_a_candidate_and_caches(dkt):
-    """The autotuner times every candidate tile size that fits, returns the fastest
-    per precision (for one and for eight columns), and answers again from its cache."""
+    """The autotuner times every candidate tile size that fits (with eight columns
+    also every columns-per-launch choice), returns the fastest per precision,
+    and answers again from its cache."""
     import importlib.util as ilu
     import os
     torch = _torch()
@@ -138,11 +139,48 @@ def test_autotuner_picks_a_candidate_and_caches(dkt):
     da = dkt.DA(dkt.balance_tree(dkt.sphere_tree(3, 30, 2, 6, (0.5, 0.5, 0.5), 0.3), 1), 1)
     for nrhs in (1, 8):
         r = at.autotune(dkt, da, reps=3, warmup=1, nrhs=nrhs)
-        assert set(r["ms"]) <= set(at.CANDIDATES) and r["ms"]
+        assert r["ms"] and {T for T, _ in r["ms"]} <= set(at.CANDIDATES)
+        assert {c for _, c in r["ms"]} == set(at._column_candidates(nrhs))
         for prec in ("f64", "f32"):
-            best = r["best"][prec]
-            assert best in r["ms"]
-            assert r["ms"][best][prec] == min(v[prec] for v in r["ms"].values())
+            key = (r["best"][prec], r["columns"][prec])
+            assert key in r["ms"]
+            assert r["ms"][key][prec] == min(v[prec] for v in r["ms"].values())
         assert at.autotune(dkt, da, nrhs=nrhs) is r
         plan = dkt.Plan(da, dkt.OP_STIFFNESS, tile_leaves=r["best"]["f32"])
         assert plan.info()["tile_leaves"] == r["best"]["f32"]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("dtype", ["f64", "f32"])
+def test_batch_columns_per_launch_bitwise(dkt, dtype):
+    """Capping the columns per launch (1 runs the single-column kernels once per
+    column over the same [N][nrhs] vectors; 2 and 4 run narrower multi-column
+    launches), and column-major [nrhs][N] vectors (one contiguous column per
+    launch), give every column bitwise the result of the default batch, for
+    p = 1 and p = 2 trees and for conforming plans."""
+    torch = _torch()
+    dt = torch.float64 if dtype == "f64" else torch.float32
+    das = [dkt.DA(dkt.balance_tree(dkt.sphere_tree(3, 30, 2, lev, c, 0.2), 1), p)
+           for lev, c, p in ((6, (0.4, 0.5, 0.5), 1), (5, (0.6, 0.4, 0.5), 2), (6, (0.5, 0.6, 0.4), 1))]
+    plans = [dkt.Plan(da, dkt.OP_STIFFNESS) for da in das]
+    plans.append(dkt.Plan(das[0], dkt.OP_STIFFNESS, conforming=True))
+    nrhs = 8
+    gen = torch.Generator(device="cpu").manual_seed(7)
+    xs = [(torch.rand(pl.n_nodes, nrhs, generator=gen, dtype=torch.float64) * 2 - 1).to(dt).cuda() for pl in plans]
+    ref = [torch.empty_like(x) for x in xs]
+    dkt.Batch(plans, xs, ref, masked=True).run()
+    torch.cuda.synchronize()
+    for cols in (1, 2, 4):
+        ys = [torch.full_like(x, float("nan")) for x in xs]
+        dkt.Batch(plans, xs, ys, masked=True, columns=cols).run()
+        torch.cuda.synchronize()
+        for k, (y, r) in enumerate(zip(ys, ref)):
+            assert torch.equal(y, r), (cols, k)
+    xc = [x.t().contiguous() for x in xs]  # [nrhs, N]
+    yc = [torch.full_like(x, float("nan")) for x in xc]
+    dkt.Batch(plans, xc, yc, masked=True, column_major=True).run()
+    torch.cuda.synchronize()
+    for k, (y, r) in enumerate(zip(yc, ref)):
+        assert torch.equal(y.t(), r), ("column-major", k)
+    with pytest.raises(dkt.DktError):
+        dkt.Batch(plans, xs, ref, columns=3)
