@@ -77,17 +77,23 @@ __device__ __forceinline__ void conf_tile_body(const ConfArgs &A, int tile, cons
 
   // 1. Stage the nodes, then evaluate the hanging points' rows.
   const bool active = tid < th.nleaf;
-  uint16_t loc[NPE];
+  // The element's local point per rank, two per register; loc_of(r) unpacks
+  // one where it is used.  Kept unpacked, the nPe indices would each hold a
+  // register from the gather through the rank phases (27 in 3D p = 2).
+  constexpr int NW = (NPE + 1) / 2;
+  uint32_t lw[NW];
   int lev = 0;
   if (active) {
     const uint16_t *lp = A.leaf_loc + (size_t)(th.leaf0 + tid) * NPE;
 #pragma unroll
-    for (int r = 0; r < NPE; r++) loc[r] = lp[r];
+    for (int w = 0; w < NW; w++)
+      lw[w] = (uint32_t)lp[2 * w] | (2 * w + 1 < NPE ? (uint32_t)lp[2 * w + 1] << 16 : 0u);
     lev = A.leaf_lev[th.leaf0 + tid];
   } else {
 #pragma unroll
-    for (int r = 0; r < NPE; r++) loc[r] = 0;
+    for (int w = 0; w < NW; w++) lw[w] = 0u;
   }
+  auto loc_of = [&](int r) -> int { return (int)((lw[r >> 1] >> ((r & 1) * 16)) & 0xFFFFu); };
   const int npts = th.nnode + th.nv;
   // Staging in batches of kCB per work-item: every load of a batch is issued
   // before its stores, so a batch costs one memory round trip, not kCB.
@@ -135,7 +141,7 @@ __device__ __forceinline__ void conf_tile_body(const ConfArgs &A, int tile, cons
     if (active) {
       T u[NPE];
 #pragma unroll
-      for (int r = 0; r < NPE; r++) u[r] = x_loc[loc[r] * nq + q];
+      for (int r = 0; r < NPE; r++) u[r] = x_loc[loc_of(r) * nq + q];
       if constexpr (CAP) {
         // Per-level element matrices (row-major, v = K u; the level carries
         // the element size, so no scale): the lanes of a wavefront mostly
@@ -157,11 +163,18 @@ __device__ __forceinline__ void conf_tile_body(const ConfArgs &A, int tile, cons
 #pragma unroll
           for (int i = 0; i < NPE; i++) v[i] = scale * u[i];
       } else {
-        q2_apply<T, DIM>(u, kref, A.op, scale);
+        // The element scale 2^(-level hp) as an exponent (one register, not
+        // an FP64 pair); scaling by a power of two gives the same bits in any
+        // position.
+        q2_apply<T, DIM>(u, kref, A.op, T(1));
 #pragma unroll
-        for (int i = 0; i < NPE; i++) v[i] = u[i];
+        for (int i = 0; i < NPE; i++) v[i] = ldexp(u[i], -lev * A.hp);
       }
     }
+    // The packed indices are opaque from here on, so the rank phases unpack
+    // them again instead of keeping the gather's unpacked copies alive.
+#pragma unroll
+    for (int w = 0; w < NW; w++) asm volatile("" : "+v"(lw[w]));
     if (nq == 1) {
       __syncthreads();
       for (int j = tid; j < npts; j += nthr) y_loc[j] = T(0);
@@ -169,7 +182,7 @@ __device__ __forceinline__ void conf_tile_body(const ConfArgs &A, int tile, cons
     }
 #pragma unroll
     for (int r = 0; r < NPE; r++) {
-      if (active) y_loc[loc[r] * nq + q] += v[r];
+      if (active) y_loc[loc_of(r) * nq + q] += v[r];
       __syncthreads();
     }
   }
