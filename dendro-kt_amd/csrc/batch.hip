@@ -220,7 +220,7 @@ struct ConfGroup : GroupBase {
     mem.arena = arena;
     constexpr int NPE = Pow<P + 1, DIM>::v;
     const bool f64 = sizeof(T) == 8;
-    auto lds_of = [](const dkt_plan *pl, int nq) { return conf_lds_bytes<T, NPE>(pl->max_nodes, nq); };
+    auto lds_of = [](const dkt_plan *pl, int nq) { return conf_lds_bytes<T, NPE>(pl->max_nodes, nq, pl->nw); };
     int chunk = colmajor ? 1 : (max_cols > 0 ? std::min(nrhs, max_cols) : nrhs);
     for (dkt_plan *pl : pls) {
       if (lds_of(pl, 1) > kLdsLimit) throw std::invalid_argument("tile does not fit in LDS");
@@ -252,11 +252,10 @@ struct ConfGroup : GroupBase {
         A.leaf_loc = pl->leaf_loc.p;
         A.leaf_lev = pl->leaf_lev.p;
         A.virt_off = pl->virt_off.p;
-        A.term_node = pl->term_node.p;
-        A.term_svirt = pl->term_svirt.p;
-        A.seg_node = pl->seg_node.p;
-        A.seg_off = pl->seg_off.p;
-        A.nterm = (int)pl->nterm;
+        A.term_pk = pl->term_pk.p;
+        A.term_spk = pl->term_spk.p;
+        A.seg = pl->seg.p;
+        A.nw = pl->nw;
         A.dir_in = masked == 1 ? pl->dirichlet.p : nullptr;
         A.dir_out = masked ? pl->dirichlet.p : nullptr;
         A.cancel = cancel.empty() ? nullptr : cancel[k];

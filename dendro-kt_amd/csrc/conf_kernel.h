@@ -16,11 +16,17 @@
 //   4. write-out as tile_kernel (owned nodes to y, shared ones to partial slots
 //      summed by fixup_kernel).
 // Every sum runs in an order fixed by the plan: bitwise reproducible.
-// LDS holds the local point values only.  The constraint terms are read from
-// global memory where they are used: in row order for step 1, and a second
-// copy sorted by node, cut into one segment per node, for step 3.  Staging the
-// terms cost 14 bytes of LDS per term (28.6 KB of the 49.6 KB of a 256-element
-// 3D p = 2 tile in FP64), which held the kernel to three workgroups per CU.
+// LDS holds the local point values and the plan's weight table only.  The
+// constraint terms are read from global memory where they are used: in row
+// order for step 1, and a second copy sorted by node, cut into one segment per
+// node, for step 3.  Staging the terms cost 14 bytes of LDS per term (28.6 KB
+// of the 49.6 KB of a 256-element 3D p = 2 tile in FP64), which held the
+// kernel to three workgroups per CU.
+// A term is one 32-bit word: its point (tile-local node, or hanging point for
+// the sorted copy) in the low half and the index of its weight in the plan's
+// table of distinct weights in the high half.  The weights are products of the
+// 1D interpolation weights (9 distinct values in 3D p = 2, 3 in 4D p = 1), so
+// the table replaces 8 bytes of FP64 weight per term and copy.
 #pragma once
 #include "plan_common.h"
 
@@ -31,25 +37,32 @@ struct ConfTileHdr {
   int32_t node0, nnode;  // mesh nodes (tile_nodes / tile_out); local points [0, nnode)
   int32_t v0, nv;        // hanging points (virt_off), local points [nnode, nnode + nv)
   int32_t t0, nt;        // constraint terms of the hanging points' rows
-  int32_t s0, ns;        // segments of the node-sorted terms (seg_node / seg_off), one per node with terms
+  int32_t s0, ns;        // segments of the node-sorted terms (seg), one per node with terms
 };
 
 struct ConfArgs {
   const ConfTileHdr *tiles;
   const int32_t *tile_nodes, *tile_out;
-  const uint16_t *leaf_loc;   // [E][nPe] local point of every rank
-  const uint8_t *leaf_lev;    // [E] element level
-  const uint16_t *virt_off;    // [hanging point] tile-relative first term of its row
-  const uint16_t *term_node;   // tile-local node per term, row order
-  const uint16_t *term_svirt;  // hanging point (0-based within the tile) per term, sorted by node
-  const uint16_t *seg_node;    // [segment] tile-local node
-  const uint16_t *seg_off;     // [segment] tile-relative first sorted term
+  const uint16_t *leaf_loc;   // [E][conf_row(nPe)] local point of every rank (odd nPe: then the level)
+  const uint8_t *leaf_lev;    // [E] element level (even nPe)
+  const uint16_t *virt_off;  // [hanging point] tile-relative first term of its row
+  const uint32_t *term_pk;   // per term, row order: tile-local node | weight index << 16
+  const uint32_t *term_spk;  // per term, sorted by node: hanging point (0-based in the tile) | weight index << 16
+  const uint32_t *seg;       // [segment] tile-local node | tile-relative first sorted term << 16
   const uint8_t *dir_in, *dir_out;
   const int *cancel;
   int op, hp, nq, ld;
   int max_points, tile_leaves;
-  int nterm;  // term weights: [0, nterm) in row order, then [nterm, 2 nterm) sorted by node
+  int nw;  // entries of the weight table (term_w)
 };
+
+// Largest weight table a conforming plan may have (LDS: 2 KB in FP64).
+constexpr int kMaxConfWeights = 256;
+
+// 16-bit entries per element in leaf_loc: nPe local points, and for odd nPe
+// (p = 2) the element level in the one more entry that makes the row a whole
+// number of 32-bit words; even nPe keep the level in leaf_lev.
+__host__ __device__ constexpr int conf_row(int npe) { return npe % 2 ? npe + 1 : npe; }
 
 // One tile of one plan (the workgroup may have more work-items than the
 // plan's tile_leaves; they idle in the element phases and help in the loops).
@@ -73,27 +86,33 @@ __device__ __forceinline__ void conf_tile_body(const ConfArgs &A, int tile, cons
   T *x_loc = reinterpret_cast<T *>(carve((size_t)A.max_points * nq * sizeof(T)));
   // One column: the accumulator reuses x_loc once every element has read it.
   T *y_loc = nq == 1 ? x_loc : reinterpret_cast<T *>(carve((size_t)A.max_points * nq * sizeof(T)));
+  T *wt = reinterpret_cast<T *>(carve((size_t)A.nw * sizeof(T)));  // the weight table
   const int tid = threadIdx.x, nthr = blockDim.x;
+  // The weight table's first batch, loaded with the stage loads and stored after them.
+  const T w_first = tid < A.nw ? term_w[tid] : T(0);
 
   // 1. Stage the nodes, then evaluate the hanging points' rows.
   const bool active = tid < th.nleaf;
   // The element's local point per rank, two per register; loc_of(r) unpacks
   // one where it is used.  Kept unpacked, the nPe indices would each hold a
   // register from the gather through the rank phases (27 in 3D p = 2).
-  constexpr int NW = (NPE + 1) / 2;
+  // Rows are whole 32-bit words, read as such; for odd nPe the last word's
+  // upper half is the level (level_of), so it needs no register of its own.
+  constexpr int NW = conf_row(NPE) / 2;
+  constexpr bool LEV_IN_ROW = NPE % 2 == 1;
   uint32_t lw[NW];
   int lev = 0;
   if (active) {
-    const uint16_t *lp = A.leaf_loc + (size_t)(th.leaf0 + tid) * NPE;
+    const uint32_t *lp = reinterpret_cast<const uint32_t *>(A.leaf_loc + (size_t)(th.leaf0 + tid) * conf_row(NPE));
 #pragma unroll
-    for (int w = 0; w < NW; w++)
-      lw[w] = (uint32_t)lp[2 * w] | (2 * w + 1 < NPE ? (uint32_t)lp[2 * w + 1] << 16 : 0u);
-    lev = A.leaf_lev[th.leaf0 + tid];
+    for (int w = 0; w < NW; w++) lw[w] = lp[w];
+    if constexpr (!LEV_IN_ROW) lev = A.leaf_lev[th.leaf0 + tid];
   } else {
 #pragma unroll
     for (int w = 0; w < NW; w++) lw[w] = 0u;
   }
   auto loc_of = [&](int r) -> int { return (int)((lw[r >> 1] >> ((r & 1) * 16)) & 0xFFFFu); };
+  auto level_of = [&]() -> int { return LEV_IN_ROW ? (int)(lw[NW - 1] >> 16) : lev; };
   const int npts = th.nnode + th.nv;
   // Staging in batches of kCB per work-item: every load of a batch is issued
   // before its stores, so a batch costs one memory round trip, not kCB.
@@ -118,22 +137,26 @@ __device__ __forceinline__ void conf_tile_body(const ConfArgs &A, int tile, cons
         if (g[b] >= 0) x_loc[(j0 + b * nthr) * nq + q] = z[b] ? T(0) : xv[b];
     }
   }
+  if (tid < A.nw) wt[tid] = w_first;
+  for (int i = tid + nthr; i < A.nw; i += nthr) wt[i] = term_w[i];
   if (nq > 1)
     for (int j = tid; j < npts * nq; j += nthr) y_loc[j] = T(0);
   __syncthreads();
   // Hanging point values: each row over the staged nodes, terms in row order.
-  const uint16_t *tnode = A.term_node + th.t0;
-  const T *tw = term_w + th.t0;
+  const uint32_t *tpk = A.term_pk + th.t0;
   for (int v = tid; v < th.nv; v += nthr) {
     const int k0 = A.virt_off[th.v0 + v], k1 = v + 1 < th.nv ? A.virt_off[th.v0 + v + 1] : th.nt;
     for (int q = 0; q < nq; q++) {
       T s = T(0);
-      for (int k = k0; k < k1; k++) s += tw[k] * x_loc[tnode[k] * nq + q];
+      for (int k = k0; k < k1; k++) {
+        const uint32_t e = tpk[k];
+        s += wt[e >> 16] * x_loc[(int)(e & 0xFFFFu) * nq + q];
+      }
       x_loc[(th.nnode + v) * nq + q] = s;
     }
   }
   __syncthreads();
-  const T scale = (T)ldexp(1.0, -lev * A.hp);
+  const T scale = (T)ldexp(1.0, -level_of() * A.hp);
 
   // 2. Elements.
   for (int q = 0; q < nq; q++) {
@@ -146,7 +169,7 @@ __device__ __forceinline__ void conf_tile_body(const ConfArgs &A, int tile, cons
         // Per-level element matrices (row-major, v = K u; the level carries
         // the element size, so no scale): the lanes of a wavefront mostly
         // share a level, so each entry is one broadcast load.
-        const T *K = kref + (size_t)lev * NPE * NPE;
+        const T *K = kref + (size_t)level_of() * NPE * NPE;
 #pragma unroll
         for (int i = 0; i < NPE; i++) {
           T acc = T(0);
@@ -167,8 +190,10 @@ __device__ __forceinline__ void conf_tile_body(const ConfArgs &A, int tile, cons
         // an FP64 pair); scaling by a power of two gives the same bits in any
         // position.
         q2_apply<T, DIM>(u, kref, A.op, T(1));
+        if constexpr (LEV_IN_ROW) asm volatile("" : "+v"(lw[NW - 1]));  // the level is read here, not kept
+        const int ex = -level_of() * A.hp;
 #pragma unroll
-        for (int i = 0; i < NPE; i++) v[i] = ldexp(u[i], -lev * A.hp);
+        for (int i = 0; i < NPE; i++) v[i] = ldexp(u[i], ex);
       }
     }
     // The packed indices are opaque from here on, so the rank phases unpack
@@ -188,14 +213,17 @@ __device__ __forceinline__ void conf_tile_body(const ConfArgs &A, int tile, cons
   }
 
   // 3. Transposed rows: per node, its terms in sorted order (one segment per node).
-  const uint16_t *svirt = A.term_svirt + th.t0;
-  const T *sw = term_w + A.nterm + th.t0;
+  const uint32_t *spk = A.term_spk + th.t0;
   for (int g = tid; g < th.ns; g += nthr) {
-    const int j = A.seg_node[th.s0 + g];
-    const int m0 = A.seg_off[th.s0 + g], m1 = g + 1 < th.ns ? A.seg_off[th.s0 + g + 1] : th.nt;
+    const uint32_t sg = A.seg[th.s0 + g];
+    const int j = (int)(sg & 0xFFFFu), m0 = (int)(sg >> 16);
+    const int m1 = g + 1 < th.ns ? (int)(A.seg[th.s0 + g + 1] >> 16) : th.nt;
     for (int q = 0; q < nq; q++) {
       T acc = T(0);
-      for (int m = m0; m < m1; m++) acc += sw[m] * y_loc[(th.nnode + svirt[m]) * nq + q];
+      for (int m = m0; m < m1; m++) {
+        const uint32_t e = spk[m];
+        acc += wt[e >> 16] * y_loc[(th.nnode + (int)(e & 0xFFFFu)) * nq + q];
+      }
       y_loc[j * nq + q] += acc;
     }
   }
@@ -234,11 +262,11 @@ __global__ void __launch_bounds__(kTileThreads) conf_tile_kernel(ConfArgs A, con
   conf_tile_body<T, DIM, P, CAP>(A, xcd_tile(blockIdx.x, gridDim.x), kref, term_w, x, y, partial);
 }
 
-// LDS bytes of conf_tile_kernel for nq right-hand sides.
+// LDS bytes of conf_tile_kernel for nq right-hand sides and a weight table of nw entries.
 template <typename T, int NPE>
-inline size_t conf_lds_bytes(int max_points, int nq) {
+inline size_t conf_lds_bytes(int max_points, int nq, int nw) {
   auto r16 = [](size_t b) { return (b + 15) & ~(size_t)15; };
-  return (nq == 1 ? 1 : 2) * r16((size_t)max_points * nq * sizeof(T));
+  return (nq == 1 ? 1 : 2) * r16((size_t)max_points * nq * sizeof(T)) + r16((size_t)nw * sizeof(T));
 }
 
 }  // namespace dkt

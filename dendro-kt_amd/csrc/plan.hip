@@ -964,9 +964,26 @@ void build_conf_tiled(dkt_plan *pl, const MeshParams &mp, const Traversal &tr, i
   std::vector<int32_t> last_tile(N, -1), local(N, -1);
   std::vector<ConfTileHdr> hdr(ntiles);
   std::vector<int32_t> tnodes;
-  std::vector<uint16_t> lloc(L * (size_t)npe, 0), voff, tnode, tvirt, tsvirt, sgnode, sgoff;
+  const int rs = conf_row(npe);  // leaf_loc row: npe local points (odd npe: then the level)
+  std::vector<uint16_t> lloc(L * (size_t)rs, 0), voff, tnode, tvirt;
+  std::vector<uint32_t> sg;
   std::vector<uint8_t> llev(L, 0);
-  std::vector<double> tw, tsw;
+  // Terms as words: point | weight index << 16 (row order, then sorted by node);
+  // the distinct weights, in order of first use, form the plan's table.
+  std::vector<uint32_t> tpk, tspk;
+  std::vector<uint16_t> twi;
+  std::vector<double> wtab;
+  std::unordered_map<uint64_t, uint16_t> widx;
+  auto weight_index = [&](double w) -> uint16_t {
+    uint64_t bits;
+    std::memcpy(&bits, &w, sizeof bits);
+    auto it = widx.find(bits);
+    if (it != widx.end()) return it->second;
+    if ((int)wtab.size() >= kMaxConfWeights) throw std::runtime_error("too many distinct constraint weights");
+    widx.emplace(bits, (uint16_t)wtab.size());
+    wtab.push_back(w);
+    return (uint16_t)(wtab.size() - 1);
+  };
   std::vector<std::pair<size_t, int32_t>> vref;
   int max_points = 0, max_virt = 0, max_terms = 0;
   for (size_t t = 0; t < ntiles; t++) {
@@ -989,11 +1006,12 @@ void build_conf_tiled(dkt_plan *pl, const MeshParams &mp, const Traversal &tr, i
     vref.clear();
     for (size_t e = bounds[t]; e < bounds[t + 1]; e++) {
       llev[e] = (uint8_t)tr.leaves[e].lev;
+      if (rs > npe) lloc[e * rs + npe] = (uint16_t)tr.leaves[e].lev;
       rank_ranges(e);
       for (int r = 0; r < npe; r++) {
         const int32_t n = tr.slot[e * npe + r];
         if (n >= 0) {
-          lloc[e * npe + r] = loc_of(n);
+          lloc[e * rs + r] = loc_of(n);
           continue;
         }
         auto ins = vmap.emplace(key_of(e, r), (int32_t)(voff.size() - H.v0));
@@ -1002,10 +1020,10 @@ void build_conf_tiled(dkt_plan *pl, const MeshParams &mp, const Traversal &tr, i
           for (int64_t k = rk0[r]; k < rk1[r]; k++) {
             tnode.push_back(loc_of(tr.cf_node[k]));
             tvirt.push_back((uint16_t)ins.first->second);
-            tw.push_back(tr.cf_w[k]);
+            twi.push_back(weight_index(tr.cf_w[k]));
           }
         }
-        vref.push_back({e * npe + r, ins.first->second});
+        vref.push_back({e * rs + r, ins.first->second});
       }
     }
     H.nnode = nn;
@@ -1018,23 +1036,23 @@ void build_conf_tiled(dkt_plan *pl, const MeshParams &mp, const Traversal &tr, i
     for (int k = 0; k < H.nt; k++) perm[k] = (uint16_t)k;
     std::stable_sort(perm.begin(), perm.end(),
                      [&](uint16_t a, uint16_t b) { return tnode[H.t0 + a] < tnode[H.t0 + b]; });
-    H.s0 = (int32_t)sgnode.size();
+    H.s0 = (int32_t)sg.size();
+    for (int k = 0; k < H.nt; k++)
+      tpk.push_back((uint32_t)tnode[H.t0 + k] | (uint32_t)twi[H.t0 + k] << 16);
     for (int i = 0; i < H.nt; i++) {
       const size_t k = H.t0 + perm[i];
-      tsvirt.push_back(tvirt[k]);
-      tsw.push_back(tw[k]);
+      tspk.push_back((uint32_t)tvirt[k] | (uint32_t)twi[k] << 16);
       if (i == 0 || tnode[H.t0 + perm[i - 1]] != tnode[k]) {
-        sgnode.push_back(tnode[k]);
-        sgoff.push_back((uint16_t)i);
+        sg.push_back((uint32_t)tnode[k] | (uint32_t)i << 16);
       }
     }
-    H.ns = (int32_t)(sgnode.size() - H.s0);
+    H.ns = (int32_t)(sg.size() - H.s0);
     max_points = std::max(max_points, nn + H.nv);
     max_virt = std::max(max_virt, H.nv);
     max_terms = std::max(max_terms, H.nt);
   }
   // Halve the tiles while one right-hand side in FP64 does not fit in LDS.
-  pl->lds_bytes = conf_lds_bytes<double, 4>(max_points, 1);
+  pl->lds_bytes = conf_lds_bytes<double, 4>(max_points, 1, (int)wtab.size());
   if (pl->lds_bytes > (size_t)kLdsLimit && T > 16) return build_conf_tiled(pl, mp, tr, T / 2);
   // Owned nodes go to y; a node in several tiles gets a partial slot per tile
   // (node-major, tile order), summed by fixup_kernel.
@@ -1066,21 +1084,19 @@ void build_conf_tiled(dkt_plan *pl, const MeshParams &mp, const Traversal &tr, i
   pl->tile_nodes.upload(nonempty(tnodes));
   pl->tile_out.upload(nonempty(tout));
   pl->leaf_loc.upload(nonempty(lloc));
-  pl->leaf_lev.upload(nonempty(llev));
+  if (rs == npe) pl->leaf_lev.upload(nonempty(llev));
   pl->virt_off.upload(nonempty(voff));
-  pl->term_node.upload(nonempty(tnode));
-  pl->term_svirt.upload(nonempty(tsvirt));
-  pl->seg_node.upload(nonempty(sgnode));
-  pl->seg_off.upload(nonempty(sgoff));
-  std::vector<double> wboth(tw);
-  wboth.insert(wboth.end(), tsw.begin(), tsw.end());
-  pl->nterm = tw.size();
-  pl->term_w64.upload(nonempty(wboth));
-  pl->term_w32.upload(cast<float>(nonempty(wboth)));
+  pl->term_pk.upload(nonempty(tpk));
+  pl->term_spk.upload(nonempty(tspk));
+  pl->seg.upload(nonempty(sg));
+  pl->nw = (int)wtab.size();
+  pl->term_w64.upload(nonempty(wtab));
+  pl->term_w32.upload(cast<float>(nonempty(wtab)));
   pl->snode.upload(nonempty(snode));
   pl->soff.upload(soff);
-  pl->map_bytes = (double)(hdr.size() * sizeof(ConfTileHdr) + tnodes.size() * 8 + lloc.size() * 2 + llev.size() +
-                           voff.size() * 2 + tnode.size() * 20 + sgnode.size() * 4 + snode.size() * 8);
+  pl->map_bytes = (double)(hdr.size() * sizeof(ConfTileHdr) + tnodes.size() * 8 + lloc.size() * 2 + (rs == npe ? llev.size() : 0) +
+                           voff.size() * 2 + tpk.size() * 8 + wtab.size() * 8 + sg.size() * 4 +
+                           snode.size() * 8);
 }
 
 // ---------------------------------------------------------------------------
@@ -1196,7 +1212,7 @@ void run_conf(dkt_plan *pl, int nrhs, const T *x, T *y, int mask, hipStream_t st
   const T *kmat = P == 2 && !pl->per_level ? (f64 ? (const T *)pl->kfac64.p : (const T *)pl->kfac32.p)
                                            : (f64 ? (const T *)pl->kmat64.p : (const T *)pl->kmat32.p);
   const T *tw = f64 ? (const T *)pl->term_w64.p : (const T *)pl->term_w32.p;
-  auto lds_for = [&](int nq) { return conf_lds_bytes<T, NPE>(pl->max_nodes, nq); };
+  auto lds_for = [&](int nq) { return conf_lds_bytes<T, NPE>(pl->max_nodes, nq, pl->nw); };
   if (lds_for(1) > kLdsLimit) throw std::invalid_argument("tile does not fit in LDS");
   int chunk = 1;
   while (chunk < nrhs && lds_for(chunk + 1) <= kLdsLimit) chunk++;
@@ -1208,10 +1224,9 @@ void run_conf(dkt_plan *pl, int nrhs, const T *x, T *y, int mask, hipStream_t st
   A.leaf_loc = pl->leaf_loc.p;
   A.leaf_lev = pl->leaf_lev.p;
   A.virt_off = pl->virt_off.p;
-  A.term_node = pl->term_node.p;
-  A.term_svirt = pl->term_svirt.p;
-  A.seg_node = pl->seg_node.p;
-  A.seg_off = pl->seg_off.p;
+  A.term_pk = pl->term_pk.p;
+  A.term_spk = pl->term_spk.p;
+  A.seg = pl->seg.p;
   A.dir_in = mask == 1 ? pl->dirichlet.p : nullptr;
   A.dir_out = mask ? pl->dirichlet.p : nullptr;
   A.cancel = cancel;
@@ -1220,7 +1235,7 @@ void run_conf(dkt_plan *pl, int nrhs, const T *x, T *y, int mask, hipStream_t st
   A.ld = nrhs;
   A.max_points = pl->max_nodes;
   A.tile_leaves = pl->tile_leaves;
-  A.nterm = (int)pl->nterm;
+  A.nw = pl->nw;
   for (int q0 = 0; q0 < nrhs; q0 += chunk) {
     const int nq = std::min(chunk, nrhs - q0);
     A.nq = nq;

@@ -49,7 +49,8 @@ struct dkt_plan {
   size_t nbuf = 0;
   // tiled path
   int tile_leaves = dkt::kTileThreads;
-  size_t ntiles = 0, nshared = 0, npartial = 0, nterm = 0;
+  size_t ntiles = 0, nshared = 0, npartial = 0;
+  int nw = 0;  // conforming: entries of the term weight table
   int max_nodes = 0, max_grp = 0, max_he = 0, max_rd = 0, max_cr = 0, max_virt = 0, max_terms = 0, max_hang = 0;
   size_t lds_bytes = 0;  // LDS per workgroup of the tiled kernel, FP64, one column
   dkt::DevBuf<dkt::TileHdr> tiles;
@@ -60,9 +61,11 @@ struct dkt_plan {
   dkt::DevBuf<uint64_t> blocks;
   // conforming tiles
   dkt::DevBuf<dkt::ConfTileHdr> ctiles;
-  dkt::DevBuf<uint16_t> virt_off, term_node, term_svirt, seg_node, seg_off;
+  dkt::DevBuf<uint16_t> virt_off;
+  dkt::DevBuf<uint32_t> seg;  // node-sorted term segments: node | first term << 16
+  dkt::DevBuf<uint32_t> term_pk, term_spk;  // point | weight index << 16, row order and sorted by node
   dkt::DevBuf<uint8_t> leaf_lev;
-  dkt::DevBuf<double> term_w64;  // row order, then sorted by node (ConfArgs::nterm each)
+  dkt::DevBuf<double> term_w64;  // the distinct term weights (nw)
   dkt::DevBuf<float> term_w32;
   dkt::DevBuf<double> carry_w64;
   dkt::DevBuf<float> carry_w32;
