@@ -316,7 +316,9 @@ def serve_leg(dkt, nreq, rates=(50, 200, 800)):
     L_inf), the conforming operator, an HBM admission budget of 200 GB, plans built
     before the timed runs.  Continuous batching (2 workers of 8 slots) with
     mixed-precision CG (FP32 iteration, FP64 reliable updates) and with FP64 CG,
-    each at every rate on the same request list; latency runs from arrival to solution on the host."""
+    each at every rate on the same request list; latency runs from arrival to solution on the host.
+    Then the lowest rate once more with mixed CG on an empty plan cache (cold start: the
+    server builds each mesh and its plans on first use, inside the measured latency)."""
     sv = _pkg_mod("dkt_serve", "serve.py")
     wl = _pkg_mod("dkt_workloads", "workloads.py")
     pool = [spec for spec, _ in wl.serve_pool(sv, 10, 1e3, 1e6, seed=0)]
@@ -364,6 +366,39 @@ def serve_leg(dkt, nreq, rates=(50, 200, 800)):
                 "iters_mean": round(float(np.mean([r.stats["iters"] for r in done])), 1),
                 "kinds": {k: sum(1 for r in done if r.kind == k) for k in ("poisson", "heat", "spacetime")}}
             log(f"  {out[f'{name}_{rate}_per_s']}")
+    # Cold start: the lowest rate's request list again, mixed CG, on a fresh plan
+    # cache, so the first request on each mesh waits for its mesh and plans to be
+    # built by the server's builder threads.
+    rate = rates[0]
+    rng = np.random.default_rng(rate)
+    drawn = []
+    for k in range(nreq):
+        m = pool[int(rng.integers(len(pool)))]
+        kind = ("heat" if m.dim == 3 else "spacetime") if k % 2 else "poisson"
+        drawn.append((m, int(rng.integers(1 << 30)), kind))
+    cold = sv.PlanCache(dkt, 200e9, "conforming")
+    srv = sv.Server(dkt, n_streams=2, batch=8, cache=cold)
+    try:
+        reqs = [sv.Request(m, rhs_seed=seed, kind=kind, max_iter=50000,
+                           mode=sv.CG_F64 if kind == "spacetime" else sv.CG_MIXED) for m, seed, kind in drawn]
+        log(f"serving {nreq} requests at {rate}/s (mixed, batched, cold plan cache)")
+        done, wall = sv.run_open_loop(srv, reqs, rate, seed=rate)
+    finally:
+        srv.close()
+    errors = [r.error for r in done if r.error]
+    key = f"mixed_{rate}_per_s_cold"
+    if errors:
+        out[key] = {"errors": errors[:3]}
+    else:
+        e2e = [(r.finished - r.submitted) * 1e3 for r in done]
+        built = [(r.finished - r.submitted) * 1e3 for r in done if r.plan_built]
+        out[key] = {"solves_per_s": round(len(done) / wall, 2),
+                    "e2e_ms": {k: round(v, 1) for k, v in sv.percentiles(e2e).items()},
+                    "plans_built": len(built),
+                    "e2e_ms_building": {k: round(v, 1) for k, v in sv.percentiles(built).items()} if built else None,
+                    "max_resid": max(r.stats["resid"] for r in done),
+                    "not_converged": sum(1 for r in done if not r.stats.get("converged"))}
+    log(f"  {out[key]}")
     return out
 
 
