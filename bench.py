@@ -21,7 +21,7 @@ Prints one JSON line (rank 0):
                    (oracle/plan_cpu.c) timed on this host on the full workload,
                    OpenMP on the host's share of cores and on one thread, with
                    the reference's own single-core figure recorded in the build
-                   container (the reference does not travel to the GPU box).
+                   container (the bench itself never runs the reference).
 Further legs: FP32 matvec, CG (4D, reference operator) and level-9 3D p=2
 CG (conforming operator), FP64 and mixed precision; the batched 64-tree matvec;
 served CG under Poisson arrivals (continuous batching, mixed vs FP64).
@@ -66,8 +66,8 @@ def build_mesh(dkt, npoints, maxd, seed):
     return da, t2 - t0
 
 
-# The reference itself, timed in the build container (it does not travel to
-# the GPU box): the Hilbert build of oracle/Makefile's `fast` target (library
+# The reference itself, timed in the build container (the bench never runs
+# it): the Hilbert build of oracle/Makefile's `fast` target (library
 # and driver at -O3, the translation unit with the functions that fall off the
 # end at -O0), one MPI rank, one thread, 3 matvecs after 1 warm-up, on samples
 # of this generator:
