@@ -359,6 +359,9 @@ void build_simple(dkt_plan *pl, const MeshParams &mp, Traversal &tr) {
   pl->csr_off.upload(cnt);
   pl->csr_src.upload(src);
   pl->nbuf = conf ? pl->L * (size_t)npe + t_node.size() : (pl->L + pl->H) * (size_t)npe;
+  const size_t pslots = tr.pslot.empty() || conf ? std::max<size_t>(tr.pslot.size(), npe) : tr.pslot.size();
+  pl->map_bytes = (double)(tr.slot.size() * 4 + pl->L * 6 + pslots * 4 + t_off.size() * 4 + t_node.size() * 17 +
+                           cnt.size() * 8 + src.size() * 4);
 }
 
 // ---------------------------------------------------------------------------

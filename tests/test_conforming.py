@@ -159,6 +159,28 @@ def test_conforming_tiled_layout(dkt, dim, p, seed, tile):
     assert info[8] <= info[19] * info[3] + 1024  # local points: element nodes and hanging points
 
 
+@pytest.mark.parametrize("dim,p,seed", CASES)
+@pytest.mark.parametrize("conforming", [True, False])
+def test_plan_maps_independent_of_operator(dkt, dim, p, seed, conforming):
+    """The plans a request kind pairs on one mesh (stiffness and mass; two sets of
+    captured per-level matrices) have maps of the same size, which lets the serving
+    layer's admission use one dry run per mesh.  Conforming plans tile the same way
+    for every operator; on the reference traversal captured matrices take the
+    per-element path."""
+    da, _ = _da(dkt, dim, p, seed)
+    npe = (p + 1) ** dim
+    caps = [np.tile(k * np.eye(npe), (10, 1, 1)).ravel() for k in (1.0, 2.0)]
+
+    def size(op, m=None):
+        return dkt.Plan(da, op, op_data=m, conforming=conforming, dry_run=True).info()["map_bytes"]
+
+    st, ms = size(dkt.OP_STIFFNESS), size(dkt.OP_MASS)
+    c1, c2 = size(dkt.OP_CAPTURED, caps[0]), size(dkt.OP_CAPTURED, caps[1])
+    assert st > 0 and ms == st and c1 > 0 and c2 == c1
+    if conforming:
+        assert c1 == st
+
+
 @pytest.mark.gpu
 @pytest.mark.parametrize("dim,p,seed", CASES)
 def test_conforming_tiled_matches_simple(dkt, dim, p, seed):
