@@ -923,12 +923,38 @@ Traversal build_conforming(const MeshParams &mp, const std::vector<Oct> &elems, 
       return (size_t)(h ^ (h >> 31));
     }
   };
-  std::unordered_map<Key, int32_t, KeyHash> at;
-  at.reserve(ns.nodes.size() * 2);
-  for (size_t i = 0; i < ns.nodes.size(); i++) {
-    Key k{0, 0, 0, 0, 0xFFFFFFFFu};
-    for (int d = 0; d < dim; d++) k[d] = ns.nodes[i].x[d];
-    at.emplace(k, (int32_t)i);
+  // Node ids by coordinates: an open-addressing table, filled without a
+  // per-node allocation and probed by every element rank.
+  struct NodeIndex {
+    std::vector<std::array<u32, 4>> key;
+    std::vector<int32_t> id;
+    size_t mask = 0;
+    static size_t hash(const u32 *x) {
+      u64 h = 0x9E3779B97F4A7C15ull;
+      for (int d = 0; d < 4; d++) h = (h ^ x[d]) * 0xBF58476D1CE4E5B9ull;
+      return (size_t)(h ^ (h >> 31));
+    }
+    int32_t find(const u32 *x) const {
+      for (size_t i = hash(x) & mask;; i = (i + 1) & mask) {
+        if (id[i] < 0) return -1;
+        if (key[i][0] == x[0] && key[i][1] == x[1] && key[i][2] == x[2] && key[i][3] == x[3]) return id[i];
+      }
+    }
+  } at;
+  {
+    size_t cap = 16;
+    while (cap < ns.nodes.size() * 2) cap <<= 1;
+    at.key.assign(cap, {0, 0, 0, 0});
+    at.id.assign(cap, -1);
+    at.mask = cap - 1;
+    for (size_t n = 0; n < ns.nodes.size(); n++) {
+      u32 x[4] = {0, 0, 0, 0};
+      for (int d = 0; d < dim; d++) x[d] = ns.nodes[n].x[d];
+      size_t i = NodeIndex::hash(x) & at.mask;
+      while (at.id[i] >= 0) i = (i + 1) & at.mask;  // node positions are distinct
+      at.key[i] = {x[0], x[1], x[2], x[3]};
+      at.id[i] = (int32_t)n;
+    }
   }
   double ip[2][9];
   interp_1d(p, 0, ip[0]);
@@ -942,11 +968,9 @@ Traversal build_conforming(const MeshParams &mp, const std::vector<Oct> &elems, 
     auto m = memo.find(key);
     if (m != memo.end()) return m->second;
     Combo v;
-    Key nk = key;
-    nk[4] = 0xFFFFFFFFu;
-    auto f = at.find(nk);
-    if (f != at.end()) {
-      v.push_back({f->second, 1.0});
+    const u32 xp[4] = {key[0], key[1], key[2], key[3]};
+    if (const int32_t f = at.find(xp); f >= 0) {
+      v.push_back({f, 1.0});
     } else {
       if (lev == 0) throw std::runtime_error("conforming constraint: a root node is missing");
       const u32 len = 1u << (mp.maxd - lev), step = len / (u32)p;
@@ -1021,9 +1045,8 @@ Traversal build_conforming(const MeshParams &mp, const std::vector<Oct> &elems, 
               k[d] = cell.x[d] + (u32)(rr % n1) * step;
               rr /= n1;
             }
-            auto f = at.find(k);
-            if (f != at.end()) {
-              tr.slot[e * npe + r] = f->second;
+            if (const int32_t f = at.find(k.data()); f >= 0) {
+              tr.slot[e * npe + r] = f;
               continue;
             }
             for (const auto &tw : resolve(k.data(), cell.x, (int)cell.lev, memo)) {
@@ -1041,23 +1064,36 @@ Traversal build_conforming(const MeshParams &mp, const std::vector<Oct> &elems, 
   }
   for (const auto &er : errs)
     if (!er.empty()) throw std::runtime_error(er);
+  // Concatenation in element order: chunk offsets first, then each chunk
+  // copied to its place in parallel.
+  std::vector<size_t> rec0(nchunks + 1, 0), term0(nchunks + 1, 0);
+  for (size_t ci = 0; ci < nchunks; ci++) {
+    size_t nrec = 0;
+    for (uint32_t c : chunks[ci].count) nrec += c ? 1 : 0;
+    rec0[ci + 1] = rec0[ci] + nrec;
+    term0[ci + 1] = term0[ci] + chunks[ci].node.size();
+  }
+  tr.cf_off.assign(rec0[nchunks] + 1, 0);
+  tr.cf_rank.resize(term0[nchunks]);
+  tr.cf_node.resize(term0[nchunks]);
+  tr.cf_w.resize(term0[nchunks]);
+#pragma omp parallel for schedule(dynamic, 1)
   for (size_t ci = 0; ci < nchunks; ci++) {
     ChunkTerms &ct = chunks[ci];
-    const size_t e0 = ci * kChunk;
-    size_t k = 0;
+    const size_t e0 = ci * kChunk, t0 = term0[ci];
+    std::copy(ct.rank.begin(), ct.rank.end(), tr.cf_rank.begin() + t0);
+    std::copy(ct.node.begin(), ct.node.end(), tr.cf_node.begin() + t0);
+    std::copy(ct.w.begin(), ct.w.end(), tr.cf_w.begin() + t0);
+    size_t h = rec0[ci], k = t0;
     for (size_t i = 0; i < ct.count.size(); i++) {
       if (!ct.count[i]) continue;
-      tr.hang_of[e0 + i] = (int32_t)(tr.cf_off.size() - 1);
-      for (uint32_t m = 0; m < ct.count[i]; m++, k++) {
-        tr.cf_rank.push_back(ct.rank[k]);
-        tr.cf_node.push_back(ct.node[k]);
-        tr.cf_w.push_back(ct.w[k]);
-      }
-      tr.cf_off.push_back((int64_t)tr.cf_node.size());
+      tr.hang_of[e0 + i] = (int32_t)h;
+      k += ct.count[i];
+      tr.cf_off[++h] = (int64_t)k;
     }
     ct = ChunkTerms();
   }
-  tr.pslot.assign((tr.cf_off.size() - 1) * (size_t)npe, -1);
+  // (pslot stays empty: conforming records have no parent-level slots.)
   tr.cf_ready = true;
   return tr;
 }

@@ -312,8 +312,8 @@ void build_simple(dkt_plan *pl, const MeshParams &mp, Traversal &tr) {
   pl->child.upload(tr.child);
   // Conforming: the constraint terms carry the whole value of a missing node,
   // so no parent-level interpolation is added.
-  pl->pslot.upload(tr.pslot.empty() || conf ? std::vector<int32_t>(std::max<size_t>(tr.pslot.size(), npe), -1)
-                                            : tr.pslot);
+  const size_t pslots = std::max<size_t>(conf ? (t_off.size() - 1) * npe : tr.pslot.size(), npe);
+  pl->pslot.upload(tr.pslot.empty() || conf ? std::vector<int32_t>(pslots, -1) : tr.pslot);
   pl->hc_off.upload(std::vector<int32_t>(t_off.begin(), t_off.end()));
   pl->hc_rank.upload(nonempty(t_rank));
   pl->hc_node.upload(nonempty(t_node));
@@ -359,7 +359,6 @@ void build_simple(dkt_plan *pl, const MeshParams &mp, Traversal &tr) {
   pl->csr_off.upload(cnt);
   pl->csr_src.upload(src);
   pl->nbuf = conf ? pl->L * (size_t)npe + t_node.size() : (pl->L + pl->H) * (size_t)npe;
-  const size_t pslots = tr.pslot.empty() || conf ? std::max<size_t>(tr.pslot.size(), npe) : tr.pslot.size();
   pl->map_bytes = (double)(tr.slot.size() * 4 + pl->L * 6 + pslots * 4 + t_off.size() * 4 + t_node.size() * 17 +
                            cnt.size() * 8 + src.size() * 4);
 }
@@ -379,6 +378,31 @@ struct TiledHost {
   int max_nodes = 0, max_grp = 0, max_he = 0, max_rd = 0, max_cr = 0, max_hang = 0;
   size_t nshared = 0, npartial = 0;
   std::vector<double> stats;
+};
+
+// Tile-local numbering of global node ids (first touch), an open-addressing
+// map that is cleared in time proportional to its use.
+struct LocalMap {
+  std::vector<int32_t> key, val;
+  std::vector<uint32_t> used;
+  LocalMap() : key(1 << 15, -1), val(1 << 15, 0) {}
+  void clear() {
+    for (uint32_t u : used) key[u] = -1;
+    used.clear();
+  }
+  // Returns the slot of n (key[slot] == n after the call) and whether it was new.
+  uint32_t find(int32_t n, bool *fresh) {
+    const uint32_t mask = (uint32_t)key.size() - 1;
+    uint32_t h = ((uint32_t)n * 2654435761u) & mask;
+    while (key[h] != -1 && key[h] != n) h = (h + 1) & mask;
+    *fresh = key[h] == -1;
+    if (*fresh) {
+      if (used.size() * 2 >= key.size()) throw std::runtime_error("tile has too many nodes");
+      key[h] = n;
+      used.push_back(h);
+    }
+    return h;
+  }
 };
 
 TiledHost build_tiled_host(const MeshParams &mp, const Traversal &tr, size_t N, int T, int n_groups) {
@@ -498,28 +522,6 @@ TiledHost build_tiled_host(const MeshParams &mp, const Traversal &tr, size_t N, 
     double cone_sum = 0, cone_n = 0, span = -1;
   };
   // Tile-local node numbers: open addressing over global node ids.
-  struct LocalMap {
-    std::vector<int32_t> key, val;
-    std::vector<uint32_t> used;
-    LocalMap() : key(1 << 15, -1), val(1 << 15, 0) {}
-    void clear() {
-      for (uint32_t u : used) key[u] = -1;
-      used.clear();
-    }
-    // Returns the slot of n (key[slot] == n after the call) and whether it was new.
-    uint32_t find(int32_t n, bool *fresh) {
-      const uint32_t mask = (uint32_t)key.size() - 1;
-      uint32_t h = ((uint32_t)n * 2654435761u) & mask;
-      while (key[h] != -1 && key[h] != n) h = (h + 1) & mask;
-      *fresh = key[h] == -1;
-      if (*fresh) {
-        if (used.size() * 2 >= key.size()) throw std::runtime_error("tile has too many nodes");
-        key[h] = n;
-        used.push_back(h);
-      }
-      return h;
-    }
-  };
   std::vector<TileOut> outs(ntiles);
   auto build_tile = [&](size_t t, TileOut &to, LocalMap &lm, std::unordered_map<uint64_t, Combo> &memo) {
     const size_t l0 = bounds[t], l1 = bounds[t + 1];
@@ -964,95 +966,184 @@ void build_conf_tiled(dkt_plan *pl, const MeshParams &mp, const Traversal &tr, i
     if (bounds.back() != L) bounds.push_back(L);
   }
   const size_t ntiles = bounds.size() - 1;
-  std::vector<int32_t> last_tile(N, -1), local(N, -1);
-  std::vector<ConfTileHdr> hdr(ntiles);
-  std::vector<int32_t> tnodes;
   const int rs = conf_row(npe);  // leaf_loc row: npe local points (odd npe: then the level)
-  std::vector<uint16_t> lloc(L * (size_t)rs, 0), voff, tnode, tvirt;
-  std::vector<uint32_t> sg;
+  std::vector<uint16_t> lloc(L * (size_t)rs, 0);
   std::vector<uint8_t> llev(L, 0);
-  // Terms as words: point | weight index << 16 (row order, then sorted by node);
-  // the distinct weights, in order of first use, form the plan's table.
-  std::vector<uint32_t> tpk, tspk;
-  std::vector<uint16_t> twi;
-  std::vector<double> wtab;
-  std::unordered_map<uint64_t, uint16_t> widx;
+  // The plan's weight table: the distinct constraint weights, ordered by their
+  // bits; a term stores its weight's index.
+  std::vector<uint64_t> wbits;
+  {
+    const size_t nw_terms = tr.cf_w.size(), chunk = 1 << 16;
+    const size_t nchunks = (nw_terms + chunk - 1) / chunk;
+    std::vector<std::vector<uint64_t>> part(nchunks);
+    std::atomic<size_t> next_chunk{0};
+    auto scan = [&]() {
+      for (size_t c = next_chunk++; c < nchunks; c = next_chunk++) {
+        std::vector<uint64_t> &d = part[c];
+        uint64_t last = 0;
+        for (size_t k = c * chunk; k < std::min(nw_terms, (c + 1) * chunk); k++) {
+          uint64_t b;
+          std::memcpy(&b, &tr.cf_w[k], sizeof b);
+          if ((!d.empty() && b == last) || std::find(d.begin(), d.end(), b) != d.end()) continue;
+          d.push_back(b);
+          last = b;
+          if (d.size() > (size_t)kMaxConfWeights) break;
+        }
+      }
+    };
+    {
+      const unsigned hw = std::thread::hardware_concurrency();
+      std::vector<std::thread> pool;
+      for (unsigned i = 1; i < std::max(1u, std::min(16u, hw ? hw : 1u)); i++) pool.emplace_back(scan);
+      scan();
+      for (auto &th : pool) th.join();
+    }
+    for (const auto &d : part) wbits.insert(wbits.end(), d.begin(), d.end());
+    std::sort(wbits.begin(), wbits.end());
+    wbits.erase(std::unique(wbits.begin(), wbits.end()), wbits.end());
+    if ((int)wbits.size() > kMaxConfWeights) throw std::runtime_error("too many distinct constraint weights");
+  }
+  std::vector<double> wtab(wbits.size());
+  for (size_t i = 0; i < wbits.size(); i++) std::memcpy(&wtab[i], &wbits[i], sizeof(double));
   auto weight_index = [&](double w) -> uint16_t {
-    uint64_t bits;
-    std::memcpy(&bits, &w, sizeof bits);
-    auto it = widx.find(bits);
-    if (it != widx.end()) return it->second;
-    if ((int)wtab.size() >= kMaxConfWeights) throw std::runtime_error("too many distinct constraint weights");
-    widx.emplace(bits, (uint16_t)wtab.size());
-    wtab.push_back(w);
-    return (uint16_t)(wtab.size() - 1);
+    uint64_t b;
+    std::memcpy(&b, &w, sizeof b);
+    return (uint16_t)(std::lower_bound(wbits.begin(), wbits.end(), b) - wbits.begin());
   };
-  std::vector<std::pair<size_t, int32_t>> vref;
-  int max_points = 0, max_virt = 0, max_terms = 0;
-  for (size_t t = 0; t < ntiles; t++) {
-    ConfTileHdr &H = hdr[t];
-    H.leaf0 = (int32_t)bounds[t];
-    H.nleaf = (int32_t)(bounds[t + 1] - bounds[t]);
-    H.node0 = (int32_t)tnodes.size();
-    H.v0 = (int32_t)voff.size();
-    H.t0 = (int32_t)tnode.size();
+  // Tiles are independent given their bounds: built in parallel into per-tile
+  // records, then concatenated in tile order.
+  struct ConfTileOut {
+    std::vector<int32_t> nodes;  // tile-local node -> mesh node (first touch)
+    std::vector<uint16_t> voff;  // per hanging point: first term (tile-relative)
+    std::vector<uint32_t> tpk, tspk, sg;
+  };
+  struct Scratch {
+    LocalMap lm;
+    std::unordered_map<Key, int32_t, KeyHash> vm;
+    std::vector<std::pair<size_t, int32_t>> vref;
+    std::vector<int64_t> rk0, rk1;
+    std::vector<uint16_t> tnode, tvirt, twi, perm;
+  };
+  std::vector<ConfTileOut> outs(ntiles);
+  auto build_tile = [&](size_t t, ConfTileOut &to, Scratch &sc) {
+    sc.lm.clear();
+    sc.vm.clear();
+    sc.vref.clear();
+    sc.tnode.clear();
+    sc.tvirt.clear();
+    sc.twi.clear();
+    sc.rk0.assign(npe, 0);
+    sc.rk1.assign(npe, 0);
     int32_t nn = 0;
     auto loc_of = [&](int32_t n) -> uint16_t {
-      if (last_tile[n] != (int32_t)t) {
-        last_tile[n] = (int32_t)t;
-        local[n] = nn++;
-        tnodes.push_back(n);
+      bool fresh;
+      const uint32_t h = sc.lm.find(n, &fresh);
+      if (fresh) {
+        sc.lm.val[h] = nn++;
+        to.nodes.push_back(n);
       }
-      return (uint16_t)local[n];
+      return (uint16_t)sc.lm.val[h];
     };
-    vmap.clear();
-    vref.clear();
     for (size_t e = bounds[t]; e < bounds[t + 1]; e++) {
       llev[e] = (uint8_t)tr.leaves[e].lev;
       if (rs > npe) lloc[e * rs + npe] = (uint16_t)tr.leaves[e].lev;
-      rank_ranges(e);
+      // terms of rank r: cf rows hold each element's ranks in order
+      std::fill(sc.rk0.begin(), sc.rk0.end(), 0);
+      std::fill(sc.rk1.begin(), sc.rk1.end(), 0);
+      if (const int h = tr.hang_of[e]; h >= 0)
+        for (int64_t k = tr.cf_off[h]; k < tr.cf_off[h + 1]; k++) {
+          const int r = tr.cf_rank[k];
+          if (sc.rk1[r] == 0) sc.rk0[r] = k;
+          sc.rk1[r] = k + 1;
+        }
       for (int r = 0; r < npe; r++) {
         const int32_t n = tr.slot[e * npe + r];
         if (n >= 0) {
           lloc[e * rs + r] = loc_of(n);
           continue;
         }
-        auto ins = vmap.emplace(key_of(e, r), (int32_t)(voff.size() - H.v0));
+        auto ins = sc.vm.emplace(key_of(e, r), (int32_t)to.voff.size());
         if (ins.second) {
-          voff.push_back((uint16_t)(tnode.size() - H.t0));
-          for (int64_t k = rk0[r]; k < rk1[r]; k++) {
-            tnode.push_back(loc_of(tr.cf_node[k]));
-            tvirt.push_back((uint16_t)ins.first->second);
-            twi.push_back(weight_index(tr.cf_w[k]));
+          to.voff.push_back((uint16_t)sc.tnode.size());
+          for (int64_t k = sc.rk0[r]; k < sc.rk1[r]; k++) {
+            sc.tnode.push_back(loc_of(tr.cf_node[k]));
+            sc.tvirt.push_back((uint16_t)ins.first->second);
+            sc.twi.push_back(weight_index(tr.cf_w[k]));
           }
         }
-        vref.push_back({e * rs + r, ins.first->second});
+        sc.vref.push_back({e * rs + r, ins.first->second});
       }
     }
-    H.nnode = nn;
-    H.nv = (int32_t)(voff.size() - H.v0);
-    H.nt = (int32_t)(tnode.size() - H.t0);
-    if (nn + H.nv >= 0xFFFF) throw std::runtime_error("tile has too many points");
-    for (const auto &vr : vref) lloc[vr.first] = (uint16_t)(nn + vr.second);
+    const int nv = (int)to.voff.size(), nt = (int)sc.tnode.size();
+    if (nn + nv >= 0xFFFF) throw std::runtime_error("tile has too many points");
+    for (const auto &vr : sc.vref) lloc[vr.first] = (uint16_t)(nn + vr.second);
+    to.tpk.resize(nt);
+    for (int k = 0; k < nt; k++) to.tpk[k] = (uint32_t)sc.tnode[k] | (uint32_t)sc.twi[k] << 16;
     // The terms again, sorted by node (row order within a node), one segment per node.
-    std::vector<uint16_t> perm(H.nt);
-    for (int k = 0; k < H.nt; k++) perm[k] = (uint16_t)k;
-    std::stable_sort(perm.begin(), perm.end(),
-                     [&](uint16_t a, uint16_t b) { return tnode[H.t0 + a] < tnode[H.t0 + b]; });
-    H.s0 = (int32_t)sg.size();
-    for (int k = 0; k < H.nt; k++)
-      tpk.push_back((uint32_t)tnode[H.t0 + k] | (uint32_t)twi[H.t0 + k] << 16);
-    for (int i = 0; i < H.nt; i++) {
-      const size_t k = H.t0 + perm[i];
-      tspk.push_back((uint32_t)tvirt[k] | (uint32_t)twi[k] << 16);
-      if (i == 0 || tnode[H.t0 + perm[i - 1]] != tnode[k]) {
-        sg.push_back((uint32_t)tnode[k] | (uint32_t)i << 16);
-      }
+    sc.perm.resize(nt);
+    for (int k = 0; k < nt; k++) sc.perm[k] = (uint16_t)k;
+    std::stable_sort(sc.perm.begin(), sc.perm.end(),
+                     [&](uint16_t a, uint16_t b) { return sc.tnode[a] < sc.tnode[b]; });
+    to.tspk.resize(nt);
+    for (int i = 0; i < nt; i++) {
+      const int k = sc.perm[i];
+      to.tspk[i] = (uint32_t)sc.tvirt[k] | (uint32_t)sc.twi[k] << 16;
+      if (i == 0 || sc.tnode[sc.perm[i - 1]] != sc.tnode[k]) to.sg.push_back((uint32_t)sc.tnode[k] | (uint32_t)i << 16);
     }
-    H.ns = (int32_t)(sg.size() - H.s0);
-    max_points = std::max(max_points, nn + H.nv);
+  };
+  {
+    const unsigned hw = std::thread::hardware_concurrency();
+    const int nthreads = (int)std::max(1u, std::min(16u, hw ? hw : 1u));
+    std::atomic<size_t> next{0};
+    std::exception_ptr err;
+    std::mutex err_mu;
+    auto worker = [&]() {
+      try {
+        Scratch sc;
+        while (true) {
+          const size_t t0 = next.fetch_add(16);
+          if (t0 >= ntiles) break;
+          for (size_t t = t0; t < std::min(ntiles, t0 + 16); t++) build_tile(t, outs[t], sc);
+        }
+      } catch (...) {
+        std::lock_guard<std::mutex> g(err_mu);
+        if (!err) err = std::current_exception();
+        next.store(ntiles);
+      }
+    };
+    std::vector<std::thread> pool;
+    for (int i = 1; i < nthreads; i++) pool.emplace_back(worker);
+    worker();
+    for (auto &th : pool) th.join();
+    if (err) std::rethrow_exception(err);
+  }
+  std::vector<ConfTileHdr> hdr(ntiles);
+  std::vector<int32_t> tnodes;
+  std::vector<uint16_t> voff;
+  std::vector<uint32_t> tpk, tspk, sg;
+  int max_points = 0, max_virt = 0, max_terms = 0;
+  for (size_t t = 0; t < ntiles; t++) {
+    ConfTileOut &to = outs[t];
+    ConfTileHdr &H = hdr[t];
+    H.leaf0 = (int32_t)bounds[t];
+    H.nleaf = (int32_t)(bounds[t + 1] - bounds[t]);
+    H.node0 = (int32_t)tnodes.size();
+    H.nnode = (int32_t)to.nodes.size();
+    H.v0 = (int32_t)voff.size();
+    H.nv = (int32_t)to.voff.size();
+    H.t0 = (int32_t)tpk.size();
+    H.nt = (int32_t)to.tpk.size();
+    H.s0 = (int32_t)sg.size();
+    H.ns = (int32_t)to.sg.size();
+    tnodes.insert(tnodes.end(), to.nodes.begin(), to.nodes.end());
+    voff.insert(voff.end(), to.voff.begin(), to.voff.end());
+    tpk.insert(tpk.end(), to.tpk.begin(), to.tpk.end());
+    tspk.insert(tspk.end(), to.tspk.begin(), to.tspk.end());
+    sg.insert(sg.end(), to.sg.begin(), to.sg.end());
+    max_points = std::max(max_points, H.nnode + H.nv);
     max_virt = std::max(max_virt, H.nv);
     max_terms = std::max(max_terms, H.nt);
+    ConfTileOut().nodes.swap(to.nodes);  // release as we go
   }
   // Halve the tiles while one right-hand side in FP64 does not fit in LDS.
   pl->lds_bytes = conf_lds_bytes<double, 4>(max_points, 1, (int)wtab.size());
