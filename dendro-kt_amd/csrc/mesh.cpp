@@ -172,15 +172,14 @@ u128 sfc_key(const MeshParams &mp, const Oct &o) {
   const SfcTables &t = sfc_tables(mp.dim, mp.curve);
   int rot = 0;
   u128 key = 0;
-  for (int l = 1; l <= mp.maxd; l++) {
-    unsigned digit = 0;
-    if (l <= (int)o.lev) {
-      const unsigned m = morton_digit(o, mp.dim, mp.maxd, l);
-      digit = t.inv[rot * t.nch + m];
-      rot = t.child[rot * t.nch + m];
-    }
-    key = (key << mp.dim) | digit;
+  const int lev = std::min((int)o.lev, mp.maxd);
+  for (int l = 1; l <= lev; l++) {
+    const unsigned m = morton_digit(o, mp.dim, mp.maxd, l);
+    key = (key << mp.dim) | t.inv[rot * t.nch + m];
+    rot = t.child[rot * t.nch + m];
   }
+  // the levels below the octant's own contribute zero digits
+  if (lev < mp.maxd) key <<= (unsigned)(mp.dim * (mp.maxd - lev));
   return key;
 }
 
