@@ -270,11 +270,65 @@ struct Builder {
 std::vector<Oct> construct_tree(const MeshParams &mp, std::vector<Oct> &pts, size_t max_pts) {
   const SfcTables &t = sfc_tables(mp.dim, mp.curve);
   auto k = keyed(mp, pts);
-  std::vector<Oct> out;
+  // The first levels of the recursion are expanded here, in visiting order,
+  // into independent subtrees (a leaf is a subtree of one octant); the
+  // subtrees are built in parallel and concatenated in that order, which is
+  // the order of the sequential recursion.
+  struct Task {
+    size_t b, e;
+    Oct cell;
+    int rot;
+    bool leaf;
+  };
+  std::vector<Task> tasks;
+  const int split_levels = mp.dim >= 4 ? 2 : 3;
+  std::function<void(size_t, size_t, const Oct &, int)> expand = [&](size_t b, size_t e, const Oct &cell, int rot) {
+    const int l = (int)cell.lev + 1;
+    while (b < e && (int)k[b].lev < l) b++;
+    const int sh = mp.dim * (mp.maxd - l);
+    const u32 len = 1u << (mp.maxd - l);
+    size_t s = b;
+    for (int cs = 0; cs < t.nch; cs++) {
+      size_t f = s;
+      while (f < e && (unsigned)((k[f].key >> sh) & (u128)(t.nch - 1)) == (unsigned)cs) f++;
+      const unsigned m = t.perm[rot * t.nch + cs];
+      Oct ch;
+      ch.lev = (u32)l;
+      for (int d = 0; d < 4; d++) ch.x[d] = d < mp.dim ? cell.x[d] + (((m >> d) & 1u) ? len : 0u) : 0u;
+      const int crot = t.child[rot * t.nch + m];
+      if (l < mp.maxd && f - s > max_pts) {
+        if (l < split_levels)
+          expand(s, f, ch, crot);
+        else
+          tasks.push_back({s, f, ch, crot, false});
+      } else {
+        tasks.push_back({s, f, ch, crot, true});
+      }
+      s = f;
+    }
+  };
   Oct root{};
   root.lev = 0;
-  Builder b{mp, t, k, max_pts, out};
-  b.rec(0, k.size(), root, 0);
+  expand(0, k.size(), root, 0);
+  std::vector<std::vector<Oct>> part(tasks.size());
+#pragma omp parallel for schedule(dynamic, 1)
+  for (size_t i = 0; i < tasks.size(); i++) {
+    const Task &tk = tasks[i];
+    if (tk.leaf) {
+      part[i].push_back(tk.cell);
+    } else {
+      Builder bl{mp, t, k, max_pts, part[i]};
+      bl.rec(tk.b, tk.e, tk.cell, tk.rot);
+    }
+  }
+  size_t total = 0;
+  for (const auto &v : part) total += v.size();
+  std::vector<Oct> out;
+  out.reserve(total);
+  for (auto &v : part) {
+    out.insert(out.end(), v.begin(), v.end());
+    std::vector<Oct>().swap(v);
+  }
   return out;
 }
 
