@@ -2,6 +2,7 @@
 #include <cstdio>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <exception>
 #include <string>
 
@@ -26,10 +27,19 @@ struct dkt_da {
   MeshParams mp;
   std::vector<Oct> elems;
   NodeSet ns;
-  Traversal tr;
-  std::unique_ptr<Traversal> conf;  // conforming element records, built on first use
+  // Built on first use, so a mesh pays only for the operator it is used with:
+  // the reference traversal (reference-semantics plans, traversal queries)
+  // and the conforming element records (conforming plans).
+  mutable std::mutex mu;
+  mutable std::unique_ptr<Traversal> ref, conf;
+  Traversal &traversal() const {
+    std::lock_guard<std::mutex> g(mu);
+    if (!ref) ref = std::make_unique<Traversal>(flatten_traversal(mp, ns.nodes));
+    return *ref;
+  }
   const Traversal &conforming() const {
-    if (!conf) const_cast<dkt_da *>(this)->conf = std::make_unique<Traversal>(build_conforming(mp, elems, ns));
+    std::lock_guard<std::mutex> g(mu);
+    if (!conf) conf = std::make_unique<Traversal>(build_conforming(mp, elems, ns));
     return *conf;
   }
 };
@@ -166,7 +176,6 @@ extern "C" int dkt_da_create(const dkt_octs *tree, int order, dkt_da **out) {
         if (order > 1 && (int)e.lev >= da->mp.maxd)
           throw std::invalid_argument("order 2 needs every element level below maxdepth");
       da->ns = enumerate_nodes(da->mp, da->elems);
-      da->tr = flatten_traversal(da->mp, da->ns.nodes);
     } catch (...) {
       delete da;
       throw;
@@ -186,11 +195,11 @@ extern "C" size_t dkt_da_count(const dkt_da *da, int what) {
     case 0: return da->elems.size();
     case 1: return da->ns.nodes.size();
     case 2: return da->ns.bdy.size();
-    case 3: return da->tr.leaves.size();
-    case 4: return da->tr.pslot.size() / (size_t)da->tr.nPe;
-    case 5: return (size_t)da->tr.nPe;
-    case 6: compute_stale_terms(da->mp, const_cast<dkt_da *>(da)->tr); return da->tr.stale_needed;
-    case 7: compute_stale_terms(da->mp, const_cast<dkt_da *>(da)->tr); return da->tr.hc_node.size();
+    case 3: return da->traversal().leaves.size();
+    case 4: return da->traversal().pslot.size() / (size_t)da->traversal().nPe;
+    case 5: return (size_t)da->traversal().nPe;
+    case 6: compute_stale_terms(da->mp, da->traversal()); return da->traversal().stale_needed;
+    case 7: compute_stale_terms(da->mp, da->traversal()); return da->traversal().hc_node.size();
     case 8: return da->conforming().cf_node.size();
     case 9: return da->conforming().cf_off.size() - 1;
     default: return 0;
@@ -226,7 +235,7 @@ extern "C" int dkt_da_traversal(const dkt_da *da, uint32_t *cells, uint8_t *chil
                                 int32_t *pslot) {
   return guarded([&] {
     if (!da) throw std::invalid_argument("null pointer");
-    const Traversal &t = da->tr;
+    const Traversal &t = da->traversal();
     if (cells) copy_octs(t.leaves, da->mp.dim, cells);
     if (child) std::memcpy(child, t.child.data(), t.child.size());
     if (slot) std::memcpy(slot, t.slot.data(), t.slot.size() * sizeof(int32_t));
@@ -238,8 +247,8 @@ extern "C" int dkt_da_traversal(const dkt_da *da, uint32_t *cells, uint8_t *chil
 extern "C" int dkt_da_stale_terms(const dkt_da *da, int64_t *off, uint8_t *rank, int32_t *node, double *w) {
   return guarded([&] {
     if (!da) throw std::invalid_argument("null pointer");
-    compute_stale_terms(da->mp, const_cast<dkt_da *>(da)->tr);
-    const Traversal &t = da->tr;
+    compute_stale_terms(da->mp, da->traversal());
+    const Traversal &t = da->traversal();
     if (off) std::memcpy(off, t.hc_off.data(), t.hc_off.size() * sizeof(int64_t));
     if (rank) std::memcpy(rank, t.hc_rank.data(), t.hc_rank.size());
     if (node) std::memcpy(node, t.hc_node.data(), t.hc_node.size() * sizeof(int32_t));
@@ -335,7 +344,7 @@ extern "C" int dkt_da_write_vtu(const dkt_da *da, const char *path, const double
 namespace dkt {
 const MeshParams &da_params(const dkt_da *da) { return da->mp; }
 const NodeSet &da_nodes(const dkt_da *da) { return da->ns; }
-const Traversal &da_traversal(const dkt_da *da) { return da->tr; }
-Traversal &da_traversal_mut(const dkt_da *da) { return const_cast<dkt_da *>(da)->tr; }
+const Traversal &da_traversal(const dkt_da *da) { return da->traversal(); }
+Traversal &da_traversal_mut(const dkt_da *da) { return da->traversal(); }
 Traversal &da_conforming_mut(const dkt_da *da) { return const_cast<Traversal &>(da->conforming()); }
 }  // namespace dkt

@@ -1406,7 +1406,7 @@ extern "C" int dkt_plan_create(const dkt_da *da, int op, const double *op_data, 
     pl->op = op;
     pl->N = ns.nodes.size();
     pl->L = tr.leaves.size();
-    pl->H = tr.pslot.size() / tr.nPe;
+    pl->H = conforming ? tr.cf_off.size() - 1 : tr.pslot.size() / tr.nPe;
     pl->simple = (flags & DKT_PLAN_SIMPLE) != 0;
     pl->dry = (flags & DKT_PLAN_DRYRUN) != 0;
     pl->conforming = conforming;
