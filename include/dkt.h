@@ -97,7 +97,9 @@ int dkt_keyed_uniform(uint64_t seed, int dim, const uint32_t *nodes, size_t n, d
 
 /* ---- Mesh (DA) ---------------------------------------------------------- */
 /* Builds the CG node list of a sorted, 2:1 balanced complete tree in the
- * reference's node order, and flattens the matvec traversal over it.
+ * reference's node order.  The matvec traversal over them is flattened on
+ * first use (a reference-semantics plan, counts 3-7, dkt_da_traversal), the
+ * conforming element records likewise (a conforming plan, counts 8-9).
  * Replaces ot::DA<dim>::DA(tree, nEle, comm, order) (src/oda.cpp:45-151). */
 int dkt_da_create(const dkt_octs *tree, int order, dkt_da **out);
 int dkt_da_destroy(dkt_da *da);
