@@ -979,8 +979,11 @@ Traversal build_conforming(const MeshParams &mp, const std::vector<Oct> &elems, 
   // Node ids by coordinates: an open-addressing table, filled without a
   // per-node allocation and probed by every element rank.
   struct NodeIndex {
-    std::vector<std::array<u32, 4>> key;
-    std::vector<int32_t> id;
+    struct Slot {
+      u32 x[4];
+      int32_t id;
+    };
+    std::vector<Slot> slot;
     size_t mask = 0;
     static size_t hash(const u32 *x) {
       u64 h = 0x9E3779B97F4A7C15ull;
@@ -989,24 +992,32 @@ Traversal build_conforming(const MeshParams &mp, const std::vector<Oct> &elems, 
     }
     int32_t find(const u32 *x) const {
       for (size_t i = hash(x) & mask;; i = (i + 1) & mask) {
-        if (id[i] < 0) return -1;
-        if (key[i][0] == x[0] && key[i][1] == x[1] && key[i][2] == x[2] && key[i][3] == x[3]) return id[i];
+        const Slot &s = slot[i];
+        if (s.id < 0) return -1;
+        if (s.x[0] == x[0] && s.x[1] == x[1] && s.x[2] == x[2] && s.x[3] == x[3]) return s.id;
       }
     }
   } at;
   {
     size_t cap = 16;
     while (cap < ns.nodes.size() * 2) cap <<= 1;
-    at.key.assign(cap, {0, 0, 0, 0});
-    at.id.assign(cap, -1);
+    at.slot.assign(cap, NodeIndex::Slot{{0, 0, 0, 0}, -1});
     at.mask = cap - 1;
+    // Filled in parallel: a node claims the first free slot of its probe
+    // sequence with a compare-and-swap on the id, then writes its key (node
+    // positions are distinct; lookups start after the fill).
+#pragma omp parallel for schedule(static)
     for (size_t n = 0; n < ns.nodes.size(); n++) {
       u32 x[4] = {0, 0, 0, 0};
       for (int d = 0; d < dim; d++) x[d] = ns.nodes[n].x[d];
-      size_t i = NodeIndex::hash(x) & at.mask;
-      while (at.id[i] >= 0) i = (i + 1) & at.mask;  // node positions are distinct
-      at.key[i] = {x[0], x[1], x[2], x[3]};
-      at.id[i] = (int32_t)n;
+      for (size_t i = NodeIndex::hash(x) & at.mask;; i = (i + 1) & at.mask) {
+        int32_t expected = -1;
+        if (__atomic_compare_exchange_n(&at.slot[i].id, &expected, (int32_t)n, false, __ATOMIC_RELAXED,
+                                        __ATOMIC_RELAXED)) {
+          for (int d = 0; d < 4; d++) at.slot[i].x[d] = x[d];
+          break;
+        }
+      }
     }
   }
   double ip[2][9];
