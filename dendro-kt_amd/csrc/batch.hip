@@ -579,6 +579,24 @@ GroupBase *make_group(int kind, int dim, int p, const std::vector<dkt_plan *> &p
                       const std::vector<const void *> &xs, const std::vector<void *> &ys, int nrhs, int masked,
                       const std::vector<const int *> &cancel = {}, DevArena *arena = nullptr, int max_cols = 0,
                       bool colmajor = false) {
+  if (colmajor && nrhs > 1) {
+    // Column-major vectors: column q of plan k is a contiguous vector of its
+    // own, so each (column, plan) pair becomes one single-column record and
+    // every column runs in the same tile launch and the same fix-up launch.
+    std::vector<dkt_plan *> pl2;
+    std::vector<const void *> xs2;
+    std::vector<void *> ys2;
+    std::vector<const int *> cs2;
+    for (int q = 0; q < nrhs; q++)
+      for (size_t k = 0; k < pls.size(); k++) {
+        const size_t off = (size_t)q * pls[k]->N * sizeof(T);
+        pl2.push_back(pls[k]);
+        xs2.push_back((const unsigned char *)xs[k] + off);
+        ys2.push_back((unsigned char *)ys[k] + off);
+        if (!cancel.empty()) cs2.push_back(cancel[k]);
+      }
+    return make_group<T>(kind, dim, p, pl2, xs2, ys2, 1, masked, cs2, arena, 0, true);
+  }
 #define DKT_GROUP(D, PP)                                                                                     \
   if (dim == D && p == PP) {                                                                                 \
     if (kind == 2)                                                                                           \

@@ -325,9 +325,10 @@ typedef struct dkt_batch dkt_batch;
  * column's result is bitwise the same either way. */
 #define DKT_BATCH_COLUMNS_SHIFT 8
 /* x[k], y[k] column-major, [nrhs][N_k] (column q at q * N_k), instead of
- * [N_k][nrhs]: every launch carries one contiguous column (the single-column
- * kernels, whose reads of x are then unit-stride).  Bitwise the results of the
- * node-major layout, column by column. */
+ * [N_k][nrhs]: every (column, plan) pair is a single-column record (the
+ * single-column kernels, whose reads of x are then unit-stride), and all
+ * columns of a (dim, p) group share one tile launch and one fix-up launch.
+ * Bitwise the results of the node-major layout, column by column. */
 #define DKT_BATCH_COLUMN_MAJOR 4
 int dkt_batch_create(dkt_plan *const *plans, int nplans, int dtype, int nrhs, const void *const *x, void *const *y,
                      int masked, int flags, dkt_batch **out);
