@@ -1,12 +1,13 @@
 // Batched matvec over many independent meshes (no counterpart in the
 // reference, which runs one fem::matvec per call, FEM/include/matvec.h:231).
 //
-// Plans of the same (dim, p) run as one launch: workgroup b finds its plan by
-// a binary search over the plans' tile offsets and runs that plan's tile with
-// the plan's maps and vectors (a device table of BatchPlan records), followed
-// by one batched fix-up launch (per chunk of right-hand sides that fits the
-// tiles' LDS).  With DKT_BATCH_GRAPH the launches of every
-// (dim, p) group are captured once into a hipGraph and replayed.  Each plan's
+// Plans of the same (dim, p) run as one launch: workgroup b finds its record
+// by a binary search over the records' tile offsets and runs that tile with
+// the record's maps and vectors (a device table of BatchPlan records, one per
+// plan and chunk of right-hand sides that fits the tiles' LDS, each chunk with
+// partial slots of its own), followed by one batched fix-up launch.  With
+// DKT_BATCH_GRAPH the two launches of every (dim, p) group are captured once
+// into a hipGraph and replayed.  Each plan's
 // result is bitwise the same as its own dkt_plan_matvec: the tile work and
 // the fix-up order are the plan's.
 #include <algorithm>
@@ -57,13 +58,14 @@ struct GroupMem {
 template <typename T, int DIM, int P>
 struct Group : GroupBase {
   GroupMem mem;
-  // One table of plan records per column chunk (columns [q0, q0 + nq)).
-  std::vector<BatchPlan<T> *> tables;
+  // One record per (column chunk, plan): columns [q0, q0 + nq) of the plan,
+  // every chunk in the same launch with partial slots of its own.
+  BatchPlan<T> *table = nullptr;
   int *tile_start = nullptr, *block_start = nullptr;
   T *partial = nullptr;
-  int nplans = 0, ntiles = 0, nblocks = 0, threads = 0;
-  std::vector<size_t> lds;
-  std::vector<int> nqs;  // columns per chunk
+  int nrec = 0, ntiles = 0, nblocks = 0, threads = 0;
+  size_t lds = 0;
+  bool one = false;  // every record has one column
 
   Group(const std::vector<dkt_plan *> &pls, const std::vector<const void *> &xs, const std::vector<void *> &ys,
         int nrhs, int masked, const std::vector<const int *> &cancel, DevArena *arena = nullptr, int max_cols = 0,
@@ -81,25 +83,28 @@ struct Group : GroupBase {
       if (tile_lds_bytes<T, NPE>(dims_of(pl), 1) > kLdsLimit) throw std::invalid_argument("tile does not fit in LDS");
       while (chunk > 1 && tile_lds_bytes<T, NPE>(dims_of(pl), chunk) > kLdsLimit) chunk--;
     }
+    const int nchunks = (nrhs + chunk - 1) / chunk;
     std::vector<int> ts{0}, bs{0};
     std::vector<size_t> poff;
     size_t ptot = 0;
-    for (dkt_plan *pl : pls) {
-      threads = std::max(threads, pl->tile_leaves);
-      poff.push_back(ptot);
-      ptot += pl->npartial * (size_t)chunk;
-      ts.push_back(ts.back() + (int)pl->ntiles);
-      bs.push_back(bs.back() + (int)((pl->nshared + 255) / 256));
-    }
+    for (int c = 0; c < nchunks; c++)
+      for (dkt_plan *pl : pls) {
+        threads = std::max(threads, pl->tile_leaves);
+        poff.push_back(ptot);
+        ptot += pl->npartial * (size_t)chunk;
+        ts.push_back(ts.back() + (int)pl->ntiles);
+        bs.push_back(bs.back() + (int)((pl->nshared + 255) / 256));
+      }
     partial = mem.alloc<T>(std::max<size_t>(1, ptot));
-    for (int q0 = 0; q0 < nrhs; q0 += chunk) {
-      const int nq = std::min(chunk, nrhs - q0);
-      std::vector<BatchPlan<T>> hp(pls.size());
-      size_t need = 0;
+    one = chunk == 1;
+    std::vector<BatchPlan<T>> hp(pls.size() * nchunks);
+    for (int c = 0; c < nchunks; c++) {
+      const int q0 = c * chunk, nq = std::min(chunk, nrhs - q0);
       for (size_t k = 0; k < pls.size(); k++) {
         dkt_plan *pl = pls[k];
-        need = std::max(need, tile_lds_bytes<T, NPE>(dims_of(pl), nq));
-        TileArgs &A = hp[k].A;
+        BatchPlan<T> &r = hp[c * pls.size() + k];
+        lds = std::max(lds, tile_lds_bytes<T, NPE>(dims_of(pl), nq));
+        TileArgs &A = r.A;
         A.tiles = pl->tiles.p;
         A.tile_nodes = pl->tile_nodes.p;
         A.tile_out = pl->tile_out.p;
@@ -118,22 +123,20 @@ struct Group : GroupBase {
         A.nq = nq;
         A.ld = colmajor ? 1 : nrhs;
         A.dims = dims_of(pl);
-        hp[k].kref = P == 2 ? (f64 ? (const T *)pl->kfac64.p : (const T *)pl->kfac32.p)
-                            : (f64 ? (const T *)pl->kmat64.p : (const T *)pl->kmat32.p);
-        hp[k].carry_w = f64 ? (const T *)pl->carry_w64.p : (const T *)pl->carry_w32.p;
+        r.kref = P == 2 ? (f64 ? (const T *)pl->kfac64.p : (const T *)pl->kfac32.p)
+                        : (f64 ? (const T *)pl->kmat64.p : (const T *)pl->kmat32.p);
+        r.carry_w = f64 ? (const T *)pl->carry_w64.p : (const T *)pl->carry_w32.p;
         const size_t coff = colmajor ? (size_t)q0 * pl->N : (size_t)q0;
-        hp[k].x = (const T *)xs[k] + coff;
-        hp[k].y = (T *)ys[k] + coff;
-        hp[k].partial = partial + poff[k];
-        hp[k].snode = pl->snode.p;
-        hp[k].soff = pl->soff.p;
-        hp[k].nshared = (int)pl->nshared;
+        r.x = (const T *)xs[k] + coff;
+        r.y = (T *)ys[k] + coff;
+        r.partial = partial + poff[c * pls.size() + k];
+        r.snode = pl->snode.p;
+        r.soff = pl->soff.p;
+        r.nshared = (int)pl->nshared;
       }
-      tables.push_back(mem.upload(hp));
-      lds.push_back(need);
-      nqs.push_back(nq);
     }
-    nplans = (int)pls.size();
+    table = mem.upload(hp);
+    nrec = (int)hp.size();
     ntiles = ts.back();
     nblocks = bs.back();
     tile_start = mem.upload(ts);
@@ -141,18 +144,15 @@ struct Group : GroupBase {
   }
 
   void launch(hipStream_t st) override {
-    for (size_t c = 0; c < tables.size(); c++) {
-      if (ntiles && nqs[c] == 1)
-        hipLaunchKernelGGL((tile_kernel_batch<T, DIM, P, true>), dim3(ntiles), dim3(threads), lds[c], st,
-                           tables[c], tile_start, nplans);
-      else if (ntiles)
-        hipLaunchKernelGGL((tile_kernel_batch<T, DIM, P, false>), dim3(ntiles), dim3(threads), lds[c], st,
-                           tables[c], tile_start, nplans);
-      if (nblocks)
-        hipLaunchKernelGGL((fixup_kernel_batch<T>), dim3(nblocks), dim3(256), 0, st, tables[c], block_start,
-                           nplans);
-      HIP_OK(hipGetLastError());
-    }
+    if (ntiles && one)
+      hipLaunchKernelGGL((tile_kernel_batch<T, DIM, P, true>), dim3(ntiles), dim3(threads), lds, st, table,
+                         tile_start, nrec);
+    else if (ntiles)
+      hipLaunchKernelGGL((tile_kernel_batch<T, DIM, P, false>), dim3(ntiles), dim3(threads), lds, st, table,
+                         tile_start, nrec);
+    if (nblocks)
+      hipLaunchKernelGGL((fixup_kernel_batch<T>), dim3(nblocks), dim3(256), 0, st, table, block_start, nrec);
+    HIP_OK(hipGetLastError());
   }
 };
 
@@ -201,18 +201,19 @@ __global__ void __launch_bounds__(kTileThreads) conf_tile_kernel_batch(const Con
   conf_tile_body<T, DIM, P, CAP>(bp.A, t - tile_start[k], bp.kref, bp.term_w, bp.x, bp.y, bp.partial);
 }
 
-// Conforming plans of one (dim, p) (CAP: with per-level captured matrices): one tile launch and one fix-up launch per
-// column chunk that fits the tiles' LDS.  Each result is bitwise the plan's
-// own matvec (same tile work, same fix-up order).
+// Conforming plans of one (dim, p) (CAP: with per-level captured matrices):
+// one tile launch and one fix-up launch, with one record per (column chunk
+// that fits the tiles' LDS, plan).  Each result is bitwise the plan's own
+// matvec (same tile work, same fix-up order).
 template <typename T, int DIM, int P, bool CAP>
 struct ConfGroup : GroupBase {
   GroupMem mem;
-  std::vector<ConfBatchPlan<T> *> tables;
-  std::vector<FixRec<T> *> fixes;
+  ConfBatchPlan<T> *table = nullptr;
+  FixRec<T> *fixes = nullptr;
   int *tile_start = nullptr, *block_start = nullptr;
   T *partial = nullptr;
-  int nplans = 0, ntiles = 0, nblocks = 0, threads = 0;
-  std::vector<size_t> lds;
+  int nrec = 0, ntiles = 0, nblocks = 0, threads = 0;
+  size_t lds = 0;
 
   ConfGroup(const std::vector<dkt_plan *> &pls, const std::vector<const void *> &xs, const std::vector<void *> &ys,
             int nrhs, int masked, const std::vector<const int *> &cancel, DevArena *arena = nullptr,
@@ -226,26 +227,28 @@ struct ConfGroup : GroupBase {
       if (lds_of(pl, 1) > kLdsLimit) throw std::invalid_argument("tile does not fit in LDS");
       while (chunk > 1 && lds_of(pl, chunk) > kLdsLimit) chunk--;
     }
+    const int nchunks = (nrhs + chunk - 1) / chunk;
     std::vector<int> ts{0}, bs{0};
     std::vector<size_t> poff;
     size_t ptot = 0;
-    for (dkt_plan *pl : pls) {
-      threads = std::max(threads, pl->tile_leaves);
-      poff.push_back(ptot);
-      ptot += pl->npartial * (size_t)chunk;
-      ts.push_back(ts.back() + (int)pl->ntiles);
-      bs.push_back(bs.back() + (int)((pl->nshared + 255) / 256));
-    }
+    for (int c = 0; c < nchunks; c++)
+      for (dkt_plan *pl : pls) {
+        threads = std::max(threads, pl->tile_leaves);
+        poff.push_back(ptot);
+        ptot += pl->npartial * (size_t)chunk;
+        ts.push_back(ts.back() + (int)pl->ntiles);
+        bs.push_back(bs.back() + (int)((pl->nshared + 255) / 256));
+      }
     partial = mem.alloc<T>(std::max<size_t>(1, ptot));
-    for (int q0 = 0; q0 < nrhs; q0 += chunk) {
-      const int nq = std::min(chunk, nrhs - q0);
-      std::vector<ConfBatchPlan<T>> hp(pls.size());
-      std::vector<FixRec<T>> hf(pls.size());
-      size_t need = 0;
+    std::vector<ConfBatchPlan<T>> hp(pls.size() * nchunks);
+    std::vector<FixRec<T>> hf(pls.size() * nchunks);
+    for (int c = 0; c < nchunks; c++) {
+      const int q0 = c * chunk, nq = std::min(chunk, nrhs - q0);
       for (size_t k = 0; k < pls.size(); k++) {
         dkt_plan *pl = pls[k];
-        need = std::max(need, lds_of(pl, nq));
-        ConfArgs &A = hp[k].A;
+        const size_t ri = c * pls.size() + k;
+        lds = std::max(lds, lds_of(pl, nq));
+        ConfArgs &A = hp[ri].A;
         A.tiles = pl->ctiles.p;
         A.tile_nodes = pl->tile_nodes.p;
         A.tile_out = pl->tile_out.p;
@@ -265,21 +268,20 @@ struct ConfGroup : GroupBase {
         A.ld = colmajor ? 1 : nrhs;
         A.max_points = pl->max_nodes;
         A.tile_leaves = pl->tile_leaves;
-        hp[k].kref = P == 2 && !pl->per_level ? (f64 ? (const T *)pl->kfac64.p : (const T *)pl->kfac32.p)
-                                              : (f64 ? (const T *)pl->kmat64.p : (const T *)pl->kmat32.p);
-        hp[k].term_w = f64 ? (const T *)pl->term_w64.p : (const T *)pl->term_w32.p;
+        hp[ri].kref = P == 2 && !pl->per_level ? (f64 ? (const T *)pl->kfac64.p : (const T *)pl->kfac32.p)
+                                               : (f64 ? (const T *)pl->kmat64.p : (const T *)pl->kmat32.p);
+        hp[ri].term_w = f64 ? (const T *)pl->term_w64.p : (const T *)pl->term_w32.p;
         const size_t coff = colmajor ? (size_t)q0 * pl->N : (size_t)q0;
-        hp[k].x = (const T *)xs[k] + coff;
-        hp[k].y = (T *)ys[k] + coff;
-        hp[k].partial = partial + poff[k];
-        hf[k] = FixRec<T>{pl->snode.p, pl->soff.p, partial + poff[k], (T *)ys[k] + coff, A.dir_out, A.cancel,
-                          nq, A.ld, (int)pl->nshared};
+        hp[ri].x = (const T *)xs[k] + coff;
+        hp[ri].y = (T *)ys[k] + coff;
+        hp[ri].partial = partial + poff[ri];
+        hf[ri] = FixRec<T>{pl->snode.p, pl->soff.p, partial + poff[ri], (T *)ys[k] + coff, A.dir_out, A.cancel,
+                           nq, A.ld, (int)pl->nshared};
       }
-      tables.push_back(mem.upload(hp));
-      fixes.push_back(mem.upload(hf));
-      lds.push_back(need);
     }
-    nplans = (int)pls.size();
+    table = mem.upload(hp);
+    fixes = mem.upload(hf);
+    nrec = (int)hp.size();
     ntiles = ts.back();
     nblocks = bs.back();
     tile_start = mem.upload(ts);
@@ -287,15 +289,12 @@ struct ConfGroup : GroupBase {
   }
 
   void launch(hipStream_t st) override {
-    for (size_t c = 0; c < tables.size(); c++) {
-      if (ntiles)
-        hipLaunchKernelGGL((conf_tile_kernel_batch<T, DIM, P, CAP>), dim3(ntiles), dim3(threads), lds[c], st,
-                           tables[c], tile_start, nplans);
-      if (nblocks)
-        hipLaunchKernelGGL((fixup_kernel_rec<T>), dim3(nblocks), dim3(256), 0, st, fixes[c], block_start,
-                           nplans);
-      HIP_OK(hipGetLastError());
-    }
+    if (ntiles)
+      hipLaunchKernelGGL((conf_tile_kernel_batch<T, DIM, P, CAP>), dim3(ntiles), dim3(threads), lds, st, table,
+                         tile_start, nrec);
+    if (nblocks)
+      hipLaunchKernelGGL((fixup_kernel_rec<T>), dim3(nblocks), dim3(256), 0, st, fixes, block_start, nrec);
+    HIP_OK(hipGetLastError());
   }
 };
 
@@ -579,24 +578,6 @@ GroupBase *make_group(int kind, int dim, int p, const std::vector<dkt_plan *> &p
                       const std::vector<const void *> &xs, const std::vector<void *> &ys, int nrhs, int masked,
                       const std::vector<const int *> &cancel = {}, DevArena *arena = nullptr, int max_cols = 0,
                       bool colmajor = false) {
-  if (colmajor && nrhs > 1) {
-    // Column-major vectors: column q of plan k is a contiguous vector of its
-    // own, so each (column, plan) pair becomes one single-column record and
-    // every column runs in the same tile launch and the same fix-up launch.
-    std::vector<dkt_plan *> pl2;
-    std::vector<const void *> xs2;
-    std::vector<void *> ys2;
-    std::vector<const int *> cs2;
-    for (int q = 0; q < nrhs; q++)
-      for (size_t k = 0; k < pls.size(); k++) {
-        const size_t off = (size_t)q * pls[k]->N * sizeof(T);
-        pl2.push_back(pls[k]);
-        xs2.push_back((const unsigned char *)xs[k] + off);
-        ys2.push_back((unsigned char *)ys[k] + off);
-        if (!cancel.empty()) cs2.push_back(cancel[k]);
-      }
-    return make_group<T>(kind, dim, p, pl2, xs2, ys2, 1, masked, cs2, arena, 0, true);
-  }
 #define DKT_GROUP(D, PP)                                                                                     \
   if (dim == D && p == PP) {                                                                                 \
     if (kind == 2)                                                                                           \
