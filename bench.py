@@ -242,7 +242,7 @@ def batch_leg(dkt, torch, ntrees):
     right-hand sides each, FP32, tile sizes and columns per launch from the
     autotuner, one hipGraph replay per step (per (dim, p): its tile launches and
     fix-ups), with the vectors node-major ([N][8]) and column-major ([8][N]; one
-    column per launch), the faster layout reported, against the same plans
+    column per tile record), the faster layout reported, against the same plans
     launched one by one."""
     wl = _pkg_mod("dkt_workloads", "workloads.py")
     at = _pkg_mod("dkt_autotune", "autotune.py")
@@ -286,7 +286,7 @@ def batch_leg(dkt, torch, ntrees):
         b = dkt.Batch(plans, xs, ys, graph=True, columns=c)
         per_cols[c] = timed(lambda: b.run(s.cuda_stream))
     # The same columns stored column-major ([nrhs][N] per tree): one
-    # contiguous column per launch.
+    # contiguous column per tile record.
     xc = [x.t().contiguous() for x in xs]
     yc = [torch.empty_like(x) for x in xc]
     bc = dkt.Batch(plans, xc, yc, graph=True, column_major=True)

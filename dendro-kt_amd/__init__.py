@@ -530,10 +530,11 @@ class Batch:
     ``xs``/``ys`` are torch device tensors [N_k] or [N_k, nrhs] (one dtype);
     with ``graph=True`` the launches are captured into a hipGraph on the first
     run (needs a non-default stream) and replayed afterwards.  ``columns``
-    caps the right-hand sides per launch (a power of two; 1 runs the
-    single-column kernels once per column; 0 = as many as fit in LDS).  With
-    ``column_major=True`` the vectors are [nrhs, N_k] and every launch carries
-    one contiguous column.
+    caps the right-hand sides per tile record (a power of two; 1 runs the
+    single-column kernels, a record per column; 0 = as many as fit in LDS);
+    every record of a (dim, p) group runs in the same launch.  With
+    ``column_major=True`` the vectors are [nrhs, N_k], one contiguous column
+    per record.
     """
 
     def __init__(self, plans, xs, ys, masked=False, graph=False, persistent=False, columns=0, column_major=False):

@@ -317,12 +317,14 @@ typedef struct dkt_batch dkt_batch;
  * dkt_batch_create refuses the flag (its launch did not finish on the device
  * and the cause is not found). */
 #define DKT_BATCH_PERSISTENT 2
-/* Columns per launch for nrhs > 1: flags |= (log2(columns) + 1) <<
+/* Columns per tile record for nrhs > 1: flags |= (log2(columns) + 1) <<
  * DKT_BATCH_COLUMNS_SHIFT with columns in {1, 2, 4, ..., 64}; 0 = as many
- * as the tiles' LDS allows.  One column per launch runs the single-column
- * tile kernels (fewer registers, more waves per SIMD) once per column, over
- * the same [N][nrhs] vectors; the autotuner chooses it per (dim, p).  Each
- * column's result is bitwise the same either way. */
+ * as the tiles' LDS allows.  The columns are cut into chunks of that many,
+ * each chunk a record of the same launch (one tile launch and one fix-up
+ * launch per (dim, p) group).  One column per record runs the single-column
+ * tile kernels (fewer registers, more waves per SIMD) over the same
+ * [N][nrhs] vectors; the autotuner chooses it per (dim, p).  Each column's
+ * result is bitwise the same either way. */
 #define DKT_BATCH_COLUMNS_SHIFT 8
 /* x[k], y[k] column-major, [nrhs][N_k] (column q at q * N_k), instead of
  * [N_k][nrhs]: every (column, plan) pair is a single-column record (the
