@@ -31,7 +31,7 @@ def _column_candidates(nrhs):
 
 
 def autotune(dkt, da, candidates=CANDIDATES, reps=10, warmup=3, nrhs=1):
-    """Fastest tile size (and, for nrhs > 1, columns per launch) per precision
+    """Fastest tile size (and, for nrhs > 1, columns per tile record) per precision
     for matvecs with `nrhs` columns on `da`: dict(best={"f64": T, "f32": T},
     columns={"f64": c, "f32": c}, ms={(T, c): {"f64": ms, "f32": ms}})."""
     import torch
