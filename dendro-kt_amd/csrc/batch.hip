@@ -55,15 +55,75 @@ struct GroupMem {
   }
 };
 
+// Shared nodes of one plan of a batch (the fix-up of either kind of tile).
+template <typename T>
+struct FixRec {
+  const int32_t *snode, *soff;
+  const T *partial;
+  T *y;
+  const uint8_t *dir_out;
+  const int *cancel;
+  int nq, ld, nshared;
+};
+
+// Shared nodes of a batch: workgroup b sums block b - block_start[k] of plan k.
+template <typename T>
+__global__ void __launch_bounds__(256) fixup_kernel_rec(const FixRec<T> *__restrict__ recs,
+                                                        const int *__restrict__ block_start, int nplans) {
+  const int k = find_range(block_start, nplans, blockIdx.x);
+  const FixRec<T> &fr = recs[k];
+  const int i = (blockIdx.x - block_start[k]) * 256 + threadIdx.x;
+  if (i >= fr.nshared || (fr.cancel && *fr.cancel)) return;
+  const int n = fr.snode[i];
+  const int a = fr.soff[i], b = fr.soff[i + 1];
+  const bool z = fr.dir_out && fr.dir_out[n];
+  for (int q = 0; q < fr.nq; q++) fr.y[(size_t)n * fr.ld + q] = z ? T(0) : sum_partials(fr.partial, a, b, fr.nq, q);
+}
+
+// The fix-up records of a group (one per tile record, host copies): a batch
+// sums the shared nodes of all its groups in one launch after their tiles.
+template <typename T>
+struct FixSource {
+  virtual ~FixSource() = default;
+  std::vector<FixRec<T>> fix;
+};
+
+struct FixLaunchBase {
+  virtual ~FixLaunchBase() = default;
+  virtual void launch(hipStream_t st) = 0;
+};
+
+template <typename T>
+struct FixAll : FixLaunchBase {
+  GroupMem mem;
+  FixRec<T> *table = nullptr;
+  int *block_start = nullptr;
+  int nrec = 0, nblocks = 0;
+  FixAll(const std::vector<FixRec<T>> &recs, DevArena *arena) {
+    mem.arena = arena;
+    std::vector<int> bs{0};
+    for (const FixRec<T> &r : recs) bs.push_back(bs.back() + (r.nshared + 255) / 256);
+    nrec = (int)recs.size();
+    nblocks = bs.back();
+    table = mem.upload(recs);
+    block_start = mem.upload(bs);
+  }
+  void launch(hipStream_t st) override {
+    if (!nblocks) return;
+    hipLaunchKernelGGL((fixup_kernel_rec<T>), dim3(nblocks), dim3(256), 0, st, table, block_start, nrec);
+    HIP_OK(hipGetLastError());
+  }
+};
+
 template <typename T, int DIM, int P>
-struct Group : GroupBase {
+struct Group : GroupBase, FixSource<T> {
   GroupMem mem;
   // One record per (column chunk, plan): columns [q0, q0 + nq) of the plan,
   // every chunk in the same launch with partial slots of its own.
   BatchPlan<T> *table = nullptr;
-  int *tile_start = nullptr, *block_start = nullptr;
+  int *tile_start = nullptr;
   T *partial = nullptr;
-  int nrec = 0, ntiles = 0, nblocks = 0, threads = 0;
+  int nrec = 0, ntiles = 0, threads = 0;
   size_t lds = 0;
   bool one = false;  // every record has one column
 
@@ -84,7 +144,7 @@ struct Group : GroupBase {
       while (chunk > 1 && tile_lds_bytes<T, NPE>(dims_of(pl), chunk) > kLdsLimit) chunk--;
     }
     const int nchunks = (nrhs + chunk - 1) / chunk;
-    std::vector<int> ts{0}, bs{0};
+    std::vector<int> ts{0};
     std::vector<size_t> poff;
     size_t ptot = 0;
     for (int c = 0; c < nchunks; c++)
@@ -93,7 +153,6 @@ struct Group : GroupBase {
         poff.push_back(ptot);
         ptot += pl->npartial * (size_t)chunk;
         ts.push_back(ts.back() + (int)pl->ntiles);
-        bs.push_back(bs.back() + (int)((pl->nshared + 255) / 256));
       }
     partial = mem.alloc<T>(std::max<size_t>(1, ptot));
     one = chunk == 1;
@@ -133,14 +192,14 @@ struct Group : GroupBase {
         r.snode = pl->snode.p;
         r.soff = pl->soff.p;
         r.nshared = (int)pl->nshared;
+        this->fix.push_back(FixRec<T>{pl->snode.p, pl->soff.p, r.partial, r.y, A.dir_out, A.cancel, nq, A.ld,
+                                      (int)pl->nshared});
       }
     }
     table = mem.upload(hp);
     nrec = (int)hp.size();
     ntiles = ts.back();
-    nblocks = bs.back();
     tile_start = mem.upload(ts);
-    block_start = mem.upload(bs);
   }
 
   void launch(hipStream_t st) override {
@@ -150,36 +209,9 @@ struct Group : GroupBase {
     else if (ntiles)
       hipLaunchKernelGGL((tile_kernel_batch<T, DIM, P, false>), dim3(ntiles), dim3(threads), lds, st, table,
                          tile_start, nrec);
-    if (nblocks)
-      hipLaunchKernelGGL((fixup_kernel_batch<T>), dim3(nblocks), dim3(256), 0, st, table, block_start, nrec);
     HIP_OK(hipGetLastError());
   }
 };
-
-// Shared nodes of one plan of a batch (the fix-up of either kind of tile).
-template <typename T>
-struct FixRec {
-  const int32_t *snode, *soff;
-  const T *partial;
-  T *y;
-  const uint8_t *dir_out;
-  const int *cancel;
-  int nq, ld, nshared;
-};
-
-// Shared nodes of a batch: workgroup b sums block b - block_start[k] of plan k.
-template <typename T>
-__global__ void __launch_bounds__(256) fixup_kernel_rec(const FixRec<T> *__restrict__ recs,
-                                                        const int *__restrict__ block_start, int nplans) {
-  const int k = find_range(block_start, nplans, blockIdx.x);
-  const FixRec<T> &fr = recs[k];
-  const int i = (blockIdx.x - block_start[k]) * 256 + threadIdx.x;
-  if (i >= fr.nshared || (fr.cancel && *fr.cancel)) return;
-  const int n = fr.snode[i];
-  const int a = fr.soff[i], b = fr.soff[i + 1];
-  const bool z = fr.dir_out && fr.dir_out[n];
-  for (int q = 0; q < fr.nq; q++) fr.y[(size_t)n * fr.ld + q] = z ? T(0) : sum_partials(fr.partial, a, b, fr.nq, q);
-}
 
 // A conforming plan of a batch.
 template <typename T>
@@ -206,13 +238,12 @@ __global__ void __launch_bounds__(kTileThreads) conf_tile_kernel_batch(const Con
 // that fits the tiles' LDS, plan).  Each result is bitwise the plan's own
 // matvec (same tile work, same fix-up order).
 template <typename T, int DIM, int P, bool CAP>
-struct ConfGroup : GroupBase {
+struct ConfGroup : GroupBase, FixSource<T> {
   GroupMem mem;
   ConfBatchPlan<T> *table = nullptr;
-  FixRec<T> *fixes = nullptr;
-  int *tile_start = nullptr, *block_start = nullptr;
+  int *tile_start = nullptr;
   T *partial = nullptr;
-  int nrec = 0, ntiles = 0, nblocks = 0, threads = 0;
+  int nrec = 0, ntiles = 0, threads = 0;
   size_t lds = 0;
 
   ConfGroup(const std::vector<dkt_plan *> &pls, const std::vector<const void *> &xs, const std::vector<void *> &ys,
@@ -228,7 +259,7 @@ struct ConfGroup : GroupBase {
       while (chunk > 1 && lds_of(pl, chunk) > kLdsLimit) chunk--;
     }
     const int nchunks = (nrhs + chunk - 1) / chunk;
-    std::vector<int> ts{0}, bs{0};
+    std::vector<int> ts{0};
     std::vector<size_t> poff;
     size_t ptot = 0;
     for (int c = 0; c < nchunks; c++)
@@ -237,11 +268,11 @@ struct ConfGroup : GroupBase {
         poff.push_back(ptot);
         ptot += pl->npartial * (size_t)chunk;
         ts.push_back(ts.back() + (int)pl->ntiles);
-        bs.push_back(bs.back() + (int)((pl->nshared + 255) / 256));
       }
     partial = mem.alloc<T>(std::max<size_t>(1, ptot));
     std::vector<ConfBatchPlan<T>> hp(pls.size() * nchunks);
-    std::vector<FixRec<T>> hf(pls.size() * nchunks);
+    std::vector<FixRec<T>> &hf = this->fix;
+    hf.resize(pls.size() * nchunks);
     for (int c = 0; c < nchunks; c++) {
       const int q0 = c * chunk, nq = std::min(chunk, nrhs - q0);
       for (size_t k = 0; k < pls.size(); k++) {
@@ -280,20 +311,15 @@ struct ConfGroup : GroupBase {
       }
     }
     table = mem.upload(hp);
-    fixes = mem.upload(hf);
     nrec = (int)hp.size();
     ntiles = ts.back();
-    nblocks = bs.back();
     tile_start = mem.upload(ts);
-    block_start = mem.upload(bs);
   }
 
   void launch(hipStream_t st) override {
     if (ntiles)
       hipLaunchKernelGGL((conf_tile_kernel_batch<T, DIM, P, CAP>), dim3(ntiles), dim3(threads), lds, st, table,
                          tile_start, nrec);
-    if (nblocks)
-      hipLaunchKernelGGL((fixup_kernel_rec<T>), dim3(nblocks), dim3(256), 0, st, fixes, block_start, nrec);
     HIP_OK(hipGetLastError());
   }
 };
@@ -607,10 +633,22 @@ std::map<std::tuple<int, int, int>, std::vector<int>> by_kind_of(dkt_plan *const
   return by_kind;
 }
 
+// One fix-up launch for the shared nodes of every group (the groups' tiles
+// have all run before it; their partial slots and outputs are disjoint).
+template <typename T>
+std::unique_ptr<FixLaunchBase> fix_all(const std::vector<std::unique_ptr<GroupBase>> &groups, DevArena *arena) {
+  std::vector<FixRec<T>> recs;
+  for (const auto &g : groups)
+    if (auto *fs = dynamic_cast<FixSource<T> *>(g.get())) recs.insert(recs.end(), fs->fix.begin(), fs->fix.end());
+  return std::make_unique<FixAll<T>>(recs, arena);
+}
+
 struct GroupsLauncher : MatvecLauncher {
   std::vector<std::unique_ptr<GroupBase>> groups;
+  std::unique_ptr<FixLaunchBase> fixes;
   void launch(hipStream_t st) override {
     for (auto &g : groups) g->launch(st);
+    if (fixes) fixes->launch(st);
   }
 };
 
@@ -635,11 +673,17 @@ std::unique_ptr<MatvecLauncher> dkt::batch_matvec(int dtype, const std::vector<d
     out->groups.emplace_back(dtype == 0 ? make_group<double>(conf, dim, p, pls, xs, ys, 1, mask, cs, arena)
                                         : make_group<float>(conf, dim, p, pls, xs, ys, 1, mask, cs, arena));
   }
+  out->fixes = dtype == 0 ? fix_all<double>(out->groups, arena) : fix_all<float>(out->groups, arena);
   return out;
 }
 
 struct dkt_batch {
   std::vector<std::unique_ptr<GroupBase>> groups;
+  std::unique_ptr<FixLaunchBase> fixes;  // the shared nodes of every group, after their tiles
+  void launch(hipStream_t st) {
+    for (auto &g : groups) g->launch(st);
+    if (fixes) fixes->launch(st);
+  }
   std::function<unsigned int()> error_flag;  // persistent launches: the spin-limit flag
   bool use_graph = false;
   hipGraph_t graph = nullptr;
@@ -705,6 +749,7 @@ extern "C" int dkt_batch_create(dkt_plan *const *plans, int nplans, int dtype, i
                          : make_group<float>(conf, dim, p, pls, xs, ys, nrhs, masked, {}, nullptr, max_cols, cm);
       b->groups.emplace_back(g);
     }
+    b->fixes = dtype == 0 ? fix_all<double>(b->groups, nullptr) : fix_all<float>(b->groups, nullptr);
     b->use_graph = (flags & DKT_BATCH_GRAPH) != 0;
     *out = b.release();
   });
@@ -715,14 +760,14 @@ extern "C" int dkt_batch_run(dkt_batch *b, void *stream) {
     if (!b) throw std::invalid_argument("null pointer");
     hipStream_t st = (hipStream_t)stream;
     if (!b->use_graph) {
-      for (auto &g : b->groups) g->launch(st);
+      b->launch(st);
       return;
     }
     if (!st) throw std::invalid_argument("graph replay needs a non-default stream");
     if (!b->exec) {
       HIP_OK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
       try {
-        for (auto &g : b->groups) g->launch(st);
+        b->launch(st);
       } catch (...) {
         hipGraph_t dropped = nullptr;
         (void)hipStreamEndCapture(st, &dropped);

@@ -573,21 +573,6 @@ __global__ void __launch_bounds__(256) fixup_kernel(int ns, const int32_t *__res
   for (int q = 0; q < nq; q++) y[(size_t)n * ld + q] = z ? T(0) : sum_partials(partial, a, b, nq, q);
 }
 
-// Shared nodes of a batch: workgroup b sums block b - block_start[k] of plan k.
-template <typename T>
-__global__ void __launch_bounds__(256) fixup_kernel_batch(const BatchPlan<T> *__restrict__ plans,
-                                                          const int *__restrict__ block_start, int nplans) {
-  const int k = find_range(block_start, nplans, blockIdx.x);
-  const BatchPlan<T> &bp = plans[k];
-  const int i = (blockIdx.x - block_start[k]) * 256 + threadIdx.x;
-  if (i >= bp.nshared || (bp.A.cancel && *bp.A.cancel)) return;
-  const int nq = bp.A.nq, ld = bp.A.ld;
-  const int n = bp.snode[i];
-  const int a = bp.soff[i], b = bp.soff[i + 1];
-  const bool z = bp.A.dir_out && bp.A.dir_out[n];
-  for (int q = 0; q < nq; q++) bp.y[(size_t)n * ld + q] = z ? T(0) : sum_partials(bp.partial, a, b, nq, q);
-}
-
 // LDS bytes of one tile for nq right-hand sides (layout of tile_kernel).
 template <typename T, int NPE>
 inline size_t tile_lds_bytes(const TileDims &D, int nq) {
