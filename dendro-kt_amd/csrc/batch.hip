@@ -684,6 +684,42 @@ struct dkt_batch {
     for (auto &g : groups) g->launch(st);
     if (fixes) fixes->launch(st);
   }
+  // Graph capture: the groups' tile launches are independent (their records,
+  // partial slots and outputs are disjoint), so each goes on a branch of its
+  // own, forked from and joined back to the caller's stream before the
+  // fix-up; the replayed graph may then run them side by side.  The branch
+  // streams have the default priority.
+  std::vector<hipStream_t> branch;
+  std::vector<hipEvent_t> done;
+  hipEvent_t fork = nullptr;
+  // Streams and events of the branches, made before the capture starts.
+  void prepare_branches() {
+    if (fork || groups.size() < 2) return;
+    HIP_OK(hipEventCreateWithFlags(&fork, hipEventDisableTiming));
+    for (size_t g = 1; g < groups.size(); g++) {
+      hipStream_t s2 = nullptr;
+      HIP_OK(hipStreamCreateWithFlags(&s2, hipStreamNonBlocking));
+      branch.push_back(s2);
+      hipEvent_t e = nullptr;
+      HIP_OK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+      done.push_back(e);
+    }
+  }
+  void capture(hipStream_t st) {
+    if (groups.size() < 2 || !fork) {
+      launch(st);
+      return;
+    }
+    HIP_OK(hipEventRecord(fork, st));
+    for (size_t g = 1; g < groups.size(); g++) {
+      HIP_OK(hipStreamWaitEvent(branch[g - 1], fork, 0));
+      groups[g]->launch(branch[g - 1]);
+      HIP_OK(hipEventRecord(done[g - 1], branch[g - 1]));
+    }
+    groups[0]->launch(st);
+    for (size_t g = 1; g < groups.size(); g++) HIP_OK(hipStreamWaitEvent(st, done[g - 1], 0));
+    if (fixes) fixes->launch(st);
+  }
   std::function<unsigned int()> error_flag;  // persistent launches: the spin-limit flag
   bool use_graph = false;
   hipGraph_t graph = nullptr;
@@ -691,6 +727,9 @@ struct dkt_batch {
   ~dkt_batch() {
     if (exec) (void)hipGraphExecDestroy(exec);
     if (graph) (void)hipGraphDestroy(graph);
+    for (hipEvent_t e : done) (void)hipEventDestroy(e);
+    for (hipStream_t s2 : branch) (void)hipStreamDestroy(s2);
+    if (fork) (void)hipEventDestroy(fork);
   }
 };
 
@@ -765,9 +804,10 @@ extern "C" int dkt_batch_run(dkt_batch *b, void *stream) {
     }
     if (!st) throw std::invalid_argument("graph replay needs a non-default stream");
     if (!b->exec) {
+      b->prepare_branches();
       HIP_OK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
       try {
-        b->launch(st);
+        b->capture(st);
       } catch (...) {
         hipGraph_t dropped = nullptr;
         (void)hipStreamEndCapture(st, &dropped);
