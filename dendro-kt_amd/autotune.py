@@ -9,11 +9,14 @@ waves per SIMD (one column runs the single-column kernels).  ``autotune``
 builds the plan of a sample mesh with each candidate tile size, times the
 FP64 and FP32 matvecs with HIP events on the launch stream (with nrhs > 1 as
 a one-plan batch for each candidate column count), and returns the fastest
-choice per precision.  Results are cached per (dim, order, nrhs) for the
-process and can be stored to a JSON file, so a server tunes once and passes
-``tile_leaves`` (and ``columns`` to Batch) to later plans.
+choice per precision.  Results are cached per (dim, order, nrhs, operator,
+mesh size class) for the process -- the size class is the element count's
+power of four, so a small mesh does not reuse the choice made on a large one
+-- and can be stored to a JSON file, so a server tunes once per class and
+passes ``tile_leaves`` (and ``columns`` to Batch) to later plans.
 """
 import json
+import math
 import os
 
 CANDIDATES = (64, 128, 256)
@@ -30,12 +33,18 @@ def _column_candidates(nrhs):
     return tuple(out) + (0,)  # 0: as many as fit in LDS
 
 
-def autotune(dkt, da, candidates=CANDIDATES, reps=10, warmup=3, nrhs=1):
+def size_class(n_elements):
+    """The mesh size class of the cache key: floor(log4(elements))."""
+    return int(math.log(max(1, n_elements), 4))
+
+
+def autotune(dkt, da, candidates=CANDIDATES, reps=10, warmup=3, nrhs=1, conforming=False):
     """Fastest tile size (and, for nrhs > 1, columns per tile record) per precision
-    for matvecs with `nrhs` columns on `da`: dict(best={"f64": T, "f32": T},
-    columns={"f64": c, "f32": c}, ms={(T, c): {"f64": ms, "f32": ms}})."""
+    for matvecs with `nrhs` columns on `da` (reference-semantics plans, or the
+    conforming operator): dict(best={"f64": T, "f32": T}, columns={"f64": c,
+    "f32": c}, ms={(T, c): {"f64": ms, "f32": ms}})."""
     import torch
-    key = (da.dim, da.order, nrhs)
+    key = (da.dim, da.order, nrhs, int(bool(conforming)), size_class(da.n_elements))
     if key in _cache:
         return _cache[key]
     x64 = torch.tensor(dkt.keyed_uniform(0, da.nodes()), device="cuda")
@@ -44,7 +53,7 @@ def autotune(dkt, da, candidates=CANDIDATES, reps=10, warmup=3, nrhs=1):
     times = {}
     for T in candidates:
         try:
-            plan = dkt.Plan(da, dkt.OP_STIFFNESS, tile_leaves=T)
+            plan = dkt.Plan(da, dkt.OP_STIFFNESS, tile_leaves=T, conforming=conforming)
         except dkt.DktError:
             continue  # does not fit in LDS
         for c in _column_candidates(nrhs):
@@ -89,8 +98,8 @@ def autotune(dkt, da, candidates=CANDIDATES, reps=10, warmup=3, nrhs=1):
 
 def save(path):
     out = {}
-    for k, v in _cache.items():
-        out[f"{k[0]}d_p{k[1]}_q{k[2]}"] = dict(v, ms={f"{T}_{c}": r for (T, c), r in v["ms"].items()})
+    for (dim, p, q, conf, sc), v in _cache.items():
+        out[f"{dim}d_p{p}_q{q}_c{conf}_s{sc}"] = dict(v, ms={f"{T}_{c}": r for (T, c), r in v["ms"].items()})
     with open(path, "w") as f:
         json.dump(out, f, indent=1)
 
@@ -101,8 +110,10 @@ def load(path):
     with open(path) as f:
         for k, v in json.load(f).items():
             dim, rest = k.split("d_p")
-            p, q = rest.split("_q")
+            p, rest = rest.split("_q")
+            q, rest = rest.split("_c")
+            conf, sc = rest.split("_s")
             v["best"] = {n: int(t) for n, t in v["best"].items()}
             v["columns"] = {n: int(c) for n, c in v.get("columns", {"f64": 0, "f32": 0}).items()}
             v["ms"] = {tuple(int(a) for a in t.split("_")): r for t, r in v["ms"].items()}
-            _cache[(int(dim), int(p), int(q))] = v
+            _cache[(int(dim), int(p), int(q), int(conf), int(sc))] = v

@@ -126,8 +126,8 @@ def test_persistent_batch_matches_per_plan(dkt, dtype, nrhs, graph):
 @pytest.mark.gpu
 def test_autotuner_picks_a_candidate_and_caches(dkt):
     """The autotuner times every candidate tile size that fits (with eight columns
-    also every columns-per-launch choice), returns the fastest per precision,
-    and answers again from its cache."""
+    also every columns-per-record choice), returns the fastest per precision,
+    and answers again from its cache; conforming plans are tuned separately."""
     import importlib.util as ilu
     import os
     torch = _torch()
@@ -148,6 +148,12 @@ def test_autotuner_picks_a_candidate_and_caches(dkt):
         assert at.autotune(dkt, da, nrhs=nrhs) is r
         plan = dkt.Plan(da, dkt.OP_STIFFNESS, tile_leaves=r["best"]["f32"])
         assert plan.info()["tile_leaves"] == r["best"]["f32"]
+    # the conforming operator is tuned and cached on its own
+    rc = at.autotune(dkt, da, reps=3, warmup=1, conforming=True)
+    assert rc is not at.autotune(dkt, da, nrhs=1) and rc is at.autotune(dkt, da, conforming=True)
+    assert {T for T, _ in rc["ms"]} <= set(at.CANDIDATES)
+    plan = dkt.Plan(da, dkt.OP_STIFFNESS, conforming=True, tile_leaves=rc["best"]["f64"])
+    assert plan.info()["tile_leaves"] == rc["best"]["f64"]
 
 
 @pytest.mark.gpu

@@ -140,3 +140,29 @@ def test_concurrent_misses_build_once():
     assert fb.device == [(7, 10.0)] and sum(1 for _, built in leases if built) == 1
     for lease, _ in leases:
         lease.release()
+
+
+def _autotune_mod():
+    spec = importlib.util.spec_from_file_location("dkt_autotune_host", os.path.join(REPO, "dendro-kt_amd",
+                                                                                   "autotune.py"))
+    m = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(m)
+    return m
+
+
+def test_autotune_cache_key_and_file_round_trip(tmp_path):
+    """The autotuner's cache is keyed by (dim, order, columns, operator, size
+    class), the size class being the element count's power of four, and a
+    saved cache loads back to the same entries."""
+    at = _autotune_mod()
+    assert [at.size_class(n) for n in (1, 3, 4, 15, 16, 1000, 1 << 20)] == [0, 0, 1, 1, 2, 4, 10]
+    entry = {"best": {"f64": 128, "f32": 256}, "columns": {"f64": 0, "f32": 1},
+             "ms": {(128, 0): {"f64": 1.0, "f32": 0.7}, (256, 1): {"f64": 1.2, "f32": 0.6}}}
+    at._cache[(4, 1, 8, 0, 9)] = entry
+    at._cache[(3, 2, 1, 1, 5)] = dict(entry, columns={"f64": 0, "f32": 0})
+    path = str(tmp_path / "tune.json")
+    at.save(path)
+    saved = dict(at._cache)
+    at._cache.clear()
+    at.load(path)
+    assert at._cache == saved
