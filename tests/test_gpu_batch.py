@@ -159,11 +159,11 @@ def test_autotuner_picks_a_candidate_and_caches(dkt):
 @pytest.mark.gpu
 @pytest.mark.parametrize("dtype", ["f64", "f32"])
 def test_batch_columns_per_launch_bitwise(dkt, dtype):
-    """Capping the columns per launch (1 runs the single-column kernels once per
-    column over the same [N][nrhs] vectors; 2 and 4 run narrower multi-column
-    launches), and column-major [nrhs][N] vectors (one contiguous column per
-    launch), give every column bitwise the result of the default batch, for
-    p = 1 and p = 2 trees and for conforming plans."""
+    """Capping the columns per tile record (1 runs the single-column kernels, a
+    record per column, over the same [N][nrhs] vectors; 2 and 4 run narrower
+    multi-column records), and column-major [nrhs][N] vectors (one contiguous
+    column per record), give every column bitwise the result of the default
+    batch, for p = 1 and p = 2 trees and for conforming plans."""
     torch = _torch()
     dt = torch.float64 if dtype == "f64" else torch.float32
     das = [dkt.DA(dkt.balance_tree(dkt.sphere_tree(3, 30, 2, lev, c, 0.2), 1), p)
