@@ -300,9 +300,11 @@ int dkt_cg_engine_destroy(dkt_cg_engine *engine);
 /* ---- Batches ----------------------------------------------------------- */
 /* One launch per (dim, p) over the tiles of many plans (independent meshes),
  * each with its own device vectors x[k], y[k] of [N_k][nrhs] (dtype 0 f64,
- * 1 f32), plus one batched fix-up launch.  flags DKT_BATCH_GRAPH: the launches
- * are captured into a hipGraph on the first dkt_batch_run and replayed after
- * (the stream must not be the default stream).  Plans and vectors are bound at
+ * 1 f32), plus one fix-up launch for the shared nodes of every plan.  flags
+ * DKT_BATCH_GRAPH: the launches are captured into a hipGraph on the first
+ * dkt_batch_run (the (dim, p) launches on parallel branches, joined before the
+ * fix-up) and replayed after (the stream must not be the default stream;
+ * the branches use streams of default priority).  Plans and vectors are bound at
  * creation and must outlive the batch.  Every y[k] equals dkt_plan_matvec of
  * plan k bit for bit.  No counterpart in the reference (one fem::matvec per
  * call, FEM/include/matvec.h:231-240). */
