@@ -115,7 +115,11 @@ __device__ __forceinline__ void conf_tile_body(const ConfArgs &A, int tile, cons
   auto level_of = [&]() -> int { return LEV_IN_ROW ? (int)(lw[NW - 1] >> 16) : lev; };
   const int npts = th.nnode + th.nv;
   // Staging in batches of kCB per work-item: every load of a batch is issued
-  // before its stores, so a batch costs one memory round trip, not kCB.
+  // before any of its values is used, so a batch costs one memory round trip
+  // after its node ids, not kCB.  The first column's values are loaded before
+  // the Dirichlet flags, and the flags unconditionally (entries past the
+  // tile's nodes read node 0's): a flag tested inside a per-entry branch is
+  // waited for there, one round trip per entry.
   constexpr int kCB = 4;
   for (int j0 = tid; j0 < th.nnode; j0 += kCB * nthr) {
     int g[kCB];
@@ -124,17 +128,22 @@ __device__ __forceinline__ void conf_tile_body(const ConfArgs &A, int tile, cons
       const int j = j0 + b * nthr;
       g[b] = j < th.nnode ? A.tile_nodes[th.node0 + j] : -1;
     }
-    for (int q = 0; q < nq; q++) {
-      T xv[kCB];
-      bool z[kCB];
+    T xv[kCB];
 #pragma unroll
-      for (int b = 0; b < kCB; b++) {
-        z[b] = g[b] >= 0 && A.dir_in && A.dir_in[g[b]];
-        xv[b] = g[b] >= 0 ? x[(size_t)g[b] * A.ld + q] : T(0);
-      }
+    for (int b = 0; b < kCB; b++) xv[b] = g[b] >= 0 ? x[(size_t)g[b] * A.ld] : T(0);
+    uint32_t d[kCB];
+#pragma unroll
+    for (int b = 0; b < kCB; b++) d[b] = 0u;
+    if (A.dir_in)
+#pragma unroll
+      for (int b = 0; b < kCB; b++) d[b] = A.dir_in[g[b] >= 0 ? g[b] : 0];
+    for (int q = 0; q < nq; q++) {
+      if (q > 0)
+#pragma unroll
+        for (int b = 0; b < kCB; b++) xv[b] = g[b] >= 0 ? x[(size_t)g[b] * A.ld + q] : T(0);
 #pragma unroll
       for (int b = 0; b < kCB; b++)
-        if (g[b] >= 0) x_loc[(j0 + b * nthr) * nq + q] = z[b] ? T(0) : xv[b];
+        if (g[b] >= 0) x_loc[(j0 + b * nthr) * nq + q] = d[b] ? T(0) : xv[b];
     }
   }
   if (tid < A.nw) wt[tid] = w_first;
@@ -142,15 +151,28 @@ __device__ __forceinline__ void conf_tile_body(const ConfArgs &A, int tile, cons
   if (nq > 1)
     for (int j = tid; j < npts * nq; j += nthr) y_loc[j] = T(0);
   __syncthreads();
-  // Hanging point values: each row over the staged nodes, terms in row order.
+  // Hanging point values: each row over the staged nodes, terms in row order,
+  // read kTB at a time with all loads of a batch issued together.  Entries
+  // past the row's end repeat its last term with the value 0: they add an
+  // exact +-0 to a sum that starts at +0 and so is never -0 (round to nearest
+  // cancels to +0), which leaves its bits those of the row's own terms.  A
+  // guarded add instead lets the compiler move the load into the guard and
+  // wait for it there.
+  constexpr int kTB = 4;
   const uint32_t *tpk = A.term_pk + th.t0;
   for (int v = tid; v < th.nv; v += nthr) {
     const int k0 = A.virt_off[th.v0 + v], k1 = v + 1 < th.nv ? A.virt_off[th.v0 + v + 1] : th.nt;
     for (int q = 0; q < nq; q++) {
       T s = T(0);
-      for (int k = k0; k < k1; k++) {
-        const uint32_t e = tpk[k];
-        s += wt[e >> 16] * x_loc[(int)(e & 0xFFFFu) * nq + q];
+      for (int k = k0; k < k1; k += kTB) {
+        uint32_t e[kTB];
+#pragma unroll
+        for (int b = 0; b < kTB; b++) e[b] = tpk[min(k + b, k1 - 1)];
+#pragma unroll
+        for (int b = 0; b < kTB; b++) {
+          const T xv = x_loc[(int)(e[b] & 0xFFFFu) * nq + q];
+          s += wt[e[b] >> 16] * (k + b < k1 ? xv : T(0));
+        }
       }
       x_loc[(th.nnode + v) * nq + q] = s;
     }
@@ -220,9 +242,15 @@ __device__ __forceinline__ void conf_tile_body(const ConfArgs &A, int tile, cons
     const int m1 = g + 1 < th.ns ? (int)(A.seg[th.s0 + g + 1] >> 16) : th.nt;
     for (int q = 0; q < nq; q++) {
       T acc = T(0);
-      for (int m = m0; m < m1; m++) {
-        const uint32_t e = spk[m];
-        acc += wt[e >> 16] * y_loc[(th.nnode + (int)(e & 0xFFFFu)) * nq + q];
+      for (int m = m0; m < m1; m += kTB) {
+        uint32_t e[kTB];
+#pragma unroll
+        for (int b = 0; b < kTB; b++) e[b] = spk[min(m + b, m1 - 1)];
+#pragma unroll
+        for (int b = 0; b < kTB; b++) {  // padding adds an exact +-0, as in the rows
+          const T yv = y_loc[(th.nnode + (int)(e[b] & 0xFFFFu)) * nq + q];
+          acc += wt[e[b] >> 16] * (m + b < m1 ? yv : T(0));
+        }
       }
       y_loc[j * nq + q] += acc;
     }
@@ -237,15 +265,18 @@ __device__ __forceinline__ void conf_tile_body(const ConfArgs &A, int tile, cons
       const int j = j0 + b * nthr;
       o[b] = j < th.nnode ? A.tile_out[th.node0 + j] : INT32_MIN;
     }
-    bool z[kCB];
+    uint32_t d[kCB];
 #pragma unroll
-    for (int b = 0; b < kCB; b++) z[b] = o[b] >= 0 && A.dir_out && A.dir_out[o[b]];
+    for (int b = 0; b < kCB; b++) d[b] = 0u;
+    if (A.dir_out)
+#pragma unroll
+      for (int b = 0; b < kCB; b++) d[b] = A.dir_out[o[b] >= 0 ? o[b] : 0];
 #pragma unroll
     for (int b = 0; b < kCB; b++) {
       const int j = j0 + b * nthr;
       if (o[b] == INT32_MIN) continue;
       if (o[b] >= 0) {
-        for (int q = 0; q < nq; q++) y[(size_t)o[b] * A.ld + q] = z[b] ? T(0) : y_loc[j * nq + q];
+        for (int q = 0; q < nq; q++) y[(size_t)o[b] * A.ld + q] = d[b] ? T(0) : y_loc[j * nq + q];
       } else {
         for (int q = 0; q < nq; q++) partial[(size_t)(-o[b] - 1) * nq + q] = y_loc[j * nq + q];
       }
