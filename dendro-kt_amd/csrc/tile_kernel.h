@@ -242,7 +242,10 @@ __device__ __forceinline__ void tile_body(const TileArgs &A, int tile, const T *
     he_first[b] = k < th.nhe ? A.hentries[th.he0 + k] : 0u;
   }
   const uint16_t rd_first = tid < th.nrd ? A.readers[th.rd0 + tid] : kNone;
-  // Node gathers in batches of kNB per work-item, all loads issued before use.
+  // Node gathers in batches of kNB per work-item, all loads issued before use:
+  // the first column's values, then the Dirichlet flags, unconditionally
+  // (entries past the tile's nodes read node 0's; a flag tested inside a
+  // per-entry branch is waited for there, one round trip per entry).
   for (int j0 = tid; j0 < th.nnode; j0 += kNB * nthr) {
     int g[kNB];
 #pragma unroll
@@ -250,18 +253,24 @@ __device__ __forceinline__ void tile_body(const TileArgs &A, int tile, const T *
       const int j = j0 + b * nthr;
       g[b] = j < th.nnode ? A.tile_nodes[th.node0 + j] : -1;
     }
-    bool z[kNB];
+    T xv[kNB];
 #pragma unroll
-    for (int b = 0; b < kNB; b++) z[b] = g[b] >= 0 && A.dir_in && A.dir_in[g[b]];
+    for (int b = 0; b < kNB; b++) xv[b] = g[b] >= 0 ? x[(size_t)g[b] * A.ld] : T(0);
+    uint32_t d[kNB];
+#pragma unroll
+    for (int b = 0; b < kNB; b++) d[b] = 0u;
+    if (A.dir_in)
+#pragma unroll
+      for (int b = 0; b < kNB; b++) d[b] = A.dir_in[g[b] >= 0 ? g[b] : 0];
     for (int q = 0; q < nq; q++) {
-      T xv[kNB];
+      if (q > 0)
 #pragma unroll
-      for (int b = 0; b < kNB; b++) xv[b] = g[b] >= 0 ? x[(size_t)g[b] * A.ld + q] : T(0);
+        for (int b = 0; b < kNB; b++) xv[b] = g[b] >= 0 ? x[(size_t)g[b] * A.ld + q] : T(0);
 #pragma unroll
       for (int b = 0; b < kNB; b++) {
         const int j = j0 + b * nthr;
         if (g[b] >= 0) {
-          x_loc[j * nq + q] = z[b] ? T(0) : xv[b];
+          x_loc[j * nq + q] = d[b] ? T(0) : xv[b];
           if constexpr (!ONE) y_loc[j * nq + q] = T(0);
         }
       }
@@ -286,14 +295,32 @@ __device__ __forceinline__ void tile_body(const TileArgs &A, int tile, const T *
   __syncthreads();
 
   // 2. Parent-buffer state at the tile start, every column: slot s sums its
-  //    terms (a header of slot offsets, then the terms by slot) in plan order.
+  //    terms (a header of slot offsets, then the terms by slot) in plan order,
+  //    read kNB at a time with all loads of a batch issued together.  Entries
+  //    past the slot's last term repeat it with the value 0: an exact +-0
+  //    added to a sum that starts at +0 and so is never -0, which leaves its
+  //    bits those of the slot's own terms.
   if (tid < NPE) {
     const uint32_t *ck = A.carry_key + th.cr0;
     const T *cwp = carry_w + th.cr0;
     const int lo = th.ncr > 0 ? (int)ck[tid] : 0, hi = th.ncr > 0 ? (int)ck[tid + 1] : 0;
     for (int q = 0; q < nq; q++) {
       T acc = T(0);
-      for (int c = lo; c < hi; c++) acc += cwp[c] * x_loc[(ck[c] & 0xFFFFu) * nq + q];
+      for (int c = lo; c < hi; c += kNB) {
+        uint32_t kk[kNB];
+        T cwv[kNB];
+#pragma unroll
+        for (int b = 0; b < kNB; b++) {
+          const int cc = min(c + b, hi - 1);
+          kk[b] = ck[cc];
+          cwv[b] = cwp[cc];
+        }
+#pragma unroll
+        for (int b = 0; b < kNB; b++) {
+          const T xv = x_loc[(kk[b] & 0xFFFFu) * nq + q];
+          acc += cwv[b] * (c + b < hi ? xv : T(0));
+        }
+      }
       cst[tid * nq + q] = acc;
     }
   }
@@ -473,9 +500,12 @@ __device__ __forceinline__ void tile_body(const TileArgs &A, int tile, const T *
       const int j = j0 + b * nthr;
       o[b] = j0 == tid ? o_first[b] : (j < th.nnode ? A.tile_out[th.node0 + j] : INT32_MIN);
     }
-    bool z[kNB];
+    uint32_t d[kNB];  // Dirichlet flags, loaded unconditionally as in the stage
 #pragma unroll
-    for (int b = 0; b < kNB; b++) z[b] = o[b] >= 0 && A.dir_out && A.dir_out[o[b]];
+    for (int b = 0; b < kNB; b++) d[b] = 0u;
+    if (A.dir_out)
+#pragma unroll
+      for (int b = 0; b < kNB; b++) d[b] = A.dir_out[o[b] >= 0 ? o[b] : 0];
 #pragma unroll
     for (int b = 0; b < kNB; b++) {
       const int j = j0 + b * nthr;
@@ -483,7 +513,7 @@ __device__ __forceinline__ void tile_body(const TileArgs &A, int tile, const T *
       for (int q = 0; q < nq; q++) {
         const T acc = y_loc[j * nq + q];
         if (o[b] >= 0)
-          y[(size_t)o[b] * A.ld + q] = z[b] ? T(0) : acc;
+          y[(size_t)o[b] * A.ld + q] = d[b] ? T(0) : acc;
         else
           partial[(size_t)(-o[b] - 1) * nq + q] = acc;
       }
