@@ -96,6 +96,12 @@ __host__ __device__ inline int cg_blocks(int n) {
 #define CG_LOOP(i, n) \
   for (int i = blockIdx.x * kCgThreads + threadIdx.x; i < (n); i += cg_blocks(n) * kCgThreads)
 
+// The iteration kernels' loops take kCgUnroll of a work-item's entries at a
+// time with every load issued before any use (one memory round trip per batch
+// where the plain loop waits once per entry), then handle them in index order:
+// each work-item's sums keep the order, and the bits, of the plain loop.
+constexpr int kCgUnroll = 4;
+
 // The bodies of the iteration kernels run one solve's block blockIdx.x of
 // kCgBlocks; the single-solve kernels and the batched ones (one solve per
 // blockIdx.y) share them, so a solve's arithmetic and reduction order do not
@@ -156,7 +162,19 @@ __device__ __forceinline__ void dot_body(int n, const T *__restrict__ a, const T
   if (flag[0]) return;
   __shared__ double sh[kCgThreads];
   double acc = 0;
-  CG_LOOP(i, n) acc += (double)a[i] * (double)b[i];
+  const int stride = cg_blocks(n) * kCgThreads;
+  int i = blockIdx.x * kCgThreads + threadIdx.x;
+  for (; i < n - (kCgUnroll - 1) * stride; i += kCgUnroll * stride) {
+    T av[kCgUnroll], bv[kCgUnroll];
+#pragma unroll
+    for (int u = 0; u < kCgUnroll; u++) {
+      av[u] = a[i + u * stride];
+      bv[u] = b[i + u * stride];
+    }
+#pragma unroll
+    for (int u = 0; u < kCgUnroll; u++) acc += (double)av[u] * (double)bv[u];
+  }
+  for (; i < n; i += stride) acc += (double)a[i] * (double)b[i];
   acc = block_sum(acc, sh);
   if (threadIdx.x == 0) part[blockIdx.x] = acc;
 }
@@ -185,7 +203,29 @@ __device__ __forceinline__ void update_xr_body(int n, T *__restrict__ x, T *__re
   }
   const T alpha = (T)a64;
   double rr = 0, mr = 0;
-  CG_LOOP(i, n) {
+  const int stride = cg_blocks(n) * kCgThreads;
+  int i = blockIdx.x * kCgThreads + threadIdx.x;
+  for (; i < n - (kCgUnroll - 1) * stride; i += kCgUnroll * stride) {
+    T xv[kCgUnroll], pv[kCgUnroll], rv[kCgUnroll], av[kCgUnroll];
+#pragma unroll
+    for (int u = 0; u < kCgUnroll; u++) {
+      const int k = i + u * stride;
+      xv[u] = x[k];
+      pv[u] = p[k];
+      rv[u] = r[k];
+      av[u] = ap[k];
+    }
+#pragma unroll
+    for (int u = 0; u < kCgUnroll; u++) {
+      const int k = i + u * stride;
+      x[k] = xv[u] + alpha * pv[u];
+      const T ri = rv[u] - alpha * av[u];
+      r[k] = ri;
+      rr += (double)ri * (double)ri;
+      mr = fmax(mr, (double)fabs(ri));
+    }
+  }
+  for (; i < n; i += stride) {
     x[i] += alpha * p[i];
     const T ri = r[i] - alpha * ap[i];
     r[i] = ri;
@@ -222,7 +262,19 @@ __device__ __forceinline__ void update_p_body(int n, T *__restrict__ p, const T 
   const bool done = resid <= tol;
   if (!done) {
     const T beta = (T)(rr1 / scal[S_RR0 + par]);
-    CG_LOOP(i, n) p[i] = r[i] + beta * p[i];
+    const int stride = cg_blocks(n) * kCgThreads;
+    int i = blockIdx.x * kCgThreads + threadIdx.x;
+    for (; i < n - (kCgUnroll - 1) * stride; i += kCgUnroll * stride) {
+      T rv[kCgUnroll], pv[kCgUnroll];
+#pragma unroll
+      for (int u = 0; u < kCgUnroll; u++) {
+        rv[u] = r[i + u * stride];
+        pv[u] = p[i + u * stride];
+      }
+#pragma unroll
+      for (int u = 0; u < kCgUnroll; u++) p[i + u * stride] = rv[u] + beta * pv[u];
+    }
+    for (; i < n; i += stride) p[i] = r[i] + beta * p[i];
   }
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     scal[S_RR0 + (par ^ 1)] = rr1;
